@@ -1,0 +1,359 @@
+"""ctypes binding of libfarm.so (include/farm_capi.h, include/farm_host.h).
+
+This is the host-side mirror of the reference's query driver: load polygon
+sets (.dt / WKT), generate candidate pairs with the host R-tree, rasterize on
+the host, upload to the GPU and run the polygon-join kernels.  The library is
+required: importing this module raises if libfarm.so is missing, and there is
+no CPU fallback for any device call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfarm.so")
+
+FARM_OK = 0
+FARM_EINVAL = 1
+FARM_ENOMEM = 2
+FARM_EDEVICE = 3
+FARM_ECAPACITY = 4
+FARM_ESTATE = 5
+
+MODE_EXACT_PLAIN = 0
+MODE_EXACT_BATCHED = 1
+MODE_APPROX_HEURISTIC = 2
+MODE_APPROX_LEARNED = 3
+
+
+class FarmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libfarm error {code}: {msg}")
+        self.code = code
+
+
+class RasterHdr(C.Structure):
+    _fields_ = [
+        ("mbr", C.c_double * 4),
+        ("step", C.c_double * 2),
+        ("dimx", C.c_int32),
+        ("dimy", C.c_int32),
+        ("pix_start", C.c_uint64),
+        ("pixoff_start", C.c_uint64),
+        ("seq_start", C.c_uint64),
+        ("vert_start", C.c_uint64),
+        ("nverts", C.c_uint32),
+        ("nseqs", C.c_uint32),
+        ("vline_start", C.c_uint64),
+        ("vnode_start", C.c_uint64),
+        ("reserved", C.c_uint64 * 2),
+    ]
+
+
+assert C.sizeof(RasterHdr) == 128
+
+HDR_DTYPE = np.dtype(
+    [
+        ("mbr", "<f8", (4,)),
+        ("step", "<f8", (2,)),
+        ("dimx", "<i4"),
+        ("dimy", "<i4"),
+        ("pix_start", "<u8"),
+        ("pixoff_start", "<u8"),
+        ("seq_start", "<u8"),
+        ("vert_start", "<u8"),
+        ("nverts", "<u4"),
+        ("nseqs", "<u4"),
+        ("vline_start", "<u8"),
+        ("vnode_start", "<u8"),
+        ("reserved", "<u8", (2,)),
+    ]
+)
+assert HDR_DTYPE.itemsize == 128
+
+
+class PolySet(C.Structure):
+    _fields_ = [
+        ("npolys", C.c_int64),
+        ("hdr", C.POINTER(RasterHdr)),
+        ("status", C.POINTER(C.c_uint8)),
+        ("nstatus", C.c_uint64),
+        ("pixoff", C.POINTER(C.c_uint32)),
+        ("npixoff", C.c_uint64),
+        ("seqs", C.POINTER(C.c_uint32)),
+        ("nseqs", C.c_uint64),
+        ("x", C.POINTER(C.c_float)),
+        ("y", C.POINTER(C.c_float)),
+        ("nverts", C.c_uint64),
+        ("voff", C.POINTER(C.c_uint32)),
+        ("nvoff", C.c_uint64),
+        ("vnodes", C.POINTER(C.c_double)),
+        ("nvnodes", C.c_uint64),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("tile_a", C.c_int32),
+        ("tile_b", C.c_int32),
+        ("order", C.c_int32),
+        ("approx_threshold", C.c_float),
+        ("merge_threshold", C.c_float),
+        ("reserved", C.c_int32 * 2),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("pairs", C.c_uint64),
+        ("in_hits", C.c_uint64),
+        ("bb_pixel_pairs", C.c_uint64),
+        ("refine_items", C.c_uint64),
+        ("segment_tests", C.c_uint64),
+        ("filter_bytes", C.c_uint64),
+        ("refine_bytes", C.c_uint64),
+        ("found", C.c_uint64),
+        ("filter_ms", C.c_double),
+        ("refine_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Every entry point declared in include/farm_capi.h and include/farm_host.h.
+EXPORTS = {
+    "farm_version": (C.c_char_p, []),
+    "farm_open": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "farm_close": (C.c_int, [C.c_void_p]),
+    "farm_last_error": (C.c_char_p, [C.c_void_p]),
+    "farm_upload_polygons": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
+    "farm_upload_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "farm_intersect": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)]),
+    "farm_intersect_dev": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)],
+    ),
+    "farm_hostset_from_rings": (C.c_void_p, [C.c_int64, C.c_void_p, C.c_void_p]),
+    "farm_hostset_load_dt": (C.c_void_p, [C.c_char_p]),
+    "farm_hostset_load_wkt": (C.c_void_p, [C.c_char_p]),
+    "farm_hostset_save_dt": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "farm_hostset_concat": (C.c_void_p, [C.c_void_p, C.c_void_p]),
+    "farm_hostset_free": (None, [C.c_void_p]),
+    "farm_hostset_size": (C.c_int64, [C.c_void_p]),
+    "farm_hostset_ring": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    "farm_hostset_mbbs": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "farm_hostset_rasterize": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "farm_hostset_view": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
+    "farm_hostset_areas": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
+    "farm_hostset_raster_failures": (C.c_int64, [C.c_void_p]),
+    "farm_candidates": (
+        C.c_int64,
+        [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    ),
+    "farm_free": (None, [C.c_void_p]),
+}
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class HostSet:
+    """A polygon set on the host (rings as stored in .dt files)."""
+
+    def __init__(self, handle: int):
+        if not handle:
+            raise FarmError(FARM_EINVAL, "could not build polygon set")
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_rings(cls, ring_off: np.ndarray, xy: np.ndarray) -> "HostSet":
+        ring_off = np.ascontiguousarray(ring_off, dtype=np.uint64)
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        return cls(lib.farm_hostset_from_rings(len(ring_off) - 1, _ptr(ring_off), _ptr(xy)))
+
+    @classmethod
+    def load_dt(cls, path: str) -> "HostSet":
+        return cls(lib.farm_hostset_load_dt(path.encode()))
+
+    @classmethod
+    def load_wkt(cls, path: str) -> "HostSet":
+        return cls(lib.farm_hostset_load_wkt(path.encode()))
+
+    @classmethod
+    def concat(cls, a: "HostSet", b: "HostSet") -> "HostSet":
+        return cls(lib.farm_hostset_concat(a._h, b._h))
+
+    def save_dt(self, path: str) -> None:
+        rc = lib.farm_hostset_save_dt(self._h, path.encode())
+        if rc:
+            raise FarmError(rc, f"cannot write {path}")
+
+    def __len__(self) -> int:
+        return int(lib.farm_hostset_size(self._h))
+
+    def ring(self, i: int) -> np.ndarray:
+        p = C.c_void_p()
+        n = C.c_int64()
+        rc = lib.farm_hostset_ring(self._h, i, C.byref(p), C.byref(n))
+        if rc:
+            raise FarmError(rc, f"no polygon {i}")
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n.value, 2)).copy()
+
+    def mbbs(self) -> np.ndarray:
+        out = np.zeros((len(self), 4), dtype=np.float64)
+        lib.farm_hostset_mbbs(self._h, _ptr(out))
+        return out
+
+    def rasterize(self, vpr: int = 10, nthreads: int | None = None) -> None:
+        rc = lib.farm_hostset_rasterize(self._h, vpr, nthreads or default_threads())
+        if rc:
+            raise FarmError(rc, "rasterize failed")
+
+    def raster_failures(self) -> int:
+        return int(lib.farm_hostset_raster_failures(self._h))
+
+    def view(self) -> PolySet:
+        ps = PolySet()
+        rc = lib.farm_hostset_view(self._h, C.byref(ps))
+        if rc:
+            raise FarmError(rc, "polygon set is not rasterized")
+        return ps
+
+    def layout(self) -> dict:
+        """Numpy views (no copy) of the device layout, valid while self lives."""
+        ps = self.view()
+
+        def arr(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,))
+
+        hdr = np.ctypeslib.as_array(C.cast(ps.hdr, C.POINTER(C.c_uint8)), shape=(ps.npolys * 128,)).view(HDR_DTYPE)
+        return {
+            "hdr": hdr,
+            "status": arr(ps.status, ps.nstatus, np.uint8),
+            "pixoff": arr(ps.pixoff, ps.npixoff, np.uint32),
+            "seqs": arr(ps.seqs, 2 * ps.nseqs, np.uint32).reshape(-1, 2),
+            "x": arr(ps.x, ps.nverts, np.float32),
+            "y": arr(ps.y, ps.nverts, np.float32),
+            "voff": arr(ps.voff, ps.nvoff, np.uint32),
+            "vnodes": arr(ps.vnodes, ps.nvnodes, np.float64),
+        }
+
+    def areas(self, i: int, npix: int) -> np.ndarray:
+        p = C.c_void_p()
+        rc = lib.farm_hostset_areas(self._h, i, C.byref(p))
+        if rc:
+            raise FarmError(rc, "no areas")
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(npix,)).copy()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.farm_hostset_free(h)
+            self._h = None
+
+
+def candidates(a: HostSet, b: HostSet, expand_km: float = 0.0, skip_self: bool = False,
+               nthreads: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """All (i in a, j in b) with inclusively overlapping MBBs, sorted by (j, i)."""
+    pa = C.c_void_p()
+    pb = C.c_void_p()
+    n = lib.farm_candidates(a._h, b._h, expand_km, int(skip_self), nthreads or default_threads(), C.byref(pa),
+                            C.byref(pb))
+    if n < 0:
+        raise FarmError(FARM_ENOMEM, "candidate generation failed")
+    try:
+        ia = np.ctypeslib.as_array(C.cast(pa, C.POINTER(C.c_uint32)), shape=(max(n, 1),))[:n].copy()
+        ib = np.ctypeslib.as_array(C.cast(pb, C.POINTER(C.c_uint32)), shape=(max(n, 1),))[:n].copy()
+    finally:
+        lib.farm_free(pa)
+        lib.farm_free(pb)
+    return ia, ib
+
+
+@dataclass
+class IntersectResult:
+    bits: np.ndarray | None
+    stats: dict
+
+
+class Device:
+    """One libfarm handle on one GPU (one HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib.farm_open(device, C.byref(h))
+        if rc:
+            raise FarmError(rc, f"farm_open({device}) failed")
+        self._h = h
+        self.npairs = 0
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc:
+            raise FarmError(rc, f"{what}: {lib.farm_last_error(self._h).decode()}")
+
+    def upload_polygons(self, s: HostSet) -> None:
+        ps = s.view()
+        self._check(lib.farm_upload_polygons(self._h, C.byref(ps)), "upload_polygons")
+
+    def upload_pairs(self, a: np.ndarray, b: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        self._check(lib.farm_upload_pairs(self._h, _ptr(a), _ptr(b), len(a)), "upload_pairs")
+        self.npairs = len(a)
+
+    def intersect(self, begin: int = 0, end: int | None = None, mode: int = MODE_EXACT_BATCHED,
+                  tile_a: int = 0, want_bits: bool = True) -> IntersectResult:
+        end = self.npairs if end is None else end
+        o = Opts(mode=mode, tile_a=tile_a)
+        st = Stats()
+        out = np.zeros(max(end - begin, 1), dtype=np.uint8) if want_bits else None
+        rc = lib.farm_intersect(self._h, begin, end, C.byref(o), _ptr(out) if want_bits else None, C.byref(st))
+        self._check(rc, "intersect")
+        return IntersectResult(out[: end - begin] if want_bits else None, st.as_dict())
+
+    def intersect_dev(self, out_dev_ptr: int, begin: int = 0, end: int | None = None,
+                      mode: int = MODE_EXACT_BATCHED, tile_a: int = 0) -> dict:
+        end = self.npairs if end is None else end
+        o = Opts(mode=mode, tile_a=tile_a)
+        st = Stats()
+        rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr), C.byref(st))
+        self._check(rc, "intersect_dev")
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.farm_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,423 @@
+// C ABI of libfarm.so: device memory, uploads and the intersect driver.
+// Declarations and their reference counterparts: include/farm_capi.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <hipcub/hipcub.hpp>
+#include <string>
+#include <vector>
+
+#include "device_layout.h"
+
+namespace farm {
+__global__ void k_filter(DevSet, const uint32_t *, const uint32_t *, uint32_t, int, int, uint32_t *, const uint32_t *,
+                         PixPair *, uint32_t *, unsigned long long *);
+__global__ void k_bucket_hist(DevSet, const uint32_t *, const uint32_t *, const PixPair *, uint32_t, const float *,
+                              uint8_t *, uint32_t *);
+__global__ void k_bucket_scan(uint32_t *, uint32_t *);
+__global__ void k_bucket_scatter(const uint8_t *, uint32_t, uint32_t *, uint32_t *);
+template <int E>
+__global__ void k_tiles(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
+template <int E>
+__global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
+                         const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+__global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
+__global__ void k_pixedges(DevSet, int64_t, uint32_t *);
+__global__ void k_finalize(const uint32_t *, uint32_t, uint8_t *, unsigned long long *);
+}  // namespace farm
+
+using namespace farm;
+
+namespace {
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct farm_handle_s {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::string err;
+    // polygon set
+    int64_t npolys = 0;
+    DevBuf hdr, status, pixoff, seqs, x, y, voff, vnodes, pixedges;
+    uint64_t nstatus = 0;
+    // pairs
+    uint64_t npairs = 0;
+    DevBuf pa, pb;
+    // per-call scratch
+    DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, ctr, hist, cursor, scan_tmp;
+};
+
+namespace {
+
+int fail(farm_handle h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((h), FARM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                     \
+    } while (0)
+
+int ensure(farm_handle h, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return FARM_OK;
+    if (b.p) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(h, FARM_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    b.cap = bytes;
+    return FARM_OK;
+}
+
+int upload(farm_handle h, DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(h, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(h, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return FARM_OK;
+}
+
+template <class T>
+T *P(DevBuf &b) {
+    return reinterpret_cast<T *>(b.p);
+}
+
+DevSet devset(farm_handle h) {
+    DevSet S;
+    S.hdr = P<farm_raster_hdr>(h->hdr);
+    S.status = P<uint8_t>(h->status);
+    S.pixoff = P<uint32_t>(h->pixoff);
+    S.seqs = P<uint2>(h->seqs);
+    S.x = P<float>(h->x);
+    S.y = P<float>(h->y);
+    S.pixedges = P<uint32_t>(h->pixedges);
+    S.voff = P<uint32_t>(h->voff);
+    S.vnodes = P<double>(h->vnodes);
+    return S;
+}
+
+unsigned blocks_for(uint64_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
+
+int fill_u32(farm_handle h, void *p, uint64_t n, uint32_t v) {
+    if (!n) return FARM_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(256), 0, h->stream, (uint32_t *)p, n, v);
+    HIPCHK(h, hipGetLastError());
+    return FARM_OK;
+}
+
+template <class T>
+int exclusive_scan(farm_handle h, const T *in, T *out, uint64_t n) {
+    size_t tmp = 0;
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, h->stream));
+    int rc = ensure(h, h->scan_tmp, tmp);
+    if (rc) return rc;
+    HIPCHK(h, hipcub::DeviceScan::ExclusiveSum(h->scan_tmp.p, tmp, in, out, (int)n, h->stream));
+    return FARM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *farm_version(void) { return "farm-mi355x 0.1 (gfx950)"; }
+
+int farm_open(int device, farm_handle *out) {
+    if (!out) return FARM_EINVAL;
+    *out = nullptr;
+    farm_handle h = new farm_handle_s();
+    h->device = device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        delete h;
+        return FARM_EDEVICE;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        delete h;
+        return FARM_EDEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete h;
+        return FARM_EDEVICE;
+    }
+    h->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return FARM_EDEVICE;
+    }
+    for (auto &e : h->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete h;
+            return FARM_EDEVICE;
+        }
+    *out = h;
+    return FARM_OK;
+}
+
+int farm_close(farm_handle h) {
+    if (!h) return FARM_EINVAL;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->voff,
+                      &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->result, &h->npp,      &h->pp_off,
+                      &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->ctr,    &h->hist,
+                      &h->cursor, &h->scan_tmp};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FARM_OK;
+}
+
+const char *farm_last_error(farm_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
+    if (!h || !ps || ps->npolys < 0) return fail(h, FARM_EINVAL, "bad polygon set");
+    HIPCHK(h, hipSetDevice(h->device));
+    // validate the layout on the host before any kernel reads it
+    for (int64_t i = 0; i < ps->npolys; i++) {
+        const farm_raster_hdr &r = ps->hdr[i];
+        const uint64_t npix = (uint64_t)r.dimx * (uint64_t)r.dimy;
+        if (r.dimx <= 0 || r.dimy <= 0 || r.pix_start + npix > ps->nstatus ||
+            r.pixoff_start + npix + 1 > ps->npixoff || r.seq_start + r.nseqs > ps->nseqs ||
+            r.vert_start + r.nverts > ps->nverts || r.nverts < 2 || r.vline_start + r.dimx + 2 > ps->nvoff ||
+            !(r.step[0] > 0) || !(r.step[1] > 0))
+            return fail(h, FARM_EINVAL, "polygon %lld: raster header out of range", (long long)i);
+        const uint32_t *po = ps->pixoff + r.pixoff_start;
+        for (uint64_t p = 0; p <= npix; p++)
+            if (po[p] > r.nseqs || (p < npix && po[p] > po[p + 1]))
+                return fail(h, FARM_EINVAL, "polygon %lld: pixel offsets out of range", (long long)i);
+        const uint32_t *sq = ps->seqs + 2 * r.seq_start;
+        for (uint32_t k = 0; k < r.nseqs; k++)
+            if ((uint64_t)sq[2 * k] + sq[2 * k + 1] + 1 > r.nverts)
+                return fail(h, FARM_EINVAL, "polygon %lld: edge sequence out of range", (long long)i);
+    }
+    int rc;
+    if ((rc = upload(h, h->hdr, ps->hdr, sizeof(farm_raster_hdr) * ps->npolys))) return rc;
+    if ((rc = upload(h, h->status, ps->status, ps->nstatus))) return rc;
+    if ((rc = upload(h, h->pixoff, ps->pixoff, 4 * ps->npixoff))) return rc;
+    if ((rc = upload(h, h->seqs, ps->seqs, 8 * ps->nseqs))) return rc;
+    if ((rc = upload(h, h->x, ps->x, 4 * ps->nverts))) return rc;
+    if ((rc = upload(h, h->y, ps->y, 4 * ps->nverts))) return rc;
+    if ((rc = upload(h, h->voff, ps->voff, 4 * ps->nvoff))) return rc;
+    if ((rc = upload(h, h->vnodes, ps->vnodes, 8 * ps->nvnodes))) return rc;
+    if ((rc = ensure(h, h->pixedges, 4 * ps->nstatus))) return rc;
+    h->npolys = ps->npolys;
+    h->nstatus = ps->nstatus;
+    if (ps->npolys > 0) {
+        hipLaunchKernelGGL(k_pixedges, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h), ps->npolys,
+                           P<uint32_t>(h->pixedges));
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FARM_OK;
+}
+
+int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint64_t n) {
+    if (!h || (n && (!a || !b))) return fail(h, FARM_EINVAL, "bad pair arrays");
+    if (h->npolys <= 0) return fail(h, FARM_ESTATE, "upload polygons before pairs");
+    for (uint64_t i = 0; i < n; i++)
+        if ((int64_t)a[i] >= h->npolys || (int64_t)b[i] >= h->npolys)
+            return fail(h, FARM_EINVAL, "pair %llu references polygon outside the set", (unsigned long long)i);
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc;
+    if ((rc = upload(h, h->pa, a, 4 * n))) return rc;
+    if ((rc = upload(h, h->pb, b, 4 * n))) return rc;
+    h->npairs = n;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FARM_OK;
+}
+
+static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
+                         farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
+    if (begin > end || end > h->npairs) return fail(h, FARM_EINVAL, "pair window [%llu,%llu) outside [0,%llu)",
+                                                    (unsigned long long)begin, (unsigned long long)end,
+                                                    (unsigned long long)h->npairs);
+    const uint64_t n64 = end - begin;
+    if (n64 > 0xffffffffull) return fail(h, FARM_ECAPACITY, "batch larger than 2^32 pairs");
+    const uint32_t n = (uint32_t)n64;
+    const int mode = opts ? opts->mode : FARM_MODE_EXACT_BATCHED;
+    int E = opts && opts->tile_a ? opts->tile_a : 2;
+    if (E != 1 && E != 2 && E != 4) return fail(h, FARM_EINVAL, "tile_a must be 1, 2 or 4");
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc;
+    const uint32_t *pa = P<uint32_t>(h->pa) + begin;
+    const uint32_t *pb = P<uint32_t>(h->pb) + begin;
+    if ((rc = ensure(h, h->result, 4ull * n + 4))) return rc;
+    if ((rc = ensure(h, h->ctr, 8 * 16))) return rc;
+    if ((rc = fill_u32(h, h->ctr.p, 32, 0))) return rc;
+    DevSet S = devset(h);
+    unsigned long long *ctr = P<unsigned long long>(h->ctr);
+    HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+    float filter_ms = 0, refine_ms = 0;
+    if (n == 0) {
+        // nothing to do
+    } else if (mode == FARM_MODE_EXACT_PLAIN) {
+        if ((rc = fill_u32(h, h->result.p, n, 0))) return rc;
+        HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+        hipLaunchKernelGGL(k_plain, dim3(n), dim3(256), 0, h->stream, S, pa, pb, n, P<uint32_t>(h->result), ctr);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+    } else if (mode == FARM_MODE_EXACT_BATCHED || mode == FARM_MODE_APPROX_HEURISTIC) {
+        const int approx = mode == FARM_MODE_APPROX_HEURISTIC;
+        if ((rc = ensure(h, h->npp, 4ull * (n + 1)))) return rc;
+        if ((rc = ensure(h, h->pp_off, 4ull * (n + 1)))) return rc;
+        if ((rc = fill_u32(h, h->npp.p, n + 1, 0))) return rc;
+        hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 0, approx,
+                           P<uint32_t>(h->npp), (const uint32_t *)nullptr, (PixPair *)nullptr,
+                           P<uint32_t>(h->result), ctr);
+        HIPCHK(h, hipGetLastError());
+        uint32_t npp_total = 0;
+        if (!approx) {
+            if ((rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->npp), P<uint32_t>(h->pp_off), n + 1))) return rc;
+            HIPCHK(h, hipMemcpyAsync(&npp_total, P<uint32_t>(h->pp_off) + n, 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+        }
+        HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+        if (npp_total > 0) {
+            if ((rc = ensure(h, h->pps, sizeof(PixPair) * (size_t)npp_total))) return rc;
+            if ((rc = ensure(h, h->bucket, npp_total))) return rc;
+            if ((rc = ensure(h, h->order, 4ull * npp_total))) return rc;
+            if ((rc = ensure(h, h->tiles, 4ull * (npp_total + 1)))) return rc;
+            if ((rc = ensure(h, h->tile_off, 4ull * (npp_total + 1)))) return rc;
+            if ((rc = ensure(h, h->hist, 4 * 64))) return rc;
+            if ((rc = ensure(h, h->cursor, 4 * 64))) return rc;
+            hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 1, 0,
+                               P<uint32_t>(h->npp), (const uint32_t *)P<uint32_t>(h->pp_off), P<PixPair>(h->pps),
+                               P<uint32_t>(h->result), ctr);
+            HIPCHK(h, hipGetLastError());
+            if ((rc = fill_u32(h, h->hist.p, 64, 0))) return rc;
+            hipLaunchKernelGGL(k_bucket_hist, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream, S, pa, pb,
+                               (const PixPair *)P<PixPair>(h->pps), npp_total, (const float *)nullptr,
+                               P<uint8_t>(h->bucket), P<uint32_t>(h->hist));
+            HIPCHK(h, hipGetLastError());
+            hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, h->stream, P<uint32_t>(h->hist),
+                               P<uint32_t>(h->cursor));
+            HIPCHK(h, hipGetLastError());
+            hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
+                               (const uint8_t *)P<uint8_t>(h->bucket), npp_total, P<uint32_t>(h->cursor),
+                               P<uint32_t>(h->order));
+            HIPCHK(h, hipGetLastError());
+            if ((rc = fill_u32(h, P<uint32_t>(h->tiles) + npp_total, 1, 0))) return rc;
+            if (E == 1)
+                hipLaunchKernelGGL(k_tiles<1>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
+                                   (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
+                                   npp_total, P<uint32_t>(h->tiles), ctr);
+            else if (E == 2)
+                hipLaunchKernelGGL(k_tiles<2>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
+                                   (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
+                                   npp_total, P<uint32_t>(h->tiles), ctr);
+            else
+                hipLaunchKernelGGL(k_tiles<4>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
+                                   (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
+                                   npp_total, P<uint32_t>(h->tiles), ctr);
+            HIPCHK(h, hipGetLastError());
+            if ((rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->tiles), P<uint32_t>(h->tile_off), npp_total + 1)))
+                return rc;
+            if ((rc = fill_u32(h, h->cursor.p, 1, 0))) return rc;
+            const unsigned grid = (unsigned)(h->num_cus * 8);
+            HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+#define LAUNCH_REFINE(EE)                                                                                    \
+    hipLaunchKernelGGL(k_refine<EE>, dim3(grid), dim3(256), 0, h->stream, S, pa, pb,                        \
+                       (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),       \
+                       (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, P<uint32_t>(h->cursor),      \
+                       P<uint32_t>(h->result), ctr)
+            if (E == 1) LAUNCH_REFINE(1);
+            else if (E == 2) LAUNCH_REFINE(2);
+            else LAUNCH_REFINE(4);
+#undef LAUNCH_REFINE
+            HIPCHK(h, hipGetLastError());
+        } else {
+            HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+        }
+    } else {
+        return fail(h, FARM_EINVAL, "mode %d not supported by farm_intersect", mode);
+    }
+    if (n) {
+        HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
+        hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n)), dim3(256), 0, h->stream,
+                           (const uint32_t *)P<uint32_t>(h->result), n, out_dev, ctr);
+        HIPCHK(h, hipGetLastError());
+    }
+    unsigned long long c[16];
+    HIPCHK(h, hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (c[7] > 0xffffffffull)
+        return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed 32-bit offsets; use smaller batches", c[7]);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->pairs = n;
+        st->filter_bytes = c[0];
+        st->in_hits = c[1];
+        st->segment_tests = c[2];
+        st->refine_items = c[3];
+        st->refine_bytes = c[4];
+        st->found = c[5];
+        if (n) {
+            float t01 = 0, t23 = 0, t03 = 0;
+            HIPCHK(h, hipEventElapsedTime(&t01, h->ev[0], h->ev[1]));
+            HIPCHK(h, hipEventElapsedTime(&t23, h->ev[2], h->ev[3]));
+            HIPCHK(h, hipEventElapsedTime(&t03, h->ev[0], h->ev[3]));
+            filter_ms = t01;
+            refine_ms = t23;
+            st->filter_ms = filter_ms;
+            st->refine_ms = refine_ms;
+            st->total_ms = t03;
+        }
+    }
+    return FARM_OK;
+}
+
+int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
+                       farm_stats *st) {
+    return run_intersect(h, begin, end, opts, out_dev, st);
+}
+
+int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out,
+                   farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (!out) return run_intersect(h, begin, end, opts, nullptr, st);
+    const uint64_t n = end > begin ? end - begin : 0;
+    DevBuf tmp;
+    int rc = ensure(h, tmp, n ? n : 1);
+    if (rc) return rc;
+    rc = run_intersect(h, begin, end, opts, (uint8_t *)tmp.p, st);
+    if (rc == FARM_OK && n) {
+        hipError_t e = hipMemcpy(out, tmp.p, n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(h, FARM_EDEVICE, "copy of results failed: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(tmp.p);
+    return rc;
+}
+
+}  // extern "C"

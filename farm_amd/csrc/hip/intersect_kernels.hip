@@ -1,0 +1,495 @@
+// Exact and approximate intersect kernels for gfx950 (wave64).
+//
+// Semantics: Ideal::intersect of the reference CPU path (Ideal.cpp:1329-1422):
+//   TRUE iff some non-OUT source pixel pa in the source window of the target
+//   MBB and some non-OUT target pixel pb in the target window of pa's box have
+//   an IN pixel on either side, or some BORDER x BORDER pair (pa, pb) has an
+//   edge of one of pa's edge sequences crossing an edge of one of pb's
+//   (segment_intersect, geometry_computation.h:215-237, binary32).
+// The answer is an OR, so any evaluation order (and any early exit) returns
+// the reference's bit.
+#include "device_layout.h"
+
+namespace farm {
+
+// ------------------------------------------------------------------------
+// Raster filter: one lane per candidate pair.
+// pass 0: count BORDER x BORDER pixel pairs, decide IN hits (and the
+//         reference -a heuristic when approx != 0)
+// pass 1: emit the pixel pairs at the offsets of an exclusive scan.
+// The window is enumerated in full (no early exit), so the status bytes read
+// equal the algorithmic filter bytes.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__restrict__ pa_ids,
+                                                 const uint32_t *__restrict__ pb_ids, uint32_t npairs, int pass,
+                                                 int approx, uint32_t *__restrict__ npp,
+                                                 const uint32_t *__restrict__ pp_off, PixPair *__restrict__ pps,
+                                                 uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long bytes = 0, inhit = 0;
+    if (k < npairs) {
+        const uint32_t ia = pa_ids[k], ib = pb_ids[k];
+        const farm_raster_hdr A = S.hdr[ia];
+        const farm_raster_hdr B = S.hdr[ib];
+        const uint8_t *sa = S.status + A.pix_start;
+        const uint8_t *sb = S.status + B.pix_start;
+        const uint32_t *ea = S.pixedges + A.pix_start;
+        const uint32_t *pofa = S.pixoff + A.pixoff_start;
+        const Window wa = window_of(A, B.mbr[0], B.mbr[1], B.mbr[2], B.mbr[3]);
+        const double area_a = A.step[0] * A.step[1], area_b = B.step[0] * B.step[1];
+        const double amax = fmax(area_a, area_b);
+        bool in_hit = false, heur = false;
+        uint32_t cnt = 0;
+        uint32_t base = 0;
+        if (pass == 1) {
+            if (npp[k] == 0) return;  // decided by the count pass or no BORDER pair
+            base = pp_off[k];
+        }
+        bytes = 8 + 2 * 56;
+        for (int x = wa.x0; x <= wa.x1; x++) {
+            for (int y = wa.y0; y <= wa.y1; y++) {
+                const int pa = y * A.dimx + x;
+                const uint8_t fa = sa[pa];
+                bytes += 1;
+                const int st_a = show(fa);
+                if (st_a == 0) continue;
+                double lx, ly, hx, hy;
+                pixel_box(A, x, y, lx, ly, hx, hy);
+                const Window wb = window_of(B, lx, ly, hx, hy);
+                for (int u = wb.x0; u <= wb.x1; u++) {
+                    for (int v = wb.y0; v <= wb.y1; v++) {
+                        const int pb = v * B.dimx + u;
+                        const uint8_t fb = sb[pb];
+                        bytes += 1;
+                        const int st_b = show(fb);
+                        if (st_b == 0) continue;
+                        if (st_a == 2 || st_b == 2) {
+                            in_hit = true;
+                            continue;
+                        }
+                        if (approx) {
+                            const double prob = (apx_area(fa, area_a) + apx_area(fb, area_b)) / amax;
+                            if (prob > 1.0) heur = true;
+                            continue;
+                        }
+                        if (pass == 1) {
+                            // edge slots of pa: sum of (len + 1) over its sequences
+                            const uint32_t nseq = pofa[pa + 1] - pofa[pa];
+                            pps[base + cnt] = PixPair{k, (uint32_t)pa, (uint32_t)pb, ea[pa] + nseq};
+                        }
+                        cnt++;
+                    }
+                }
+            }
+        }
+        if (pass == 0) {
+            const bool decided = in_hit || heur;
+            npp[k] = decided ? 0u : cnt;
+            result[k] = decided ? 1u : 0u;
+            inhit = in_hit ? 1 : 0;
+        }
+    }
+    if (pass == 0) {
+        // wave-level reduction of the counters, one atomic per wave
+        for (int o = 32; o > 0; o >>= 1) {
+            bytes += __shfl_xor(bytes, o);
+            inhit += __shfl_xor(inhit, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&ctr[0], bytes);
+            atomicAdd(&ctr[1], inhit);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Cost order: counting sort of the pixel pairs by floor(log2(cost)),
+// heaviest bucket first.  cost = edge slots of pa x edges of pb.
+// ------------------------------------------------------------------------
+constexpr int kBuckets = 40;
+
+__device__ __forceinline__ int cost_bucket(uint64_t cost) {
+    int b = cost ? 63 - __builtin_clzll(cost) : 0;
+    return min(b, kBuckets - 1);
+}
+
+__global__ __launch_bounds__(256) void k_bucket_hist(DevSet S, const uint32_t *__restrict__ pa_ids,
+                                                      const uint32_t *__restrict__ pb_ids,
+                                                      const PixPair *__restrict__ pps, uint32_t n,
+                                                      const float *__restrict__ pred_cost,
+                                                      uint8_t *__restrict__ bucket, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const PixPair p = pps[i];
+        int b;
+        if (pred_cost) {
+            const float c = fmaxf(pred_cost[p.pair], 0.f);
+            b = min((int)(c * 1.4426950408889634f), kBuckets - 1);
+        } else {
+            const farm_raster_hdr B = S.hdr[pb_ids[p.pair]];
+            const uint64_t eb = S.pixedges[B.pix_start + p.pb];
+            b = cost_bucket((uint64_t)p.slots * eb);
+        }
+        bucket[i] = (uint8_t)b;
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kBuckets; j += blockDim.x)
+        if (h[j]) atomicAdd(&hist[j], h[j]);
+}
+
+// one block: descending-bucket exclusive scan of the histogram
+__global__ void k_bucket_scan(uint32_t *__restrict__ hist, uint32_t *__restrict__ cursor) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int b = kBuckets - 1; b >= 0; b--) {
+            cursor[b] = acc;
+            acc += hist[b];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bucket_scatter(const uint8_t *__restrict__ bucket, uint32_t n,
+                                                         uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint32_t pos = atomicAdd(&cursor[bucket[i]], 1u);
+        order[pos] = i;
+    }
+}
+
+// tiles per ordered pixel pair (written at position i, scanned afterwards)
+// (the u64 total in ctr[7] lets the host detect a 32-bit offset overflow)
+template <int E>
+__global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
+                                                uint32_t n, uint32_t *__restrict__ tiles,
+                                                unsigned long long *__restrict__ ctr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long t = 0;
+    if (i < n) {
+        const uint32_t slots = pps[order[i]].slots;
+        const uint32_t per = 64u * E;
+        t = slots > 1 ? (slots - 1 + per - 1) / per : 0;
+        tiles[i] = (uint32_t)t;
+    }
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if ((threadIdx.x & 63) == 0 && t) atomicAdd(&ctr[7], t);
+}
+
+// ------------------------------------------------------------------------
+// Persistent refinement.  Each wave takes the next tile from a global
+// cursor.  A tile is 64 lanes x E consecutive edge slots of the source
+// pixel's sequences (consecutive lanes share one vertex); the target pixel's
+// edges stream through scalar registers (every lane needs the same target
+// vertex), so no LDS is used.  Per target vertex q and source edge (a, b):
+//   O(q)   = a.cross(b, q)      -- reused as o1/o2 of consecutive edges
+//   o(p)   = c.cross(p, d)      -- per source vertex p, for target edge (c,d)
+// and a superset filter (o1*o2 <= 0 or o3 == 0) and o3*o4 <= 0 sends the
+// rare candidates to the exact reference predicate.
+// ------------------------------------------------------------------------
+template <int E>
+__global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_ids,
+                                                 const uint32_t *__restrict__ pb_ids,
+                                                 const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
+                                                 const uint32_t *__restrict__ tile_off, uint32_t npp,
+                                                 uint32_t *__restrict__ cursor, uint32_t *__restrict__ result,
+                                                 unsigned long long *__restrict__ ctr) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t total = tile_off[npp];
+    unsigned long long tests = 0, items = 0, rbytes = 0;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(cursor, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= total) break;
+        // ordered pixel pair owning tile t: largest q with tile_off[q] <= t
+        uint32_t lo = 0, hi = npp;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tile_off[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t tile = t - tile_off[lo];
+        const PixPair P = pps[__builtin_amdgcn_readfirstlane(order[lo])];
+        const uint32_t pair = __builtin_amdgcn_readfirstlane(P.pair);
+        if (__hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        items++;
+        const farm_raster_hdr A = S.hdr[pa_ids[pair]];
+        const farm_raster_hdr B = S.hdr[pb_ids[pair]];
+        const uint32_t *pofa = S.pixoff + A.pixoff_start;
+        const uint32_t *pofb = S.pixoff + B.pixoff_start;
+        const uint2 *sqa = S.seqs + A.seq_start;
+        const uint2 *sqb = S.seqs + B.seq_start;
+        const float *ax_ = S.x + A.vert_start;
+        const float *ay_ = S.y + A.vert_start;
+        const float *bx_ = S.x + B.vert_start;
+        const float *by_ = S.y + B.vert_start;
+        const uint32_t a_s0 = pofa[P.pa], a_s1 = pofa[P.pa + 1];
+        const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
+        const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
+
+        // source vertices of this lane: slots first .. first + E
+        const uint32_t first = tile * 64u * E + (uint32_t)lane * E;
+        float px[E + 1], py[E + 1];
+        bool ev[E];
+        {
+            uint32_t k = a_s0, sbase = 0;
+            uint2 sq = sqa[k];
+            for (int m = 0; m <= E; m++) {
+                const uint32_t s = first + m;
+                while (k + 1 < a_s1 && s >= sbase + sq.y + 1) {
+                    sbase += sq.y + 1;
+                    k++;
+                    sq = sqa[k];
+                }
+                uint32_t o = s - sbase;
+                const bool inside = s < sbase + sq.y + 1;
+                if (!inside) o = sq.y;  // past the last slot: clamp (no edge)
+                const uint32_t v = sq.x + o;
+                px[m] = ax_[v];
+                py[m] = ay_[v];
+                if (m < E) ev[m] = inside && o < sq.y;
+            }
+        }
+        float bax[E], bay[E];
+        int nvalid = 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            bax[e] = px[e + 1] - px[e];
+            bay[e] = py[e + 1] - py[e];
+            nvalid += ev[e] ? 1 : 0;
+        }
+        bool hit = false;
+        uint32_t steps = 0;
+        bool stop = false;
+        for (uint32_t j = b_s0; j < b_s1 && !stop; j++) {
+            const uint2 sq = sqb[j];
+            const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
+            const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
+            float cx = bx_[v0], cy = by_[v0];
+            float Op[E];
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                float l = bax[e] * (cy - py[e]);
+                float r = bay[e] * (cx - px[e]);
+                Op[e] = l - r;
+            }
+            for (uint32_t m = 1; m <= L; m++) {
+                const float dx = bx_[v0 + m], dy = by_[v0 + m];
+                const float ddx = dx - cx, ddy = dy - cy;
+                float o[E + 1];
+#pragma unroll
+                for (int e = 0; e <= E; e++) {
+                    float l = (px[e] - cx) * ddy;
+                    float r = (py[e] - cy) * ddx;
+                    o[e] = l - r;
+                }
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    float l = bax[e] * (dy - py[e]);
+                    float r = bay[e] * (dx - px[e]);
+                    const float On = l - r;
+                    const bool maybe = ev[e] && (o[e] * o[e + 1] <= 0.f) && ((Op[e] * On <= 0.f) || (o[e] == 0.f));
+                    if (maybe) {
+                        if (seg_hit(Op[e], On, o[e], o[e + 1], px[e], py[e], px[e + 1], py[e + 1], cx, cy, dx, dy))
+                            hit = true;
+                    }
+                    Op[e] = On;
+                }
+                cx = dx;
+                cy = dy;
+                tests += nvalid;
+                if ((++steps & 31u) == 0) {
+                    if (__any(hit)) {
+                        stop = true;
+                        break;
+                    }
+                    if (__hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        stop = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (__any(hit)) {
+            if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            // algorithmic refine bytes of this tile: its source vertices plus
+            // the target pixel's sequences once
+            const uint32_t nb = b_s1 - b_s0;
+            rbytes += 16 + 8ull * nb + 8ull * (S.pixedges[B.pix_start + P.pb] + nb);
+        }
+        rbytes += (unsigned long long)nvalid * 8;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        tests += __shfl_xor(tests, o);
+        rbytes += __shfl_xor(rbytes, o);
+    }
+    if (lane == 0) {
+        atomicAdd(&ctr[2], tests);
+        atomicAdd(&ctr[3], items);
+        atomicAdd(&ctr[4], rbytes);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Plain kernel: one workgroup (256 lanes) per candidate pair, the direct
+// formulation.  The window is scanned for IN overlaps by all lanes; then each
+// BORDER x BORDER pixel pair is refined by striding the lanes over the
+// flattened (source edge, target edge) grid.
+// ------------------------------------------------------------------------
+constexpr int kPlainCap = 2048;  // edges per side held in LDS per chunk
+
+__global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restrict__ pa_ids,
+                                                const uint32_t *__restrict__ pb_ids, uint32_t npairs,
+                                                uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr) {
+    __shared__ uint32_t ea[kPlainCap];
+    __shared__ uint32_t eb[kPlainCap];
+    __shared__ int s_flag;
+    const uint32_t k = blockIdx.x;
+    if (k >= npairs) return;
+    const int tid = threadIdx.x;
+    const farm_raster_hdr A = S.hdr[pa_ids[k]];
+    const farm_raster_hdr B = S.hdr[pb_ids[k]];
+    const uint8_t *sa = S.status + A.pix_start;
+    const uint8_t *sb = S.status + B.pix_start;
+    const uint32_t *pofa = S.pixoff + A.pixoff_start;
+    const uint32_t *pofb = S.pixoff + B.pixoff_start;
+    const uint2 *sqa = S.seqs + A.seq_start;
+    const uint2 *sqb = S.seqs + B.seq_start;
+    const float *AX = S.x + A.vert_start, *AY = S.y + A.vert_start;
+    const float *BX = S.x + B.vert_start, *BY = S.y + B.vert_start;
+    const Window wa = window_of(A, B.mbr[0], B.mbr[1], B.mbr[2], B.mbr[3]);
+    const int nwx = wa.x1 - wa.x0 + 1, nwy = wa.y1 - wa.y0 + 1;
+    const int nwa = nwx * nwy;
+    if (tid == 0) s_flag = 0;
+    __syncthreads();
+    // IN overlap scan: lanes over source window pixels
+    for (int w = tid; w < nwa; w += blockDim.x) {
+        const int x = wa.x0 + w / nwy, y = wa.y0 + w % nwy;
+        const int st_a = show(sa[y * A.dimx + x]);
+        if (st_a == 0) continue;
+        double lx, ly, hx, hy;
+        pixel_box(A, x, y, lx, ly, hx, hy);
+        const Window wb = window_of(B, lx, ly, hx, hy);
+        for (int u = wb.x0; u <= wb.x1; u++)
+            for (int v = wb.y0; v <= wb.y1; v++) {
+                const int st_b = show(sb[v * B.dimx + u]);
+                if (st_b != 0 && (st_a == 2 || st_b == 2)) s_flag = 1;
+            }
+    }
+    __syncthreads();
+    if (s_flag) {
+        if (tid == 0) {
+            result[k] = 1;
+            atomicAdd(&ctr[1], 1ull);
+        }
+        return;
+    }
+    unsigned long long tests = 0;
+    bool hit = false;
+    // refinement: uniform loop over the BORDER x BORDER pixel pairs
+    for (int w = 0; w < nwa && !hit; w++) {
+        const int x = wa.x0 + w / nwy, y = wa.y0 + w % nwy;
+        const int pa = y * A.dimx + x;
+        if (show(sa[pa]) != 1) continue;
+        double lx, ly, hx, hy;
+        pixel_box(A, x, y, lx, ly, hx, hy);
+        const Window wb = window_of(B, lx, ly, hx, hy);
+        for (int u = wb.x0; u <= wb.x1 && !hit; u++)
+            for (int v = wb.y0; v <= wb.y1 && !hit; v++) {
+                const int pb = v * B.dimx + u;
+                if (show(sb[pb]) != 1) continue;
+                // flatten the edge lists (vertex index of each edge start)
+                int na = 0, nb = 0;
+                for (uint32_t s = pofa[pa]; s < pofa[pa + 1]; s++) na += sqa[s].y;
+                for (uint32_t s = pofb[pb]; s < pofb[pb + 1]; s++) nb += sqb[s].y;
+                for (int ca = 0; ca < na && !hit; ca += kPlainCap) {
+                    for (int cb = 0; cb < nb && !hit; cb += kPlainCap) {
+                        const int la = min(kPlainCap, na - ca), lb = min(kPlainCap, nb - cb);
+                        __syncthreads();
+                        if (tid == 0) {
+                            int c = 0, o = 0;
+                            for (uint32_t s = pofa[pa]; s < pofa[pa + 1]; s++)
+                                for (uint32_t e = 0; e < sqa[s].y; e++, c++)
+                                    if (c >= ca && c < ca + la) ea[o++] = sqa[s].x + e;
+                            c = 0;
+                            o = 0;
+                            for (uint32_t s = pofb[pb]; s < pofb[pb + 1]; s++)
+                                for (uint32_t e = 0; e < sqb[s].y; e++, c++)
+                                    if (c >= cb && c < cb + lb) eb[o++] = sqb[s].x + e;
+                        }
+                        __syncthreads();
+                        const int n = la * lb;
+                        const int iters = (n + (int)blockDim.x - 1) / (int)blockDim.x;  // uniform
+                        for (int it = 0; it < iters; it++) {
+                            const int idx = it * (int)blockDim.x + tid;
+                            if (idx < n) {
+                                const int i = idx / lb, j = idx % lb;
+                                const uint32_t va = ea[i], vb = eb[j];
+                                tests++;
+                                if (seg_intersect(AX[va], AY[va], AX[va + 1], AY[va + 1], BX[vb], BY[vb], BX[vb + 1],
+                                                  BY[vb + 1]))
+                                    hit = true;
+                            }
+                            if ((it & 15) == 15) {
+                                if (__syncthreads_or(hit)) {
+                                    hit = true;
+                                    break;
+                                }
+                            }
+                        }
+                        hit = __syncthreads_or(hit);
+                    }
+                }
+            }
+    }
+    for (int o = 32; o > 0; o >>= 1) tests += __shfl_xor(tests, o);
+    if ((tid & 63) == 0) atomicAdd(&ctr[2], tests);
+    if (tid == 0 && hit) result[k] = 1;
+}
+
+// per-pixel edge count of every BORDER pixel (sum of its sequence lengths)
+__global__ __launch_bounds__(256) void k_pixedges(DevSet S, int64_t npolys, uint32_t *__restrict__ out) {
+    const int64_t p = blockIdx.x;
+    if (p >= npolys) return;
+    const farm_raster_hdr H = S.hdr[p];
+    const int n = H.dimx * H.dimy;
+    const uint32_t *pof = S.pixoff + H.pixoff_start;
+    const uint2 *sq = S.seqs + H.seq_start;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t c = 0;
+        if (show(S.status[H.pix_start + i]) == 1)
+            for (uint32_t s = pof[i]; s < pof[i + 1]; s++) c += sq[s].y;
+        out[H.pix_start + i] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ result, uint32_t n,
+                                                   uint8_t *__restrict__ out, unsigned long long *__restrict__ ctr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long f = 0;
+    if (i < n) {
+        const uint32_t r = result[i] ? 1u : 0u;
+        if (out) out[i] = (uint8_t)r;
+        f = r;
+    }
+    for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
+    if ((threadIdx.x & 63) == 0 && f) atomicAdd(&ctr[5], f);
+}
+
+template __global__ void k_tiles<1>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
+template __global__ void k_tiles<2>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
+template __global__ void k_tiles<4>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
+template __global__ void k_refine<1>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
+                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+template __global__ void k_refine<2>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
+                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+template __global__ void k_refine<4>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
+                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+
+}  // namespace farm

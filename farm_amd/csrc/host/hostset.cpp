@@ -1,0 +1,635 @@
+// Host polygon storage (.dt / WKT), raster packing into the device layout,
+// and candidate-pair generation with a host R-tree.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/farm_host.h"
+#include "raster.h"
+
+struct farm_hostset {
+    // stored rings (as in the file; no appended vertex)
+    std::vector<uint64_t> ring_off{0};
+    std::vector<float> xy;
+    // holes, kept for .dt round trips only
+    std::vector<uint64_t> hole_first{0};  // per polygon: range into hole_off
+    std::vector<uint64_t> hole_off{0};
+    std::vector<float> hole_xy;
+    std::vector<double> mbb;  // 4 per polygon
+    // device layout (after rasterize)
+    bool rastered = false;
+    int64_t raster_fail = -1;
+    std::vector<farm_raster_hdr> hdr;
+    std::vector<uint8_t> status;
+    std::vector<double> areas;  // per pixel, indexed like status
+    std::vector<uint32_t> pixoff, seqs, voff;
+    std::vector<double> vnodes;
+    std::vector<float> vx, vy;
+};
+
+namespace {
+
+// VertexSequence::getMBR (vertex_sequence.cpp:152-167) over the decoded ring
+// (stored vertices plus the repeated first one, which changes nothing).
+void ring_mbb(const float *xy, uint64_t n, double *m) {
+    double lx = 100000.0, ly = 100000.0, hx = -100000.0, hy = -100000.0;
+    for (uint64_t k = 0; k < n; k++) {
+        lx = std::min(lx, (double)xy[2 * k]);
+        hx = std::max(hx, (double)xy[2 * k]);
+        ly = std::min(ly, (double)xy[2 * k + 1]);
+        hy = std::max(hy, (double)xy[2 * k + 1]);
+    }
+    const float bias = 1e-9f;
+    m[0] = lx - bias;
+    m[1] = ly - bias;
+    m[2] = hx + bias;
+    m[3] = hy + bias;
+}
+
+void push_polygon(farm_hostset *s, const float *xy, uint64_t nv) {
+    s->xy.insert(s->xy.end(), xy, xy + 2 * nv);
+    s->ring_off.push_back(s->xy.size() / 2);
+    s->hole_first.push_back(s->hole_off.size() - 1);
+    double m[4];
+    ring_mbb(xy, nv, m);
+    s->mbb.insert(s->mbb.end(), m, m + 4);
+}
+
+int64_t npoly(const farm_hostset *s) { return (int64_t)s->ring_off.size() - 1; }
+
+// ---------------------------------------------------------------- WKT ----
+// read_float (util.h:129-144): skip to the next number character, take the
+// run of [-.0-9e], convert with stof (strtof rounding).
+bool is_num(char c) { return c == '-' || c == '.' || (c <= '9' && c >= '0') || c == 'e'; }
+float read_float(const char *w, size_t &off, size_t len, bool &ok) {
+    while (off < len && !is_num(w[off])) off++;
+    char tmp[100];
+    int k = 0;
+    while (off < len && is_num(w[off]) && k < 99) tmp[k++] = w[off++];
+    tmp[k] = 0;
+    if (k == 0) {
+        ok = false;
+        return 0.f;
+    }
+    return strtof(tmp, nullptr);
+}
+void skip_sp(const char *w, size_t &off, size_t len) {
+    while (off < len && (w[off] == ' ' || w[off] == '\t' || w[off] == '\n' || w[off] == '\r')) off++;
+}
+// read_vertices (storage.cpp:637-684): vertex count = commas before ')' + 1
+bool read_ring(const char *w, size_t &off, size_t len, std::vector<float> &out) {
+    skip_sp(w, off, len);
+    if (off >= len || w[off++] != '(') return false;
+    size_t c = off;
+    int nv = 0;
+    while (c < len && w[c++] != ')')
+        if (c < len && w[c] == ',') nv++;
+    nv++;
+    out.clear();
+    bool ok = true;
+    for (int i = 0; i < nv; i++) {
+        out.push_back(read_float(w, off, len, ok));
+        out.push_back(read_float(w, off, len, ok));
+    }
+    skip_sp(w, off, len);
+    if (off >= len || w[off++] != ')') return false;
+    return ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+farm_hostset *farm_hostset_from_rings(int64_t n, const uint64_t *ring_off, const float *xy) {
+    farm_hostset *s = new farm_hostset();
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t b = ring_off[i], e = ring_off[i + 1];
+        if (e < b) {
+            delete s;
+            return nullptr;
+        }
+        if (e - b + 1 < 3) continue;  // storage.cpp:341
+        push_polygon(s, xy + 2 * b, e - b);
+    }
+    return s;
+}
+
+farm_hostset *farm_hostset_load_dt(const char *path) {
+    FILE *f = fopen(path, "rb");
+    if (!f) return nullptr;
+    fseek(f, 0, SEEK_END);
+    long fsz = ftell(f);
+    if (fsz < 8) {
+        fclose(f);
+        return nullptr;
+    }
+    std::vector<char> buf((size_t)fsz);
+    fseek(f, 0, SEEK_SET);
+    size_t rd = fread(buf.data(), 1, (size_t)fsz, f);
+    fclose(f);
+    if (rd != (size_t)fsz) return nullptr;
+    uint64_t n;
+    memcpy(&n, buf.data() + fsz - 8, 8);
+    const uint64_t meta_bytes = 48;  // PolygonMeta (MyPolygon.h:29-35)
+    if (n == 0 || n * meta_bytes + 8 > (uint64_t)fsz) return nullptr;
+    const char *meta = buf.data() + fsz - 8 - n * meta_bytes;
+    uint64_t first_off, last_off;
+    uint32_t last_size;
+    memcpy(&first_off, meta + 8, 8);
+    memcpy(&last_off, meta + (n - 1) * meta_bytes + 8, 8);
+    memcpy(&last_size, meta + (n - 1) * meta_bytes, 4);
+    const uint64_t data_end = last_off + last_size;
+    if (data_end > (uint64_t)fsz - 8 - n * meta_bytes) return nullptr;
+    farm_hostset *s = new farm_hostset();
+    // polygons are decoded back to back (storage.cpp:320-360,
+    // MyPolygon::decode MyPolygon.cpp:747-759)
+    uint64_t off = first_off;
+    auto rd64 = [&](uint64_t &v) -> bool {
+        if (off + 8 > data_end) return false;
+        memcpy(&v, buf.data() + off, 8);
+        off += 8;
+        return true;
+    };
+    while (off < data_end) {
+        uint64_t nh, nv;
+        if (!rd64(nh) || !rd64(nv) || nv == 0 || off + nv * 8 > data_end) {
+            delete s;
+            return nullptr;
+        }
+        const float *ring = reinterpret_cast<const float *>(buf.data() + off);
+        off += nv * 8;
+        std::vector<std::pair<const float *, uint64_t>> holes;
+        for (uint64_t h = 0; h < nh; h++) {
+            uint64_t hn;
+            if (!rd64(hn) || off + hn * 8 > data_end) {
+                delete s;
+                return nullptr;
+            }
+            holes.push_back({reinterpret_cast<const float *>(buf.data() + off), hn});
+            off += hn * 8;
+        }
+        if (nv + 1 < 3) continue;
+        push_polygon(s, ring, nv);
+        for (auto &hh : holes) {
+            s->hole_xy.insert(s->hole_xy.end(), hh.first, hh.first + 2 * hh.second);
+            s->hole_off.push_back(s->hole_xy.size() / 2);
+        }
+        s->hole_first.back() = s->hole_off.size() - 1;
+    }
+    return s;
+}
+
+farm_hostset *farm_hostset_load_wkt(const char *path) {
+    FILE *f = fopen(path, "rb");
+    if (!f) return nullptr;
+    farm_hostset *s = new farm_hostset();
+    std::string line;
+    std::vector<float> ring;
+    std::vector<std::vector<float>> holes;
+    char chunk[1 << 16];
+    auto handle = [&](const std::string &l) -> bool {
+        if (l.empty()) return true;
+        size_t off = l.find('(');
+        if (off == std::string::npos) return false;
+        const char *w = l.c_str();
+        size_t len = l.size();
+        skip_sp(w, off, len);
+        if (w[off++] != '(') return false;
+        if (!read_ring(w, off, len, ring)) return false;
+        skip_sp(w, off, len);
+        holes.clear();
+        while (off < len && w[off] == ',') {
+            off++;
+            std::vector<float> h;
+            if (!read_ring(w, off, len, h)) return false;
+            holes.push_back(std::move(h));
+            skip_sp(w, off, len);
+        }
+        // the WKT loader keeps every polygon; the .dt loader drops tiny rings
+        s->xy.insert(s->xy.end(), ring.begin(), ring.end());
+        s->ring_off.push_back(s->xy.size() / 2);
+        double m[4];
+        ring_mbb(ring.data(), ring.size() / 2, m);
+        s->mbb.insert(s->mbb.end(), m, m + 4);
+        for (auto &h : holes) {
+            s->hole_xy.insert(s->hole_xy.end(), h.begin(), h.end());
+            s->hole_off.push_back(s->hole_xy.size() / 2);
+        }
+        s->hole_first.push_back(s->hole_off.size() - 1);
+        return true;
+    };
+    bool ok = true;
+    while (ok) {
+        size_t got = fread(chunk, 1, sizeof chunk, f);
+        for (size_t i = 0; i < got && ok; i++) {
+            if (chunk[i] == '\n') {
+                ok = handle(line);
+                line.clear();
+            } else {
+                line.push_back(chunk[i]);
+            }
+        }
+        if (got < sizeof chunk) break;
+    }
+    if (ok && !line.empty()) ok = handle(line);
+    fclose(f);
+    if (!ok) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+// dump_polygons_to_file (storage.cpp:158-193); layout of MyPolygon::encode
+// (MyPolygon.cpp:736-745) and PolygonMeta (MyPolygon.h:29-35).
+int farm_hostset_save_dt(const farm_hostset *s, const char *path) {
+    FILE *f = fopen(path, "wb");
+    if (!f) return FARM_EINVAL;
+    const int64_t n = npoly(s);
+    std::vector<char> meta((size_t)n * 48);
+    uint64_t cur = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const uint64_t nv = s->ring_off[i + 1] - s->ring_off[i];
+        const uint64_t h0 = s->hole_first[i], h1 = s->hole_first[i + 1];
+        const uint64_t nh = h1 - h0;
+        uint64_t size = 8 + 8 + 8 * nv;
+        fwrite(&nh, 8, 1, f);
+        fwrite(&nv, 8, 1, f);
+        fwrite(s->xy.data() + 2 * s->ring_off[i], 8, nv, f);
+        for (uint64_t h = h0; h < h1; h++) {
+            uint64_t hn = s->hole_off[h + 1] - s->hole_off[h];
+            fwrite(&hn, 8, 1, f);
+            fwrite(s->hole_xy.data() + 2 * s->hole_off[h], 8, hn, f);
+            size += 8 + 8 * hn;
+        }
+        char *m = meta.data() + i * 48;
+        uint32_t sz32 = (uint32_t)size, nv32 = (uint32_t)nv;
+        memcpy(m, &sz32, 4);
+        memcpy(m + 4, &nv32, 4);
+        memcpy(m + 8, &cur, 8);
+        memcpy(m + 16, &s->mbb[4 * i], 32);
+        cur += size;
+    }
+    fwrite(meta.data(), 1, meta.size(), f);
+    uint64_t nn = (uint64_t)n;
+    fwrite(&nn, 8, 1, f);
+    int rc = ferror(f) ? FARM_EINVAL : FARM_OK;
+    fclose(f);
+    return rc;
+}
+
+farm_hostset *farm_hostset_concat(const farm_hostset *a, const farm_hostset *b) {
+    farm_hostset *s = new farm_hostset();
+    for (const farm_hostset *src : {a, b}) {
+        const uint64_t vbase = s->xy.size() / 2, hbase = s->hole_off.size() - 1, hxy = s->hole_xy.size() / 2;
+        s->xy.insert(s->xy.end(), src->xy.begin(), src->xy.end());
+        for (size_t i = 1; i < src->ring_off.size(); i++) s->ring_off.push_back(src->ring_off[i] + vbase);
+        s->hole_xy.insert(s->hole_xy.end(), src->hole_xy.begin(), src->hole_xy.end());
+        for (size_t i = 1; i < src->hole_off.size(); i++) s->hole_off.push_back(src->hole_off[i] + hxy);
+        for (size_t i = 1; i < src->hole_first.size(); i++) s->hole_first.push_back(src->hole_first[i] + hbase);
+        s->mbb.insert(s->mbb.end(), src->mbb.begin(), src->mbb.end());
+    }
+    return s;
+}
+
+void farm_hostset_free(farm_hostset *s) { delete s; }
+int64_t farm_hostset_size(const farm_hostset *s) { return npoly(s); }
+
+int farm_hostset_ring(const farm_hostset *s, int64_t i, const float **xy, int64_t *nv) {
+    if (i < 0 || i >= npoly(s)) return FARM_EINVAL;
+    *xy = s->xy.data() + 2 * s->ring_off[i];
+    *nv = (int64_t)(s->ring_off[i + 1] - s->ring_off[i]);
+    return FARM_OK;
+}
+
+int farm_hostset_mbbs(const farm_hostset *s, double *out) {
+    memcpy(out, s->mbb.data(), s->mbb.size() * sizeof(double));
+    return FARM_OK;
+}
+
+int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
+    if (vpr <= 0) return FARM_EINVAL;
+    const int64_t n = npoly(s);
+    if (nthreads < 1) nthreads = 1;
+    std::vector<farm::RasterOut> outs(n);
+    std::atomic<int64_t> next(0);
+    auto work = [&]() {
+        farm::RasterScratch *sc = farm::raster_scratch_new();
+        std::vector<float> ring;
+        for (;;) {
+            int64_t b = next.fetch_add(8);
+            if (b >= n) break;
+            for (int64_t i = b; i < std::min(n, b + 8); i++) {
+                const uint64_t r0 = s->ring_off[i], r1 = s->ring_off[i + 1];
+                ring.assign(s->xy.begin() + 2 * r0, s->xy.begin() + 2 * r1);
+                ring.push_back(ring[0]);  // VertexSequence::decode appends p[0]
+                ring.push_back(ring[1]);
+                farm::build_raster(ring.data(), (int)(r1 - r0 + 1), &s->mbb[4 * i], vpr, outs[i], sc);
+            }
+        }
+        farm::raster_scratch_free(sc);
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+
+    // pack: prefix sums, then parallel copy
+    s->hdr.assign(n, farm_raster_hdr());
+    uint64_t np = 0, npo = 0, nsq = 0, nvt = 0, nvl = 0, nvn = 0;
+    s->raster_fail = -1;
+    for (int64_t i = 0; i < n; i++) {
+        const farm::RasterOut &o = outs[i];
+        if (!o.ok && s->raster_fail < 0) s->raster_fail = i;
+        farm_raster_hdr &h = s->hdr[i];
+        memset(&h, 0, sizeof h);
+        h.mbr[0] = o.g.low[0];
+        h.mbr[1] = o.g.low[1];
+        h.mbr[2] = o.g.high[0];
+        h.mbr[3] = o.g.high[1];
+        h.step[0] = o.g.step[0];
+        h.step[1] = o.g.step[1];
+        h.dimx = o.g.dimx;
+        h.dimy = o.g.dimy;
+        h.pix_start = np;
+        h.pixoff_start = npo;
+        h.seq_start = nsq;
+        h.vert_start = nvt;
+        h.nverts = (uint32_t)(s->ring_off[i + 1] - s->ring_off[i] + 1);
+        h.nseqs = (uint32_t)(o.seqs.size() / 2);
+        h.vline_start = nvl;
+        h.vnode_start = nvn;
+        np += o.status.size();
+        npo += o.pixoff.size();
+        nsq += o.seqs.size() / 2;
+        nvt += h.nverts;
+        nvl += o.voff.size();
+        nvn += o.vnodes.size();
+    }
+    s->status.assign(np, 0);
+    s->areas.assign(np, 0.0);
+    s->pixoff.assign(npo, 0);
+    s->seqs.assign(2 * nsq, 0);
+    s->vx.assign(nvt, 0.f);
+    s->vy.assign(nvt, 0.f);
+    s->voff.assign(nvl, 0);
+    s->vnodes.assign(nvn, 0.0);
+    next = 0;
+    auto pack = [&]() {
+        for (;;) {
+            int64_t b = next.fetch_add(64);
+            if (b >= n) break;
+            for (int64_t i = b; i < std::min(n, b + 64); i++) {
+                farm::RasterOut &o = outs[i];
+                const farm_raster_hdr &h = s->hdr[i];
+                std::copy(o.status.begin(), o.status.end(), s->status.begin() + h.pix_start);
+                std::copy(o.areas.begin(), o.areas.end(), s->areas.begin() + h.pix_start);
+                std::copy(o.pixoff.begin(), o.pixoff.end(), s->pixoff.begin() + h.pixoff_start);
+                std::copy(o.seqs.begin(), o.seqs.end(), s->seqs.begin() + 2 * h.seq_start);
+                std::copy(o.voff.begin(), o.voff.end(), s->voff.begin() + h.vline_start);
+                std::copy(o.vnodes.begin(), o.vnodes.end(), s->vnodes.begin() + h.vnode_start);
+                const uint64_t r0 = s->ring_off[i];
+                const uint32_t nvs = h.nverts - 1;
+                for (uint32_t k = 0; k < nvs; k++) {
+                    s->vx[h.vert_start + k] = s->xy[2 * (r0 + k)];
+                    s->vy[h.vert_start + k] = s->xy[2 * (r0 + k) + 1];
+                }
+                s->vx[h.vert_start + nvs] = s->xy[2 * r0];
+                s->vy[h.vert_start + nvs] = s->xy[2 * r0 + 1];
+                farm::RasterOut().status.swap(o.status);  // release early
+                std::vector<double>().swap(o.areas);
+                std::vector<uint32_t>().swap(o.seqs);
+            }
+        }
+    };
+    th.clear();
+    for (int t = 0; t < nthreads; t++) th.emplace_back(pack);
+    for (auto &t : th) t.join();
+    s->rastered = true;
+    return FARM_OK;
+}
+
+int farm_hostset_view(const farm_hostset *s, farm_polyset *o) {
+    if (!s->rastered) return FARM_ESTATE;
+    o->npolys = npoly(s);
+    o->hdr = s->hdr.data();
+    o->status = s->status.data();
+    o->nstatus = s->status.size();
+    o->pixoff = s->pixoff.data();
+    o->npixoff = s->pixoff.size();
+    o->seqs = s->seqs.data();
+    o->nseqs = s->seqs.size() / 2;
+    o->x = s->vx.data();
+    o->y = s->vy.data();
+    o->nverts = s->vx.size();
+    o->voff = s->voff.data();
+    o->nvoff = s->voff.size();
+    o->vnodes = s->vnodes.data();
+    o->nvnodes = s->vnodes.size();
+    return FARM_OK;
+}
+
+int farm_hostset_areas(const farm_hostset *s, int64_t i, const double **areas) {
+    if (!s->rastered || i < 0 || i >= npoly(s)) return FARM_EINVAL;
+    *areas = s->areas.data() + s->hdr[i].pix_start;
+    return FARM_OK;
+}
+
+int64_t farm_hostset_raster_failures(const farm_hostset *s) { return s->raster_fail; }
+
+void farm_free(void *p) { free(p); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------ R-tree ----
+// Sort-Tile-Recursive bulk-loaded R-tree over double boxes.  Any correct
+// spatial index yields the same candidate set: every pair whose MBBs overlap
+// inclusively (RTree::Overlap, RTree.h:1567-1580).
+namespace {
+
+struct RNode {
+    double b[4];
+    int32_t first, count;  // children (internal) or entries (leaf)
+    int32_t leaf;
+};
+
+struct RTree {
+    static const int M = 16;
+    std::vector<RNode> nodes;
+    std::vector<int32_t> items;
+    int root = -1;
+
+    void build(const double *boxes, int64_t n) {
+        nodes.clear();
+        items.resize(n);
+        for (int64_t i = 0; i < n; i++) items[i] = (int32_t)i;
+        if (n == 0) return;
+        auto cx = [&](int32_t i) { return boxes[4 * i] + boxes[4 * i + 2]; };
+        auto cy = [&](int32_t i) { return boxes[4 * i + 1] + boxes[4 * i + 3]; };
+        const int64_t nleaf = (n + M - 1) / M;
+        const int64_t slices = (int64_t)std::ceil(std::sqrt((double)nleaf));
+        std::sort(items.begin(), items.end(), [&](int32_t a, int32_t b) { return cx(a) < cx(b); });
+        const int64_t per_slice = slices * M;
+        for (int64_t s0 = 0; s0 < n; s0 += per_slice) {
+            int64_t s1 = std::min(n, s0 + per_slice);
+            std::sort(items.begin() + s0, items.begin() + s1, [&](int32_t a, int32_t b) { return cy(a) < cy(b); });
+        }
+        std::vector<int32_t> level;
+        for (int64_t s0 = 0; s0 < n; s0 += M) {
+            RNode nd;
+            nd.first = (int32_t)s0;
+            nd.count = (int32_t)std::min<int64_t>(M, n - s0);
+            nd.leaf = 1;
+            nd.b[0] = nd.b[1] = 1e300;
+            nd.b[2] = nd.b[3] = -1e300;
+            for (int k = 0; k < nd.count; k++) {
+                const double *bb = boxes + 4 * items[s0 + k];
+                nd.b[0] = std::min(nd.b[0], bb[0]);
+                nd.b[1] = std::min(nd.b[1], bb[1]);
+                nd.b[2] = std::max(nd.b[2], bb[2]);
+                nd.b[3] = std::max(nd.b[3], bb[3]);
+            }
+            level.push_back((int32_t)nodes.size());
+            nodes.push_back(nd);
+        }
+        while (level.size() > 1) {
+            std::vector<int32_t> up;
+            // children of one parent must be contiguous in `nodes`
+            std::vector<RNode> moved;
+            for (size_t s0 = 0; s0 < level.size(); s0 += M) {
+                RNode nd;
+                nd.leaf = 0;
+                nd.count = (int32_t)std::min<size_t>(M, level.size() - s0);
+                nd.first = level[s0];
+                nd.b[0] = nd.b[1] = 1e300;
+                nd.b[2] = nd.b[3] = -1e300;
+                for (int k = 0; k < nd.count; k++) {
+                    const RNode &c = nodes[level[s0 + k]];
+                    nd.b[0] = std::min(nd.b[0], c.b[0]);
+                    nd.b[1] = std::min(nd.b[1], c.b[1]);
+                    nd.b[2] = std::max(nd.b[2], c.b[2]);
+                    nd.b[3] = std::max(nd.b[3], c.b[3]);
+                }
+                moved.push_back(nd);
+            }
+            for (auto &nd : moved) {
+                up.push_back((int32_t)nodes.size());
+                nodes.push_back(nd);
+            }
+            level.swap(up);
+        }
+        root = level[0];
+    }
+
+    static bool overlap(const double *a, const double *b) {
+        return !(a[0] > b[2] || b[0] > a[2] || a[1] > b[3] || b[1] > a[3]);
+    }
+
+    template <class F>
+    void search(const double *q, const double *boxes, F &&emit) const {
+        if (root < 0) return;
+        int32_t stack[128];
+        int sp = 0;
+        stack[sp++] = root;
+        while (sp) {
+            const RNode &nd = nodes[stack[--sp]];
+            if (!overlap(nd.b, q)) continue;
+            if (nd.leaf) {
+                for (int k = 0; k < nd.count; k++) {
+                    int32_t it = items[nd.first + k];
+                    if (overlap(boxes + 4 * it, q)) emit(it);
+                }
+            } else {
+                for (int k = 0; k < nd.count; k++) stack[sp++] = nd.first + k;
+            }
+        }
+    }
+};
+
+const double kDegPerKmLat = 360.0 / 40076.0;
+
+// degree_per_kilometer_longitude (util.h:73-96): the 900-entry table equals
+// 360 / (sin((90 - k/10) * PI / 180) * 40076), PI = 3.14159265, printed with
+// six decimals; it is rebuilt from that formula.
+struct Dpk {
+    double v[900];
+    Dpk() {
+        const double PI = 3.14159265;
+        char buf[64];
+        for (int k = 0; k < 900; k++) {
+            double f = 360.0 / (std::sin((90 - k / 10.0) * PI / 180) * 40076);
+            snprintf(buf, sizeof buf, "%.6f", f);
+            v[k] = strtod(buf, nullptr);
+        }
+    }
+    double lon(double lat) const {
+        double a = std::abs(lat);
+        if (a == 90) a = 89.9;
+        return v[(int)(a * 10)];
+    }
+};
+const Dpk kDpk;
+
+}  // namespace
+
+extern "C" int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b, double expand_km, int skip_self,
+                                   int nthreads, uint32_t **pa, uint32_t **pb) {
+    const int64_t na = npoly(a), nb = npoly(b);
+    RTree rt;
+    rt.build(a->mbb.data(), na);
+    if (nthreads < 1) nthreads = 1;
+    const int64_t chunk = 256;
+    const int64_t nchunks = (nb + chunk - 1) / chunk;
+    std::vector<std::vector<uint32_t>> res_a(nchunks), res_b(nchunks);
+    std::atomic<int64_t> next(0);
+    auto work = [&]() {
+        std::vector<uint32_t> hits;
+        for (;;) {
+            int64_t c = next.fetch_add(1);
+            if (c >= nchunks) break;
+            for (int64_t j = c * chunk; j < std::min(nb, (c + 1) * chunk); j++) {
+                double q[4];
+                memcpy(q, &b->mbb[4 * j], sizeof q);
+                if (expand_km > 0) {  // box::expand (Box.cpp:200-214)
+                    double sx = expand_km * kDpk.lon(q[1]);
+                    double sy = expand_km * kDegPerKmLat;
+                    q[0] -= sx;
+                    q[2] += sx;
+                    q[1] -= sy;
+                    q[3] += sy;
+                }
+                hits.clear();
+                rt.search(q, a->mbb.data(), [&](int32_t i) {
+                    if (skip_self && i == j) return;
+                    hits.push_back((uint32_t)i);
+                });
+                std::sort(hits.begin(), hits.end());
+                for (uint32_t i : hits) {
+                    res_a[c].push_back(i);
+                    res_b[c].push_back((uint32_t)j);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+    int64_t total = 0;
+    for (auto &v : res_a) total += (int64_t)v.size();
+    *pa = (uint32_t *)malloc(sizeof(uint32_t) * std::max<int64_t>(total, 1));
+    *pb = (uint32_t *)malloc(sizeof(uint32_t) * std::max<int64_t>(total, 1));
+    if (!*pa || !*pb) return -1;
+    int64_t o = 0;
+    for (int64_t c = 0; c < nchunks; c++) {
+        std::copy(res_a[c].begin(), res_a[c].end(), *pa + o);
+        std::copy(res_b[c].begin(), res_b[c].end(), *pb + o);
+        o += (int64_t)res_a[c].size();
+    }
+    return total;
+}

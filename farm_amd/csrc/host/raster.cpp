@@ -1,0 +1,563 @@
+// Host builder of the IDEAL raster model for one polygon.
+//
+// Semantics follow the reference CPU raster builder exactly (same pixel grid,
+// same BORDER marking walk, same edge sequences, same scanline parity, same
+// clipped-area fill classes):
+//   grid sizing          MyRaster::init_raster(int)      MyRaster.cpp:8-33
+//   edge walk            Ideal::evaluate_edges           Ideal.cpp:252-538
+//   edge sequences       Ideal::process_crosses          Ideal.cpp:90-183
+//   grid-line nodes      Ideal::process_intersection     Ideal.cpp:185-238
+//   CSR fill-in          Ideal::process_pixels_null      Ideal.cpp:35-45
+//   IN/OUT               Ideal::scanline_reandering      Ideal.cpp:540-573
+//   fill classes         Ideal::calculate_fullness       Ideal.cpp:820-918
+//                        (intersectionDualX/Y Ideal.cpp:575-818,
+//                         computePolygonArea geometry_computation.h:275-297,
+//                         encodePixelArea Ideal.cpp:64-88)
+// The data structures are this project's own: events are bucketed per pixel
+// with a stable counting sort instead of per-pixel std::map vectors, the
+// clipped strips are built once per column from the edges that reach it, and
+// the tolerant point dedup runs through a hash set whenever tolerant and
+// exact float equality provably coincide.
+#include "raster.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace farm {
+
+namespace {
+
+constexpr int kCC = 16;  // fill categories (MyRaster.h:33)
+constexpr uint8_t kBorderRaw = 1;
+constexpr uint8_t kInRaw = 2;
+
+struct P2 {
+    float x, y;
+};
+
+inline int dbl2int(double v) {  // util.h:98-106
+    int i = (int)v;
+    if (std::abs(1.0 * (i + 1) - v) < 0.00000001) i++;
+    return i;
+}
+
+// Point::operator== / != (Point.h:32-33): float difference within 1e-9
+inline bool same_pt(const P2 &a, const P2 &b) {
+    float dx = a.x - b.x, dy = a.y - b.y;
+    return std::fabs(dx) < 1e-9 && std::fabs(dy) < 1e-9;
+}
+
+struct Event {
+    int pix;
+    int edge;
+    int leave;  // 1 LEAVE, 0 ENTER
+};
+struct Node {
+    int line;
+    double v;
+};
+
+}  // namespace
+
+struct RasterScratch {
+    std::vector<Event> ev, ev_sorted;
+    std::vector<int> cnt;
+    std::vector<Node> hn, vn;
+    std::vector<P2> strip, cell;
+    std::vector<uint64_t> hash;
+    std::vector<int> strip_start;
+    std::vector<P2> strips;
+    std::vector<int> strip_col;
+};
+
+RasterScratch *raster_scratch_new() { return new RasterScratch(); }
+void raster_scratch_free(RasterScratch *s) { delete s; }
+
+RasterGrid size_grid(const double mbb[4], int num_pixels) {
+    RasterGrid g;
+    g.low[0] = mbb[0];
+    g.low[1] = mbb[1];
+    g.high[0] = mbb[2];
+    g.high[1] = mbb[3];
+    const double w = mbb[2] - mbb[0], h = mbb[3] - mbb[1];
+    const double aspect = std::abs(h / w);
+    g.dimx = (int)std::pow(num_pixels / aspect, 0.5);
+    g.dimy = (int)(g.dimx * aspect);
+    if (g.dimx == 0) g.dimx = 1;
+    if (g.dimy == 0) g.dimy = 1;
+    g.step[0] = w / g.dimx;
+    g.step[1] = h / g.dimy;
+    if (g.step[0] < 0.00001) {
+        g.step[0] = 0.00001;
+        g.dimx = (int)(w / g.step[0] + 1);
+    }
+    if (g.step[1] < 0.00001) {
+        g.step[1] = 0.00001;
+        g.dimy = (int)(h / g.step[1] + 1);
+    }
+    return g;
+}
+
+namespace {
+
+// Point dedup of TempPolygon::addPoint (MyPolygon.h:49-57).  When every
+// coordinate the strip can hold is >= 2^-6 (and all share a sign), two
+// different floats differ by at least 2^-29 > 1e-9, so tolerant equality is
+// exact equality and a hash set answers it.
+struct Dedup {
+    bool exact;
+    std::vector<uint64_t> *tab;
+    uint64_t mask = 0;
+    size_t n = 0;
+    void reset(size_t cap) {
+        if (!exact) return;
+        size_t sz = 64;
+        while (sz < cap * 2) sz <<= 1;
+        tab->assign(sz, ~0ull);
+        mask = sz - 1;
+        n = 0;
+    }
+    static uint64_t key(const P2 &p) {
+        uint32_t a, b;
+        std::memcpy(&a, &p.x, 4);
+        std::memcpy(&b, &p.y, 4);
+        return ((uint64_t)a << 32) | b;
+    }
+    // returns true if p was not present (and records it)
+    bool insert(std::vector<P2> &v, const P2 &p) {
+        if (!exact) {
+            for (const P2 &q : v)
+                if (same_pt(q, p)) return false;
+            v.push_back(p);
+            return true;
+        }
+        uint64_t k = key(p);
+        uint64_t hsh = k * 0x9E3779B97F4A7C15ull;
+        size_t i = (hsh >> 20) & mask;
+        while ((*tab)[i] != ~0ull) {
+            if ((*tab)[i] == k) return false;
+            i = (i + 1) & mask;
+        }
+        if ((n + 1) * 2 > tab->size()) {  // grow and rehash
+            std::vector<uint64_t> old;
+            old.swap(*tab);
+            tab->assign(old.size() * 2, ~0ull);
+            mask = tab->size() - 1;
+            for (uint64_t o : old) {
+                if (o == ~0ull) continue;
+                size_t j = ((o * 0x9E3779B97F4A7C15ull) >> 20) & mask;
+                while ((*tab)[j] != ~0ull) j = (j + 1) & mask;
+                (*tab)[j] = o;
+            }
+            i = (hsh >> 20) & mask;
+            while ((*tab)[i] != ~0ull) i = (i + 1) & mask;
+        }
+        (*tab)[i] = k;
+        n++;
+        v.push_back(p);
+        return true;
+    }
+};
+
+inline bool within(double lo, double hi, double v) { return !(v < lo || v > hi); }
+
+}  // namespace
+
+void build_raster(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out, RasterScratch *S) {
+    out.ok = true;
+    const int npix_req = nv / vpr;
+    out.g = size_grid(mbb, npix_req);
+    const RasterGrid &g = out.g;
+    const int dimx = g.dimx, dimy = g.dimy;
+    const int npix = dimx * dimy;
+    const double x0 = g.low[0], y0 = g.low[1];
+    const double sx = g.step[0], sy = g.step[1];
+    const P2 *p = reinterpret_cast<const P2 *>(ring);
+
+    out.status.assign(npix, 0);
+    out.areas.assign(npix, 0.0);
+    out.pixoff.assign((size_t)npix + 1, 0);
+    out.voff.assign((size_t)dimx + 2, 0);
+    out.hoff.assign((size_t)dimy + 2, 0);
+    out.seqs.clear();
+    out.vnodes.clear();
+    out.hnodes.clear();
+
+    auto &ev = S->ev;
+    auto &hn = S->hn;
+    auto &vn = S->vn;
+    ev.clear();
+    hn.clear();
+    vn.clear();
+    uint8_t *st = out.status.data();
+
+    auto pid = [&](int x, int y) -> int {
+        if (x < 0 || x >= dimx || y < 0 || y >= dimy) {
+            out.ok = false;
+            x = std::min(std::max(x, 0), dimx - 1);
+            y = std::min(std::max(y, 0), dimy - 1);
+        }
+        return y * dimx + x;
+    };
+    // one pixel-to-pixel step of the walk: LEAVE the old pixel, ENTER the new
+    auto step_to = [&](int ox, int oy, int nx, int ny, int e) {
+        int a = pid(ox, oy), b = pid(nx, ny);
+        st[a] = kBorderRaw;
+        ev.push_back({a, e, 1});
+        ev.push_back({b, e, 0});
+        st[b] = kBorderRaw;
+    };
+
+    // ---- edge walk (BORDER pixels, crossings, grid-line nodes) ----
+    for (int e = 0; e < nv - 1 && out.ok; e++) {
+        const double ax = p[e].x, ay = p[e].y, bx = p[e + 1].x, by = p[e + 1].y;
+        int cx0 = dbl2int((ax - x0) / sx), cx1 = dbl2int((bx - x0) / sx);
+        int cy0 = dbl2int((ay - y0) / sy), cy1 = dbl2int((by - y0) / sy);
+        if (cx0 == dimx) cx0--;
+        if (cx1 == dimx) cx1--;
+        if (cy0 == dimy) cy0--;
+        if (cy1 == dimy) cy1--;
+        if (cx0 >= dimx || cx1 >= dimx || cy0 >= dimy || cy1 >= dimy) {
+            out.ok = false;
+            break;
+        }
+        st[pid(cx0, cy0)] = kBorderRaw;
+        st[pid(cx1, cy1)] = kBorderRaw;
+        if (cx0 == cx1 && cy0 == cy1) continue;
+
+        if (ay == by) {  // horizontal edge
+            const int d = cx0 < cx1 ? 1 : -1;
+            for (int x = cx0; x != cx1; x += d) {
+                vn.push_back({d > 0 ? x + 1 : x, ay});
+                step_to(x, cy0, x + d, cy0, e);
+            }
+            continue;
+        }
+        if (ax == bx) {  // vertical edge
+            const int d = cy0 < cy1 ? 1 : -1;
+            for (int y = cy0; y != cy1; y += d) {
+                hn.push_back({d > 0 ? y + 1 : y, ax});
+                step_to(cx0, y, cx0, y + d, e);
+            }
+            continue;
+        }
+        // general edge: follow the line y = k*x + c through the grid
+        const double k = (ay - by) / (ax - bx);
+        const double c = (ax * by - bx * ay) / (ax - bx);
+        const int dx = cx0 < cx1 ? 1 : -1, dy = cy0 < cy1 ? 1 : -1;
+        const int ylo = std::min(cy0, cy1), yhi = std::max(cy0, cy1);
+        const int xlo = std::min(cx0, cx1), xhi = std::max(cx0, cx1);
+        int x = cx0, y = cy0;
+        int64_t guard = 4LL * ((int64_t)dimx + dimy) + 16;
+        while ((x != cx1 || y != cy1) && out.ok) {
+            if (guard-- <= 0) {
+                out.ok = false;
+                break;
+            }
+            bool moved = false;
+            if (x != cx1) {
+                const double xv = dx > 0 ? ((double)x + 1) * sx + x0 : (double)x * sx + x0;
+                const double yv = xv * k + c;
+                const double q = (yv - y0) / sy;
+                if (std::abs(q - std::round(q)) < 1e-9) {
+                    // the line passes a grid corner: step diagonally; only the
+                    // rightward walks record a vertical node (Ideal.cpp:397-423).
+                    // An edge that stays in one pixel row matches none of the
+                    // reference's four diagonal cases and its walk never ends.
+                    if (cy0 == cy1) {
+                        out.ok = false;
+                        break;
+                    }
+                    if (dx > 0) vn.push_back({dy > 0 ? x + 1 : x, yv});
+                    step_to(x, y, x + dx, y + dy, e);
+                    x += dx;
+                    y += dy;
+                    continue;
+                }
+                int ry = (int)((yv - y0) / sy);
+                ry = std::min(std::max(ry, ylo), yhi);
+                if (ry == y) {
+                    moved = true;
+                    vn.push_back({dx > 0 ? x + 1 : x, yv});
+                    step_to(x, y, x + dx, y, e);
+                    x += dx;
+                }
+            }
+            if (y != cy1) {
+                const double yv = dy > 0 ? (y + 1) * sy + y0 : y * sy + y0;
+                const double xv = (yv - c) / k;
+                int rx = (int)((xv - x0) / sx);
+                rx = std::min(std::max(rx, xlo), xhi);
+                if (rx == x) {
+                    moved = true;
+                    hn.push_back({dy > 0 ? y + 1 : y, xv});
+                    step_to(x, y, x, y + dy, e);
+                    y += dy;
+                }
+            }
+            if (!moved) out.ok = false;
+        }
+    }
+    if (!out.ok) return;
+
+    // ---- edge sequences per pixel ----
+    auto &cnt = S->cnt;
+    if (ev.empty()) {
+        // The whole ring lies in one pixel.  The reference records
+        // (0, num_vertices) here, one edge more than the ring has, and its
+        // refinement reads one vertex past the array; the real edges are kept.
+        out.seqs.push_back(0);
+        out.seqs.push_back((uint32_t)(nv - 1));
+        out.pixoff[0] = 0;
+    } else {
+        cnt.assign((size_t)npix + 1, 0);
+        for (const Event &v : ev) cnt[v.pix + 1]++;
+        for (int i = 0; i < npix; i++) cnt[i + 1] += cnt[i];
+        auto &sorted = S->ev_sorted;
+        sorted.resize(ev.size());
+        {
+            std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+            for (const Event &v : ev) sorted[pos[v.pix]++] = v;
+        }
+        uint32_t nseq = 0;
+        for (int pix = 0; pix < npix && out.ok; pix++) {
+            int b = cnt[pix], m = cnt[pix + 1] - b;
+            if (m == 0) continue;
+            const Event *E = &sorted[b];
+            out.pixoff[pix] = nseq;
+            // an odd list is closed with the opposite type of its last entry
+            auto type_at = [&](int i) { return i < m ? E[i].leave : !E[m - 1].leave; };
+            auto edge_at = [&](int i) { return i < m ? E[i].edge : E[m - 1].edge; };
+            const int mm = m + (m & 1);
+            auto push = [&](int s, int t) {
+                if (t - s + 1 <= 0) out.ok = false;
+                out.seqs.push_back((uint32_t)s);
+                out.seqs.push_back((uint32_t)(t - s + 1));
+                nseq++;
+            };
+            int i = 0, last = mm - 1;
+            if (type_at(0) == 1) {  // the ring starts inside this pixel
+                if (type_at(last) != 0) out.ok = false;
+                push(0, edge_at(0));
+                push(edge_at(last), nv - 2);
+                i = 1;
+                last--;
+            }
+            for (; i <= last; i++) {
+                if (type_at(i) != 0) out.ok = false;
+                if (i == last || type_at(i + 1) == 0) {
+                    push(edge_at(i), edge_at(i));
+                } else {
+                    push(edge_at(i), edge_at(i + 1));
+                    i++;
+                }
+            }
+        }
+        if (nseq >= 65536) out.ok = false;  // Ideal.cpp:32
+        // a BORDER pixel that got no crossing has no defined sequence range
+        for (int pix = 0; pix < npix; pix++)
+            if (st[pix] == kBorderRaw && cnt[pix + 1] == cnt[pix]) out.ok = false;
+    }
+    if (!out.ok) return;
+    const uint32_t nseq_total = (uint32_t)(out.seqs.size() / 2);
+    out.pixoff[npix] = nseq_total;
+    for (int i = npix - 1; i >= 0; i--)
+        if (st[i] != kBorderRaw) out.pixoff[i] = out.pixoff[i + 1];
+
+    // ---- grid-line nodes: bucket per line, sort each line ----
+    auto fill_lines = [&](std::vector<Node> &nodes, int nlines, std::vector<uint32_t> &off, std::vector<double> &vals,
+                          bool vertical) {
+        vals.resize(nodes.size());
+        std::vector<int> c(nlines + 2, 0);
+        for (const Node &n : nodes) c[n.line + 1]++;
+        for (int i = 0; i <= nlines; i++) c[i + 1] += c[i];
+        std::vector<int> pos(c.begin(), c.end() - 1);
+        for (const Node &n : nodes) vals[pos[n.line]++] = n.v;
+        // lines without nodes keep offset 0 (Grid_line.cpp:3-7)
+        for (int l = 0; l <= nlines; l++) {
+            if (c[l + 1] > c[l]) {
+                off[l] = (uint32_t)c[l];
+                std::sort(vals.begin() + c[l], vals.begin() + c[l + 1]);
+            }
+        }
+        if (vertical) {
+            off[nlines + 1] = (uint32_t)nodes.size();
+            off[nlines] = (uint32_t)nodes.size();
+        } else {
+            off[nlines] = (uint32_t)nodes.size();
+        }
+    };
+    fill_lines(hn, dimy, out.hoff, out.hnodes, false);
+    fill_lines(vn, dimx, out.voff, out.vnodes, true);
+
+    // ---- scanline parity: IN/OUT for rows >= 1 ----
+    for (int y = 1; y < dimy; y++) {
+        bool inside = false;
+        uint32_t i = out.hoff[y], j = out.hoff[y + 1];
+        for (int x = 0; x < dimx; x++) {
+            uint8_t &s = st[y * dimx + x];
+            if (s != kBorderRaw) {
+                s = inside ? kInRaw : 0;
+                continue;
+            }
+            int pass = 0;
+            const double right = x0 + sx * (x + 1);
+            while (i < j && out.hnodes[i] <= right) {
+                pass++;
+                i++;
+            }
+            if (pass & 1) inside = !inside;
+        }
+    }
+
+    // ---- fill classes: clip to each column strip, then to each pixel ----
+    const double pix_area = sx * sy;
+    const bool exact_dedup = (mbb[0] > 0.0160 && mbb[1] > 0.0160);
+    Dedup dd{exact_dedup, &S->hash};
+    auto &strips = S->strips;
+    auto &sstart = S->strip_start;
+    auto &scol = S->strip_col;
+    strips.clear();
+    sstart.clear();
+    scol.clear();
+
+    {
+        double Xi = x0, Xi1 = Xi + sx;
+        const double kx = Xi + dimx * sx;
+        int col = 0;
+        while (Xi1 < kx + 1e-9) {
+            const size_t base = strips.size();
+            S->strip.clear();
+            dd.reset(64);
+            const float fXi = (float)Xi, fXi1 = (float)Xi1;
+            for (int e = 0; e < nv - 1; e++) {
+                const P2 A = p[e], B = p[e + 1];
+                double yi, yi1;
+                if (A.x == B.x) {
+                    yi = std::min(A.y, B.y);
+                    yi1 = yi;
+                } else {
+                    const float den = B.x - A.x;
+                    const double slope = ((double)B.y - A.y) / den;
+                    yi = slope * (Xi - B.x) + B.y;
+                    yi1 = slope * (Xi1 - B.x) + B.y;
+                }
+                const P2 q0 = {fXi, (float)yi}, q1 = {fXi1, (float)yi1};
+                const float mny = std::min(A.y, B.y), mxy = std::max(A.y, B.y);
+                const float mnx = std::min(A.x, B.x), mxx = std::max(A.x, B.x);
+                const bool in0 = within(mny, mxy, q0.y) && within(mnx, mxx, q0.x);
+                const bool in1 = within(mny, mxy, q1.y) && within(mnx, mxx, q1.x);
+                if (within(Xi, Xi1, A.x)) dd.insert(S->strip, A);
+                if (A.x < B.x) {
+                    if (in0) dd.insert(S->strip, q0);
+                    if (in1) dd.insert(S->strip, q1);
+                } else {
+                    if (in1) dd.insert(S->strip, q1);
+                    if (in0) dd.insert(S->strip, q0);
+                }
+                if (within(Xi, Xi1, B.x)) dd.insert(S->strip, B);
+            }
+            if (!S->strip.empty()) {
+                S->strip.push_back(S->strip.front());
+                sstart.push_back((int)base);
+                scol.push_back(col);
+                strips.insert(strips.end(), S->strip.begin(), S->strip.end());
+            }
+            Xi += sx;
+            Xi1 = Xi + sx;
+            col++;
+        }
+        sstart.push_back((int)strips.size());
+    }
+
+    const double ky = y0 + dimy * sy;
+    auto &cell = S->cell;
+    for (size_t si = 0; si + 1 < sstart.size(); si++) {
+        const P2 *V = strips.data() + sstart[si];
+        const int nvs = sstart[si + 1] - sstart[si];
+        const int col = scol[si];
+        double Yi = y0, Yi1 = Yi + sy;
+        int row = 0;
+        while (Yi1 < ky + 1e-9) {
+            cell.clear();
+            for (int k = 0; k + 1 < nvs; k++) {
+                const P2 A = V[k], B = V[k + 1];
+                bool skip = false, ok0 = true, ok1 = true;
+                double xi = 0, xi1 = 0;
+                if (A.x == B.x) {
+                    xi = A.x;
+                    xi1 = A.x;
+                } else {
+                    const float den = B.x - A.x;
+                    const double slope = ((double)B.y - A.y) / den;
+                    if (A.y == B.y) {
+                        if (A.y == Yi) {
+                            xi = A.x;
+                            ok1 = false;
+                        } else if (A.y == Yi1) {
+                            ok0 = false;
+                            xi1 = A.x;
+                        } else {
+                            skip = true;
+                        }
+                    } else {
+                        xi = ((Yi - B.y) / slope) + B.x;
+                        xi1 = ((Yi1 - B.y) / slope) + B.x;
+                    }
+                }
+                if (within(Yi, Yi1, A.y)) cell.push_back(A);
+                if (skip) continue;
+                const P2 q0 = {(float)xi, (float)Yi}, q1 = {(float)xi1, (float)Yi1};
+                const float mny = std::min(A.y, B.y), mxy = std::max(A.y, B.y);
+                const float mnx = std::min(A.x, B.x), mxx = std::max(A.x, B.x);
+                const bool in0 = ok0 && within(mnx, mxx, q0.x) && within(mny, mxy, q0.y);
+                const bool in1 = ok1 && within(mnx, mxx, q1.x) && within(mny, mxy, q1.y);
+                if (A.y < B.y) {
+                    if (in0) cell.push_back(q0);
+                    if (in1) cell.push_back(q1);
+                } else {
+                    if (in1) cell.push_back(q1);
+                    if (in0) cell.push_back(q0);
+                }
+            }
+            if (!cell.empty() && !same_pt(cell.front(), cell.back())) cell.push_back(cell.front());
+            double area = 0.0;
+            if (cell.size() >= 3) {
+                for (size_t k = 0; k + 1 < cell.size(); k++)
+                    area += ((double)cell[k].x * cell[k + 1].y - (double)cell[k + 1].x * cell[k].y);
+                area = std::abs(area) / 2.0;
+            }
+            // encodePixelArea
+            int cls;
+            const double ratio = area / pix_area;
+            if (std::fabs(ratio - 1.0) < 1e-9) cls = kCC - 1;
+            else if (std::fabs(ratio) < 1e-9) cls = 0;
+            else {
+                cls = (int)std::ceil(ratio * (kCC - 2));
+                if (cls >= kCC) cls = kCC - 1;
+                cls = (uint8_t)cls;
+            }
+            if (col >= dimx || row >= dimy) {
+                out.ok = false;
+                return;
+            }
+            const int id = row * dimx + col;
+            if (st[id] == kBorderRaw) {
+                st[id] = (uint8_t)std::max(1, std::min(cls, kCC - 2));
+                out.areas[id] = area;
+            } else if (st[id] == kInRaw) {
+                st[id] = kCC - 1;
+                out.areas[id] = pix_area;
+            } else {
+                st[id] = 0;
+                out.areas[id] = 0.0;
+            }
+            Yi += sy;
+            Yi1 = Yi + sy;
+            row++;
+        }
+    }
+}
+
+}  // namespace farm

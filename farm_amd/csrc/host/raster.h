@@ -1,0 +1,39 @@
+// Host raster builder of the IDEAL model (one polygon at a time).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace farm {
+
+struct RasterGrid {
+    double low[2], high[2];
+    double step[2];
+    int dimx = 0, dimy = 0;
+};
+
+struct RasterOut {
+    RasterGrid g;
+    std::vector<uint8_t> status;   // dimx*dimy fill classes (0 OUT, 15 IN)
+    std::vector<double> areas;     // clipped polygon area per pixel
+    std::vector<uint32_t> pixoff;  // dimx*dimy+1, relative seq index
+    std::vector<uint32_t> seqs;    // (first edge, #edges) pairs
+    std::vector<uint32_t> voff;    // dimx+2
+    std::vector<double> vnodes;
+    std::vector<uint32_t> hoff;    // dimy+2
+    std::vector<double> hnodes;
+    bool ok = true;                // false: the reference raster builder asserts here
+};
+
+// Scratch buffers reused across polygons by one worker thread.
+struct RasterScratch;
+RasterScratch *raster_scratch_new();
+void raster_scratch_free(RasterScratch *s);
+
+// ring: nv points (x,y interleaved) incl. the repeated first vertex;
+// mbb: the loader's MBB (low x, low y, high x, high y).
+void build_raster(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out, RasterScratch *s);
+
+// Grid sizing of MyRaster::init_raster(int) for num_pixels pixels.
+RasterGrid size_grid(const double mbb[4], int num_pixels);
+
+}  // namespace farm

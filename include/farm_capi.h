@@ -1,0 +1,161 @@
+/*
+ * farm_capi.h -- C ABI of the MI355X polygon-join library (libfarm.so).
+ *
+ * This is the drop-in boundary that replaces the reference's GPU entry points
+ * declared in src/include/Ideal.h:205-213 (ResetDevice, cuda_create_buffer,
+ * preprocess_for_gpu, cuda_intersect, cuda_intersection, cuda_within_polygon)
+ * and the undeclared cuda_intersection_a (src/cuda/query_intersection.cu:756).
+ * The reference passes a query_context* holding every host/device buffer
+ * (src/include/query_context.h:171-229); here the same data crosses as plain
+ * pointers and sizes, owned by an opaque handle.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: FARM_OK (0) or an FARM_E* code;
+ *    farm_last_error() gives a message.  Nothing calls exit().
+ *  - One HIP stream per handle; calls on one handle are serialised by the
+ *    caller (not re-entrant).  Separate handles are independent.
+ *  - Input arrays are host memory unless the name says "dev"; the library
+ *    copies them and the caller keeps ownership.
+ *  - Polygon ids: the uploaded set is one array; the reference GPU path also
+ *    stores sources then targets in one array and shifts target ids by the
+ *    number of sources (src/cuda/preprocessing.cu:32).
+ */
+#ifndef FARM_CAPI_H
+#define FARM_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FARM_OK 0
+#define FARM_EINVAL 1     /* bad argument / inconsistent layout */
+#define FARM_ENOMEM 2     /* device allocation failed */
+#define FARM_EDEVICE 3    /* HIP runtime / launch error */
+#define FARM_ECAPACITY 4  /* a bounded buffer would overflow */
+#define FARM_ESTATE 5     /* call order violated (e.g. no polygons uploaded) */
+
+/* Category count of the raster fill classes used by the CPU reference
+ * (src/include/MyRaster.h:33): 0 = OUT, 15 = IN, 1..14 = BORDER. */
+#define FARM_CATEGORY_COUNT 16
+
+/*
+ * Per-polygon raster header (the IDEAL model of one polygon).  Replaces the
+ * reference's IdealOffset (src/include/Ideal.h:41-50) + RasterInfo
+ * (src/include/MyRaster.h:14-21) pair.  128 bytes; the first 64 bytes are
+ * what the raster filter reads.
+ */
+typedef struct farm_raster_hdr {
+    double mbr[4];        /* raster box: low x, low y, high x, high y */
+    double step[2];       /* pixel width / height */
+    int32_t dimx, dimy;   /* pixel grid; pixel id = y * dimx + x */
+    uint64_t pix_start;   /* first status byte in farm_polyset.status */
+    /* --- refinement --- */
+    uint64_t pixoff_start;  /* first entry in pixoff (dimx*dimy + 1 entries) */
+    uint64_t seq_start;     /* first entry in seqs */
+    uint64_t vert_start;    /* first vertex in x / y */
+    uint32_t nverts;        /* ring length incl. the repeated first vertex */
+    uint32_t nseqs;
+    uint64_t vline_start;   /* first entry in voff (dimx + 2 entries) */
+    uint64_t vnode_start;   /* first entry in vnodes */
+    uint64_t reserved[2];
+} farm_raster_hdr;
+
+/*
+ * A rasterised polygon set in device layout (SoA of CSR), built on the host.
+ *  status[pix_start + id]            fill class byte per pixel
+ *  pixoff[pixoff_start + id .. +1]   edge-sequence range of pixel id, relative
+ *                                    to seq_start (valid for BORDER pixels)
+ *  seqs[2*(seq_start + k)]           first edge (relative to vert_start)
+ *  seqs[2*(seq_start + k) + 1]       number of edges
+ *  x/y[vert_start + v]               ring vertices (f32)
+ *  voff/vnodes                       vertical grid-line crossings, per line
+ *                                    sorted (used by point-in-polygon)
+ */
+typedef struct farm_polyset {
+    int64_t npolys;
+    const farm_raster_hdr *hdr;
+    const uint8_t *status;
+    uint64_t nstatus;
+    const uint32_t *pixoff;
+    uint64_t npixoff;
+    const uint32_t *seqs; /* 2 * nseqs */
+    uint64_t nseqs;
+    const float *x;
+    const float *y;
+    uint64_t nverts;
+    const uint32_t *voff;
+    uint64_t nvoff;
+    const double *vnodes;
+    uint64_t nvnodes;
+} farm_polyset;
+
+/* Query modes of farm_intersect. */
+#define FARM_MODE_EXACT_PLAIN 0    /* one candidate pair per workgroup */
+#define FARM_MODE_EXACT_BATCHED 1  /* filter pass + cost-ordered persistent refine */
+#define FARM_MODE_APPROX_HEURISTIC 2 /* reference -a: IN overlap or BORDER pair prob > 1 */
+#define FARM_MODE_APPROX_LEARNED 3   /* MLP logit on pair features, threshold in opts */
+
+typedef struct farm_opts {
+    int32_t mode;
+    int32_t tile_a;        /* 0 = autotuned default */
+    int32_t tile_b;        /* 0 = autotuned default */
+    int32_t order;         /* batched refine order: 0 exact-cost, 1 learned cost, 2 none */
+    float approx_threshold;/* learned -a decision threshold on the logit */
+    float merge_threshold; /* -m, accepted for compatibility; exact results do not depend on it */
+    int32_t reserved[2];
+} farm_opts;
+
+/* Counters of the work actually executed (device counters). */
+typedef struct farm_stats {
+    uint64_t pairs;            /* candidate pairs processed */
+    uint64_t in_hits;          /* pairs decided by an IN pixel overlap */
+    uint64_t bb_pixel_pairs;   /* BORDER x BORDER pixel pairs enumerated */
+    uint64_t refine_items;     /* refinement work items executed */
+    uint64_t segment_tests;    /* segment x segment tests executed */
+    uint64_t filter_bytes;     /* algorithmic filter bytes (full window enumeration) */
+    uint64_t refine_bytes;     /* algorithmic refinement bytes */
+    uint64_t found;            /* pairs reported true */
+    double filter_ms;          /* device time of the filter kernel(s) */
+    double refine_ms;          /* device time of the refinement kernel(s) */
+    double total_ms;           /* device time of the whole call */
+} farm_stats;
+
+typedef struct farm_handle_s *farm_handle;
+
+/* Open a handle on HIP device `device` (replaces ResetDevice +
+ * cuda_create_buffer, src/cuda/preprocessing.cu:5-126,362-379). */
+int farm_open(int device, farm_handle *out);
+int farm_close(farm_handle h);
+const char *farm_last_error(farm_handle h);
+
+/* Upload a rasterised polygon set (replaces preprocess_for_gpu's packing and
+ * host-to-device copies, src/cuda/preprocessing.cu:128-343). */
+int farm_upload_polygons(farm_handle h, const farm_polyset *ps);
+
+/* Upload candidate pairs (ids index the uploaded set); replaces the pair
+ * upload at src/cuda/preprocessing.cu:352-358. */
+int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint64_t n);
+
+/* Exact / approximate intersect over pairs [begin, end) (replaces
+ * cuda_intersect, src/cuda/query_intersect.cu:335-464, whose batch window is
+ * query_context::index/index_end).  out[k - begin] = 1 if pair k intersects.
+ * out may be NULL (counts only, as the reference, via stats->found). */
+int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out,
+                   farm_stats *stats);
+
+/* Device-pointer variant for callers that keep results resident (bench):
+ * out_dev is a device buffer of at least end - begin bytes. */
+int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
+                       farm_stats *stats);
+
+/* Library version string. */
+const char *farm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FARM_CAPI_H */

@@ -1,0 +1,71 @@
+/*
+ * farm_host.h -- host side of the MI355X polygon-join library (libfarm.so):
+ * .dt / WKT polygon storage, candidate-pair generation (host R-tree) and the
+ * host raster builder that produces the device layout of farm_capi.h.
+ *
+ * These replace the reference's host preprocessing:
+ *   farm_hostset_load_dt      load_binary_file   src/storage/storage.cpp:361-449
+ *   farm_hostset_save_dt      dump_polygons_to_file  src/storage/storage.cpp:158-193
+ *   farm_hostset_load_wkt     load_polygon_wkt   src/storage/storage.cpp:45-94
+ *   farm_hostset_rasterize    preprocess / process_rasterization
+ *                             src/geometry/preprocessing.cpp:4-87 (+ Ideal.cpp:920-955)
+ *   farm_candidates           rtree_query + MySearchCallback
+ *                             src/query/intersect_cpu.cpp:7-36, within_polygon_cpu.cpp:8-48
+ */
+#ifndef FARM_HOST_H
+#define FARM_HOST_H
+
+#include "farm_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct farm_hostset farm_hostset;
+
+/* Build from rings as stored in .dt files (closing vertex included; the
+ * loader appends the first vertex once more, vertex_sequence.cpp:206-218).
+ * Rings with fewer than 2 stored vertices are dropped (storage.cpp:341). */
+farm_hostset *farm_hostset_from_rings(int64_t n, const uint64_t *ring_off, const float *xy);
+/* Ids follow file order (the reference's multi-threaded loader may permute
+ * files larger than 10 MiB, storage.cpp:355-357,447). */
+farm_hostset *farm_hostset_load_dt(const char *path);
+/* One polygon per line: "POLYGON ((x y, ...), (hole), ...)"; coordinates are
+ * parsed to float as std::stof does (util.h:129-144).  Holes are kept in the
+ * file but not used by queries (MyPolygon.cpp:754-757). */
+farm_hostset *farm_hostset_load_wkt(const char *path);
+int farm_hostset_save_dt(const farm_hostset *s, const char *path);
+/* Concatenate (sources then targets) before rasterizing. */
+farm_hostset *farm_hostset_concat(const farm_hostset *a, const farm_hostset *b);
+void farm_hostset_free(farm_hostset *s);
+int64_t farm_hostset_size(const farm_hostset *s);
+/* Stored ring of polygon i (without the appended copy of vertex 0). */
+int farm_hostset_ring(const farm_hostset *s, int64_t i, const float **xy, int64_t *nv);
+/* MBB (low x, low y, high x, high y) with the float 1e-9 bias of
+ * VertexSequence::getMBR (vertex_sequence.cpp:152-167). */
+int farm_hostset_mbbs(const farm_hostset *s, double *out4n);
+
+/* Rasterize every polygon with vpr vertices per pixel (-v), nthreads host
+ * threads (-n).  Fills the device layout returned by farm_hostset_view. */
+int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads);
+int farm_hostset_view(const farm_hostset *s, farm_polyset *out);
+/* Per-pixel clipped areas of polygon i (dimx*dimy doubles), for tests. */
+int farm_hostset_areas(const farm_hostset *s, int64_t i, const double **areas);
+/* Index of the first polygon the reference raster builder would reject
+ * (an assert), or -1. */
+int64_t farm_hostset_raster_failures(const farm_hostset *s);
+
+/* Candidate pairs: every (i in a, j in b) whose MBBs overlap inclusively
+ * (RTree::Overlap, src/index/RTree.h:1567-1580).  expand_km > 0 expands b's
+ * MBB with box::expand (Box.cpp:200-214); skip_self drops i == j (within
+ * self-join).  Pairs come out sorted by (j, i).  Arrays are allocated by the
+ * library; free them with farm_free. */
+int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b, double expand_km, int skip_self,
+                        int nthreads, uint32_t **pa, uint32_t **pb);
+void farm_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FARM_HOST_H */

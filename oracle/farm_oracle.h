@@ -1,0 +1,75 @@
+/*
+ * farm_oracle.h -- CPU restatement of the reference FARM CPU path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (farm_amd/) links,
+ * includes or calls this code; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, and only as the checker / baseline.
+ *
+ * The restatement follows the reference sources function by function; each
+ * function in farm_oracle.cpp cites the reference file:line it restates.
+ *
+ * Parity pinning: the reference publishes no golden vectors and cannot be
+ * built in this image (it needs boost headers that are absent), so every
+ * function here is pinned by tests/test_oracle_*.py against closed forms,
+ * invariants and brute force (see the header of farm_oracle.cpp for the
+ * function -> check map).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes returned by every orc_* entry point. */
+#define ORC_OK 0
+#define ORC_EINVAL 1
+/* the reference itself would abort (an assert fires) on this input */
+#define ORC_EREF_ABORT 2
+
+typedef struct orc_set orc_set;
+
+/* Build a polygon set from rings exactly as stored in a .dt file (the ring
+ * includes the WKT closing vertex; decode appends p[0] once more).
+ * ring_off has n+1 entries indexing xy pairs. Rings with fewer than 2 stored
+ * vertices are dropped, as the loader does. */
+orc_set *orc_create(int64_t n, const uint64_t *ring_off, const float *xy);
+void orc_destroy(orc_set *s);
+int64_t orc_size(const orc_set *s);
+
+/* Rasterize every polygon: init_raster(nv / vpr) then Ideal::rasterization.
+ * hierarchy != 0 additionally runs grid_align + layering(nlow) and merges
+ * layer status (the -h path).  Returns ORC_OK or ORC_EREF_ABORT (index of
+ * the first offending polygon in *bad). */
+int orc_rasterize(orc_set *s, int vpr, int hierarchy, int nlow, int nthreads, int64_t *bad);
+
+/* Raster accessors for parity tests. */
+int orc_raster_info(const orc_set *s, int64_t i, double *mbr4, int *dims2, double *steps2,
+                    int64_t *counts /* [nstatus, nseqs, nvnodes, nhnodes, nverts] */);
+int orc_raster_copy(const orc_set *s, int64_t i, uint8_t *status, uint32_t *offset,
+                    uint32_t *seqs /* 2 per seq */, uint32_t *voff, double *vnodes,
+                    uint32_t *hoff, double *hnodes, double *areas);
+
+/* Exact intersect over candidate pairs (source set a, target set b):
+ * out[k] = Ideal::intersect(a[pa[k]], b[pb[k]]).  mode 0 = exact,
+ * mode 1 = reference -a heuristic (IN overlap or a BORDER pixel pair with
+ * probability > 1).  nthreads workers take batches of 10 pairs.
+ * stats (may be NULL): [pixel_pairs_bb, segment_tests, in_hits]. */
+int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa,
+                  const uint32_t *pb, uint8_t *out, int mode, int nthreads, int64_t *stats);
+
+/* Candidate generation by brute force: all (i, j) with inclusively
+ * overlapping MBBs; writes up to cap pairs, returns the total count. */
+int64_t orc_candidates(const orc_set *a, const orc_set *b, int64_t cap, uint32_t *pa, uint32_t *pb);
+
+/* MBB of polygon i (low x, low y, high x, high y) as the loader computes it. */
+int orc_mbb(const orc_set *s, int64_t i, double *mbr4);
+
+/* Primitive restatements exposed for unit tests. */
+int orc_segment_intersect(const float *abcd /* ax ay bx by cx cy dx dy */);
+int orc_double_to_int(double v);
+double orc_point_to_segment_distance(const float *p, const float *p1, const float *p2, int geography);
+
+#ifdef __cplusplus
+}
+#endif

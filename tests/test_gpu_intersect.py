@@ -1,0 +1,95 @@
+"""GPU parity: exact intersect (plain and batched kernels, all tile widths)
+and the reference -a heuristic, bit-exact against the oracle."""
+import numpy as np
+import pytest
+
+from farm_amd import capi
+from tests.helpers import Workload, config1, rings_from_list
+
+pytestmark = pytest.mark.gpu
+
+MODES = [
+    (capi.MODE_EXACT_PLAIN, 0),
+    (capi.MODE_EXACT_BATCHED, 1),
+    (capi.MODE_EXACT_BATCHED, 2),
+    (capi.MODE_EXACT_BATCHED, 4),
+]
+
+
+def _check(dev, w: Workload, modes=MODES):
+    want, st = w.oracle(0)
+    assert not (want == 2).any(), "reference would abort on this input"
+    w.upload(dev)
+    for mode, tile in modes:
+        print(f"  mode={mode} tile={tile} pairs={len(w.pa)}", flush=True)
+        r = dev.intersect(mode=mode, tile_a=tile)
+        diff = np.nonzero(r.bits != want)[0]
+        assert diff.size == 0, f"mode {mode} tile {tile}: {diff.size} pairs differ, first {diff[:10]}"
+        assert r.stats["found"] == int(want.sum())
+    return want, st
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_config1_full(device, seed):
+    print(f"config1 seed={seed}", flush=True)
+    w = config1(seed)
+    want, _ = _check(device, w)
+    assert 0 < want.sum() < len(want)
+
+
+def test_config1_many_seeds(device):
+    # 20 smaller seeded instances, every kernel variant
+    for seed in range(100, 120):
+        print(f"config1-small seed={seed}", flush=True)
+        _check(device, config1(seed, n=400))
+
+
+def test_heuristic_mode_matches_oracle(device):
+    w = config1(5)
+    want, _ = w.oracle(1)
+    w.upload(device)
+    r = device.intersect(mode=capi.MODE_APPROX_HEURISTIC)
+    assert np.array_equal(r.bits, want)
+
+
+def test_batch_windows_concatenate(device):
+    w = config1(9, n=600)
+    want, _ = w.oracle(0)
+    w.upload(device)
+    n = len(w.pa)
+    cuts = [0, 1, 7, n // 3, n // 2 + 5, n]
+    got = np.concatenate([device.intersect(b, e).bits for b, e in zip(cuts[:-1], cuts[1:])])
+    assert np.array_equal(got, want)
+    empty = device.intersect(3, 3)
+    assert empty.bits.size == 0 and empty.stats["found"] == 0
+
+
+def _square(x0, y0, x1, y1):
+    return np.array([[x0, y0], [x1, y0], [x1, y1], [x0, y1]], dtype=np.float32)
+
+
+def test_degenerate_shapes(device):
+    # shared edges, touching corners, collinear overlaps, containment,
+    # identical polygons, grid-aligned coordinates and one-pixel rasters
+    a = [
+        _square(10, 10, 20, 20),
+        _square(30, 30, 31, 31),
+        np.array([[50, 50], [60, 50], [55, 58]], np.float32),
+        _square(70, 70, 90, 90),
+        _square(1, 1, 1.5, 1.5),
+        np.array([[5, 5], [6, 5], [6, 6], [5.5, 5.5], [5, 6]], np.float32),
+    ]
+    b = [
+        _square(20, 12, 25, 18),       # shares the edge x = 20
+        _square(31, 31, 32, 32),       # touches at one corner
+        np.array([[55, 58], [65, 58], [60, 66]], np.float32),  # touches a vertex
+        _square(75, 75, 80, 80),       # strictly inside a[3]
+        _square(1, 1, 1.5, 1.5),       # identical
+        _square(5.25, 5.25, 5.75, 5.75),
+        _square(12, 21, 18, 22),       # near miss
+        _square(20.0000001, 10, 21, 20),
+    ]
+    w = Workload(rings_from_list(a), rings_from_list(b), vpr=1)
+    _check(device, w)
+    w = Workload(rings_from_list(a), rings_from_list(b), vpr=10)
+    _check(device, w)
