@@ -153,6 +153,7 @@ EXPORTS = {
     "farm_hostset_view": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
     "farm_hostset_areas": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
     "farm_hostset_raster_failures": (C.c_int64, [C.c_void_p]),
+    "farm_hostset_raster_ok": (C.c_int, [C.c_void_p, C.c_void_p]),
     "farm_candidates": (
         C.c_int64,
         [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
@@ -238,7 +239,15 @@ class HostSet:
             raise FarmError(rc, "rasterize failed")
 
     def raster_failures(self) -> int:
+        """Index of the first polygon the reference raster builder rejects, or -1."""
         return int(lib.farm_hostset_raster_failures(self._h))
+
+    def raster_ok(self) -> np.ndarray:
+        out = np.zeros(len(self), dtype=np.uint8)
+        rc = lib.farm_hostset_raster_ok(self._h, _ptr(out))
+        if rc:
+            raise FarmError(rc, "polygon set is not rasterized")
+        return out.astype(bool)
 
     def view(self) -> PolySet:
         ps = PolySet()

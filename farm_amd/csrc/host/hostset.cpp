@@ -26,6 +26,7 @@ struct farm_hostset {
     // device layout (after rasterize)
     bool rastered = false;
     int64_t raster_fail = -1;
+    std::vector<uint8_t> raster_ok;  // per polygon
     std::vector<farm_raster_hdr> hdr;
     std::vector<uint8_t> status;
     std::vector<double> areas;  // per pixel, indexed like status
@@ -344,9 +345,13 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
     s->hdr.assign(n, farm_raster_hdr());
     uint64_t np = 0, npo = 0, nsq = 0, nvt = 0, nvl = 0, nvn = 0;
     s->raster_fail = -1;
+    s->raster_ok.assign(n, 1);
     for (int64_t i = 0; i < n; i++) {
         const farm::RasterOut &o = outs[i];
-        if (!o.ok && s->raster_fail < 0) s->raster_fail = i;
+        if (!o.ok) {
+            s->raster_ok[i] = 0;
+            if (s->raster_fail < 0) s->raster_fail = i;
+        }
         farm_raster_hdr &h = s->hdr[i];
         memset(&h, 0, sizeof h);
         h.mbr[0] = o.g.low[0];
@@ -442,6 +447,12 @@ int farm_hostset_areas(const farm_hostset *s, int64_t i, const double **areas) {
 }
 
 int64_t farm_hostset_raster_failures(const farm_hostset *s) { return s->raster_fail; }
+
+int farm_hostset_raster_ok(const farm_hostset *s, uint8_t *out) {
+    if (!s->rastered) return FARM_ESTATE;
+    memcpy(out, s->raster_ok.data(), s->raster_ok.size());
+    return FARM_OK;
+}
 
 void farm_free(void *p) { free(p); }
 

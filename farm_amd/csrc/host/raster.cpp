@@ -164,7 +164,8 @@ inline bool within(double lo, double hi, double v) { return !(v < lo || v > hi);
 
 }  // namespace
 
-void build_raster(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out, RasterScratch *S) {
+static void build_raster_impl(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out,
+                              RasterScratch *S) {
     out.ok = true;
     const int npix_req = nv / vpr;
     out.g = size_grid(mbb, npix_req);
@@ -558,6 +559,24 @@ void build_raster(const float *ring, int nv, const double mbb[4], int vpr, Raste
             row++;
         }
     }
+}
+
+void build_raster(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out, RasterScratch *S) {
+    build_raster_impl(ring, nv, mbb, vpr, out, S);
+    if (out.ok) return;
+    // The reference raster builder asserts or never terminates on this ring.
+    // Keep the grid but make the polygon empty (every pixel OUT, no edge
+    // sequences) so that no query reads a half-built raster; the failure is
+    // reported through farm_hostset_raster_failures.
+    const int npix = out.g.dimx * out.g.dimy;
+    out.status.assign(npix, 0);
+    out.areas.assign(npix, 0.0);
+    out.pixoff.assign((size_t)npix + 1, 0);
+    out.seqs.clear();
+    out.voff.assign((size_t)out.g.dimx + 2, 0);
+    out.vnodes.clear();
+    out.hoff.assign((size_t)out.g.dimy + 2, 0);
+    out.hnodes.clear();
 }
 
 }  // namespace farm

@@ -1,0 +1,144 @@
+// Shared command-line handling of the drop-in executables.
+//
+// Flags follow the reference's get_parameters (src/geometry/query_context.cpp:
+// 227-272): -s/--source, -t/--target, -n/--threads, -v/--vpr,
+// -b/--batch_size, -m/--merge_threshold, --NLow, -u/--unorll_size, and the
+// switches -r/--rasterize, -g/--gpu, -a/--approximation, -h/--hierachy.  The
+// reference's -w and -d never reach its option table (query_context.cpp:
+// 245-247 end the chain with a comma); here they are registered:
+// -w/--category_bits b selects 2^b fill categories for the approximate modes
+// (exact results do not depend on it) and -d/--within_distance sets the
+// within distance in km (default 10, the reference's fixed value).
+#pragma once
+#include <sys/time.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "../../../include/farm_capi.h"
+#include "../../../include/farm_host.h"
+
+struct DriverArgs {
+    std::string source, target;
+    int threads = (int)std::thread::hardware_concurrency();
+    int vpr = 10;
+    unsigned long long batch = 0;
+    float merge_threshold = 0.9f;
+    int nlow = 1;
+    int unroll = 16;
+    int category_bits = 4;
+    int within_km = 10;
+    bool rasterize = false, gpu = false, approx = false, hierarchy = false;
+    int device = 0;
+    int tile = 0;
+    bool plain = false;
+};
+
+inline double now_ms() {
+    struct timeval t;
+    gettimeofday(&t, nullptr);
+    return t.tv_sec * 1000.0 + t.tv_usec / 1000.0;
+}
+
+inline void logf(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vfprintf(stderr, fmt, ap);
+    va_end(ap);
+    fputc('\n', stderr);
+}
+
+inline void usage(const char *prog) {
+    printf("%s usage:\n"
+           "  -s, --source PATH       source polygons (.dt)\n"
+           "  -t, --target PATH       target polygons (.dt)\n"
+           "  -n, --threads N         host threads\n"
+           "  -v, --vpr N             vertices per raster pixel (default 10)\n"
+           "  -b, --batch_size N      candidate pairs per GPU batch (0 = all)\n"
+           "  -m, --merge_threshold X accepted for compatibility (exact results do not depend on it)\n"
+           "  -w, --category_bits B   2^B fill categories for approximate modes (default 4)\n"
+           "  -d, --within_distance D within distance in km (default 10)\n"
+           "  --NLow N                coarsest hierarchy layer size (default 1)\n"
+           "  -u, --unorll_size N     accepted for compatibility\n"
+           "  -r, -g, -a, -h          rasterize / gpu / approximation / hierarchy\n"
+           "  --plain                 one-pair-per-workgroup kernel instead of the batched one\n"
+           "  --tile E                source edges per lane in the batched kernel (1, 2, 4)\n"
+           "  --device N              HIP device\n",
+           prog);
+}
+
+inline bool parse_args(int argc, char **argv, DriverArgs &a) {
+    for (int i = 1; i < argc; i++) {
+        std::string k = argv[i], v;
+        size_t eq = k.find('=');
+        bool has_eq = k.rfind("--", 0) == 0 && eq != std::string::npos;
+        if (has_eq) {
+            v = k.substr(eq + 1);
+            k = k.substr(0, eq);
+        }
+        auto val = [&](void) -> const char * {
+            if (has_eq) return v.c_str();
+            if (i + 1 >= argc) {
+                printf("Undefined Option!\n");
+                exit(0);
+            }
+            return argv[++i];
+        };
+        if (k == "--help") {
+            usage(argv[0]);
+            exit(0);
+        } else if (k == "-r" || k == "--rasterize") a.rasterize = true;
+        else if (k == "-g" || k == "--gpu") a.gpu = true;
+        else if (k == "-a" || k == "--approximation") a.approx = true;
+        else if (k == "-h" || k == "--hierachy") a.hierarchy = true;
+        else if (k == "--plain") a.plain = true;
+        else if (k == "-s" || k == "--source") a.source = val();
+        else if (k == "-t" || k == "--target") a.target = val();
+        else if (k == "-n" || k == "--threads") a.threads = atoi(val());
+        else if (k == "-v" || k == "--vpr") a.vpr = atoi(val());
+        else if (k == "-b" || k == "--batch_size") a.batch = strtoull(val(), nullptr, 10);
+        else if (k == "-m" || k == "--merge_threshold") a.merge_threshold = (float)atof(val());
+        else if (k == "--NLow") a.nlow = atoi(val());
+        else if (k == "-u" || k == "--unorll_size") a.unroll = atoi(val());
+        else if (k == "-w" || k == "--category_bits") a.category_bits = atoi(val());
+        else if (k == "-d" || k == "--within_distance") a.within_km = atoi(val());
+        else if (k == "--device") a.device = atoi(val());
+        else if (k == "--tile") a.tile = atoi(val());
+        else {
+            // boost::program_options rejection of the reference (query_context.cpp:250-255)
+            printf("Undefined Option!\n");
+            exit(0);
+        }
+    }
+    if (a.threads < 1) a.threads = 1;
+    if (a.vpr < 1) {
+        fprintf(stderr, "-v must be positive\n");
+        return false;
+    }
+    return true;
+}
+
+inline farm_hostset *load_or_die(const std::string &path) {
+    double t = now_ms();
+    farm_hostset *s = farm_hostset_load_dt(path.c_str());
+    if (!s) {
+        logf("%s does not exist or is not a .dt file", path.c_str());
+        exit(1);
+    }
+    logf("loaded %lld polygons from %s takes %.3f ms", (long long)farm_hostset_size(s), path.c_str(),
+         now_ms() - t);
+    return s;
+}
+
+#define DIE_IF(h, rc, what)                                                                 \
+    do {                                                                                    \
+        int rc_ = (rc);                                                                     \
+        if (rc_ != FARM_OK) {                                                               \
+            logf("%s failed (%d): %s", what, rc_, (h) ? farm_last_error(h) : "");           \
+            exit(1);                                                                        \
+        }                                                                                   \
+    } while (0)

@@ -54,6 +54,10 @@ int farm_hostset_areas(const farm_hostset *s, int64_t i, const double **areas);
 /* Index of the first polygon the reference raster builder would reject
  * (an assert), or -1. */
 int64_t farm_hostset_raster_failures(const farm_hostset *s);
+/* Per polygon: 1 if rasterized, 0 if the reference raster builder would
+ * abort or never terminate on it (the polygon is then kept as an empty
+ * raster: every pixel OUT, so every query on it answers false). */
+int farm_hostset_raster_ok(const farm_hostset *s, uint8_t *out);
 
 /* Candidate pairs: every (i in a, j in b) whose MBBs overlap inclusively
  * (RTree::Overlap, src/index/RTree.h:1567-1580).  expand_km > 0 expands b's

@@ -36,6 +36,7 @@ def _load() -> C.CDLL:
         ),
         "orc_candidates": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
         "orc_mbb": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+        "orc_failed": (C.c_int, [C.c_void_p, C.c_int64]),
         "orc_segment_intersect": (C.c_int, [C.c_void_p]),
         "orc_double_to_int": (C.c_int, [C.c_double]),
         "orc_point_to_segment_distance": (C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
@@ -73,6 +74,11 @@ class OracleSet:
         if rc == ORC_EINVAL:
             raise ValueError("bad rasterize arguments")
         return bad.value if rc == ORC_EREF_ABORT else -1
+
+    def failed(self) -> np.ndarray:
+        """Per polygon: True where the reference raster builder aborts or
+        never terminates."""
+        return np.array([lib.orc_failed(self._h, i) == 1 for i in range(len(self))], dtype=bool)
 
     def mbb(self, i: int) -> np.ndarray:
         out = np.zeros(4)

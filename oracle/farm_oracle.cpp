@@ -385,6 +385,7 @@ struct Poly {
     std::vector<double> hnodes, vnodes;
     /* hierarchy (-h) */
     bool hier = false;
+    bool failed = false; /* the reference raster builder aborts / never ends */
     std::vector<Layer> layers;
 
     int nv() const { return (int)p.size(); }
@@ -1147,6 +1148,7 @@ int orc_rasterize(orc_set *s, int vpr, int hierarchy, int nlow, int nthreads, in
                 try {
                     rasterize(s->polys[i], vpr, hierarchy != 0, nlow);
                 } catch (const RefAbort &) {
+                    s->polys[i].failed = true;
                     int64_t cur = first_bad.load();
                     while (i < cur && !first_bad.compare_exchange_weak(cur, i)) {
                     }
@@ -1219,6 +1221,11 @@ int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint
             if (s >= npairs) break;
             for (int64_t k = s; k < std::min(npairs, s + 10); k++) {
                 try {
+                    if (a->polys[pa[k]].failed || b->polys[pb[k]].failed) {
+                        out[k] = 2;
+                        err = ORC_EREF_ABORT;
+                        continue;
+                    }
                     out[k] = intersect(a->polys[pa[k]], b->polys[pb[k]], mode, qs) ? 1 : 0;
                 } catch (const RefAbort &) {
                     out[k] = 2;
@@ -1270,3 +1277,8 @@ double orc_point_to_segment_distance(const float *p, const float *p1, const floa
 }
 
 } // extern "C"
+
+extern "C" int orc_failed(const orc_set *s, int64_t i) {
+    if (i < 0 || i >= (int64_t)s->polys.size()) return -1;
+    return s->polys[i].failed ? 1 : 0;
+}

@@ -65,6 +65,9 @@ int64_t orc_candidates(const orc_set *a, const orc_set *b, int64_t cap, uint32_t
 /* MBB of polygon i (low x, low y, high x, high y) as the loader computes it. */
 int orc_mbb(const orc_set *s, int64_t i, double *mbr4);
 
+/* 1 if the reference raster builder aborts (or never ends) on polygon i. */
+int orc_failed(const orc_set *s, int64_t i);
+
 /* Primitive restatements exposed for unit tests. */
 int orc_segment_intersect(const float *abcd /* ax ay bx by cx cy dx dy */);
 int orc_double_to_int(double v);

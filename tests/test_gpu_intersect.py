@@ -16,16 +16,24 @@ MODES = [
 ]
 
 
-def _check(dev, w: Workload, modes=MODES):
+def _check(dev, w: Workload, modes=MODES, allow_ref_failures=False):
     want, st = w.oracle(0)
-    assert not (want == 2).any(), "reference would abort on this input"
+    # polygons on which the reference raster builder aborts or never ends:
+    # the product must flag exactly the same ones (and answer false on them)
+    ok = w.both.raster_ok()
+    ref_failed = np.concatenate([w.OA.failed(), w.OB.failed()])
+    assert np.array_equal(~ok, ref_failed)
+    if not allow_ref_failures:
+        assert not (want == 2).any(), "reference would abort on this input"
+    done = want != 2
     w.upload(dev)
     for mode, tile in modes:
         print(f"  mode={mode} tile={tile} pairs={len(w.pa)}", flush=True)
         r = dev.intersect(mode=mode, tile_a=tile)
-        diff = np.nonzero(r.bits != want)[0]
+        diff = np.nonzero((r.bits != want) & done)[0]
         assert diff.size == 0, f"mode {mode} tile {tile}: {diff.size} pairs differ, first {diff[:10]}"
-        assert r.stats["found"] == int(want.sum())
+        assert not r.bits[~done].any()
+        assert r.stats["found"] == int((want == 1).sum())
     return want, st
 
 
@@ -89,7 +97,8 @@ def test_degenerate_shapes(device):
         _square(12, 21, 18, 22),       # near miss
         _square(20.0000001, 10, 21, 20),
     ]
-    w = Workload(rings_from_list(a), rings_from_list(b), vpr=1)
-    _check(device, w)
-    w = Workload(rings_from_list(a), rings_from_list(b), vpr=10)
-    _check(device, w)
+    for vpr in (1, 2, 10):
+        print(f"degenerate vpr={vpr}", flush=True)
+        w = Workload(rings_from_list(a), rings_from_list(b), vpr=vpr)
+        want, _ = _check(device, w, allow_ref_failures=True)
+        assert (want != 2).sum() >= 5

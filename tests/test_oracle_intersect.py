@@ -1,0 +1,85 @@
+"""Pins the oracle's Ideal::intersect restatement against brute force: an
+all-edges x all-edges binary32 segment test evaluated independently in numpy
+(CPU only)."""
+import numpy as np
+import pytest
+
+import oracle
+from farm_amd import datagen
+from tests.helpers import rings_from_list
+
+
+def _cross(px, py, ax, ay, bx, by):
+    # Point::cross(a, b) in binary32, every operation rounded (numpy has no FMA)
+    return (ax - px) * (by - py) - (ay - py) * (bx - px)
+
+
+def _any_crossing(ra: np.ndarray, rb: np.ndarray) -> bool:
+    """Any (A edge, B edge) pair satisfying segment_intersect, A edges of the
+    decoded ring (stored ring + the repeated first vertex)."""
+    ra = np.vstack([ra, ra[:1]]).astype(np.float32)
+    rb = np.vstack([rb, rb[:1]]).astype(np.float32)
+    a, b = ra[:-1, None, :], ra[1:, None, :]
+    c, d = rb[None, :-1, :], rb[None, 1:, :]
+    o3 = _cross(c[..., 0], c[..., 1], a[..., 0], a[..., 1], d[..., 0], d[..., 1])
+    o4 = _cross(c[..., 0], c[..., 1], b[..., 0], b[..., 1], d[..., 0], d[..., 1])
+    o1 = _cross(a[..., 0], a[..., 1], b[..., 0], b[..., 1], c[..., 0], c[..., 1])
+    o2 = _cross(a[..., 0], a[..., 1], b[..., 0], b[..., 1], d[..., 0], d[..., 1])
+    col = (o3 == 0) & (o4 == 0)
+
+    def ov(p, q, r, s):
+        return np.maximum(np.minimum(p, q), np.minimum(r, s)) <= np.minimum(np.maximum(p, q), np.maximum(r, s))
+
+    hit_col = col & ov(a[..., 0], b[..., 0], c[..., 0], d[..., 0]) & ov(a[..., 1], b[..., 1], c[..., 1], d[..., 1])
+    hit = ~col & (np.sign(o1) != np.sign(o2)) & (np.sign(-o3) != np.sign(-o4))
+    return bool((hit_col | hit).any())
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_crossings_imply_intersect(seed):
+    (sa, sxy), (ta, txy) = datagen.star_pair_set(seed, 500, 500)
+    A, B = oracle.OracleSet(sa, sxy), oracle.OracleSet(ta, txy)
+    A.rasterize(10)
+    B.rasterize(10)
+    pa, pb = oracle.candidates(A, B)
+    got, _ = oracle.intersect(A, B, pa, pb)
+    n_true_no_cross = 0
+    for k in range(len(pa)):
+        ra = sxy[int(sa[pa[k]]):int(sa[pa[k] + 1])]
+        rb = txy[int(ta[pb[k]]):int(ta[pb[k] + 1])]
+        cross = _any_crossing(ra, rb)
+        if cross:
+            # every crossing edge pair lies in some BORDER x BORDER pixel pair
+            # of the windows, so refinement (or an IN pixel) must report it
+            assert got[k] == 1, (k, pa[k], pb[k])
+        elif got[k] == 1:
+            n_true_no_cross += 1  # decided by an IN pixel overlap (containment)
+    # containment without any boundary crossing is rare but exists
+    assert n_true_no_cross < 0.2 * len(pa)
+
+
+def _square(x0, y0, x1, y1):
+    return np.array([[x0, y0], [x1, y0], [x1, y1], [x0, y1]], np.float32)
+
+
+def _stars(cx, cy, r, n, seed):
+    rng = np.random.default_rng(seed)
+    th = np.sort(rng.uniform(0, 2 * np.pi, n))
+    rr = r * (1 + 0.3 * rng.uniform(-1, 1, n))
+    return np.stack([cx + rr * np.cos(th), cy + rr * np.sin(th)], 1).astype(np.float32)
+
+
+def test_containment_and_disjoint_cases():
+    outer = _stars(50, 50, 10, 400, 1)   # big polygon with IN pixels
+    inner = _stars(50, 50, 1, 60, 2)     # inside outer's IN pixels
+    far = _stars(80, 80, 1, 60, 3)       # disjoint
+    ring_around = _stars(50, 50, 12.5, 300, 4)
+    a = rings_from_list([outer, outer, ring_around])
+    b = rings_from_list([inner, far, inner])
+    A, B = oracle.OracleSet(*a), oracle.OracleSet(*b)
+    A.rasterize(10)
+    B.rasterize(10)
+    got, st = oracle.intersect(A, B, np.array([0, 1, 2]), np.array([0, 1, 2]))
+    assert got[0] == 1 and st["in_hits"] >= 1     # containment decided by IN pixels
+    assert got[1] == 0                             # MBB-disjoint pair: empty windows
+    assert got[2] == 1                             # inner sits in ring_around's IN pixels
