@@ -127,6 +127,13 @@ def main() -> int:
     # back afterwards for the parity sample.
     nb = args.steps + args.warmup
     cuts = np.linspace(0, len(pa), nb + 1).astype(np.int64)
+    tile = args.tile
+    if not tile:
+        # autotune the refinement tile width on part of the warmup window
+        t = time.time()
+        tile = dev.autotune(0, int(min(cuts[1], 1_000_000)))
+        log(f"autotuned tile width {tile} in {time.time() - t:.1f}s")
+    args.tile = tile
     for s in range(args.warmup):
         st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False).stats
         log(f"warmup step {s}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms")
@@ -165,7 +172,7 @@ def main() -> int:
             "timed_pairs": timed_pairs,
             "found": int(found),
             "scale": args.scale,
-            "tile_a": args.tile or 2,
+            "tile_a": args.tile,
         },
         "segment_tests_per_s": tests / wall,
         "filter_ms": filter_ms,

@@ -140,6 +140,8 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_int)]),
+    "farm_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "farm_hostset_from_rings": (C.c_void_p, [C.c_int64, C.c_void_p, C.c_void_p]),
     "farm_hostset_load_dt": (C.c_void_p, [C.c_char_p]),
     "farm_hostset_load_wkt": (C.c_void_p, [C.c_char_p]),
@@ -358,6 +360,33 @@ class Device:
         rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr), C.byref(st))
         self._check(rc, "intersect_dev")
         return st.as_dict()
+
+    def autotune(self, begin: int = 0, end: int | None = None, cache: str | None = None, key: str = "") -> int:
+        """Pick the refinement tile width on pairs [begin, end).  With a cache
+        path, a choice stored under `key` (a workload-shape descriptor) is
+        reused instead of re-timing."""
+        import json
+
+        if cache and key and os.path.exists(cache):
+            try:
+                known = json.load(open(cache))
+            except (OSError, ValueError):
+                known = {}
+            if key in known:
+                self._check(lib.farm_set_tile(self._h, int(known[key])), "set_tile")
+                return int(known[key])
+        end = self.npairs if end is None else end
+        best = C.c_int(0)
+        self._check(lib.farm_autotune(self._h, begin, end, C.byref(best)), "autotune")
+        if cache and key:
+            try:
+                known = json.load(open(cache)) if os.path.exists(cache) else {}
+            except (OSError, ValueError):
+                known = {}
+            known[key] = best.value
+            with open(cache, "w") as f:
+                json.dump(known, f, indent=1, sort_keys=True)
+        return best.value
 
     def close(self) -> None:
         if getattr(self, "_h", None):

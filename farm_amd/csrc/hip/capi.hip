@@ -42,6 +42,8 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+constexpr uint64_t kVertexPad = 64;  // >= the refinement chunk of target vertices
+
 }  // namespace
 
 struct farm_handle_s {
@@ -59,6 +61,7 @@ struct farm_handle_s {
     DevBuf pa, pb;
     // per-call scratch
     DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, ctr, hist, cursor, scan_tmp;
+    int tuned_tile = 4;  // source edges per lane of the refinement (farm_autotune)
 };
 
 namespace {
@@ -141,11 +144,70 @@ int exclusive_scan(farm_handle h, const T *in, T *out, uint64_t n) {
     return FARM_OK;
 }
 
+// tiles per ordered pixel pair, their scan, and the persistent refinement
+// with E source edges per lane (events ev[2]..ev[3] bracket the refinement)
+template <int E>
+int launch_tiles_refine(farm_handle h, const DevSet &S, const uint32_t *pa, const uint32_t *pb, uint32_t npp_total,
+                        unsigned long long *ctr) {
+    hipLaunchKernelGGL(k_tiles<E>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
+                       (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order), npp_total,
+                       P<uint32_t>(h->tiles), ctr);
+    HIPCHK(h, hipGetLastError());
+    int rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->tiles), P<uint32_t>(h->tile_off), npp_total + 1);
+    if (rc) return rc;
+    if ((rc = fill_u32(h, h->cursor.p, 1, 0))) return rc;
+    const unsigned grid = (unsigned)(h->num_cus * 8);
+    HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+    hipLaunchKernelGGL(k_refine<E>, dim3(grid), dim3(256), 0, h->stream, S, pa, pb,
+                       (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
+                       (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, P<uint32_t>(h->cursor),
+                       P<uint32_t>(h->result), ctr);
+    HIPCHK(h, hipGetLastError());
+    return FARM_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
+static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
+                         farm_stats *st);
+
 const char *farm_version(void) { return "farm-mi355x 0.1 (gfx950)"; }
+
+int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile) {
+    if (!h) return FARM_EINVAL;
+    // time the refinement of one representative pair window with every tile
+    // width, keep the fastest for later calls with opts->tile_a == 0
+    const int cand[] = {1, 2, 4, 8};
+    double best_ms = 1e300;
+    int best = h->tuned_tile;
+    for (int E : cand) {
+        farm_opts o;
+        memset(&o, 0, sizeof o);
+        o.mode = FARM_MODE_EXACT_BATCHED;
+        o.tile_a = E;
+        farm_stats st;
+        int rc = run_intersect(h, begin, end, &o, nullptr, &st);
+        if (rc) return rc;
+        rc = run_intersect(h, begin, end, &o, nullptr, &st);  // second run: warm caches
+        if (rc) return rc;
+        if (st.total_ms < best_ms) {
+            best_ms = st.total_ms;
+            best = E;
+        }
+    }
+    h->tuned_tile = best;
+    if (best_tile) *best_tile = best;
+    return FARM_OK;
+}
+
+int farm_set_tile(farm_handle h, int tile) {
+    if (!h) return FARM_EINVAL;
+    if (tile != 1 && tile != 2 && tile != 4 && tile != 8) return fail(h, FARM_EINVAL, "tile must be 1, 2, 4 or 8");
+    h->tuned_tile = tile;
+    return FARM_OK;
+}
 
 int farm_open(int device, farm_handle *out) {
     if (!out) return FARM_EINVAL;
@@ -225,6 +287,12 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     if ((rc = upload(h, h->status, ps->status, ps->nstatus))) return rc;
     if ((rc = upload(h, h->pixoff, ps->pixoff, 4 * ps->npixoff))) return rc;
     if ((rc = upload(h, h->seqs, ps->seqs, 8 * ps->nseqs))) return rc;
+    // vertex arrays carry kVertexPad zeroed floats past the end: the
+    // refinement kernel reads target vertices in fixed-size chunks
+    for (DevBuf *b : {&h->x, &h->y}) {
+        if ((rc = ensure(h, *b, 4 * (ps->nverts + kVertexPad)))) return rc;
+        HIPCHK(h, hipMemsetAsync((char *)b->p + 4 * ps->nverts, 0, 4 * kVertexPad, h->stream));
+    }
     if ((rc = upload(h, h->x, ps->x, 4 * ps->nverts))) return rc;
     if ((rc = upload(h, h->y, ps->y, 4 * ps->nverts))) return rc;
     if ((rc = upload(h, h->voff, ps->voff, 4 * ps->nvoff))) return rc;
@@ -267,8 +335,8 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
     if (n64 > 0xffffffffull) return fail(h, FARM_ECAPACITY, "batch larger than 2^32 pairs");
     const uint32_t n = (uint32_t)n64;
     const int mode = opts ? opts->mode : FARM_MODE_EXACT_BATCHED;
-    int E = opts && opts->tile_a ? opts->tile_a : 2;
-    if (E != 1 && E != 2 && E != 4) return fail(h, FARM_EINVAL, "tile_a must be 1, 2 or 4");
+    int E = opts && opts->tile_a ? opts->tile_a : h->tuned_tile;
+    if (E != 1 && E != 2 && E != 4 && E != 8) return fail(h, FARM_EINVAL, "tile_a must be 1, 2, 4 or 8");
     HIPCHK(h, hipSetDevice(h->device));
     int rc;
     const uint32_t *pa = P<uint32_t>(h->pa) + begin;
@@ -329,34 +397,13 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
                                P<uint32_t>(h->order));
             HIPCHK(h, hipGetLastError());
             if ((rc = fill_u32(h, P<uint32_t>(h->tiles) + npp_total, 1, 0))) return rc;
-            if (E == 1)
-                hipLaunchKernelGGL(k_tiles<1>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
-                                   (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
-                                   npp_total, P<uint32_t>(h->tiles), ctr);
-            else if (E == 2)
-                hipLaunchKernelGGL(k_tiles<2>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
-                                   (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
-                                   npp_total, P<uint32_t>(h->tiles), ctr);
-            else
-                hipLaunchKernelGGL(k_tiles<4>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
-                                   (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
-                                   npp_total, P<uint32_t>(h->tiles), ctr);
-            HIPCHK(h, hipGetLastError());
-            if ((rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->tiles), P<uint32_t>(h->tile_off), npp_total + 1)))
-                return rc;
-            if ((rc = fill_u32(h, h->cursor.p, 1, 0))) return rc;
-            const unsigned grid = (unsigned)(h->num_cus * 8);
-            HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-#define LAUNCH_REFINE(EE)                                                                                    \
-    hipLaunchKernelGGL(k_refine<EE>, dim3(grid), dim3(256), 0, h->stream, S, pa, pb,                        \
-                       (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),       \
-                       (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, P<uint32_t>(h->cursor),      \
-                       P<uint32_t>(h->result), ctr)
-            if (E == 1) LAUNCH_REFINE(1);
-            else if (E == 2) LAUNCH_REFINE(2);
-            else LAUNCH_REFINE(4);
-#undef LAUNCH_REFINE
-            HIPCHK(h, hipGetLastError());
+            switch (E) {
+                case 1: rc = launch_tiles_refine<1>(h, S, pa, pb, npp_total, ctr); break;
+                case 2: rc = launch_tiles_refine<2>(h, S, pa, pb, npp_total, ctr); break;
+                case 4: rc = launch_tiles_refine<4>(h, S, pa, pb, npp_total, ctr); break;
+                default: rc = launch_tiles_refine<8>(h, S, pa, pb, npp_total, ctr); break;
+            }
+            if (rc) return rc;
         } else {
             HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
         }

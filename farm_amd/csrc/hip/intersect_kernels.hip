@@ -152,13 +152,25 @@ __global__ void k_bucket_scan(uint32_t *__restrict__ hist, uint32_t *__restrict_
     }
 }
 
+// positions are reserved per block and bucket (one global atomic per
+// non-empty bucket per block), then handed out from LDS
 __global__ __launch_bounds__(256) void k_bucket_scatter(const uint8_t *__restrict__ bucket, uint32_t n,
                                                          uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+    __shared__ uint32_t cnt[kBuckets], base[kBuckets];
+    for (int j = threadIdx.x; j < kBuckets; j += blockDim.x) cnt[j] = 0;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = 0;
+    uint32_t local = 0;
     if (i < n) {
-        const uint32_t pos = atomicAdd(&cursor[bucket[i]], 1u);
-        order[pos] = i;
+        b = bucket[i];
+        local = atomicAdd(&cnt[b], 1u);
     }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kBuckets; j += blockDim.x)
+        if (cnt[j]) base[j] = atomicAdd(&cursor[j], cnt[j]);
+    __syncthreads();
+    if (i < n) order[base[b] + local] = i;
 }
 
 // tiles per ordered pixel pair (written at position i, scanned afterwards)
@@ -190,6 +202,8 @@ __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, 
 // and a superset filter (o1*o2 <= 0 or o3 == 0) and o3*o4 <= 0 sends the
 // rare candidates to the exact reference predicate.
 // ------------------------------------------------------------------------
+constexpr uint32_t kChunk = 16;  // target vertices per scalar-load chunk
+
 template <int E>
 __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                  const uint32_t *__restrict__ pb_ids,
@@ -263,55 +277,89 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
             nvalid += ev[e] ? 1 : 0;
         }
         bool hit = false;
-        uint32_t steps = 0;
         bool stop = false;
+        uint32_t chunks = 0;
         for (uint32_t j = b_s0; j < b_s1 && !stop; j++) {
             const uint2 sq = sqb[j];
             const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
             const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
             float cx = bx_[v0], cy = by_[v0];
-            float Op[E];
+            // tx/ty: source vertex minus the current target vertex.  By the
+            // antisymmetry of rounded subtraction, (q - a) == -(a - q), so one
+            // difference serves O(q) at this step and o(p) at the next one.
+            float tx[E + 1], ty[E + 1], Op[E];
+#pragma unroll
+            for (int v = 0; v <= E; v++) {
+                tx[v] = px[v] - cx;
+                ty[v] = py[v] - cy;
+            }
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                float l = bax[e] * (cy - py[e]);
-                float r = bay[e] * (cx - px[e]);
-                Op[e] = l - r;
+                float l = bay[e] * tx[e];
+                float r = bax[e] * ty[e];
+                Op[e] = l - r;  // a.cross(b, c), bit-identical to the reference form
             }
-            for (uint32_t m = 1; m <= L; m++) {
-                const float dx = bx_[v0 + m], dy = by_[v0 + m];
-                const float ddx = dx - cx, ddy = dy - cy;
-                float o[E + 1];
+            for (uint32_t base = 1; base <= L && !stop; base += kChunk) {
+                const uint32_t nstep = min(kChunk, L - base + 1);
+                // the target vertices of this chunk, uniform: scalar loads
+                // (the vertex arrays carry padding, so the chunk may run past L)
+                float qx[kChunk], qy[kChunk];
 #pragma unroll
-                for (int e = 0; e <= E; e++) {
-                    float l = (px[e] - cx) * ddy;
-                    float r = (py[e] - cy) * ddx;
-                    o[e] = l - r;
+                for (int m = 0; m < (int)kChunk; m++) {
+                    qx[m] = bx_[v0 + base + m];
+                    qy[m] = by_[v0 + base + m];
                 }
+                const float cx0 = cx, cy0 = cy;
+                bool maybe = false;
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    float l = bax[e] * (dy - py[e]);
-                    float r = bay[e] * (dx - px[e]);
-                    const float On = l - r;
-                    const bool maybe = ev[e] && (o[e] * o[e + 1] <= 0.f) && ((Op[e] * On <= 0.f) || (o[e] == 0.f));
+                for (int m = 0; m < (int)kChunk; m++) {
+                    if ((uint32_t)m < nstep) {
+                        const float ddx = qx[m] - cx, ddy = qy[m] - cy;
+                        float o[E + 1];
+#pragma unroll
+                        for (int v = 0; v <= E; v++) {
+                            float l = tx[v] * ddy;
+                            float r = ty[v] * ddx;
+                            o[v] = l - r;  // c.cross(p_v, d)
+                            tx[v] = px[v] - qx[m];
+                            ty[v] = py[v] - qy[m];
+                        }
+#pragma unroll
+                        for (int e = 0; e < E; e++) {
+                            float l = bay[e] * tx[e];
+                            float r = bax[e] * ty[e];
+                            const float On = l - r;  // a.cross(b, d)
+                            // superset of segment_intersect: both pairs of
+                            // orientations have opposite (or zero) signs, or
+                            // the collinear branch may apply (o3 == 0)
+                            const float g = fminf(fmaxf(o[e] * o[e + 1], Op[e] * On), fabsf(o[e]));
+                            maybe |= ev[e] && g <= 0.f;
+                            Op[e] = On;
+                        }
+                        cx = qx[m];
+                        cy = qy[m];
+                    }
+                }
+                if (__any(maybe)) {
+                    // rare: re-evaluate the chunk with the reference predicate
                     if (maybe) {
-                        if (seg_hit(Op[e], On, o[e], o[e + 1], px[e], py[e], px[e + 1], py[e + 1], cx, cy, dx, dy))
-                            hit = true;
+                        float ccx = cx0, ccy = cy0;
+                        for (uint32_t m = 0; m < nstep; m++) {
+                            const float dx = bx_[v0 + base + m], dy = by_[v0 + base + m];
+#pragma unroll
+                            for (int e = 0; e < E; e++)
+                                if (ev[e] && seg_intersect(px[e], py[e], px[e + 1], py[e + 1], ccx, ccy, dx, dy))
+                                    hit = true;
+                            ccx = dx;
+                            ccy = dy;
+                        }
                     }
-                    Op[e] = On;
+                    if (__any(hit)) stop = true;
                 }
-                cx = dx;
-                cy = dy;
-                tests += nvalid;
-                if ((++steps & 31u) == 0) {
-                    if (__any(hit)) {
-                        stop = true;
-                        break;
-                    }
-                    if (__hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        stop = true;
-                        break;
-                    }
-                }
+                tests += (unsigned long long)nvalid * nstep;
+                if (!stop && (++chunks & 1u) == 0 &&
+                    __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    stop = true;
             }
         }
         if (__any(hit)) {
@@ -491,5 +539,8 @@ template __global__ void k_refine<2>(DevSet, const uint32_t *, const uint32_t *,
                                      const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
 template __global__ void k_refine<4>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
                                      const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+template __global__ void k_refine<8>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
+                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+template __global__ void k_tiles<8>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 
 }  // namespace farm

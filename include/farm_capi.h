@@ -151,6 +151,15 @@ int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts 
 int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
                        farm_stats *stats);
 
+/* Autotuner of the batched refinement: times pairs [begin, end) with every
+ * supported tile width (source edges per lane: 1, 2, 4, 8) and keeps the
+ * fastest for later calls whose opts->tile_a is 0.  No reference
+ * counterpart (the reference hard-codes 16x16 tiles,
+ * src/cuda/query_intersect.cu:194). */
+int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile);
+/* Set the tile width used when opts->tile_a is 0 (e.g. a persisted choice). */
+int farm_set_tile(farm_handle h, int tile);
+
 /* Library version string. */
 const char *farm_version(void);
 

@@ -13,6 +13,7 @@ MODES = [
     (capi.MODE_EXACT_BATCHED, 1),
     (capi.MODE_EXACT_BATCHED, 2),
     (capi.MODE_EXACT_BATCHED, 4),
+    (capi.MODE_EXACT_BATCHED, 8),
 ]
 
 
@@ -57,6 +58,18 @@ def test_heuristic_mode_matches_oracle(device):
     want, _ = w.oracle(1)
     w.upload(device)
     r = device.intersect(mode=capi.MODE_APPROX_HEURISTIC)
+    assert np.array_equal(r.bits, want)
+
+
+def test_autotuned_tile_is_exact(device, tmp_path):
+    w = config1(8, n=800)
+    want, _ = w.oracle(0)
+    w.upload(device)
+    cache = str(tmp_path / "tune.json")
+    tile = device.autotune(cache=cache, key="config1-800")
+    assert tile in (1, 2, 4, 8)
+    assert device.autotune(cache=cache, key="config1-800") == tile  # served from the cache
+    r = device.intersect()  # tile_a = 0: the tuned width
     assert np.array_equal(r.bits, want)
 
 
