@@ -103,9 +103,14 @@ class Opts(C.Structure):
         ("tile_b", C.c_int32),
         ("order", C.c_int32),
         ("approx_threshold", C.c_float),
+        ("approx_reject", C.c_float),
         ("merge_threshold", C.c_float),
-        ("reserved", C.c_int32 * 2),
+        ("reserved", C.c_int32),
     ]
+
+
+NFEAT = 80
+MLP_NPARAMS = 27138
 
 
 class Stats(C.Structure):
@@ -140,6 +145,24 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_intersect_costs": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
+    ),
+    "farm_pair_features": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "farm_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "farm_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "farm_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "farm_mlp_loss_grad": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_float, C.c_void_p, C.c_void_p],
+    ),
+    "farm_mlp_train": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float,
+         C.c_uint64, C.c_void_p],
+    ),
+    "farm_mlp_predict_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]),
     "farm_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_int)]),
     "farm_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
     "farm_hostset_from_rings": (C.c_void_p, [C.c_int64, C.c_void_p, C.c_void_p]),
@@ -343,9 +366,14 @@ class Device:
         self.npairs = len(a)
 
     def intersect(self, begin: int = 0, end: int | None = None, mode: int = MODE_EXACT_BATCHED,
-                  tile_a: int = 0, want_bits: bool = True) -> IntersectResult:
+                  tile_a: int = 0, want_bits: bool = True, order: int = 0, accept: float = 0.0,
+                  reject: float | None = None) -> IntersectResult:
+        """Exact or approximate intersect of pairs [begin, end).  order=1 orders
+        the refinement queue by the MLP cost head; mode MODE_APPROX_LEARNED
+        accepts logits > accept, rejects logits < reject and refines the rest."""
         end = self.npairs if end is None else end
-        o = Opts(mode=mode, tile_a=tile_a)
+        o = Opts(mode=mode, tile_a=tile_a, order=order, approx_threshold=accept,
+                 approx_reject=accept if reject is None else reject)
         st = Stats()
         out = np.zeros(max(end - begin, 1), dtype=np.uint8) if want_bits else None
         rc = lib.farm_intersect(self._h, begin, end, C.byref(o), _ptr(out) if want_bits else None, C.byref(st))
@@ -360,6 +388,67 @@ class Device:
         rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr), C.byref(st))
         self._check(rc, "intersect_dev")
         return st.as_dict()
+
+    def intersect_costs(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray, dict]:
+        """Exact bits plus the segment tests executed per pair."""
+        end = self.npairs if end is None else end
+        n = end - begin
+        bits = np.zeros(max(n, 1), np.uint8)
+        tests = np.zeros(max(n, 1), np.uint32)
+        st = Stats()
+        self._check(lib.farm_intersect_costs(self._h, begin, end, _ptr(bits), _ptr(tests), C.byref(st)),
+                    "intersect_costs")
+        return bits[:n], tests[:n], st.as_dict()
+
+    def pair_features(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        end = self.npairs if end is None else end
+        n = end - begin
+        feat = np.zeros((max(n, 1), NFEAT), np.float32)
+        dec = np.zeros(max(n, 1), np.int8)
+        self._check(lib.farm_pair_features(self._h, begin, end, _ptr(feat), _ptr(dec)), "pair_features")
+        return feat[:n], dec[:n]
+
+    def mlp_set_params(self, params: np.ndarray, norm: np.ndarray | None = None) -> None:
+        params = np.ascontiguousarray(params, np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._check(lib.farm_mlp_set_params(self._h, _ptr(params), len(params), None if nm is None else _ptr(nm)),
+                    "mlp_set_params")
+
+    def mlp_get_params(self) -> np.ndarray:
+        out = np.zeros(MLP_NPARAMS, np.float32)
+        self._check(lib.farm_mlp_get_params(self._h, _ptr(out), MLP_NPARAMS), "mlp_get_params")
+        return out
+
+    def mlp_forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros((len(X), 2), np.float32)
+        self._check(lib.farm_mlp_forward(self._h, _ptr(X), len(X), _ptr(out)), "mlp_forward")
+        return out
+
+    def mlp_loss_grad(self, X, y, c, lam: float) -> tuple[float, np.ndarray]:
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        c = np.ascontiguousarray(c, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        self._check(lib.farm_mlp_loss_grad(self._h, _ptr(X), _ptr(y), _ptr(c), len(X), lam, _ptr(loss), _ptr(g)),
+                    "mlp_loss_grad")
+        return float(loss[0]), g
+
+    def mlp_train(self, X, y, c, epochs: int, batch: int, lr: float, lam: float, seed: int = 0) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        c = np.ascontiguousarray(c, np.float32)
+        losses = np.zeros(epochs, np.float32)
+        self._check(lib.farm_mlp_train(self._h, _ptr(X), _ptr(y), _ptr(c), len(X), epochs, batch, lr, lam, seed,
+                                       _ptr(losses)), "mlp_train")
+        return losses
+
+    def mlp_predict_pairs(self, begin: int = 0, end: int | None = None) -> np.ndarray:
+        end = self.npairs if end is None else end
+        out = np.zeros((max(end - begin, 1), 2), np.float32)
+        self._check(lib.farm_mlp_predict_pairs(self._h, begin, end, _ptr(out)), "mlp_predict_pairs")
+        return out[: end - begin]
 
     def autotune(self, begin: int = 0, end: int | None = None, cache: str | None = None, key: str = "") -> int:
         """Pick the refinement tile width on pairs [begin, end).  With a cache

@@ -9,11 +9,11 @@
 #include <string>
 #include <vector>
 
-#include "device_layout.h"
+#include "handle.h"
 
 namespace farm {
-__global__ void k_filter(DevSet, const uint32_t *, const uint32_t *, uint32_t, int, int, uint32_t *, const uint32_t *,
-                         PixPair *, uint32_t *, unsigned long long *);
+__global__ void k_filter(DevSet, const uint32_t *, const uint32_t *, uint32_t, int, int, const uint8_t *, uint32_t *,
+                         const uint32_t *, PixPair *, uint32_t *, unsigned long long *);
 __global__ void k_bucket_hist(DevSet, const uint32_t *, const uint32_t *, const PixPair *, uint32_t, const float *,
                               uint8_t *, uint32_t *);
 __global__ void k_bucket_scan(uint32_t *, uint32_t *);
@@ -22,117 +22,46 @@ template <int E>
 __global__ void k_tiles(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 template <int E>
 __global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                         const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+                         const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *, uint32_t *);
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
 __global__ void k_finalize(const uint32_t *, uint32_t, uint8_t *, unsigned long long *);
 }  // namespace farm
 
+extern "C" int farm_mlp_predict_pairs(farm_handle h, uint64_t begin, uint64_t end, float *out);
+
 using namespace farm;
 
-namespace {
-
+namespace farm {
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
-
-constexpr uint64_t kVertexPad = 64;  // >= the refinement chunk of target vertices
-
-}  // namespace
-
-struct farm_handle_s {
-    int device = 0;
-    int num_cus = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
-    std::string err;
-    // polygon set
-    int64_t npolys = 0;
-    DevBuf hdr, status, pixoff, seqs, x, y, voff, vnodes, pixedges;
-    uint64_t nstatus = 0;
-    // pairs
-    uint64_t npairs = 0;
-    DevBuf pa, pb;
-    // per-call scratch
-    DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, ctr, hist, cursor, scan_tmp;
-    int tuned_tile = 4;  // source edges per lane of the refinement (farm_autotune)
-};
+// learned -a decision: an IN overlap decides true, no BORDER pair decides
+// false, otherwise the intersect logit: > accept -> true, < reject -> false,
+// in between -> exact refinement (skip = 0)
+__global__ __launch_bounds__(256) void k_learned_decide(const int8_t *__restrict__ decided,
+                                                         const float *__restrict__ pred, uint32_t n, float accept,
+                                                         float reject, uint32_t *__restrict__ result,
+                                                         uint8_t *__restrict__ skip) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = decided[i];
+    const float z = pred[2 * (size_t)i];
+    uint32_t r = 0;
+    uint8_t s = 1;
+    if (d == 1) r = 1;
+    else if (d == 0) r = 0;
+    else if (z > accept) r = 1;
+    else if (z < reject) r = 0;
+    else s = 0;
+    result[i] = r;
+    skip[i] = s;
+}
+}  // namespace farm
 
 namespace {
-
-int fail(farm_handle h, int code, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    if (h) h->err = buf;
-    return code;
-}
-
-#define HIPCHK(h, call)                                                                          \
-    do {                                                                                         \
-        hipError_t e_ = (call);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail((h), FARM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                     \
-    } while (0)
-
-int ensure(farm_handle h, DevBuf &b, size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    if (b.cap >= bytes) return FARM_OK;
-    if (b.p) {
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        HIPCHK(h, hipFree(b.p));
-        b.p = nullptr;
-        b.cap = 0;
-    }
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) return fail(h, FARM_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    b.cap = bytes;
-    return FARM_OK;
-}
-
-int upload(farm_handle h, DevBuf &b, const void *src, size_t bytes) {
-    int rc = ensure(h, b, bytes);
-    if (rc) return rc;
-    if (bytes) HIPCHK(h, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream));
-    return FARM_OK;
-}
-
-template <class T>
-T *P(DevBuf &b) {
-    return reinterpret_cast<T *>(b.p);
-}
-
-DevSet devset(farm_handle h) {
-    DevSet S;
-    S.hdr = P<farm_raster_hdr>(h->hdr);
-    S.status = P<uint8_t>(h->status);
-    S.pixoff = P<uint32_t>(h->pixoff);
-    S.seqs = P<uint2>(h->seqs);
-    S.x = P<float>(h->x);
-    S.y = P<float>(h->y);
-    S.pixedges = P<uint32_t>(h->pixedges);
-    S.voff = P<uint32_t>(h->voff);
-    S.vnodes = P<double>(h->vnodes);
-    return S;
-}
-
-unsigned blocks_for(uint64_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
-
-int fill_u32(farm_handle h, void *p, uint64_t n, uint32_t v) {
-    if (!n) return FARM_OK;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(256), 0, h->stream, (uint32_t *)p, n, v);
-    HIPCHK(h, hipGetLastError());
-    return FARM_OK;
-}
 
 template <class T>
 int exclusive_scan(farm_handle h, const T *in, T *out, uint64_t n) {
@@ -148,7 +77,7 @@ int exclusive_scan(farm_handle h, const T *in, T *out, uint64_t n) {
 // with E source edges per lane (events ev[2]..ev[3] bracket the refinement)
 template <int E>
 int launch_tiles_refine(farm_handle h, const DevSet &S, const uint32_t *pa, const uint32_t *pb, uint32_t npp_total,
-                        unsigned long long *ctr) {
+                        unsigned long long *ctr, uint32_t *pair_tests) {
     hipLaunchKernelGGL(k_tiles<E>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
                        (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order), npp_total,
                        P<uint32_t>(h->tiles), ctr);
@@ -161,7 +90,7 @@ int launch_tiles_refine(farm_handle h, const DevSet &S, const uint32_t *pa, cons
     hipLaunchKernelGGL(k_refine<E>, dim3(grid), dim3(256), 0, h->stream, S, pa, pb,
                        (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
                        (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, P<uint32_t>(h->cursor),
-                       P<uint32_t>(h->result), ctr);
+                       P<uint32_t>(h->result), ctr, pair_tests);
     HIPCHK(h, hipGetLastError());
     return FARM_OK;
 }
@@ -171,7 +100,7 @@ int launch_tiles_refine(farm_handle h, const DevSet &S, const uint32_t *pa, cons
 extern "C" {
 
 static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
-                         farm_stats *st);
+                         farm_stats *st, uint32_t *pair_tests = nullptr);
 
 const char *farm_version(void) { return "farm-mi355x 0.1 (gfx950)"; }
 
@@ -249,7 +178,10 @@ int farm_close(farm_handle h) {
     DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->voff,
                       &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->result, &h->npp,      &h->pp_off,
                       &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->ctr,    &h->hist,
-                      &h->cursor, &h->scan_tmp};
+                      &h->cursor, &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
+                      &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
+                      &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
+                      &h->skipbuf};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : h->ev)
@@ -325,7 +257,7 @@ int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint6
 }
 
 static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
-                         farm_stats *st) {
+                         farm_stats *st, uint32_t *pair_tests) {
     if (!h) return FARM_EINVAL;
     if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
     if (begin > end || end > h->npairs) return fail(h, FARM_EINVAL, "pair window [%llu,%llu) outside [0,%llu)",
@@ -356,12 +288,33 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
         hipLaunchKernelGGL(k_plain, dim3(n), dim3(256), 0, h->stream, S, pa, pb, n, P<uint32_t>(h->result), ctr);
         HIPCHK(h, hipGetLastError());
         HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-    } else if (mode == FARM_MODE_EXACT_BATCHED || mode == FARM_MODE_APPROX_HEURISTIC) {
+    } else if (mode == FARM_MODE_EXACT_BATCHED || mode == FARM_MODE_APPROX_HEURISTIC ||
+               mode == FARM_MODE_APPROX_LEARNED) {
         const int approx = mode == FARM_MODE_APPROX_HEURISTIC;
+        const bool learned = mode == FARM_MODE_APPROX_LEARNED;
+        const bool learned_order = opts && opts->order == 1 && !approx;
+        const uint8_t *skip = nullptr;
+        const float *pred_cost = nullptr;
+        if (learned || learned_order) {
+            // pair features + MLP on the device (counted in the filter phase)
+            if ((rc = farm_mlp_predict_pairs(h, begin, end, nullptr))) return rc;
+            pred_cost = P<float>(h->pred) + 1;
+        }
+        if (learned) {
+            if ((rc = ensure(h, h->skipbuf, n))) return rc;
+            const float accept = opts->approx_threshold;
+            const float reject = opts->approx_reject < accept ? opts->approx_reject : accept;
+            hipLaunchKernelGGL(k_learned_decide, dim3(blocks_for(n)), dim3(256), 0, h->stream,
+                               (const int8_t *)P<int8_t>(h->decided), (const float *)P<float>(h->pred), n, accept,
+                               reject, P<uint32_t>(h->result), P<uint8_t>(h->skipbuf));
+            HIPCHK(h, hipGetLastError());
+            skip = P<uint8_t>(h->skipbuf);
+        }
         if ((rc = ensure(h, h->npp, 4ull * (n + 1)))) return rc;
         if ((rc = ensure(h, h->pp_off, 4ull * (n + 1)))) return rc;
         if ((rc = fill_u32(h, h->npp.p, n + 1, 0))) return rc;
-        hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 0, approx,
+        if (pair_tests && (rc = fill_u32(h, pair_tests, n, 0))) return rc;
+        hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 0, approx, skip,
                            P<uint32_t>(h->npp), (const uint32_t *)nullptr, (PixPair *)nullptr,
                            P<uint32_t>(h->result), ctr);
         HIPCHK(h, hipGetLastError());
@@ -380,14 +333,15 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
             if ((rc = ensure(h, h->tile_off, 4ull * (npp_total + 1)))) return rc;
             if ((rc = ensure(h, h->hist, 4 * 64))) return rc;
             if ((rc = ensure(h, h->cursor, 4 * 64))) return rc;
-            hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 1, 0,
+            hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 1, 0, skip,
                                P<uint32_t>(h->npp), (const uint32_t *)P<uint32_t>(h->pp_off), P<PixPair>(h->pps),
                                P<uint32_t>(h->result), ctr);
             HIPCHK(h, hipGetLastError());
             if ((rc = fill_u32(h, h->hist.p, 64, 0))) return rc;
             hipLaunchKernelGGL(k_bucket_hist, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream, S, pa, pb,
-                               (const PixPair *)P<PixPair>(h->pps), npp_total, (const float *)nullptr,
-                               P<uint8_t>(h->bucket), P<uint32_t>(h->hist));
+                               (const PixPair *)P<PixPair>(h->pps), npp_total,
+                               learned_order ? pred_cost : (const float *)nullptr, P<uint8_t>(h->bucket),
+                               P<uint32_t>(h->hist));
             HIPCHK(h, hipGetLastError());
             hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, h->stream, P<uint32_t>(h->hist),
                                P<uint32_t>(h->cursor));
@@ -398,10 +352,10 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
             HIPCHK(h, hipGetLastError());
             if ((rc = fill_u32(h, P<uint32_t>(h->tiles) + npp_total, 1, 0))) return rc;
             switch (E) {
-                case 1: rc = launch_tiles_refine<1>(h, S, pa, pb, npp_total, ctr); break;
-                case 2: rc = launch_tiles_refine<2>(h, S, pa, pb, npp_total, ctr); break;
-                case 4: rc = launch_tiles_refine<4>(h, S, pa, pb, npp_total, ctr); break;
-                default: rc = launch_tiles_refine<8>(h, S, pa, pb, npp_total, ctr); break;
+                case 1: rc = launch_tiles_refine<1>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
+                case 2: rc = launch_tiles_refine<2>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
+                case 4: rc = launch_tiles_refine<4>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
+                default: rc = launch_tiles_refine<8>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
             }
             if (rc) return rc;
         } else {
@@ -461,6 +415,27 @@ int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts 
     rc = run_intersect(h, begin, end, opts, (uint8_t *)tmp.p, st);
     if (rc == FARM_OK && n) {
         hipError_t e = hipMemcpy(out, tmp.p, n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(h, FARM_EDEVICE, "copy of results failed: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(tmp.p);
+    return rc;
+}
+
+int farm_intersect_costs(farm_handle h, uint64_t begin, uint64_t end, uint8_t *bits, uint32_t *tests,
+                         farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    const uint64_t n = end > begin ? end - begin : 0;
+    int rc = ensure(h, h->pair_tests, 4 * (n ? n : 1));
+    if (rc) return rc;
+    DevBuf tmp;
+    if ((rc = ensure(h, tmp, n ? n : 1))) return rc;
+    farm_opts o;
+    memset(&o, 0, sizeof o);
+    o.mode = FARM_MODE_EXACT_BATCHED;
+    rc = run_intersect(h, begin, end, &o, (uint8_t *)tmp.p, st, P<uint32_t>(h->pair_tests));
+    if (rc == FARM_OK && n) {
+        hipError_t e = hipMemcpy(bits, tmp.p, n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(tests, h->pair_tests.p, 4 * n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(h, FARM_EDEVICE, "copy of results failed: %s", hipGetErrorString(e));
     }
     (void)hipFree(tmp.p);

@@ -1,0 +1,137 @@
+// Internal state of a libfarm handle (one device, one stream) and the host
+// helpers shared by the C ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "device_layout.h"
+
+namespace farm {
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+constexpr uint64_t kVertexPad = 64;  // >= the refinement chunk of target vertices
+
+// MLP 80 -> 128 -> 128 -> 2 (intersect logit, log refinement cost)
+constexpr int kNFeat = 80;
+constexpr int kHidden = 128;
+constexpr int kOut = 2;
+constexpr int kW1 = kNFeat * kHidden, kB1 = kHidden, kW2 = kHidden * kHidden, kB2 = kHidden, kW3 = kHidden * kOut,
+              kB3 = kOut;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kW1, kOffW2 = kOffB1 + kB1, kOffB2 = kOffW2 + kW2, kOffW3 = kOffB2 + kB2,
+              kOffB3 = kOffW3 + kW3, kNParams = kOffB3 + kB3;
+
+}  // namespace farm
+
+struct farm_handle_s {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::string err;
+    // polygon set
+    int64_t npolys = 0;
+    farm::DevBuf hdr, status, pixoff, seqs, x, y, voff, vnodes, pixedges;
+    uint64_t nstatus = 0;
+    // pairs
+    uint64_t npairs = 0;
+    farm::DevBuf pa, pb;
+    // per-call scratch of the intersect driver
+    farm::DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, ctr, hist, cursor, scan_tmp;
+    farm::DevBuf pair_tests;  // per-pair executed segment tests (cost labels)
+    int tuned_tile = 4;       // source edges per lane of the refinement (farm_autotune)
+    // learned part
+    bool mlp_ready = false;
+    farm::DevBuf mlp_w, mlp_m, mlp_v, mlp_g;  // f32 master params, Adam moments, grads
+    farm::DevBuf mlp_norm;                    // per feature: mean, 1/std (2 * kNFeat)
+    farm::DevBuf mlp_x, mlp_h1, mlp_h2, mlp_z3, mlp_d3, mlp_dh2, mlp_dh1, mlp_y, mlp_c, mlp_loss;
+    farm::DevBuf feat, decided, pred;  // per-pair features / decided flags / predictions
+    farm::DevBuf skipbuf;              // pairs decided before the exact filter
+    uint64_t adam_t = 0;
+};
+
+namespace farm {
+
+inline int fail(farm_handle h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                                    \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return farm::fail((h), FARM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                              __FILE__, __LINE__);                                                         \
+    } while (0)
+
+inline int ensure(farm_handle h, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return FARM_OK;
+    if (b.p) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(h, FARM_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    b.cap = bytes;
+    return FARM_OK;
+}
+
+inline int upload(farm_handle h, DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(h, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(h, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return FARM_OK;
+}
+
+template <class T>
+T *P(DevBuf &b) {
+    return reinterpret_cast<T *>(b.p);
+}
+
+inline DevSet devset(farm_handle h) {
+    DevSet S;
+    S.hdr = P<farm_raster_hdr>(h->hdr);
+    S.status = P<uint8_t>(h->status);
+    S.pixoff = P<uint32_t>(h->pixoff);
+    S.seqs = P<uint2>(h->seqs);
+    S.x = P<float>(h->x);
+    S.y = P<float>(h->y);
+    S.pixedges = P<uint32_t>(h->pixedges);
+    S.voff = P<uint32_t>(h->voff);
+    S.vnodes = P<double>(h->vnodes);
+    return S;
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
+
+inline int fill_u32(farm_handle h, void *p, uint64_t n, uint32_t v) {
+    if (!n) return FARM_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(256), 0, h->stream, (uint32_t *)p, n, v);
+    HIPCHK(h, hipGetLastError());
+    return FARM_OK;
+}
+
+inline void free_buf(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+}  // namespace farm

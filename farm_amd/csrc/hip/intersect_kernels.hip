@@ -20,14 +20,19 @@ namespace farm {
 // The window is enumerated in full (no early exit), so the status bytes read
 // equal the algorithmic filter bytes.
 // ------------------------------------------------------------------------
+// skip (optional): pairs already decided by the caller (their result is left
+// untouched and they emit no pixel pairs).
 __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                  const uint32_t *__restrict__ pb_ids, uint32_t npairs, int pass,
-                                                 int approx, uint32_t *__restrict__ npp,
-                                                 const uint32_t *__restrict__ pp_off, PixPair *__restrict__ pps,
-                                                 uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr) {
+                                                 int approx, const uint8_t *__restrict__ skip,
+                                                 uint32_t *__restrict__ npp, const uint32_t *__restrict__ pp_off,
+                                                 PixPair *__restrict__ pps, uint32_t *__restrict__ result,
+                                                 unsigned long long *__restrict__ ctr) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long bytes = 0, inhit = 0;
-    if (k < npairs) {
+    if (k < npairs && skip && skip[k]) {
+        if (pass == 0) npp[k] = 0;
+    } else if (k < npairs) {
         const uint32_t ia = pa_ids[k], ib = pb_ids[k];
         const farm_raster_hdr A = S.hdr[ia];
         const farm_raster_hdr B = S.hdr[ib];
@@ -126,8 +131,10 @@ __global__ __launch_bounds__(256) void k_bucket_hist(DevSet S, const uint32_t *_
         const PixPair p = pps[i];
         int b;
         if (pred_cost) {
-            const float c = fmaxf(pred_cost[p.pair], 0.f);
-            b = min((int)(c * 1.4426950408889634f), kBuckets - 1);
+            // learned order: the MLP cost head predicts log2(1 + tests) of
+            // the pair; predictions are (logit, cost) pairs, hence stride 2
+            const float c = fmaxf(pred_cost[2 * (size_t)p.pair], 0.f);
+            b = min((int)c, kBuckets - 1);
         } else {
             const farm_raster_hdr B = S.hdr[pb_ids[p.pair]];
             const uint64_t eb = S.pixedges[B.pix_start + p.pb];
@@ -210,11 +217,13 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                                                  const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
                                                  const uint32_t *__restrict__ tile_off, uint32_t npp,
                                                  uint32_t *__restrict__ cursor, uint32_t *__restrict__ result,
-                                                 unsigned long long *__restrict__ ctr) {
+                                                 unsigned long long *__restrict__ ctr,
+                                                 uint32_t *__restrict__ pair_tests) {
     const int lane = threadIdx.x & 63;
     const uint32_t total = tile_off[npp];
     unsigned long long tests = 0, items = 0, rbytes = 0;
     for (;;) {
+        const unsigned long long tests_before = tests;
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(cursor, 1u);
         t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
@@ -364,6 +373,11 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         }
         if (__any(hit)) {
             if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (pair_tests) {  // executed tests of this tile, for the cost labels
+            unsigned long long tt = tests - tests_before;
+            for (int o = 32; o > 0; o >>= 1) tt += __shfl_xor(tt, o);
+            if (lane == 0) atomicAdd(&pair_tests[pair], (uint32_t)min(tt, 0xffffffffull));
         }
         if (lane == 0) {
             // algorithmic refine bytes of this tile: its source vertices plus
@@ -533,14 +547,15 @@ __global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ r
 template __global__ void k_tiles<1>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 template __global__ void k_tiles<2>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 template __global__ void k_tiles<4>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
-template __global__ void k_refine<1>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
-template __global__ void k_refine<2>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
-template __global__ void k_refine<4>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
-template __global__ void k_refine<8>(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                                     const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *);
+#define FARM_REFINE_INST(EE)                                                                                   \
+    template __global__ void k_refine<EE>(DevSet, const uint32_t *, const uint32_t *, const PixPair *,          \
+                                          const uint32_t *, const uint32_t *, uint32_t, uint32_t *, uint32_t *, \
+                                          unsigned long long *, uint32_t *);
+FARM_REFINE_INST(1)
+FARM_REFINE_INST(2)
+FARM_REFINE_INST(4)
+FARM_REFINE_INST(8)
+#undef FARM_REFINE_INST
 template __global__ void k_tiles<8>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 
 }  // namespace farm

@@ -100,12 +100,14 @@ typedef struct farm_polyset {
 
 typedef struct farm_opts {
     int32_t mode;
-    int32_t tile_a;        /* 0 = autotuned default */
-    int32_t tile_b;        /* 0 = autotuned default */
-    int32_t order;         /* batched refine order: 0 exact-cost, 1 learned cost, 2 none */
-    float approx_threshold;/* learned -a decision threshold on the logit */
+    int32_t tile_a;        /* source edges per lane of the batched refinement; 0 = autotuned */
+    int32_t tile_b;        /* reserved (target vertices stream in fixed 16-vertex chunks) */
+    int32_t order;         /* batched refine queue order: 0 exact cost bound, 1 learned cost head */
+    float approx_threshold;/* learned -a: logit above which a pair is accepted */
+    float approx_reject;   /* learned -a: logit below which a pair is rejected; pairs in
+                              [reject, threshold] fall back to exact refinement */
     float merge_threshold; /* -m, accepted for compatibility; exact results do not depend on it */
-    int32_t reserved[2];
+    int32_t reserved;
 } farm_opts;
 
 /* Counters of the work actually executed (device counters). */
@@ -150,6 +152,42 @@ int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts 
  * out_dev is a device buffer of at least end - begin bytes. */
 int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
                        farm_stats *stats);
+
+/* Exact intersect of pairs [begin, end) that also returns, per pair, the
+ * segment tests the batched refinement executed for it (the cost labels of
+ * the learned part).  No reference counterpart. */
+int farm_intersect_costs(farm_handle h, uint64_t begin, uint64_t end, uint8_t *bits, uint32_t *tests,
+                         farm_stats *stats);
+
+/* ---- learned part (no reference counterpart) ----
+ * Pair features: FARM_NFEAT floats per candidate pair from the filter's
+ * pixel windows: 64-bin histogram of BORDER x BORDER fill-ratio products,
+ * log counts of IN and BORDER pixel pairs, log vertex counts, MBR overlap
+ * ratio, log refinement cost bound, heuristic probabilities, log edge sums,
+ * pixel-area ratio, window BORDER / IN fractions, zero padding.
+ * decided[k]: 1 an IN overlap (exact answer true), 0 no BORDER pixel pair
+ * (exact answer false), -1 needs refinement. */
+#define FARM_NFEAT 80
+#define FARM_MLP_NPARAMS 27138 /* 80x128 + 128 + 128x128 + 128 + 128x2 + 2 */
+int farm_pair_features(farm_handle h, uint64_t begin, uint64_t end, float *feat, int8_t *decided);
+/* MLP 80 -> 128 (ReLU) -> 128 (ReLU) -> 2 (intersect logit, log2(1 + tests)).
+ * params: f32 master weights [W1 (80x128), b1, W2 (128x128), b2, W3 (128x2),
+ * b3], row-major; norm: per-feature mean then 1/std (2 x 80) applied to raw
+ * features on the device (NULL = identity).  Products run on MFMA with bf16
+ * operands and f32 accumulation. */
+int farm_mlp_set_params(farm_handle h, const float *params, uint64_t n, const float *norm);
+int farm_mlp_get_params(farm_handle h, float *params, uint64_t n);
+/* X: n x 80 normalised features (host); out: n x 2 */
+int farm_mlp_forward(farm_handle h, const float *X, uint64_t n, float *out);
+/* loss = mean BCE(logit, y) + lambda * mean (cost - c)^2 and its gradient
+ * (FARM_MLP_NPARAMS floats) on one batch */
+int farm_mlp_loss_grad(farm_handle h, const float *X, const float *y, const float *c, uint64_t n, float lambda,
+                       float *loss, float *grad);
+/* Adam training on the device (shuffled minibatches); epoch_loss[epochs] */
+int farm_mlp_train(farm_handle h, const float *X, const float *y, const float *c, uint64_t n, int epochs, int batch,
+                   float lr, float lambda, uint64_t seed, float *epoch_loss);
+/* features + normalisation + forward for pairs [begin, end); out: n x 2 */
+int farm_mlp_predict_pairs(farm_handle h, uint64_t begin, uint64_t end, float *out);
 
 /* Autotuner of the batched refinement: times pairs [begin, end) with every
  * supported tile width (source edges per lane: 1, 2, 4, 8) and keeps the

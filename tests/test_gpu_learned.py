@@ -1,0 +1,128 @@
+"""GPU: the learned part.  MFMA GEMMs, forward and gradients against the CPU
+reference MLP (oracle/mlp_ref.py); training; the learned -a mode against the
+exact labels and against the reference heuristic; learned queue order keeps
+exact mode bit-exact."""
+import numpy as np
+import pytest
+
+from farm_amd import capi, learned
+from oracle import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_problem(seed, m):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.15, capi.MLP_NPARAMS).astype(np.float32)
+    X = rng.normal(0, 1, (m, capi.NFEAT)).astype(np.float32)
+    y = (rng.uniform(size=m) < 0.3).astype(np.float32)
+    c = rng.normal(8, 3, m).astype(np.float32)
+    return p, X, y, c
+
+
+def test_forward_matches_reference(device):
+    # shapes cover partial 64-row tiles and the 2-wide output layer
+    for m in (1, 17, 64, 300):
+        p, X, _, _ = _rand_problem(m, m)
+        device.mlp_set_params(p)
+        got = device.mlp_forward(X)
+        want, _ = mlp_ref.forward(p.astype(np.float64), X, rounding=True)
+        scale = np.abs(want).max() + 1e-6
+        assert np.abs(got - want).max() <= 2e-4 * scale, (m, np.abs(got - want).max(), scale)
+
+
+def test_gradients_match_reference(device):
+    for m, lam in ((64, 0.1), (1000, 0.02), (4096, 0.05)):
+        p, X, y, c = _rand_problem(7 + m, m)
+        device.mlp_set_params(p)
+        L, g = device.mlp_loss_grad(X, y, c, lam)
+        Lr, gr = mlp_ref.loss_grad(p.astype(np.float64), X, y.astype(np.float64), c.astype(np.float64), lam,
+                                   rounding=True)
+        assert abs(L - Lr) <= 1e-4 * abs(Lr) + 1e-6
+        # per parameter block, relative to the block's largest gradient
+        o = 0
+        for shp in learned.SHAPES:
+            n = int(np.prod(shp))
+            blk, ref = g[o:o + n], gr[o:o + n]
+            err = np.abs(blk - ref).max() / (np.abs(ref).max() + 1e-12)
+            assert err < 3e-3, (m, shp, err)
+            o += n
+
+
+def test_training_reduces_loss(device):
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 1, (20000, capi.NFEAT)).astype(np.float32)
+    w = rng.normal(0, 1, capi.NFEAT)
+    y = (X @ w > 0).astype(np.float32)
+    c = (np.abs(X[:, :4]).sum(1) * 2).astype(np.float32)
+    device.mlp_set_params(learned.init_params(0))
+    losses = device.mlp_train(X, y, c, epochs=6, batch=1024, lr=3e-3, lam=0.05, seed=1)
+    assert losses[-1] < 0.3 * losses[0], losses
+    z = device.mlp_forward(X[:2000])
+    acc = ((z[:, 0] > 0) == (y[:2000] > 0)).mean()
+    assert acc > 0.95
+
+
+@pytest.fixture(scope="module")
+def star_model(device):
+    train_ps = learned.build_pairsets(list(range(500, 516)))
+    lab = learned.label(device, train_ps)
+    norm, losses = learned.train(device, lab, seed=0, epochs=15)
+    params = device.mlp_get_params()
+    test_ps = learned.build_pairsets(list(range(900, 908)))
+    test_lab = learned.label(device, test_ps)
+    return {"norm": norm, "params": params, "train": lab, "test": test_lab, "test_ps": test_ps, "losses": losses}
+
+
+def test_learned_approx_beats_heuristic_at_99_precision(device, star_model):
+    M = star_model
+    assert M["losses"][-1] < M["losses"][0]
+    tr, te = M["train"], M["test"]
+    device.mlp_set_params(M["params"], M["norm"])
+    # threshold chosen on the training pairs, evaluated on fresh seeds
+    device.upload_polygons(M["test_ps"].both)  # (label() left the test set uploaded)
+    device.upload_pairs(M["test_ps"].pa, M["test_ps"].pb)
+    z_te = device.mlp_predict_pairs()
+    truth = te.bits.astype(bool)
+    # training-set threshold: reload train predictions on the host copy of features
+    z_tr = device.mlp_forward(learned.apply_norm(tr.feat, M["norm"]))
+    thr = learned.threshold_for_precision(tr.decided, z_tr[:, 0], tr.bits.astype(bool), 0.995)
+    pred = learned.decisions(te.decided, z_te[:, 0], thr)
+    p_l, r_l = learned.precision_recall(pred, truth)
+    # the reference heuristic on the same pairs
+    heur = device.intersect(mode=capi.MODE_APPROX_HEURISTIC).bits.astype(bool)
+    p_h, r_h = learned.precision_recall(heur, truth)
+    # the filter alone (IN overlaps only): the method without refinement or MLP
+    p_f, r_f = learned.precision_recall(te.decided == 1, truth)
+    print(f"learned P={p_l:.4f} R={r_l:.4f} | heuristic P={p_h:.4f} R={r_h:.4f} | filter-only R={r_f:.4f}")
+    assert p_l >= 0.985
+    assert r_l > r_h + 0.05 and r_l > r_f + 0.05
+    # the device learned mode takes the same decisions as the host rule
+    r = device.intersect(mode=capi.MODE_APPROX_LEARNED, accept=thr)
+    assert np.array_equal(r.bits.astype(bool), pred)
+
+
+def test_learned_fallback_band_is_exact_inside_band(device, star_model):
+    M = star_model
+    te = M["test"]
+    device.mlp_set_params(M["params"], M["norm"])
+    device.upload_polygons(M["test_ps"].both)
+    device.upload_pairs(M["test_ps"].pa, M["test_ps"].pb)
+    z = device.mlp_predict_pairs()[:, 0]
+    lo, hi = -2.0, 2.0
+    r = device.intersect(mode=capi.MODE_APPROX_LEARNED, accept=hi, reject=lo)
+    band = (te.decided == -1) & (z >= lo) & (z <= hi)
+    assert band.sum() > 0
+    assert np.array_equal(r.bits[band], te.bits[band])
+    dec = te.decided != -1
+    assert np.array_equal(r.bits[dec], te.bits[dec])
+
+
+def test_learned_queue_order_keeps_exact_results(device, star_model):
+    M = star_model
+    device.mlp_set_params(M["params"], M["norm"])
+    device.upload_polygons(M["test_ps"].both)
+    device.upload_pairs(M["test_ps"].pa, M["test_ps"].pb)
+    a = device.intersect(order=0).bits
+    b = device.intersect(order=1).bits
+    assert np.array_equal(a, b) and np.array_equal(a, M["test"].bits)
