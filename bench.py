@@ -106,6 +106,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-pairs", type=int, default=1_000_000)
+    ap.add_argument("--label-pairs", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the pair list would be sharded "
@@ -134,13 +138,41 @@ def main() -> int:
         tile = dev.autotune(0, int(min(cuts[1], 1_000_000)))
         log(f"autotuned tile width {tile} in {time.time() - t:.1f}s")
     args.tile = tile
-    for s in range(args.warmup):
-        st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False).stats
+
+    # Warmup doubles as the learned part's training data: the exact path
+    # labels the warmup window (bits + executed tests per pair), the MLP is
+    # trained on the device on the pairs the filter leaves undecided.
+    order = 0
+    learned_info = None
+    if not args.no_learned:
+        from farm_amd import learned
+
+        t = time.time()
+        w0 = int(cuts[0])
+        w1 = int(min(cuts[1], cuts[0] + args.label_pairs))
+        lbits, ltests, _ = dev.intersect_costs(w0, w1)
+        feat, dec = dev.pair_features(w0, w1)
+        und = np.nonzero(dec == -1)[0]
+        rng = np.random.default_rng(args.seed + 11)
+        take = np.sort(rng.choice(und, size=min(len(und), args.train_pairs), replace=False))
+        lab = learned.Labeled(feat[take], dec[take], lbits[take], ltests[take])
+        norm, losses = learned.train(dev, lab, seed=args.seed, epochs=args.epochs)
+        z_train = dev.mlp_forward(learned.apply_norm(feat, norm))[:, 0]
+        thr = learned.threshold_for_precision(dec, z_train, lbits.astype(bool), 0.99)
+        order = 1
+        learned_info = {"train_pairs": int(len(take)), "epochs": args.epochs, "loss_first": float(losses[0]),
+                        "loss_last": float(losses[-1]), "threshold": thr, "seconds": time.time() - t}
+        log(f"learned: labeled + trained on {len(take)} undecided warmup pairs in {time.time() - t:.1f}s, "
+            f"loss {losses[0]:.4f} -> {losses[-1]:.4f}, -a threshold {thr:.3f}")
+    for s in range(1 if learned_info else 0, args.warmup):
+        st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False,
+                           order=order).stats
         log(f"warmup step {s}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms")
     per_step = []
     t0 = time.time()
     for s in range(args.warmup, nb):
-        st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False).stats
+        st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False,
+                           order=order).stats
         per_step.append(st)
         log(f"step {s - args.warmup}: {cuts[s + 1] - cuts[s]} pairs, filter {st['filter_ms']:.1f} ms, "
             f"refine {st['refine_ms']:.1f} ms, total {st['total_ms']:.1f} ms, tests {st['segment_tests']:.3e}")
@@ -167,7 +199,8 @@ def main() -> int:
         "dtype": "f32 (segment predicates), f64 (raster windows)",
         "data": "synthetic, seeded",
         "config": {
-            "workload": f"exact intersect, {len(A)} complex x {len(B)} lakes, -v {args.vpr}, batched kernel",
+            "workload": f"exact intersect, {len(A)} complex x {len(B)} lakes, -v {args.vpr}, batched kernel, "
+                        f"{'learned' if order else 'cost-bound'} queue order",
             "candidate_pairs": int(len(pa)),
             "timed_pairs": timed_pairs,
             "found": int(found),
@@ -189,23 +222,64 @@ def main() -> int:
         },
     }
 
+    # comparisons on the first timed batch
+    s = args.warmup
+    b0, b1 = int(cuts[s]), int(cuts[s + 1])
+    batched_first = per_step[0]["total_ms"] / 1000.0
+    if order:
+        tq = time.time()
+        st0 = dev.intersect(b0, b1, capi.MODE_EXACT_BATCHED, args.tile, want_bits=False, order=0).stats
+        t_cost = time.time() - tq
+        line["batched_cost_bound_order"] = {"pairs_per_s": (b1 - b0) / t_cost, "batch_pairs": b1 - b0,
+                                            "found": int(st0["found"]),
+                                            "learned_order_speedup": t_cost / batched_first}
+        log(f"cost-bound queue order on step-0 batch: {t_cost * 1000:.1f} ms")
     if not args.no_plain:
-        s = args.warmup
-        b0, b1 = int(cuts[s]), int(cuts[s + 1])
         tp = time.time()
         stp = dev.intersect(b0, b1, capi.MODE_EXACT_PLAIN, want_bits=False).stats
         plain_s = time.time() - tp
         plain_rate = (b1 - b0) / plain_s
-        batched_first = per_step[0]["total_ms"] / 1000.0
         line["plain"] = {"pairs_per_s": plain_rate, "batch_pairs": b1 - b0, "found": int(stp["found"]),
                          "found_batched": int(per_step[0]["found"]),
                          "speedup_batched_vs_plain": plain_s / batched_first if batched_first > 0 else None}
         log(f"plain kernel on step-0 batch: {plain_s * 1000:.1f} ms ({plain_rate:.3e} pairs/s)")
 
+    # exact bits of every pair (untimed): parity sample and approximate-mode truth
+    bits = dev.intersect(0, len(pa), capi.MODE_EXACT_BATCHED, args.tile).bits
+    if learned_info is not None:
+        from farm_amd import learned
+
+        e1 = int(min(b1, b0 + args.label_pairs))  # evaluation window: fresh pairs of the first timed batch
+        truth = bits[b0:e1].astype(bool)
+        tq = time.time()
+        dev.intersect(b0, e1, capi.MODE_EXACT_BATCHED, args.tile, want_bits=False, order=order)
+        t_x = time.time() - tq
+        tq = time.time()
+        heur = dev.intersect(b0, e1, capi.MODE_APPROX_HEURISTIC).bits.astype(bool)
+        t_h = time.time() - tq
+        tq = time.time()
+        lrn = dev.intersect(b0, e1, capi.MODE_APPROX_LEARNED, accept=learned_info["threshold"]).bits.astype(bool)
+        t_l = time.time() - tq
+        p_h, r_h = learned.precision_recall(heur, truth)
+        p_l, r_l = learned.precision_recall(lrn, truth)
+        _, dec = dev.pair_features(b0, e1)
+        p_f, r_f = learned.precision_recall(dec == 1, truth)
+        learned_info.update({
+            "eval_pairs": e1 - b0,
+            "exact_pairs_per_s": (e1 - b0) / t_x,
+            "learned_a": {"precision": p_l, "recall": r_l, "pairs_per_s": (e1 - b0) / t_l,
+                          "speedup_vs_exact": t_x / t_l},
+            "heuristic_a": {"precision": p_h, "recall": r_h, "pairs_per_s": (e1 - b0) / t_h,
+                            "speedup_vs_exact": t_x / t_h},
+            "filter_only": {"precision": p_f, "recall": r_f},
+        })
+        line["learned"] = learned_info
+        log(f"-a learned P={p_l:.4f} R={r_l:.4f} ({t_l * 1000:.1f} ms) | heuristic P={p_h:.4f} R={r_h:.4f} "
+            f"({t_h * 1000:.1f} ms) | filter-only R={r_f:.4f}")
+
     if not args.no_cpu:
         rng = np.random.default_rng(args.seed + 7)
         res = cpu_baseline(rings, pa, pb, args.vpr, args.threads, args.cpu_seconds, rng, None)
-        bits = dev.intersect(0, len(pa), capi.MODE_EXACT_BATCHED, args.tile).bits
         samp = res["sample"]
         timed = (samp >= cuts[0]) & (samp < cuts[nb])
         mism = int((bits[samp][timed] != res["bits"][timed]).sum())

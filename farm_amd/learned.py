@@ -148,6 +148,51 @@ def precision_recall(pred: np.ndarray, truth: np.ndarray) -> tuple[float, float]
     return (tp / max(tp + fp, 1), tp / max(tp + fn, 1))
 
 
+def evaluate(train_pairs: int = 500_000, eval_pairs: int = 2_000_000, target: float = 0.99, epochs: int = 12,
+             seed: int = 1) -> dict:
+    """Learned -a on the star-pair distribution: train on `train_pairs`
+    pairs of seeded instances, pick the threshold for `target` precision on
+    them, evaluate on `eval_pairs` pairs of fresh seeds (exact labels, the
+    reference heuristic and the filter alone on the same pairs)."""
+    import time
+
+    per_instance = 4200  # candidate pairs of one 2,000 x 2,000 instance, about
+    n_tr = max(1, train_pairs // per_instance)
+    n_ev = max(1, eval_pairs // per_instance)
+    dev = capi.Device(0)
+    t = time.time()
+    tr_ps = build_pairsets([10_000 + seed * 100_000 + i for i in range(n_tr)])
+    tr = label(dev, tr_ps)
+    norm, losses = train(dev, tr, seed=seed, epochs=epochs)
+    z_tr = dev.mlp_forward(apply_norm(tr.feat, norm))[:, 0]
+    thr = threshold_for_precision(tr.decided, z_tr, tr.bits.astype(bool), target)
+    t_train = time.time() - t
+    ev_ps = build_pairsets([50_000 + seed * 100_000 + i for i in range(n_ev)])
+    ev = label(dev, ev_ps)
+    truth = ev.bits.astype(bool)
+    res = {"train_pairs": int(len(tr.bits)), "eval_pairs": int(len(ev.bits)), "threshold": thr,
+           "train_seconds": t_train, "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+    timings = {}
+    for name, kw in (("exact", dict(mode=capi.MODE_EXACT_BATCHED)),
+                     ("exact_learned_order", dict(mode=capi.MODE_EXACT_BATCHED, order=1)),
+                     ("heuristic_a", dict(mode=capi.MODE_APPROX_HEURISTIC)),
+                     ("learned_a", dict(mode=capi.MODE_APPROX_LEARNED, accept=thr))):
+        dev.intersect(**kw)  # warm
+        tq = time.time()
+        bits = dev.intersect(**kw).bits.astype(bool)
+        timings[name] = time.time() - tq
+        p, r = precision_recall(bits, truth)
+        res[name] = {"precision": p, "recall": r, "pairs_per_s": len(truth) / timings[name]}
+        if name.startswith("exact"):
+            assert np.array_equal(bits, truth)
+    for name in ("heuristic_a", "learned_a", "exact_learned_order"):
+        res[name]["speedup_vs_exact"] = timings["exact"] / timings[name]
+    p, r = precision_recall(ev.decided == 1, truth)
+    res["filter_only"] = {"precision": p, "recall": r}
+    dev.close()
+    return res
+
+
 def threshold_for_precision(decided, logit, truth, target: float = 0.99) -> float:
     """Smallest logit threshold whose overall precision reaches target."""
     und = decided == -1
@@ -164,3 +209,16 @@ def threshold_for_precision(decided, logit, truth, target: float = 0.99) -> floa
         return float(np.inf)
     k = ok[-1]  # accept the top k+1 logits
     return float(z[order][k]) - 1e-6 if k + 1 < len(z) else float(z.min()) - 1.0
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train and evaluate the learned -a mode on star-polygon pairs")
+    ap.add_argument("--train-pairs", type=int, default=500_000)
+    ap.add_argument("--eval-pairs", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args()
+    print(json.dumps(evaluate(a.train_pairs, a.eval_pairs, epochs=a.epochs, seed=a.seed)), flush=True)
