@@ -50,11 +50,28 @@ class RasterHdr(C.Structure):
         ("nseqs", C.c_uint32),
         ("vline_start", C.c_uint64),
         ("vnode_start", C.c_uint64),
-        ("reserved", C.c_uint64 * 2),
+        ("layer_start", C.c_uint64),
+        ("nlayers", C.c_int32),
+        ("flags", C.c_int32),
     ]
 
 
 assert C.sizeof(RasterHdr) == 128
+
+HDR_REJECTED = 1
+
+
+class Layer(C.Structure):
+    _fields_ = [
+        ("low", C.c_double * 2),
+        ("step", C.c_double),
+        ("dimx", C.c_int32),
+        ("dimy", C.c_int32),
+        ("status_start", C.c_uint64),
+    ]
+
+
+assert C.sizeof(Layer) == 40
 
 HDR_DTYPE = np.dtype(
     [
@@ -70,7 +87,9 @@ HDR_DTYPE = np.dtype(
         ("nseqs", "<u4"),
         ("vline_start", "<u8"),
         ("vnode_start", "<u8"),
-        ("reserved", "<u8", (2,)),
+        ("layer_start", "<u8"),
+        ("nlayers", "<i4"),
+        ("flags", "<i4"),
     ]
 )
 assert HDR_DTYPE.itemsize == 128
@@ -93,6 +112,11 @@ class PolySet(C.Structure):
         ("nvoff", C.c_uint64),
         ("vnodes", C.POINTER(C.c_double)),
         ("nvnodes", C.c_uint64),
+        ("layers", C.POINTER(Layer)),
+        ("nlayers", C.c_uint64),
+        ("lstatus", C.POINTER(C.c_uint8)),
+        ("nlstatus", C.c_uint64),
+        ("areas", C.POINTER(C.c_double)),
     ]
 
 
@@ -149,6 +173,14 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_within": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
+    ),
+    "farm_within_dev": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
+    ),
     "farm_pair_features": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
     "farm_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "farm_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -175,6 +207,8 @@ EXPORTS = {
     "farm_hostset_ring": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "farm_hostset_mbbs": (C.c_int, [C.c_void_p, C.c_void_p]),
     "farm_hostset_rasterize": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "farm_hostset_rasterize_h": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "farm_hostset_layer": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(Layer), C.POINTER(C.c_void_p)]),
     "farm_hostset_view": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
     "farm_hostset_areas": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
     "farm_hostset_raster_failures": (C.c_int64, [C.c_void_p]),
@@ -262,6 +296,24 @@ class HostSet:
         rc = lib.farm_hostset_rasterize(self._h, vpr, nthreads or default_threads())
         if rc:
             raise FarmError(rc, "rasterize failed")
+
+    def rasterize_h(self, vpr: int = 10, nlow: int = 1, nthreads: int | None = None) -> None:
+        """-h rasterization: aligned grid, coarse layers and per-pixel areas
+        (needed by Device.within)."""
+        rc = lib.farm_hostset_rasterize_h(self._h, vpr, nlow, nthreads or default_threads())
+        if rc:
+            raise FarmError(rc, "rasterize_h failed")
+
+    def layer(self, i: int, k: int) -> dict:
+        """Layer k (0 = coarsest) of polygon i after rasterize_h."""
+        L = Layer()
+        p = C.c_void_p()
+        rc = lib.farm_hostset_layer(self._h, i, k, C.byref(L), C.byref(p))
+        if rc:
+            raise FarmError(rc, f"no layer {k} of polygon {i}")
+        n = L.dimx * L.dimy
+        st = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n,)).copy()
+        return {"low": np.array(L.low[:]), "step": L.step, "dimx": L.dimx, "dimy": L.dimy, "status": st}
 
     def raster_failures(self) -> int:
         """Index of the first polygon the reference raster builder rejects, or -1."""
@@ -399,6 +451,18 @@ class Device:
         self._check(lib.farm_intersect_costs(self._h, begin, end, _ptr(bits), _ptr(tests), C.byref(st)),
                     "intersect_costs")
         return bits[:n], tests[:n], st.as_dict()
+
+    def within(self, d_km: float = 10.0, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray, dict]:
+        """Within-distance decision of pairs [begin, end): bits (1 within, 0
+        not, 2 a polygon is rejected by the raster builder), the running
+        minimum distance of the refinement, and counters."""
+        end = self.npairs if end is None else end
+        n = end - begin
+        bits = np.zeros(max(n, 1), np.uint8)
+        md = np.zeros(max(n, 1), np.float64)
+        st = Stats()
+        self._check(lib.farm_within(self._h, begin, end, d_km, _ptr(bits), _ptr(md), C.byref(st)), "within")
+        return bits[:n], md[:n], st.as_dict()
 
     def pair_features(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray]:
         end = self.npairs if end is None else end

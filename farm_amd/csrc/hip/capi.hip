@@ -181,7 +181,8 @@ int farm_close(farm_handle h) {
                       &h->cursor, &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
-                      &h->skipbuf};
+                      &h->skipbuf, &h->layers, &h->lstatus, &h->areas, &h->wctr, &h->wovf, &h->wslab,
+                      &h->wslab_big, &h->wout, &h->wmd};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : h->ev)
@@ -214,6 +215,32 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
             if ((uint64_t)sq[2 * k] + sq[2 * k + 1] + 1 > r.nverts)
                 return fail(h, FARM_EINVAL, "polygon %lld: edge sequence out of range", (long long)i);
     }
+    // hierarchy: every polygon's layers in range, square steps
+    const bool layered = ps->areas != nullptr;
+    uint64_t max_layer_pix = 0, max_border_pix = 0;
+    if (layered) {
+        for (int64_t i = 0; i < ps->npolys; i++) {
+            const farm_raster_hdr &r = ps->hdr[i];
+            if (r.flags & FARM_HDR_REJECTED) continue;  // within answers 2 without reading it
+            if (r.nlayers < 1 || r.layer_start + (uint64_t)(r.nlayers - 1) > ps->nlayers ||
+                (r.nlayers > 1 && !ps->layers) || r.step[0] != r.step[1])
+                return fail(h, FARM_EINVAL, "polygon %lld: layer header out of range", (long long)i);
+            max_layer_pix = std::max<uint64_t>(max_layer_pix, (uint64_t)r.dimx * r.dimy);
+            uint64_t nb = 0;
+            for (uint64_t p = 0; p < (uint64_t)r.dimx * r.dimy; p++) {
+                const uint8_t c = ps->status[r.pix_start + p];
+                nb += (c != 0 && c != FARM_CATEGORY_COUNT - 1);
+            }
+            max_border_pix = std::max(max_border_pix, nb);
+            for (int k = 0; k + 1 < r.nlayers; k++) {
+                const farm_layer &L = ps->layers[r.layer_start + k];
+                const uint64_t np = (uint64_t)L.dimx * L.dimy;
+                if (L.dimx <= 0 || L.dimy <= 0 || !(L.step > 0) || L.status_start + np > ps->nlstatus)
+                    return fail(h, FARM_EINVAL, "polygon %lld: layer %d out of range", (long long)i, k);
+                max_layer_pix = std::max(max_layer_pix, np);
+            }
+        }
+    }
     int rc;
     if ((rc = upload(h, h->hdr, ps->hdr, sizeof(farm_raster_hdr) * ps->npolys))) return rc;
     if ((rc = upload(h, h->status, ps->status, ps->nstatus))) return rc;
@@ -230,6 +257,15 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     if ((rc = upload(h, h->voff, ps->voff, 4 * ps->nvoff))) return rc;
     if ((rc = upload(h, h->vnodes, ps->vnodes, 8 * ps->nvnodes))) return rc;
     if ((rc = ensure(h, h->pixedges, 4 * ps->nstatus))) return rc;
+    h->has_layers = false;
+    if (layered) {
+        if ((rc = upload(h, h->layers, ps->layers, sizeof(farm_layer) * ps->nlayers))) return rc;
+        if ((rc = upload(h, h->lstatus, ps->lstatus, ps->nlstatus))) return rc;
+        if ((rc = upload(h, h->areas, ps->areas, 8 * ps->nstatus))) return rc;
+        h->has_layers = true;
+        h->max_layer_pix = max_layer_pix;
+        h->max_border_pix = max_border_pix;
+    }
     h->npolys = ps->npolys;
     h->nstatus = ps->nstatus;
     if (ps->npolys > 0) {

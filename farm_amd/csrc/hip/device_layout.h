@@ -20,6 +20,10 @@ struct DevSet {
     const uint32_t *pixedges;  // per pixel: sum of sequence lengths (BORDER), else 0
     const uint32_t *voff;
     const double *vnodes;
+    // hierarchy (-h) and per-pixel areas (within only; null otherwise)
+    const farm_layer *layers;
+    const uint8_t *lstatus;
+    const double *areas;
 };
 
 // A BORDER x BORDER pixel pair of one candidate pair.

@@ -56,6 +56,13 @@ struct farm_handle_s {
     farm::DevBuf feat, decided, pred;  // per-pair features / decided flags / predictions
     farm::DevBuf skipbuf;              // pairs decided before the exact filter
     uint64_t adam_t = 0;
+    // within (-h sets)
+    bool has_layers = false;
+    bool within_consts_ready = false;
+    uint64_t max_layer_pix = 0;   // largest layer (pixels) of the set
+    uint64_t max_border_pix = 0;  // most BORDER pixels of a finest layer
+    farm::DevBuf layers, lstatus, areas;
+    farm::DevBuf wctr, wovf, wslab, wslab_big, wout, wmd;
 };
 
 namespace farm {
@@ -116,6 +123,9 @@ inline DevSet devset(farm_handle h) {
     S.pixedges = P<uint32_t>(h->pixedges);
     S.voff = P<uint32_t>(h->voff);
     S.vnodes = P<double>(h->vnodes);
+    S.layers = h->has_layers ? P<farm_layer>(h->layers) : nullptr;
+    S.lstatus = h->has_layers ? P<uint8_t>(h->lstatus) : nullptr;
+    S.areas = h->has_layers ? P<double>(h->areas) : nullptr;
     return S;
 }
 

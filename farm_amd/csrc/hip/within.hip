@@ -1,0 +1,764 @@
+// Within-distance decision of candidate pairs on gfx950.
+//
+// Follows the CPU semantics of Ideal::within (src/geometry/Ideal.cpp:2475-2643)
+// with the distance helpers point_to_segment_distance /
+// segment_to_segment_within_batch (src/include/geometry_computation.h:49-182)
+// and box::distance / box::max_distance (src/geometry/Box.cpp:89-149).  The
+// reference GPU path (src/cuda/query_within_polygon.cu) computes the
+// refinement in FP32 and decides differently; this one reproduces the CPU
+// decision bit for bit.
+//
+// One wavefront per candidate pair; all control flow of a pair is uniform
+// across the wave and the lanes split the wide loops:
+//  1. hierarchical descent: the pixel sets of both polygons are kept as
+//     compact lists of packed (x, y) ids in LDS.  The reference's queue of
+//     pixel pairs only feeds the union of its first members and the union of
+//     its second members into the next level, so the two projections carry
+//     the same information; at the last level the candidate pixel pairs are
+//     the filtered cross product of the two lists.
+//  2. box distances: box::max_distance over the 16 corner pairs reduces to
+//     the largest |dx| and |dy| (the corner sets are a full product and every
+//     step after the f32 differences is monotone), so one division per axis
+//     and one square root give the identical double.
+//  3. candidates (apx, low, high, id1, id2) live in a per-wave global slab
+//     and are taken in tuple order by argmin selection (the order of
+//     std::sort over the tuples); the refinement stops at the reference's
+//     first return, so usually after one or two candidates.
+//  4. the truncated point-segment distances of one edge-sequence pair are
+//     spread over the lanes; the minimum is taken on the squared distances
+//     and one square root follows (sqrt is monotone and correctly rounded).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+#include "handle.h"
+
+namespace farm {
+
+const double *km_lon_table();  // host table shared with candidate generation
+
+__constant__ double c_km_lon[900];  // degree_per_kilometer_longitude (util.h:73-96)
+__constant__ float c_mean[20];      // mean[20] (util.h:68)
+
+namespace {
+
+constexpr double kKmLat = 360.0 / 40076.0;  // degree_per_kilometer_latitude (util.h:72)
+constexpr double kFar = 100000.0;           // the reference's "no distance yet"
+constexpr int kWaves = 4;                   // wavefronts per workgroup
+constexpr int kPixMax = 1024;               // pixels of a layer handled in LDS
+constexpr int kBitWords = kPixMax / 32;
+constexpr uint32_t kCandLds = 2048;         // candidate slab of a wave in the LDS path
+constexpr uint32_t kChunk = 8;              // pairs per queue grab
+
+struct Cand {
+    double apx, lo, hi;
+    uint32_t id1, id2;
+};
+
+struct LGrid {
+    double lx, ly, step;
+    int dimx, dimy;
+    const uint8_t *st;
+};
+
+__device__ __forceinline__ LGrid lgrid(const DevSet &S, const farm_raster_hdr &h, int k) {
+    LGrid g;
+    if (k == h.nlayers - 1) {
+        g.lx = h.mbr[0];
+        g.ly = h.mbr[1];
+        g.step = h.step[0];
+        g.dimx = h.dimx;
+        g.dimy = h.dimy;
+        g.st = S.status + h.pix_start;
+    } else {
+        const farm_layer &L = S.layers[h.layer_start + k];
+        g.lx = L.low[0];
+        g.ly = L.low[1];
+        g.step = L.step;
+        g.dimx = L.dimx;
+        g.dimy = L.dimy;
+        g.st = S.lstatus + L.status_start;
+    }
+    return g;
+}
+
+__device__ __forceinline__ bool border(uint8_t s) { return s != 0 && s != kCC - 1; }
+
+// degree_per_kilometer_longitude(lat) (util.h:88-96); the index is clamped
+// so that |lat| > 90 cannot read outside the table
+__device__ __forceinline__ double km_lon(double lat) {
+    double a = fabs(lat);
+    if (a == 90) a = 89.9;
+    int i = (int)(a * 10);
+    return c_km_lon[min(max(i, 0), 899)];
+}
+
+// MyRaster::get_pixel_box (MyRaster.cpp:312-322), square pixels
+__device__ __forceinline__ void pbox(const LGrid &g, int x, int y, double b[4]) {
+    b[0] = g.lx + x * g.step;
+    b[1] = g.ly + y * g.step;
+    b[2] = g.lx + (x + 1) * g.step;
+    b[3] = g.ly + (y + 1) * g.step;
+}
+
+// box::distance(box, geography) (Box.cpp:89-120), b = this
+__device__ __forceinline__ double box_min_dist(const double b[4], const double t[4]) {
+    if (!(t[0] > b[2] || t[2] < b[0] || t[1] > b[3] || t[3] < b[1])) return 0.0;
+    double dx = 0, dy = 0;
+    if (t[1] > b[3]) dy = t[1] - b[3];
+    else if (t[3] < b[1]) dy = b[1] - t[3];
+    if (t[0] > b[2]) dx = t[0] - b[2];
+    else if (t[2] < b[0]) dx = b[0] - t[2];
+    dy = dy / kKmLat;
+    dx = dx / km_lon(b[1]);
+    return sqrt(dx * dx + dy * dy);
+}
+
+// box::max_distance(box, geography) (Box.cpp:122-149): corners as f32
+// points, f32 differences promoted to double; the largest |difference| per
+// axis gives the maximum over the 16 corner pairs
+__device__ __forceinline__ double box_max_dist(const double b[4], const double t[4]) {
+    const float b0 = (float)b[0], b1 = (float)b[1], b2 = (float)b[2], b3 = (float)b[3];
+    const float t0 = (float)t[0], t1 = (float)t[1], t2 = (float)t[2], t3 = (float)t[3];
+    const float fx = fmaxf(fmaxf(fabsf(b0 - t0), fabsf(b0 - t2)), fmaxf(fabsf(b2 - t0), fabsf(b2 - t2)));
+    const float fy = fmaxf(fmaxf(fabsf(b1 - t1), fabsf(b1 - t3)), fmaxf(fabsf(b3 - t1), fabsf(b3 - t3)));
+    const double dy = (double)fy / kKmLat;
+    const double dx = (double)fx / km_lon(b[1]);
+    return sqrt(dx * dx + dy * dy);
+}
+
+// point_to_segment_distance (geometry_computation.h:49-89), geography on,
+// before the square root
+__device__ __forceinline__ double pt_seg_d2(float px, float py, float ax, float ay, float bx, float by) {
+    const double A = (double)(px - ax);
+    const double B = (double)(py - ay);
+    const double C = (double)(bx - ax);
+    const double D = (double)(by - ay);
+    const double dot = A * C + B * D;
+    const double len_sq = C * C + D * D;
+    double param = -1;
+    if (len_sq != 0) param = dot / len_sq;
+    double xx, yy;
+    if (param < 0) {
+        xx = ax;
+        yy = ay;
+    } else if (param > 1) {
+        xx = bx;
+        yy = by;
+    } else {
+        xx = ax + param * C;
+        yy = ay + param * D;
+    }
+    double dx = px - xx;
+    double dy = py - yy;
+    dx = dx / km_lon(py);
+    dy = dy / kKmLat;
+    return dx * dx + dy * dy;
+}
+
+// classifyPixel(area, pixelArea, count) (geometry_computation.h:299-321)
+__device__ __forceinline__ int classify(double area, double pixel_area, int count) {
+    const double ratio = area / pixel_area;
+    if (fabs(ratio - 1.0) < 1e-9) return count - 1;
+    if (fabs(ratio) < 1e-9) return 0;
+    int idx = (int)ceil(ratio * (count - 2));
+    if (idx >= count) idx = count - 1;
+    return idx;
+}
+
+// Orders one wave's own LDS / global writes before its later reads by other
+// lanes (the candidate slab is global memory).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
+
+__device__ __forceinline__ bool tuple_less(const Cand &a, const Cand &b) {
+    if (a.apx != b.apx) return a.apx < b.apx;
+    if (a.lo != b.lo) return a.lo < b.lo;
+    if (a.hi != b.hi) return a.hi < b.hi;
+    if (a.id1 != b.id1) return a.id1 < b.id1;
+    return a.id2 < b.id2;
+}
+
+// Pixel set -> compact list of packed (x | y << 16) in id order.  bits has
+// ceil(npix / 32) words; lanes take one word each.
+__device__ __forceinline__ uint32_t compact(const uint32_t *bits, int npix, int dimx, uint32_t *list, int lane) {
+    const int nw = (npix + 31) >> 5;
+    uint32_t total = 0;
+    for (int w0 = 0; w0 < nw; w0 += 64) {
+        const int w = w0 + lane;
+        const uint32_t word = w < nw ? bits[w] : 0u;
+        const uint32_t c = __popc(word);
+        // inclusive scan of c over the wave
+        uint32_t s = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(s, o, 64);
+            if (lane >= o) s += t;
+        }
+        uint32_t pos = total + s - c;
+        uint32_t m = word;
+        while (m) {
+            const int b = __ffs(m) - 1;
+            m &= m - 1;
+            const uint32_t id = (uint32_t)(w * 32 + b);
+            const uint32_t y = id / (uint32_t)dimx, x = id - y * (uint32_t)dimx;
+            list[pos++] = x | (y << 16);
+        }
+        total += __shfl(s, 63, 64);
+    }
+    wave_sync();
+    return total;
+}
+
+__device__ __forceinline__ void clear_bits(uint32_t *bits, int npix, int lane) {
+    const int nw = (npix + 31) >> 5;
+    for (int w = lane; w < nw; w += 64) bits[w] = 0u;
+    wave_sync();
+}
+
+__device__ __forceinline__ void set_bit(uint32_t *bits, uint32_t id) { atomicOr(&bits[id >> 5], 1u << (id & 31)); }
+
+struct PairOut {
+    uint8_t bit;      // 0 / 1; 3 = the pair does not fit this path's buffers
+    double min_dist;
+    uint32_t ncand;
+    uint64_t evals;
+};
+
+// Descend one side: children (at layer g) of every parent in list (at layer
+// pg), or (same layer) the members themselves; BORDER ones go to bits.
+__device__ __forceinline__ void expand_side(const uint32_t *list, uint32_t n, bool next, const LGrid &pg,
+                                            const LGrid &g, uint32_t *bits, int lane) {
+    for (uint32_t k = lane; k < n; k += 64) {
+        const uint32_t v = list[k];
+        const int px = (int)(v & 0xffffu), py = (int)(v >> 16);
+        if (!next) {
+            const uint32_t id = (uint32_t)(py * g.dimx + px);
+            if (border(g.st[id])) set_bit(bits, id);
+            continue;
+        }
+        // the parent box shrunk by 1e-6, then retrieve_pixels'
+        // own 1e-6 (MyRaster.cpp:332-346), in that order
+        double b[4];
+        pbox(pg, px, py, b);
+        b[0] += 1e-6;
+        b[1] += 1e-6;
+        b[2] -= 1e-6;
+        b[3] -= 1e-6;
+        const int sx = off_axis(b[0] + 1e-6, g.lx, g.step, g.dimx);
+        const int sy = off_axis(b[1] + 1e-6, g.ly, g.step, g.dimy);
+        const int ex = off_axis(b[2] - 1e-6, g.lx, g.step, g.dimx);
+        const int ey = off_axis(b[3] - 1e-6, g.ly, g.step, g.dimy);
+        for (int x = sx; x <= ex; x++)
+            for (int y = sy; y <= ey; y++) {
+                const uint32_t id = (uint32_t)(y * g.dimx + x);
+                if (border(g.st[id])) set_bit(bits, id);
+            }
+    }
+    wave_sync();
+}
+
+// Truncated distance of one edge-sequence pair
+// (segment_to_segment_within_batch(vs1 = T seq, vs2 = S seq)), squared.
+__device__ __forceinline__ double seq_pair_d2(const float *tx, const float *ty, uint32_t lb, const float *sx,
+                                              const float *sy, uint32_t la, int lane) {
+    double best = DBL_MAX;
+    // points of the target sequence against the source segments
+    {
+        const uint32_t m = la - 1;
+        uint32_t i = lane / m, j = lane - (lane / m) * m;
+        const uint32_t q = 64 / m, r = 64 - q * m;
+        while (i < lb) {
+            const double d2 = pt_seg_d2(tx[i], ty[i], sx[j], sy[j], sx[j + 1], sy[j + 1]);
+            best = fmin(best, d2);
+            i += q;
+            j += r;
+            if (j >= m) {
+                j -= m;
+                i++;
+            }
+        }
+    }
+    // points of the source sequence against the target segments
+    {
+        const uint32_t m = lb - 1;
+        uint32_t i = lane / m, j = lane - (lane / m) * m;
+        const uint32_t q = 64 / m, r = 64 - q * m;
+        while (i < la) {
+            const double d2 = pt_seg_d2(sx[i], sy[i], tx[j], ty[j], tx[j + 1], ty[j + 1]);
+            best = fmin(best, d2);
+            i += q;
+            j += r;
+            if (j >= m) {
+                j -= m;
+                i++;
+            }
+        }
+    }
+    return wave_min(best);
+}
+
+// Argmin (tuple order) over the live candidates; returns its slot or -1.
+__device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lane, Cand &out) {
+    Cand best;
+    best.apx = INFINITY;
+    best.lo = INFINITY;
+    best.hi = INFINITY;
+    best.id1 = best.id2 = 0xffffffffu;
+    int bi = -1;
+    for (uint32_t k = lane; k < n; k += 64) {
+        const Cand c = cand[k];
+        if (c.apx == INFINITY) continue;  // taken
+        if (bi < 0 || tuple_less(c, best)) {
+            best = c;
+            bi = (int)k;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        Cand c;
+        c.apx = __shfl_xor(best.apx, o, 64);
+        c.lo = __shfl_xor(best.lo, o, 64);
+        c.hi = __shfl_xor(best.hi, o, 64);
+        c.id1 = __shfl_xor(best.id1, o, 64);
+        c.id2 = __shfl_xor(best.id2, o, 64);
+        const int ci = __shfl_xor(bi, o, 64);
+        if (ci >= 0 && (bi < 0 || tuple_less(c, best))) {
+            best = c;
+            bi = ci;
+        }
+    }
+    out = best;
+    return bi;
+}
+
+// Ideal::within for one pair; the wave's buffers: lists / bits (LDS or a
+// global slab) and the candidate slab.
+__device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uint32_t ib, double d, uint32_t *sl,
+                                               uint32_t *tl, uint32_t *sb, uint32_t *tb, Cand *cand, uint32_t pixcap,
+                                               uint32_t candcap, int lane) {
+    PairOut r;
+    r.bit = 0;
+    r.min_dist = kFar;
+    r.ncand = 0;
+    r.evals = 0;
+    const farm_raster_hdr hs = S.hdr[ia];
+    const farm_raster_hdr ht = S.hdr[ib];
+    const int s_level = hs.nlayers - 1, t_level = ht.nlayers - 1;
+    // every layer must fit the pixel buffers (the finest is the largest,
+    // but check all: a coarse layer may be one pixel wider)
+    for (int k = 0; k <= s_level; k++) {
+        const LGrid g = lgrid(S, hs, k);
+        if ((uint64_t)g.dimx * g.dimy > pixcap || g.dimx > 0xffff || g.dimy > 0xffff) {
+            r.bit = 3;
+            return r;
+        }
+    }
+    for (int k = 0; k <= t_level; k++) {
+        const LGrid g = lgrid(S, ht, k);
+        if ((uint64_t)g.dimx * g.dimy > pixcap || g.dimx > 0xffff || g.dimy > 0xffff) {
+            r.bit = 3;
+            return r;
+        }
+    }
+    // level 0: every pixel of both top layers
+    uint32_t ns, nt;
+    {
+        const LGrid gs = lgrid(S, hs, 0), gt = lgrid(S, ht, 0);
+        ns = (uint32_t)(gs.dimx * gs.dimy);
+        nt = (uint32_t)(gt.dimx * gt.dimy);
+        for (uint32_t k = lane; k < ns; k += 64) sl[k] = (k % gs.dimx) | ((k / gs.dimx) << 16);
+        for (uint32_t k = lane; k < nt; k += 64) tl[k] = (k % gt.dimx) | ((k / gt.dimx) << 16);
+        wave_sync();
+    }
+    int i = 0, j = 0;
+    bool final_level = false;
+    for (;;) {
+        const double s_step = lgrid(S, hs, i).step, t_step = lgrid(S, ht, j).step;
+        bool s_next = false, t_next = false;
+        if (i < s_level && (s_step >= t_step || j >= t_level)) {
+            i++;
+            s_next = true;
+        }
+        if (j < t_level && (s_step <= t_step || i >= s_level)) {
+            j++;
+            t_next = true;
+        }
+        if (ns == 0 || nt == 0) break;  // the queue of pixel pairs is empty
+        final_level = !s_next && !t_next;
+        const LGrid gs = lgrid(S, hs, i), gt = lgrid(S, ht, j);
+        const int npix_s = gs.dimx * gs.dimy, npix_t = gt.dimx * gt.dimy;
+        clear_bits(sb, npix_s, lane);
+        clear_bits(tb, npix_t, lane);
+        expand_side(sl, ns, s_next, s_next ? lgrid(S, hs, i - 1) : gs, gs, sb, lane);
+        expand_side(tl, nt, t_next, t_next ? lgrid(S, ht, j - 1) : gt, gt, tb, lane);
+        ns = compact(sb, npix_s, gs.dimx, sl, lane);
+        nt = compact(tb, npix_t, gt.dimx, tl, lane);
+        if (ns == 0 || nt == 0) break;  // no BORDER pixel on a side: nothing to refine
+        // pass 1: any BORDER box pair within d by its farthest corners
+        // decides; else the smallest such maximum bounds the useful pairs
+        double maxd = kFar;
+        bool hit = false;
+        const uint64_t total = (uint64_t)ns * nt;
+        {
+            uint32_t a = lane / nt, b = lane - (lane / nt) * nt;
+            const uint32_t q = 64 / nt, rr = 64 - q * nt;
+            while (a < ns) {
+                const uint32_t va = sl[a], vb = tl[b];
+                double b1[4], b2[4];
+                pbox(gs, (int)(va & 0xffffu), (int)(va >> 16), b1);
+                pbox(gt, (int)(vb & 0xffffu), (int)(vb >> 16), b2);
+                const double mx = box_max_dist(b1, b2);
+                hit |= (mx <= d);
+                maxd = fmin(maxd, mx);
+                a += q;
+                b += rr;
+                if (b >= nt) {
+                    b -= nt;
+                    a++;
+                }
+            }
+        }
+        if (wave_any(hit)) {
+            r.bit = 1;
+            return r;
+        }
+        maxd = wave_min(maxd);
+        // pass 2: keep pairs with min distance <= d and below maxd; their
+        // projections are the next level's sets, or (last level) they are
+        // the candidates
+        clear_bits(sb, npix_s, lane);
+        clear_bits(tb, npix_t, lane);
+        uint32_t nc = 0;
+        bool over = false;
+        const double sa = gs.step * gs.step, ta = gt.step * gt.step;
+        {
+            uint32_t a = lane / nt, b = lane - (lane / nt) * nt;
+            const uint32_t q = 64 / nt, rr = 64 - q * nt;
+            const uint64_t rounds = (total + 63) / 64;
+            for (uint64_t it = 0; it < rounds; it++) {
+                bool keep = false;
+                Cand c;
+                if (a < ns) {
+                    const uint32_t va = sl[a], vb = tl[b];
+                    const int xa = (int)(va & 0xffffu), ya = (int)(va >> 16);
+                    const int xb = (int)(vb & 0xffffu), yb = (int)(vb >> 16);
+                    double b1[4], b2[4];
+                    pbox(gs, xa, ya, b1);
+                    pbox(gt, xb, yb, b2);
+                    const double mn = box_min_dist(b1, b2);
+                    if (mn <= d && mn < maxd) {
+                        keep = true;
+                        const uint32_t id1 = (uint32_t)(ya * gs.dimx + xa), id2 = (uint32_t)(yb * gt.dimx + xb);
+                        set_bit(sb, id1);
+                        set_bit(tb, id2);
+                        if (final_level) {
+                            const double hi = box_max_dist(b1, b2);
+                            const int pf = classify(S.areas[hs.pix_start + id1] + S.areas[ht.pix_start + id2],
+                                                    sa + ta, 20);
+                            c.apx = mn + (double)c_mean[pf] * (hi - mn);
+                            c.lo = mn;
+                            c.hi = hi;
+                            c.id1 = id1;
+                            c.id2 = id2;
+                        }
+                    }
+                }
+                if (final_level) {
+                    const uint64_t m = __ballot(keep);
+                    const uint32_t pos = nc + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    if (keep) {
+                        if (pos < candcap) cand[pos] = c;
+                        else over = true;
+                    }
+                    nc += (uint32_t)__popcll(m);
+                }
+                a += q;
+                b += rr;
+                if (b >= nt) {
+                    b -= nt;
+                    a++;
+                }
+            }
+        }
+        wave_sync();
+        if (final_level) {
+            if (wave_any(over)) {
+                r.bit = 3;
+                return r;
+            }
+            r.ncand = nc;
+            break;
+        }
+        ns = compact(sb, npix_s, gs.dimx, sl, lane);
+        nt = compact(tb, npix_t, gt.dimx, tl, lane);
+    }
+    if (!final_level || r.ncand == 0) return r;
+
+    // refinement in tuple order
+    const uint32_t nc = r.ncand;
+    double min_dist = kFar;
+    const float *sxv = S.x + hs.vert_start, *syv = S.y + hs.vert_start;
+    const float *txv = S.x + ht.vert_start, *tyv = S.y + ht.vert_start;
+    for (uint32_t step = 0; step < nc; step++) {
+        Cand c;
+        const int slot = select_next(cand, nc, lane, c);
+        if (slot < 0) break;
+        if (lane == 0) cand[slot].apx = INFINITY;
+        wave_sync();
+        // reference_dist: smallest low bound among the candidates after it
+        double ref = kFar;
+        for (uint32_t k = lane; k < nc; k += 64) {
+            const Cand e = cand[k];
+            if (e.apx != INFINITY) ref = fmin(ref, e.lo);
+        }
+        ref = wave_min(ref);
+        const uint32_t s0 = S.pixoff[hs.pixoff_start + c.id1], s1 = S.pixoff[hs.pixoff_start + c.id1 + 1];
+        const uint32_t t0 = S.pixoff[ht.pixoff_start + c.id2], t1 = S.pixoff[ht.pixoff_start + c.id2 + 1];
+        for (uint32_t a = s0; a < s1; a++) {
+            const uint2 er1 = S.seqs[hs.seq_start + a];
+            for (uint32_t b = t0; b < t1; b++) {
+                const uint2 er2 = S.seqs[ht.seq_start + b];
+                if (er1.y < 2 || er2.y < 2) continue;
+                const double d2 = seq_pair_d2(txv + er2.x, tyv + er2.x, er2.y, sxv + er1.x, syv + er1.x, er1.y, lane);
+                r.evals += (uint64_t)er2.y * (er1.y - 1) + (uint64_t)er1.y * (er2.y - 1);
+                const double dist = sqrt(d2);
+                min_dist = min_dist < dist ? min_dist : dist;
+                r.min_dist = min_dist;
+                if (min_dist < d) {
+                    r.bit = 1;
+                    return r;
+                }
+                if (min_dist < ref) return r;
+            }
+        }
+    }
+    return r;
+}
+
+}  // namespace
+
+// Persistent waves take pairs in chunks from a global counter.  BIG: the
+// pair indices come from `list` (pairs the LDS path could not hold) and the
+// lists / bits live in a per-wave global slab.
+template <bool BIG>
+__global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__restrict__ pa,
+                                                const uint32_t *__restrict__ pb, uint32_t n,
+                                                const uint32_t *__restrict__ list, double d, uint8_t *__restrict__ out,
+                                                double *__restrict__ md, char *__restrict__ slab, uint64_t slab_stride,
+                                                uint32_t pixcap, uint32_t candcap, uint32_t *__restrict__ ctr,
+                                                uint32_t *__restrict__ ovf, unsigned long long *__restrict__ stats) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const uint64_t gw = (uint64_t)blockIdx.x * kWaves + wv;
+    uint32_t *sl, *tl, *sb, *tb;
+    Cand *cand;
+    if constexpr (!BIG) {
+        __shared__ uint32_t lists[kWaves][2][kPixMax];
+        __shared__ uint32_t bits[kWaves][2][kBitWords];
+        sl = lists[wv][0];
+        tl = lists[wv][1];
+        sb = bits[wv][0];
+        tb = bits[wv][1];
+        cand = reinterpret_cast<Cand *>(slab + gw * slab_stride);
+    } else {
+        char *base = slab + gw * slab_stride;
+        const uint64_t wbits = ((uint64_t)pixcap + 31) / 32;
+        sl = reinterpret_cast<uint32_t *>(base);
+        tl = sl + pixcap;
+        sb = tl + pixcap;
+        tb = sb + wbits;
+        cand = reinterpret_cast<Cand *>(base + (((2ull * pixcap + 2 * wbits) * 4 + 255) & ~255ull));
+    }
+    unsigned long long ncand = 0, evals = 0, found = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ctr[0], kChunk);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t end = min(base + kChunk, n);
+        for (uint32_t k = base; k < end; k++) {
+            const uint32_t idx = BIG ? list[k] : k;
+            const uint32_t ia = pa[idx], ib = pb[idx];
+            const farm_raster_hdr &hs = S.hdr[ia];
+            const farm_raster_hdr &ht = S.hdr[ib];
+            if (((hs.flags | ht.flags) & FARM_HDR_REJECTED) || hs.nlayers <= 0 || ht.nlayers <= 0) {
+                if (lane == 0) {
+                    out[idx] = 2;
+                    md[idx] = kFar;
+                }
+                continue;
+            }
+            const PairOut r = within_pair(S, ia, ib, d, sl, tl, sb, tb, cand, pixcap, candcap, lane);
+            if (r.bit == 3) {
+                if (lane == 0) {
+                    if (BIG) {
+                        out[idx] = 3;
+                    } else {
+                        const uint32_t slot = atomicAdd(&ctr[1], 1u);
+                        ovf[slot] = idx;
+                    }
+                }
+                continue;
+            }
+            ncand += r.ncand;
+            evals += r.evals;
+            found += r.bit;
+            if (lane == 0) {
+                out[idx] = r.bit;
+                md[idx] = r.min_dist;
+            }
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[0], ncand);
+        atomicAdd(&stats[1], evals);
+        atomicAdd(&stats[2], found);
+    }
+}
+
+template __global__ void k_within<false>(DevSet, const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, double,
+                                         uint8_t *, double *, char *, uint64_t, uint32_t, uint32_t, uint32_t *,
+                                         uint32_t *, unsigned long long *);
+template __global__ void k_within<true>(DevSet, const uint32_t *, const uint32_t *, uint32_t, const uint32_t *, double,
+                                        uint8_t *, double *, char *, uint64_t, uint32_t, uint32_t, uint32_t *,
+                                        uint32_t *, unsigned long long *);
+
+}  // namespace farm
+
+using namespace farm;
+
+namespace {
+
+int within_consts(farm_handle h) {
+    if (h->within_consts_ready) return FARM_OK;
+    static const float mean[20] = {0.0f,  0.7f,  0.62f, 0.55f, 0.52f, 0.45f, 0.42f, 0.4f,  0.38f, 0.34f,
+                                   0.31f, 0.28f, 0.25f, 0.22f, 0.18f, 0.15f, 0.1f,  0.05f, 0.0f,  0.0f};
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_km_lon), km_lon_table(), sizeof(double) * 900));
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_mean), mean, sizeof mean));
+    h->within_consts_ready = true;
+    return FARM_OK;
+}
+
+int run_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out_dev, double *md_dev,
+               farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
+    if (!h->has_layers) return fail(h, FARM_ESTATE, "within needs a set rasterized with layers (-h)");
+    if (begin > end || end > h->npairs)
+        return fail(h, FARM_EINVAL, "pair window [%llu,%llu) outside [0,%llu)", (unsigned long long)begin,
+                    (unsigned long long)end, (unsigned long long)h->npairs);
+    if (!(d_km >= 0)) return fail(h, FARM_EINVAL, "distance must be >= 0");
+    const uint64_t n64 = end - begin;
+    if (n64 > 0xffffffffull) return fail(h, FARM_ECAPACITY, "batch larger than 2^32 pairs");
+    const uint32_t n = (uint32_t)n64;
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc;
+    if ((rc = within_consts(h))) return rc;
+    if (st) memset(st, 0, sizeof *st);
+    if (st) st->pairs = n;
+    if (n == 0) return FARM_OK;
+    const uint32_t *pa = P<uint32_t>(h->pa) + begin;
+    const uint32_t *pb = P<uint32_t>(h->pb) + begin;
+    // counters: [0] queue head, [1] overflow count, [2..3] big-path head; stats
+    if ((rc = ensure(h, h->wctr, 64))) return rc;
+    if ((rc = ensure(h, h->wovf, 4ull * n))) return rc;
+    uint32_t *ctr = P<uint32_t>(h->wctr);
+    unsigned long long *wst = reinterpret_cast<unsigned long long *>(ctr + 8);
+    if ((rc = fill_u32(h, ctr, 16, 0))) return rc;
+    const unsigned nblocks = (unsigned)h->num_cus * 4;
+    const uint64_t nwaves = (uint64_t)nblocks * kWaves;
+    const uint64_t stride = (uint64_t)kCandLds * sizeof(Cand);
+    if ((rc = ensure(h, h->wslab, nwaves * stride))) return rc;
+    DevSet S = devset(h);
+    HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+    hipLaunchKernelGGL(k_within<false>, dim3(nblocks), dim3(64 * kWaves), 0, h->stream, S, pa, pb, n,
+                       (const uint32_t *)nullptr, d_km, out_dev, md_dev, P<char>(h->wslab), stride,
+                       (uint32_t)kPixMax, kCandLds, ctr, P<uint32_t>(h->wovf), wst);
+    HIPCHK(h, hipGetLastError());
+    uint32_t novf = 0;
+    HIPCHK(h, hipMemcpyAsync(&novf, ctr + 1, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (novf) {
+        // pairs whose layers or candidates exceed the LDS path: the same
+        // algorithm with per-wave global slabs sized for the largest layer
+        // of the set and a candidate bound of the border pixel counts
+        const uint32_t pixcap = (uint32_t)std::max<uint64_t>(h->max_layer_pix, 1);
+        const uint64_t candcap = std::min<uint64_t>(h->max_border_pix * h->max_border_pix + 64, 1ull << 24);
+        const uint64_t wbits = (pixcap + 31) / 32;
+        const uint64_t head = (((2ull * pixcap + 2 * wbits) * 4 + 255) & ~255ull);
+        const uint64_t bstride = head + ((candcap * sizeof(Cand) + 255) & ~255ull);
+        uint64_t bwaves = std::min<uint64_t>((uint64_t)h->num_cus * 4, std::max<uint64_t>(1, (4ull << 30) / bstride));
+        bwaves = std::min<uint64_t>(bwaves, ((uint64_t)novf + kWaves - 1) / kWaves * kWaves);
+        const unsigned bblocks = (unsigned)std::max<uint64_t>(1, bwaves / kWaves);
+        if ((rc = ensure(h, h->wslab_big, (uint64_t)bblocks * kWaves * bstride))) return rc;
+        if ((rc = fill_u32(h, ctr, 1, 0))) return rc;
+        hipLaunchKernelGGL(k_within<true>, dim3(bblocks), dim3(64 * kWaves), 0, h->stream, S, pa, pb, novf,
+                           P<uint32_t>(h->wovf), d_km, out_dev, md_dev, P<char>(h->wslab_big), bstride, pixcap,
+                           (uint32_t)candcap, ctr, P<uint32_t>(h->wovf), wst);
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+    unsigned long long hst[3] = {0, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(hst, wst, sizeof hst, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (novf) {
+        // a pair that even the global slabs cannot hold is an error, not a
+        // silent answer
+        std::vector<uint8_t> tmp(n);
+        HIPCHK(h, hipMemcpy(tmp.data(), out_dev, n, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < n; k++)
+            if (tmp[k] == 3)
+                return fail(h, FARM_ECAPACITY, "pair %llu: more candidate pixel pairs than the within buffers hold",
+                            (unsigned long long)(begin + k));
+    }
+    if (st) {
+        float ms = 0;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+        st->bb_pixel_pairs = hst[0];
+        st->segment_tests = hst[1];
+        st->found = hst[2];
+        st->refine_ms = ms;
+        st->total_ms = ms;
+    }
+    return FARM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int farm_within_dev(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out_dev, double *md_dev,
+                    farm_stats *st) {
+    if (!out_dev || !md_dev) return fail(h, FARM_EINVAL, "null output buffer");
+    return run_within(h, begin, end, d_km, out_dev, md_dev, st);
+}
+
+int farm_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out, double *min_dist,
+                farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (begin > end) return fail(h, FARM_EINVAL, "bad pair window");
+    const uint64_t n = end - begin;
+    int rc;
+    if ((rc = ensure(h, h->wout, n + 16))) return rc;
+    if ((rc = ensure(h, h->wmd, 8 * n + 16))) return rc;
+    if ((rc = run_within(h, begin, end, d_km, P<uint8_t>(h->wout), P<double>(h->wmd), st))) return rc;
+    if (n && out) HIPCHK(h, hipMemcpy(out, h->wout.p, n, hipMemcpyDeviceToHost));
+    if (n && min_dist) HIPCHK(h, hipMemcpy(min_dist, h->wmd.p, 8 * n, hipMemcpyDeviceToHost));
+    return FARM_OK;
+}
+
+}  // extern "C"

@@ -33,6 +33,10 @@ struct farm_hostset {
     std::vector<uint32_t> pixoff, seqs, voff;
     std::vector<double> vnodes;
     std::vector<float> vx, vy;
+    // hierarchy (-h)
+    bool hier = false;
+    std::vector<farm_layer> layers;
+    std::vector<uint8_t> lstatus;
 };
 
 namespace {
@@ -315,11 +319,14 @@ int farm_hostset_mbbs(const farm_hostset *s, double *out) {
     return FARM_OK;
 }
 
-int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
-    if (vpr <= 0) return FARM_EINVAL;
+namespace {
+
+int rasterize_all(farm_hostset *s, int vpr, bool hier, int nlow, int nthreads) {
+    if (vpr <= 0 || (hier && nlow < 1)) return FARM_EINVAL;
     const int64_t n = npoly(s);
     if (nthreads < 1) nthreads = 1;
     std::vector<farm::RasterOut> outs(n);
+    std::vector<std::vector<farm::LayerOut>> lays(hier ? n : 0);
     std::atomic<int64_t> next(0);
     auto work = [&]() {
         farm::RasterScratch *sc = farm::raster_scratch_new();
@@ -332,7 +339,11 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
                 ring.assign(s->xy.begin() + 2 * r0, s->xy.begin() + 2 * r1);
                 ring.push_back(ring[0]);  // VertexSequence::decode appends p[0]
                 ring.push_back(ring[1]);
-                farm::build_raster(ring.data(), (int)(r1 - r0 + 1), &s->mbb[4 * i], vpr, outs[i], sc);
+                if (hier)
+                    farm::build_raster_hier(ring.data(), (int)(r1 - r0 + 1), &s->mbb[4 * i], vpr, nlow, outs[i],
+                                            lays[i], sc);
+                else
+                    farm::build_raster(ring.data(), (int)(r1 - r0 + 1), &s->mbb[4 * i], vpr, outs[i], sc);
             }
         }
         farm::raster_scratch_free(sc);
@@ -343,8 +354,9 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
 
     // pack: prefix sums, then parallel copy
     s->hdr.assign(n, farm_raster_hdr());
-    uint64_t np = 0, npo = 0, nsq = 0, nvt = 0, nvl = 0, nvn = 0;
+    uint64_t np = 0, npo = 0, nsq = 0, nvt = 0, nvl = 0, nvn = 0, nly = 0, nls = 0;
     s->raster_fail = -1;
+    s->layers.clear();
     s->raster_ok.assign(n, 1);
     for (int64_t i = 0; i < n; i++) {
         const farm::RasterOut &o = outs[i];
@@ -370,6 +382,24 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
         h.nseqs = (uint32_t)(o.seqs.size() / 2);
         h.vline_start = nvl;
         h.vnode_start = nvn;
+        h.flags = o.ok ? 0 : FARM_HDR_REJECTED;
+        if (hier) {
+            const auto &L = lays[i];
+            h.layer_start = nly;
+            h.nlayers = (int32_t)L.size();
+            for (size_t k = 0; k + 1 < L.size(); k++) {
+                farm_layer l;
+                l.low[0] = L[k].g.low[0];
+                l.low[1] = L[k].g.low[1];
+                l.step = L[k].g.step[0];
+                l.dimx = L[k].g.dimx;
+                l.dimy = L[k].g.dimy;
+                l.status_start = nls;
+                s->layers.push_back(l);
+                nls += L[k].status.size();
+                nly++;
+            }
+        }
         np += o.status.size();
         npo += o.pixoff.size();
         nsq += o.seqs.size() / 2;
@@ -385,6 +415,7 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
     s->vy.assign(nvt, 0.f);
     s->voff.assign(nvl, 0);
     s->vnodes.assign(nvn, 0.0);
+    s->lstatus.assign(nls, 0);
     next = 0;
     auto pack = [&]() {
         for (;;) {
@@ -407,6 +438,14 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
                 }
                 s->vx[h.vert_start + nvs] = s->xy[2 * r0];
                 s->vy[h.vert_start + nvs] = s->xy[2 * r0 + 1];
+                if (hier) {
+                    for (int k = 0; k + 1 < h.nlayers; k++) {
+                        const auto &st = lays[i][k].status;
+                        std::copy(st.begin(), st.end(),
+                                  s->lstatus.begin() + s->layers[h.layer_start + k].status_start);
+                    }
+                    std::vector<farm::LayerOut>().swap(lays[i]);
+                }
                 farm::RasterOut().status.swap(o.status);  // release early
                 std::vector<double>().swap(o.areas);
                 std::vector<uint32_t>().swap(o.seqs);
@@ -417,7 +456,16 @@ int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) {
     for (int t = 0; t < nthreads; t++) th.emplace_back(pack);
     for (auto &t : th) t.join();
     s->rastered = true;
+    s->hier = hier;
     return FARM_OK;
+}
+
+}  // namespace
+
+int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads) { return rasterize_all(s, vpr, false, 1, nthreads); }
+
+int farm_hostset_rasterize_h(farm_hostset *s, int vpr, int nlow, int nthreads) {
+    return rasterize_all(s, vpr, true, nlow, nthreads);
 }
 
 int farm_hostset_view(const farm_hostset *s, farm_polyset *o) {
@@ -437,6 +485,31 @@ int farm_hostset_view(const farm_hostset *s, farm_polyset *o) {
     o->nvoff = s->voff.size();
     o->vnodes = s->vnodes.data();
     o->nvnodes = s->vnodes.size();
+    // layers and per-pixel areas are uploaded for within queries only
+    o->layers = s->hier ? s->layers.data() : nullptr;
+    o->nlayers = s->hier ? s->layers.size() : 0;
+    o->lstatus = s->hier ? s->lstatus.data() : nullptr;
+    o->nlstatus = s->hier ? s->lstatus.size() : 0;
+    o->areas = s->hier ? s->areas.data() : nullptr;
+    return FARM_OK;
+}
+
+int farm_hostset_layer(const farm_hostset *s, int64_t i, int k, farm_layer *out, const uint8_t **status) {
+    if (!s->rastered || !s->hier || i < 0 || i >= npoly(s)) return FARM_EINVAL;
+    const farm_raster_hdr &h = s->hdr[i];
+    if (k < 0 || k >= h.nlayers) return FARM_EINVAL;
+    if (k == h.nlayers - 1) {
+        out->low[0] = h.mbr[0];
+        out->low[1] = h.mbr[1];
+        out->step = h.step[0];
+        out->dimx = h.dimx;
+        out->dimy = h.dimy;
+        out->status_start = h.pix_start;
+        *status = s->status.data() + h.pix_start;
+    } else {
+        *out = s->layers[h.layer_start + k];
+        *status = s->lstatus.data() + out->status_start;
+    }
     return FARM_OK;
 }
 
@@ -588,6 +661,10 @@ struct Dpk {
 const Dpk kDpk;
 
 }  // namespace
+
+namespace farm {
+const double *km_lon_table() { return kDpk.v; }
+}  // namespace farm
 
 extern "C" int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b, double expand_km, int skip_self,
                                    int nthreads, uint32_t **pa, uint32_t **pb) {

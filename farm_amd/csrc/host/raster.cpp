@@ -164,11 +164,9 @@ inline bool within(double lo, double hi, double v) { return !(v < lo || v > hi);
 
 }  // namespace
 
-static void build_raster_impl(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out,
-                              RasterScratch *S) {
+// the raster on a given grid (out.g)
+static void build_raster_impl(const float *ring, int nv, RasterOut &out, RasterScratch *S) {
     out.ok = true;
-    const int npix_req = nv / vpr;
-    out.g = size_grid(mbb, npix_req);
     const RasterGrid &g = out.g;
     const int dimx = g.dimx, dimy = g.dimy;
     const int npix = dimx * dimy;
@@ -414,7 +412,9 @@ static void build_raster_impl(const float *ring, int nv, const double mbb[4], in
 
     // ---- fill classes: clip to each column strip, then to each pixel ----
     const double pix_area = sx * sy;
-    const bool exact_dedup = (mbb[0] > 0.0160 && mbb[1] > 0.0160);
+    // every clip point lies in the grid box; above 0.016 distinct floats are
+    // more than 1e-9 apart, so the tolerant dedup is an exact one there
+    const bool exact_dedup = (x0 > 0.0160 && y0 > 0.0160);
     Dedup dd{exact_dedup, &S->hash};
     auto &strips = S->strips;
     auto &sstart = S->strip_start;
@@ -561,9 +561,152 @@ static void build_raster_impl(const float *ring, int nv, const double mbb[4], in
     }
 }
 
+static void make_empty(RasterOut &out);
+
 void build_raster(const float *ring, int nv, const double mbb[4], int vpr, RasterOut &out, RasterScratch *S) {
-    build_raster_impl(ring, nv, mbb, vpr, out, S);
-    if (out.ok) return;
+    out.g = size_grid(mbb, nv / vpr);
+    build_raster_impl(ring, nv, out, S);
+    if (!out.ok) make_empty(out);
+}
+
+namespace {
+
+// UniversalGrid::configure(25) (UniversalGrid.h:29-35): 360 / 2^25 rounded
+// to two significant digits (roundToSignificantDigits, util.h:51-63)
+double universal_step() {
+    const double s = 360.0 / std::pow(2, 25);
+    const double d = std::ceil(std::log10(s));
+    const int power = 2 - static_cast<int>(d);
+    const double mag = std::pow(10.0, power);
+    return std::round(s * mag) / mag;
+}
+
+// MyRaster::grid_align (MyRaster.cpp:444-491): snap both steps to the
+// nearer of the two bracketing universal steps 1.1e-5 * 2^k, take the
+// smaller as the square pixel size, floor/ceil the box to it
+bool align_grid(RasterGrid &g) {
+    const double us = universal_step();
+    double x = us, y = us;
+    bool fx = false, fy = false;
+    if (g.step[0] <= x) g.step[0] = x, fx = true;
+    if (g.step[1] <= y) g.step[1] = y, fy = true;
+    for (int i = 25; i > 0; i--) {
+        if (!fx && g.step[0] > x && g.step[0] < x * 2) {
+            fx = true;
+            g.step[0] = std::abs(g.step[0] - x) < std::abs(g.step[0] - x * 2) ? x : x * 2;
+        }
+        if (!fy && g.step[1] > y && g.step[1] < y * 2) {
+            fy = true;
+            g.step[1] = std::abs(g.step[1] - y) < std::abs(g.step[1] - y * 2) ? y : y * 2;
+        }
+        if (fx && fy) break;
+        x *= 2;
+        y *= 2;
+    }
+    if (!fx || !fy) return false;
+    const double s = std::min(g.step[0], g.step[1]);
+    g.step[0] = g.step[1] = s;
+    g.low[0] = std::floor(g.low[0] / s) * s;
+    g.low[1] = std::floor(g.low[1] / s) * s;
+    g.high[0] = std::ceil(g.high[0] / s) * s;
+    g.high[1] = std::ceil(g.high[1] / s) * s;
+    g.dimx = static_cast<int>(std::round((g.high[0] - g.low[0]) / s));
+    g.dimy = static_cast<int>(std::round((g.high[1] - g.low[1]) / s));
+    return g.dimx > 0 && g.dimy > 0;
+}
+
+inline int grid_off(double v, double lo, double step, int dim) {
+    return std::min(std::max(dbl2int((v - lo) / step), 0), dim - 1);
+}
+
+}  // namespace
+
+void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, int nlow, RasterOut &out,
+                       std::vector<LayerOut> &layers, RasterScratch *S) {
+    layers.clear();
+    out.g = size_grid(mbb, nv / vpr);
+    if (!align_grid(out.g)) {
+        out.ok = false;
+        make_empty(out);
+        layers.push_back({out.g, {}});
+        return;
+    }
+    build_raster_impl(ring, nv, out, S);
+    if (!out.ok) make_empty(out);
+    const RasterGrid &f = out.g;
+    // Ideal::layering (Ideal.cpp:1061-1099): coarser layers by doubling the
+    // step until the layer has at most nlow pixels
+    const int high_layers = static_cast<int>(
+        std::ceil(std::max(std::log(f.dimx) / std::log(2.0), std::log(f.dimy) / std::log(2.0))));
+    int low_layers = high_layers;
+    double dx = f.dimx, dy = f.dimy;
+    for (int i = high_layers - 1; i >= 0; i--) {
+        if (dx * dy <= nlow) break;
+        dx = static_cast<int>(std::ceil(dx / 2.0));
+        dy = static_cast<int>(std::ceil(dy / 2.0));
+        low_layers--;
+    }
+    const int nl = high_layers - low_layers + 1;
+    layers.resize(nl);
+    layers[nl - 1].g = f;
+    for (int i = nl - 2; i >= 0; i--) {
+        RasterGrid &g = layers[i].g;
+        const double s0 = layers[i + 1].g.step[0] * 2, s1 = layers[i + 1].g.step[1] * 2;
+        g.step[0] = s0;
+        g.step[1] = s1;
+        g.low[0] = std::floor(f.low[0] / s0) * s0;
+        g.low[1] = std::floor(f.low[1] / s1) * s1;
+        g.high[0] = std::ceil(f.high[0] / s0) * s0;
+        g.high[1] = std::ceil(f.high[1] / s1) * s1;
+        g.dimx = static_cast<int>(std::round((g.high[0] - g.low[0]) / s0));
+        g.dimy = static_cast<int>(std::round((g.high[1] - g.low[1]) / s1));
+        // Ideal::merge_status (Ideal.cpp:1040-1059): each coarse pixel sums the
+        // finest layer's clipped areas over the fine pixels it covers; it is
+        // BORDER if any of them is, else it takes the first one's class
+        std::vector<uint8_t> &st = layers[i].status;
+        st.assign((size_t)g.dimx * g.dimy, 0);
+        const double carea = s0 * s1;
+        for (int x = 0; x < g.dimx; x++) {
+            for (int y = 0; y < g.dimy; y++) {
+                const double lx = g.low[0] + x * s0, ly = g.low[1] + y * s1;
+                const double hx = g.low[0] + (x + 1) * s0, hy = g.low[1] + (y + 1) * s1;
+                const int x0 = grid_off(lx, f.low[0], f.step[0], f.dimx);
+                const int y0 = grid_off(ly, f.low[1], f.step[1], f.dimy);
+                int x1 = grid_off(hx, f.low[0], f.step[0], f.dimx) - 1;
+                int y1 = grid_off(hy, f.low[1], f.step[1], f.dimy) - 1;
+                if (y1 < y0) y1 = y0;
+                if (x1 < x0) x1 = x0;
+                double area = 0.0;
+                bool border = false;
+                for (int a = x0; a <= x1; a++)
+                    for (int b = y0; b <= y1; b++) {
+                        const uint8_t c = out.status[b * f.dimx + a];
+                        if (c != 0 && c != kCC - 1) border = true;
+                        area += out.areas[b * f.dimx + a];
+                    }
+                uint8_t cls;
+                if (border) {
+                    // classifyPixel(area, pixelArea, 16) (geometry_computation.h:299-321)
+                    const double ratio = area / carea;
+                    if (std::fabs(ratio - 1.0) < 1e-9) cls = kCC - 1;
+                    else if (std::fabs(ratio) < 1e-9) cls = 0;
+                    else {
+                        int idx = static_cast<int>(std::ceil(ratio * (kCC - 2)));
+                        if (idx >= kCC) idx = kCC - 1;
+                        cls = (uint8_t)idx;
+                    }
+                } else {
+                    const uint8_t c = out.status[y0 * f.dimx + x0];
+                    cls = (c == kCC - 1) ? kCC - 1 : 0;
+                }
+                st[(size_t)y * g.dimx + x] = cls;
+            }
+        }
+    }
+}
+
+static void make_empty(RasterOut &out) {
+    out.ok = false;
     // The reference raster builder asserts or never terminates on this ring.
     // Keep the grid but make the polygon empty (every pixel OUT, no edge
     // sequences) so that no query reads a half-built raster; the failure is

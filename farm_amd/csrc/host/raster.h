@@ -36,4 +36,18 @@ void build_raster(const float *ring, int nv, const double mbb[4], int vpr, Raste
 // Grid sizing of MyRaster::init_raster(int) for num_pixels pixels.
 RasterGrid size_grid(const double mbb[4], int num_pixels);
 
+// One coarse layer of the hierarchy (-h): its grid and fill classes.
+struct LayerOut {
+    RasterGrid g;
+    std::vector<uint8_t> status;
+};
+
+// -h rasterisation: grid snapped to the universal 1.1e-5 * 2^k steps (square
+// pixels, MBB floored/ceiled to the grid), the raster built on it, and the
+// coarse layers (step doubled until dimx*dimy <= nlow) with merged fill
+// classes.  layers is coarse to fine; its last entry is out's own grid (with
+// an empty status: the finest layer's status is out.status).
+void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, int nlow, RasterOut &out,
+                       std::vector<LayerOut> &layers, RasterScratch *s);
+
 }  // namespace farm

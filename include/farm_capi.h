@@ -60,8 +60,26 @@ typedef struct farm_raster_hdr {
     uint32_t nseqs;
     uint64_t vline_start;   /* first entry in voff (dimx + 2 entries) */
     uint64_t vnode_start;   /* first entry in vnodes */
-    uint64_t reserved[2];
+    /* --- hierarchy (-h), replaces Ideal::layers (src/include/Ideal.h:66) --- */
+    uint64_t layer_start;   /* first coarse layer in farm_polyset.layers */
+    int32_t nlayers;        /* layers incl. this finest grid (coarse ones first); 0 = none */
+    int32_t flags;          /* FARM_HDR_REJECTED: the reference raster builder aborts here */
 } farm_raster_hdr;
+
+#define FARM_HDR_REJECTED 1
+
+/*
+ * One coarse layer of a polygon's hierarchy (Hraster, src/include/Hraster.h):
+ * a square-pixel grid and its fill classes.  Layer k (0 = coarsest) of
+ * polygon i is layers[hdr[i].layer_start + k] for k < nlayers - 1; the
+ * finest layer is the polygon's own raster.
+ */
+typedef struct farm_layer {
+    double low[2];          /* grid origin (low x, low y) */
+    double step;            /* pixel side */
+    int32_t dimx, dimy;
+    uint64_t status_start;  /* first byte in farm_polyset.lstatus */
+} farm_layer;
 
 /*
  * A rasterised polygon set in device layout (SoA of CSR), built on the host.
@@ -90,6 +108,13 @@ typedef struct farm_polyset {
     uint64_t nvoff;
     const double *vnodes;
     uint64_t nvnodes;
+    /* hierarchy and per-pixel areas: needed by farm_within only (NULL / 0
+     * for intersect-only sets) */
+    const farm_layer *layers;
+    uint64_t nlayers;
+    const uint8_t *lstatus;
+    uint64_t nlstatus;
+    const double *areas;    /* clipped polygon area per finest pixel, like status */
 } farm_polyset;
 
 /* Query modes of farm_intersect. */
@@ -158,6 +183,22 @@ int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_o
  * the learned part).  No reference counterpart. */
 int farm_intersect_costs(farm_handle h, uint64_t begin, uint64_t end, uint8_t *bits, uint32_t *tests,
                          farm_stats *stats);
+
+/* Within-distance decision over pairs [begin, end) (replaces
+ * cuda_within_polygon, src/cuda/query_within_polygon.cu:659-844, with the
+ * CPU semantics of Ideal::within, src/geometry/Ideal.cpp:2475-2643): the
+ * hierarchical box-distance descent over both polygons' layers, then
+ * candidate pixel pairs in dist_apx order refined with the truncated
+ * point-segment distances.  Needs a set uploaded with layers and areas.
+ * out[k - begin]: 1 within d_km, 0 not, 2 a polygon of the pair is
+ * rejected by the reference raster builder.  min_dist (may be NULL): the
+ * running minimum segment distance when the decision was taken (100000 when
+ * no distance was computed).  The device-pointer variant takes device
+ * buffers of end - begin entries. */
+int farm_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out, double *min_dist,
+                farm_stats *stats);
+int farm_within_dev(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out_dev,
+                    double *min_dist_dev, farm_stats *stats);
 
 /* ---- learned part (no reference counterpart) ----
  * Pair features: FARM_NFEAT floats per candidate pair from the filter's

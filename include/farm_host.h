@@ -48,6 +48,14 @@ int farm_hostset_mbbs(const farm_hostset *s, double *out4n);
 /* Rasterize every polygon with vpr vertices per pixel (-v), nthreads host
  * threads (-n).  Fills the device layout returned by farm_hostset_view. */
 int farm_hostset_rasterize(farm_hostset *s, int vpr, int nthreads);
+/* -h: the raster on the grid snapped to the universal steps plus the coarse
+ * layers down to at most nlow pixels (--NLow), with per-pixel areas
+ * (MyRaster::grid_align, MyRaster.cpp:444-491; Ideal::layering and
+ * merge_status, Ideal.cpp:1008-1099).  Needed by farm_within. */
+int farm_hostset_rasterize_h(farm_hostset *s, int vpr, int nlow, int nthreads);
+/* Layer k (0 = coarsest, nlayers - 1 = the raster itself) of polygon i of a
+ * set rasterized with farm_hostset_rasterize_h, for tests. */
+int farm_hostset_layer(const farm_hostset *s, int64_t i, int k, farm_layer *out, const uint8_t **status);
 int farm_hostset_view(const farm_hostset *s, farm_polyset *out);
 /* Per-pixel clipped areas of polygon i (dimx*dimy doubles), for tests. */
 int farm_hostset_areas(const farm_hostset *s, int64_t i, const double **areas);

@@ -35,8 +35,15 @@ def _load() -> C.CDLL:
             [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p],
         ),
         "orc_candidates": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+        "orc_within": (
+            C.c_int,
+            [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+             C.c_int, C.c_void_p],
+        ),
+        "orc_candidates_within": (C.c_int64, [C.c_void_p, C.c_double, C.c_int64, C.c_void_p, C.c_void_p]),
         "orc_mbb": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
         "orc_failed": (C.c_int, [C.c_void_p, C.c_int64]),
+        "orc_layer": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_segment_intersect": (C.c_int, [C.c_void_p]),
         "orc_double_to_int": (C.c_int, [C.c_double]),
         "orc_point_to_segment_distance": (C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
@@ -112,6 +119,24 @@ class OracleSet:
                             _p(r["vnodes"]), _p(r["hoff"]), _p(r["hnodes"]), _p(r["areas"]))
         return r
 
+    def nlayers(self, i: int) -> int:
+        low = np.zeros(2)
+        step = np.zeros(1)
+        dims = np.zeros(2, np.int32)
+        return int(lib.orc_layer(self._h, i, -1, _p(low), _p(step), _p(dims), None))
+
+    def layer(self, i: int, k: int) -> dict:
+        """Layer k (0 = coarsest) of polygon i after rasterize(hierarchy=True)."""
+        low = np.zeros(2)
+        step = np.zeros(1)
+        dims = np.zeros(2, np.int32)
+        nl = lib.orc_layer(self._h, i, k, _p(low), _p(step), _p(dims), None)
+        if nl < 0 or k >= nl:
+            raise IndexError((i, k))
+        st = np.zeros(int(dims[0]) * int(dims[1]), np.uint8)
+        lib.orc_layer(self._h, i, k, _p(low), _p(step), _p(dims), _p(st))
+        return {"low": low, "step": float(step[0]), "dimx": int(dims[0]), "dimy": int(dims[1]), "status": st}
+
     def __del__(self):
         if getattr(self, "_h", None):
             lib.orc_destroy(self._h)
@@ -130,6 +155,29 @@ def intersect(a: OracleSet, b: OracleSet, pa: np.ndarray, pb: np.ndarray, mode: 
     if rc == ORC_EINVAL:
         raise ValueError("pair index out of range")
     return out, {"bb_pixel_pairs": int(stats[0]), "segment_tests": int(stats[1]), "in_hits": int(stats[2])}
+
+
+def within(a: OracleSet, b: OracleSet, pa: np.ndarray, pb: np.ndarray, d_km: float = 10.0,
+           nthreads: int = 8) -> tuple[np.ndarray, np.ndarray, dict]:
+    """Ideal::within per pair (sets rasterized with hierarchy=True).
+    Returns (bits (2 = reference aborts), running min distance, stats)."""
+    pa = np.ascontiguousarray(pa, dtype=np.uint32)
+    pb = np.ascontiguousarray(pb, dtype=np.uint32)
+    out = np.zeros(len(pa), dtype=np.uint8)
+    md = np.zeros(len(pa), dtype=np.float64)
+    stats = np.zeros(3, dtype=np.int64)
+    rc = lib.orc_within(a._h, b._h, len(pa), _p(pa), _p(pb), d_km, _p(out), _p(md), nthreads, _p(stats))
+    if rc == ORC_EINVAL:
+        raise ValueError("pair index out of range")
+    return out, md, {"candidate_pixel_pairs": int(stats[0]), "distance_evals": int(stats[1])}
+
+
+def candidates_within(s: OracleSet, d_km: float = 10.0) -> tuple[np.ndarray, np.ndarray]:
+    n = lib.orc_candidates_within(s._h, d_km, 0, None, None)
+    pa = np.zeros(max(n, 1), np.uint32)
+    pb = np.zeros(max(n, 1), np.uint32)
+    lib.orc_candidates_within(s._h, d_km, n, _p(pa), _p(pb))
+    return pa[:n], pb[:n]
 
 
 def candidates(a: OracleSet, b: OracleSet) -> tuple[np.ndarray, np.ndarray]:

@@ -375,7 +375,8 @@ struct Layer {
 
 struct Poly {
     std::vector<Pt> p; /* decoded ring: stored vertices + copy of p[0] */
-    Box mbr;
+    Box mbr;           /* getMBB(); grid_align (-h) moves it */
+    Box mbb0;          /* the MBB as loaded, which the R-tree indexes */
     Grid g;
     std::vector<uint8_t> status;
     std::vector<double> areas;
@@ -1092,6 +1093,125 @@ static bool intersect(const Poly &A, const Poly &B, int mode, QStats &qs) {
     return false;
 }
 
+/* Ideal::within (Ideal.cpp:2475-2643): hierarchical BFS over layer pairs,
+ * candidate pixel pairs ordered by dist_apx, truncated segment distances.
+ * `min_dist_out` receives the running minimum distance when refinement ran
+ * (100000 otherwise).  Requires rasterize(..., hierarchy = true). */
+static bool within(const Poly &S, const Poly &T, double d, QStats &qs, double *min_dist_out) {
+    *min_dist_out = 100000.0;
+    REF_ASSERT(S.hier && T.hier && !S.layers.empty() && !T.layers.empty());
+    const int s_level = (int)S.layers.size() - 1, t_level = (int)T.layers.size() - 1;
+    std::queue<std::pair<int, int>> pairs;
+    const Grid &s0 = S.layers[0].g, &t0 = T.layers[0].g;
+    for (int a = 0; a < s0.dimx * s0.dimy; a++)
+        for (int b = 0; b < t0.dimx * t0.dimy; b++) pairs.push({a, b});
+    int i = 0, j = 0;
+    while (true) {
+        std::unordered_set<int> s_pxs, t_pxs;
+        bool s_next = false, t_next = false;
+        const double s_step = S.layers[i].g.step_x, t_step = T.layers[j].g.step_x;
+        if (i < s_level && (s_step >= t_step || j >= t_level)) {
+            i++;
+            s_next = true;
+        }
+        if (j < t_level && (s_step <= t_step || i >= s_level)) {
+            j++;
+            t_next = true;
+        }
+        const int size = (int)pairs.size();
+        if (size == 0) break;
+        const Grid &sg = S.layers[i].g, &tg = T.layers[j].g;
+        std::vector<int> tmp;
+        for (int k = 0; k < size; k++) {
+            auto pr = pairs.front();
+            pairs.pop();
+            if (s_next) {
+                const Grid &pg = S.layers[i - 1].g;
+                Box b = pg.pixel_box(pg.gx(pr.first), pg.gy(pr.first));
+                b.low[0] += 1e-6;
+                b.low[1] += 1e-6;
+                b.high[0] -= 1e-6;
+                b.high[1] -= 1e-6;
+                sg.retrieve(b, tmp);
+                for (int p : tmp)
+                    if (sg.show(p) == ST_BORDER) s_pxs.insert(p);
+            } else if (sg.show(pr.first) == ST_BORDER) {
+                s_pxs.insert(pr.first);
+            }
+            if (t_next) {
+                const Grid &pg = T.layers[j - 1].g;
+                Box b = pg.pixel_box(pg.gx(pr.second), pg.gy(pr.second));
+                b.low[0] += 1e-6;
+                b.low[1] += 1e-6;
+                b.high[0] -= 1e-6;
+                b.high[1] -= 1e-6;
+                tg.retrieve(b, tmp);
+                for (int p : tmp)
+                    if (tg.show(p) == ST_BORDER) t_pxs.insert(p);
+            } else if (tg.show(pr.second) == ST_BORDER) {
+                t_pxs.insert(pr.second);
+            }
+        }
+        double max_box_dist = 100000.0;
+        for (int id1 : s_pxs) {
+            Box b1 = sg.pixel_box(sg.gx(id1), sg.gy(id1));
+            for (int id2 : t_pxs) {
+                Box b2 = tg.pixel_box(tg.gx(id2), tg.gy(id2));
+                double mx = box_max_distance(b1, b2);
+                if (mx <= d) return true;
+                max_box_dist = std::min(max_box_dist, mx);
+            }
+        }
+        for (int id1 : s_pxs) {
+            Box b1 = sg.pixel_box(sg.gx(id1), sg.gy(id1));
+            for (int id2 : t_pxs) {
+                Box b2 = tg.pixel_box(tg.gx(id2), tg.gy(id2));
+                double mn = box_distance(b1, b2);
+                if (mn > d) continue;
+                if (mn < max_box_dist) pairs.push({id1, id2});
+            }
+        }
+        if (!s_next && !t_next) break;
+    }
+    std::vector<std::tuple<double, double, double, int, int>> cand;
+    const double sa = S.g.pixel_area(), ta = T.g.pixel_area();
+    while (!pairs.empty()) {
+        auto pr = pairs.front();
+        pairs.pop();
+        const int id1 = pr.first, id2 = pr.second;
+        const int pf = classify_pixel(S.areas[id1] + T.areas[id2], sa + ta, 20);
+        Box b1 = S.g.pixel_box(S.g.gx(id1), S.g.gy(id1));
+        Box b2 = T.g.pixel_box(T.g.gx(id2), T.g.gy(id2));
+        const double lo = box_distance(b1, b2);
+        const double hi = box_max_distance(b1, b2);
+        const double apx = lo + kMean[pf] * (hi - lo);
+        cand.emplace_back(apx, lo, hi, id1, id2);
+    }
+    std::sort(cand.begin(), cand.end());
+    qs.bb_pairs += (int64_t)cand.size();
+    double min_dist = 100000.0;
+    for (size_t pid = 0; pid < cand.size(); pid++) {
+        double reference_dist = 100000;
+        for (size_t k = pid + 1; k < cand.size(); k++) reference_dist = std::min(reference_dist, std::get<1>(cand[k]));
+        const int id1 = std::get<3>(cand[pid]), id2 = std::get<4>(cand[pid]);
+        const uint32_t n1 = S.num_seqs(id1), n2 = T.num_seqs(id2);
+        for (uint32_t a = 0; a < n1; a++) {
+            auto er1 = S.seqs[S.offset[id1] + a];
+            for (uint32_t b = 0; b < n2; b++) {
+                auto er2 = T.seqs[T.offset[id2] + b];
+                if (er1.second < 2 || er2.second < 2) continue;
+                double dist = segment_to_segment_within_batch(T.p.data() + er2.first, S.p.data() + er1.first,
+                                                              er2.second, er1.second, qs.seg_tests);
+                min_dist = std::min(dist, min_dist);
+                *min_dist_out = min_dist;
+                if (min_dist < d) return true;
+                if (min_dist < reference_dist) return false;
+            }
+        }
+    }
+    return false;
+}
+
 } // namespace
 
 /* ---------------------------------------------------------------------- */
@@ -1117,6 +1237,7 @@ orc_set *orc_create(int64_t n, const uint64_t *ring_off, const float *xy) {
         for (uint64_t k = b; k < e; k++) P.p[k - b] = {xy[2 * k], xy[2 * k + 1]};
         P.p[e - b] = P.p[0];
         P.mbr = ring_mbr(P.p);
+        P.mbb0 = P.mbr;
         s->polys.push_back(std::move(P));
     }
     return s;
@@ -1205,6 +1326,24 @@ int orc_raster_copy(const orc_set *s, int64_t i, uint8_t *status, uint32_t *offs
     return ORC_OK;
 }
 
+int orc_layer(const orc_set *s, int64_t i, int k, double *low2, double *step, int *dims2, uint8_t *status) {
+    if (i < 0 || i >= (int64_t)s->polys.size()) return -1;
+    const Poly &P = s->polys[i];
+    const int nl = (int)P.layers.size();
+    if (k < 0 || k >= nl) return nl;
+    const Grid &g = P.layers[k].g;
+    low2[0] = g.mbr.low[0];
+    low2[1] = g.mbr.low[1];
+    *step = g.step_x;
+    dims2[0] = g.dimx;
+    dims2[1] = g.dimy;
+    if (status) {
+        if (g.st) memcpy(status, g.st, (size_t)g.dimx * g.dimy);
+        else memset(status, 0, (size_t)g.dimx * g.dimy);
+    }
+    return nl;
+}
+
 int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa, const uint32_t *pb,
                   uint8_t *out, int mode, int nthreads, int64_t *stats) {
     for (int64_t k = 0; k < npairs; k++)
@@ -1248,6 +1387,71 @@ int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint
         stats[2] = total.in_hits;
     }
     return err.load();
+}
+
+int orc_within(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa, const uint32_t *pb,
+               double d_km, uint8_t *out, double *min_dist, int nthreads, int64_t *stats) {
+    for (int64_t k = 0; k < npairs; k++)
+        if (pa[k] >= a->polys.size() || pb[k] >= b->polys.size()) return ORC_EINVAL;
+    std::atomic<int64_t> next(0);
+    std::atomic<int> err(ORC_OK);
+    std::mutex mu;
+    QStats total;
+    auto work = [&]() {
+        QStats qs;
+        for (;;) {
+            int64_t s = next.fetch_add(10);
+            if (s >= npairs) break;
+            for (int64_t k = s; k < std::min(npairs, s + 10); k++) {
+                double md = 100000.0;
+                const Poly &S = a->polys[pa[k]], &T = b->polys[pb[k]];
+                if (S.failed || T.failed) {
+                    out[k] = 2;
+                    err = ORC_EREF_ABORT;
+                } else {
+                    try {
+                        out[k] = within(S, T, d_km, qs, &md) ? 1 : 0;
+                    } catch (const RefAbort &) {
+                        out[k] = 2;
+                        err = ORC_EREF_ABORT;
+                    }
+                }
+                if (min_dist) min_dist[k] = md;
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        total.bb_pairs += qs.bb_pairs;
+        total.seg_tests += qs.seg_tests;
+    };
+    if (nthreads < 1) nthreads = 1;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
+    if (stats) {
+        stats[0] = total.bb_pairs;
+        stats[1] = total.seg_tests;
+    }
+    return err.load();
+}
+
+int64_t orc_candidates_within(const orc_set *s, double d_km, int64_t cap, uint32_t *pa, uint32_t *pb) {
+    /* within_polygon_cpu rtree_query + MySearchCallback
+     * (src/query/within_polygon_cpu.cpp:8-48): every (i, j), i != j, whose
+     * MBBs overlap after expanding j's MBB by d_km; MBBs as loaded (before
+     * the hierarchy's grid_align moves them) */
+    int64_t n = 0;
+    for (size_t j = 0; j < s->polys.size(); j++) {
+        const Box q = box_expand(s->polys[j].mbb0, d_km);
+        for (size_t i = 0; i < s->polys.size(); i++) {
+            if (i == j || !box_intersect(s->polys[i].mbb0, q)) continue;
+            if (n < cap) {
+                pa[n] = (uint32_t)i;
+                pb[n] = (uint32_t)j;
+            }
+            n++;
+        }
+    }
+    return n;
 }
 
 int64_t orc_candidates(const orc_set *a, const orc_set *b, int64_t cap, uint32_t *pa, uint32_t *pb) {

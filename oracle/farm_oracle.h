@@ -50,6 +50,12 @@ int orc_raster_copy(const orc_set *s, int64_t i, uint8_t *status, uint32_t *offs
                     uint32_t *seqs /* 2 per seq */, uint32_t *voff, double *vnodes,
                     uint32_t *hoff, double *hnodes, double *areas);
 
+/* Layer k (0 = coarsest) of polygon i after orc_rasterize(..., hierarchy=1):
+ * grid origin, square step, dims and (if status != NULL) its fill classes.
+ * Returns the number of layers (k out of range: nothing written), -1 for a
+ * bad polygon index. */
+int orc_layer(const orc_set *s, int64_t i, int k, double *low2, double *step, int *dims2, uint8_t *status);
+
 /* Exact intersect over candidate pairs (source set a, target set b):
  * out[k] = Ideal::intersect(a[pa[k]], b[pb[k]]).  mode 0 = exact,
  * mode 1 = reference -a heuristic (IN overlap or a BORDER pixel pair with
@@ -57,6 +63,18 @@ int orc_raster_copy(const orc_set *s, int64_t i, uint8_t *status, uint32_t *offs
  * stats (may be NULL): [pixel_pairs_bb, segment_tests, in_hits]. */
 int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa,
                   const uint32_t *pb, uint8_t *out, int mode, int nthreads, int64_t *stats);
+
+/* Ideal::within per ordered pair (source a[pa[k]], target b[pb[k]]) with the
+ * reference's distance convention (km); needs orc_rasterize(..., hierarchy=1).
+ * out: 1 within, 0 not, 2 the reference aborts; min_dist (may be NULL): the
+ * running minimum segment distance of the refinement (100000 if none ran).
+ * stats (may be NULL): [candidate pixel pairs, point-segment distances]. */
+int orc_within(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa, const uint32_t *pb,
+               double d_km, uint8_t *out, double *min_dist, int nthreads, int64_t *stats);
+
+/* Within self-join candidates: (i, j), i != j, with i's MBB overlapping j's
+ * MBB expanded by d_km (box::expand). */
+int64_t orc_candidates_within(const orc_set *s, double d_km, int64_t cap, uint32_t *pa, uint32_t *pb);
 
 /* Candidate generation by brute force: all (i, j) with inclusively
  * overlapping MBBs; writes up to cap pairs, returns the total count. */
