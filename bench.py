@@ -93,6 +93,124 @@ def cpu_baseline(rings, pa, pb, vpr, threads, target_s, rng, plain_bits_fn):
     return results
 
 
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic FP64 operations of the within query (counted from the
+# reference formulas): a point-segment distance (geometry_computation.h:49-89)
+# is 4 f32->f64 differences, dot and squared length (6), the projection
+# parameter (1 division), the foot point (4), the offsets (2), two unit
+# divisions and the squared norm (5): 22; a box-pair distance (Box.cpp:89-149)
+# is the pixel boxes of both sides (8), the axis gaps (2), two divisions, the
+# squared norm and the square root (5): 15
+OPS_PER_PT_SEG = 22
+OPS_PER_BOX_PAIR = 15
+
+
+def within_workload(scale: float, seed: int, threads: int):
+    from farm_amd import capi, datagen
+
+    side = 20.0 * np.sqrt(scale)
+    n = max(2, int(round(300000 * scale)))
+    t = time.time()
+    ro, xy = datagen.lake_set(seed + 3, n, 0.0, side)
+    log(f"within: generated {n} lakes ({len(xy)} vertices) in a [0,{side:.1f}]^2 box in {time.time() - t:.1f}s")
+    t = time.time()
+    H = capi.HostSet.from_rings(ro, xy)
+    H.rasterize_h(10, 1, threads)
+    log(f"within: -h rasterized (-v 10, --NLow 1) in {time.time() - t:.1f}s")
+    t = time.time()
+    pa, pb = capi.candidates(H, H, expand_km=10.0, skip_self=True, nthreads=threads)
+    log(f"within: {len(pa)} ordered candidate pairs (MBB expanded by 10 km) in {time.time() - t:.1f}s")
+    return (ro, xy), H, pa, pb
+
+
+def within_cpu_baseline(ro, xy, pa, pb, threads, target_s, rng):
+    """Oracle within throughput on a random sample of the pairs (query phase)."""
+    import oracle
+    from farm_amd import datagen
+
+    n = len(pa)
+    sample = np.sort(rng.choice(n, size=min(n, 2000), replace=False))
+    while True:
+        u, inv = np.unique(np.concatenate([pa[sample], pb[sample]]), return_inverse=True)
+        O = oracle.OracleSet(*datagen.subset(ro, xy, u))
+        O.rasterize(10, hierarchy=True, nlow=1, nthreads=threads)
+        ia = inv[: len(sample)].astype(np.uint32)
+        ib = inv[len(sample):].astype(np.uint32)
+        t = time.time()
+        bits, md, st = oracle.within(O, O, ia, ib, 10.0, threads)
+        dt = time.time() - t
+        res = {"sample": sample, "bits": bits, "md": md, "seconds": dt}
+        if dt >= target_s or len(sample) >= n:
+            return res
+        grow = int(len(sample) * min(8.0, max(1.5, target_s / max(dt, 1e-3))))
+        sample = np.sort(rng.choice(n, size=min(n, grow), replace=False))
+
+
+def bench_within(args) -> dict:
+    """Within-distance self-join: 300,000 lakes in [0,20]^2, 10 km, -h."""
+    from farm_amd import capi
+
+    (ro, xy), H, pa, pb = within_workload(args.scale, args.seed, args.threads)
+    dev = capi.Device(0)
+    dev.upload_polygons(H)
+    dev.upload_pairs(pa, pb)
+    nb = args.steps + args.warmup
+    cuts = np.linspace(0, len(pa), nb + 1).astype(np.int64)
+    for s in range(args.warmup):
+        st = dev.within(10.0, int(cuts[s]), int(cuts[s + 1]), want_results=False)[2]
+        log(f"within warmup step {s}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms")
+    per = []
+    t0 = time.time()
+    for s in range(args.warmup, nb):
+        st = dev.within(10.0, int(cuts[s]), int(cuts[s + 1]), want_results=False)[2]
+        per.append(st)
+        log(f"within step {s - args.warmup}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms, "
+            f"found {st['found']}, box pairs {st['refine_items']:.3e}, distances {st['segment_tests']:.3e}")
+    wall = time.time() - t0
+    timed = int(cuts[nb] - cuts[args.warmup])
+    kms = sum(p["total_ms"] for p in per)
+    ops = sum(p["segment_tests"] * OPS_PER_PT_SEG + p["refine_items"] * OPS_PER_BOX_PAIR for p in per)
+    achieved = ops / (kms / 1000.0) / 1e12 if kms > 0 else 0.0
+    out = {
+        "metric": "within-distance candidate pairs/s",
+        "value": timed / wall,
+        "unit": "candidate pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (box and point-segment distances)",
+        "data": "synthetic, seeded",
+        "config": {"workload": f"within self-join, {len(ro) - 1} lakes, 10 km, -h --NLow 1 -v 10",
+                   "candidate_pairs": int(len(pa)), "timed_pairs": timed,
+                   "found": int(sum(p["found"] for p in per)), "scale": args.scale},
+        "point_segment_distances_per_s": sum(p["segment_tests"] for p in per) / wall,
+        "box_pair_distances_per_s": sum(p["refine_items"] for p in per) / wall,
+        "roofline": {"bound": "valu", "kernel": "k_within", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "ops_per_point_segment": OPS_PER_PT_SEG, "ops_per_box_pair": OPS_PER_BOX_PAIR},
+    }
+    if not args.no_cpu:
+        bits, md, _ = dev.within(10.0)
+        rng = np.random.default_rng(args.seed + 9)
+        res = within_cpu_baseline(ro, xy, pa, pb, args.threads, args.cpu_seconds, rng)
+        samp = res["sample"]
+        mism = int((bits[samp] != res["bits"]).sum())
+        mdm = int((md[samp].view(np.uint64) != res["md"].view(np.uint64)).sum())
+        out["cpu_baseline"] = {"value": len(samp) / res["seconds"], "unit": "candidate pairs/s",
+                               "cores": args.threads, "kind": "port",
+                               "sample": f"{len(samp)} random candidate pairs of this workload, oracle within "
+                                         f"query only (-h rasterization untimed), {res['seconds']:.1f}s"}
+        out["parity_sample"] = {"pairs": int(len(samp)), "mismatches": mism, "min_dist_mismatches": mdm}
+        log(f"within cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; mismatches {mism}, "
+            f"min-distance mismatches {mdm}")
+    dev.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,11 +228,15 @@ def main() -> int:
     ap.add_argument("--train-pairs", type=int, default=1_000_000)
     ap.add_argument("--label-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--workload", choices=["intersect", "within"], default="intersect")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the pair list would be sharded "
                                    "across GPUs by predicted cost (see DESIGN.md)", "n_gpus": args.gpus}))
         return 2
+    if args.workload == "within":
+        print(json.dumps(bench_within(args)), flush=True)
+        return 0
 
     from farm_amd import capi
 

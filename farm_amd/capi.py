@@ -452,15 +452,20 @@ class Device:
                     "intersect_costs")
         return bits[:n], tests[:n], st.as_dict()
 
-    def within(self, d_km: float = 10.0, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray, dict]:
+    def within(self, d_km: float = 10.0, begin: int = 0, end: int | None = None,
+               want_results: bool = True) -> tuple[np.ndarray | None, np.ndarray | None, dict]:
         """Within-distance decision of pairs [begin, end): bits (1 within, 0
         not, 2 a polygon is rejected by the raster builder), the running
-        minimum distance of the refinement, and counters."""
+        minimum distance of the refinement, and counters.  want_results=False
+        leaves the results on the device (counts only)."""
         end = self.npairs if end is None else end
         n = end - begin
+        st = Stats()
+        if not want_results:
+            self._check(lib.farm_within(self._h, begin, end, d_km, None, None, C.byref(st)), "within")
+            return None, None, st.as_dict()
         bits = np.zeros(max(n, 1), np.uint8)
         md = np.zeros(max(n, 1), np.float64)
-        st = Stats()
         self._check(lib.farm_within(self._h, begin, end, d_km, _ptr(bits), _ptr(md), C.byref(st)), "within")
         return bits[:n], md[:n], st.as_dict()
 

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "handle.h"
@@ -176,10 +177,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Values every lane holds alike, moved to scalar registers.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
+    return uni(v);
 }
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
@@ -192,9 +203,25 @@ __device__ __forceinline__ bool tuple_less(const Cand &a, const Cand &b) {
     return a.id2 < b.id2;
 }
 
-// Pixel set -> compact list of packed (x | y << 16) in id order.  bits has
-// ceil(npix / 32) words; lanes take one word each.
-__device__ __forceinline__ uint32_t compact(const uint32_t *bits, int npix, int dimx, uint32_t *list, int lane) {
+// Pixel lists hold packed (x | y << SH): 16-bit entries with SH = 8 in the
+// LDS path (layers up to 256 x 256), 32-bit entries with SH = 16 otherwise.
+template <int SH, class T>
+__device__ __forceinline__ T pack_xy(uint32_t x, uint32_t y) {
+    return (T)(x | (y << SH));
+}
+template <int SH, class T>
+__device__ __forceinline__ int unpack_x(T v) {
+    return (int)((uint32_t)v & ((1u << SH) - 1u));
+}
+template <int SH, class T>
+__device__ __forceinline__ int unpack_y(T v) {
+    return (int)((uint32_t)v >> SH);
+}
+
+// Pixel set -> compact list in id order.  bits has ceil(npix / 32) words;
+// lanes take one word each.
+template <int SH, class T>
+__device__ __forceinline__ uint32_t compact(const uint32_t *bits, int npix, int dimx, T *list, int lane) {
     const int nw = (npix + 31) >> 5;
     uint32_t total = 0;
     for (int w0 = 0; w0 < nw; w0 += 64) {
@@ -215,9 +242,9 @@ __device__ __forceinline__ uint32_t compact(const uint32_t *bits, int npix, int 
             m &= m - 1;
             const uint32_t id = (uint32_t)(w * 32 + b);
             const uint32_t y = id / (uint32_t)dimx, x = id - y * (uint32_t)dimx;
-            list[pos++] = x | (y << 16);
+            list[pos++] = pack_xy<SH, T>(x, y);
         }
-        total += __shfl(s, 63, 64);
+        total += uni(__shfl(s, 63, 64));
     }
     wave_sync();
     return total;
@@ -236,15 +263,17 @@ struct PairOut {
     double min_dist;
     uint32_t ncand;
     uint64_t evals;
+    uint64_t boxes;   // box-pair distances evaluated in the descent
 };
 
 // Descend one side: children (at layer g) of every parent in list (at layer
 // pg), or (same layer) the members themselves; BORDER ones go to bits.
-__device__ __forceinline__ void expand_side(const uint32_t *list, uint32_t n, bool next, const LGrid &pg,
+template <int SH, class T>
+__device__ __forceinline__ void expand_side(const T *list, uint32_t n, bool next, const LGrid &pg,
                                             const LGrid &g, uint32_t *bits, int lane) {
     for (uint32_t k = lane; k < n; k += 64) {
-        const uint32_t v = list[k];
-        const int px = (int)(v & 0xffffu), py = (int)(v >> 16);
+        const T v = list[k];
+        const int px = unpack_x<SH>(v), py = unpack_y<SH>(v);
         if (!next) {
             const uint32_t id = (uint32_t)(py * g.dimx + px);
             if (border(g.st[id])) set_bit(bits, id);
@@ -271,60 +300,138 @@ __device__ __forceinline__ void expand_side(const uint32_t *list, uint32_t n, bo
     wave_sync();
 }
 
-// Truncated distance of one edge-sequence pair
-// (segment_to_segment_within_batch(vs1 = T seq, vs2 = S seq)), squared.
-__device__ __forceinline__ double seq_pair_d2(const float *tx, const float *ty, uint32_t lb, const float *sx,
-                                              const float *sy, uint32_t la, int lane) {
-    double best = DBL_MAX;
-    // points of the target sequence against the source segments
-    {
-        const uint32_t m = la - 1;
-        uint32_t i = lane / m, j = lane - (lane / m) * m;
-        const uint32_t q = 64 / m, r = 64 - q * m;
-        while (i < lb) {
-            const double d2 = pt_seg_d2(tx[i], ty[i], sx[j], sy[j], sx[j + 1], sy[j + 1]);
-            best = fmin(best, d2);
-            i += q;
-            j += r;
-            if (j >= m) {
-                j -= m;
-                i++;
-            }
+// Edge-sequence pairs of one candidate pixel pair, 64 at a time.
+struct SeqTab {
+    uint32_t fa[64], la[64], fb[64], lb[64], w1[64], pre[65];
+    unsigned long long mins[64];  // squared distance bits (non-negative doubles order as integers)
+};
+
+// Refines one candidate pixel pair (id1, id2): the truncated distances of
+// all its edge-sequence pairs (segment_to_segment_within_batch with vs1 = the
+// target sequence, vs2 = the source one), the evaluations of up to 64
+// sequence pairs flattened over the lanes, each pair's minimum gathered in
+// LDS; then the reference's checks after every valid sequence pair
+// (running minimum < d: within; < ref: not within) become a running-minimum
+// scan and the first lane that passes one.  Returns 1 / 0 when decided, 2 to
+// go on with the next candidate.
+__device__ __forceinline__ int refine_cand(const DevSet &S, const farm_raster_hdr &hs, const farm_raster_hdr &ht,
+                                           uint32_t id1, uint32_t id2, double d, double ref, double &min_dist,
+                                           PairOut &r, SeqTab &tab, int lane) {
+    const uint32_t s0 = S.pixoff[hs.pixoff_start + id1], n1 = S.pixoff[hs.pixoff_start + id1 + 1] - s0;
+    const uint32_t t0 = S.pixoff[ht.pixoff_start + id2], n2 = S.pixoff[ht.pixoff_start + id2 + 1] - t0;
+    const uint32_t np = n1 * n2;
+    const float *sxv = S.x + hs.vert_start, *syv = S.y + hs.vert_start;
+    const float *txv = S.x + ht.vert_start, *tyv = S.y + ht.vert_start;
+    for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+        const uint32_t k = c0 + lane;
+        uint32_t la = 0, lb = 0, fa = 0, fb = 0;
+        if (k < np) {
+            const uint32_t a = k / n2, b = k - (k / n2) * n2;
+            const uint2 e1 = S.seqs[hs.seq_start + s0 + a];
+            const uint2 e2 = S.seqs[ht.seq_start + t0 + b];
+            fa = e1.x;
+            la = e1.y;
+            fb = e2.x;
+            lb = e2.y;
         }
-    }
-    // points of the source sequence against the target segments
-    {
-        const uint32_t m = lb - 1;
-        uint32_t i = lane / m, j = lane - (lane / m) * m;
-        const uint32_t q = 64 / m, r = 64 - q * m;
-        while (i < la) {
-            const double d2 = pt_seg_d2(sx[i], sy[i], tx[j], ty[j], tx[j + 1], ty[j + 1]);
-            best = fmin(best, d2);
-            i += q;
-            j += r;
-            if (j >= m) {
-                j -= m;
-                i++;
-            }
+        // sequences of fewer than 2 edges are skipped (Ideal.cpp:2630)
+        const bool ok = k < np && la >= 2 && lb >= 2;
+        const uint32_t w1 = ok ? lb * (la - 1) : 0u;
+        const uint32_t w = ok ? w1 + la * (lb - 1) : 0u;
+        uint32_t inc = w;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
         }
+        const uint32_t W = uni(__shfl(inc, 63, 64));
+        tab.fa[lane] = fa;
+        tab.la[lane] = la;
+        tab.fb[lane] = fb;
+        tab.lb[lane] = lb;
+        tab.w1[lane] = w1;
+        tab.pre[lane] = inc - w;
+        if (lane == 63) tab.pre[64] = W;
+        tab.mins[lane] = ~0ull;
+        wave_sync();
+        int kk = 0, cur = -1;
+        double acc = INFINITY;
+        for (uint32_t u = lane; u < W; u += 64) {
+            while (tab.pre[kk + 1] <= u) kk++;
+            if (kk != cur) {
+                if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
+                cur = kk;
+                acc = INFINITY;
+            }
+            const uint32_t e = u - tab.pre[kk];
+            const uint32_t fa_ = tab.fa[kk], fb_ = tab.fb[kk], w1_ = tab.w1[kk];
+            double d2;
+            if (e < w1_) {  // point i of the target sequence, segment j of the source one
+                const uint32_t m = tab.la[kk] - 1, i = e / m, j = e - (e / m) * m;
+                d2 = pt_seg_d2(txv[fb_ + i], tyv[fb_ + i], sxv[fa_ + j], syv[fa_ + j], sxv[fa_ + j + 1],
+                               syv[fa_ + j + 1]);
+            } else {  // point i of the source sequence, segment j of the target one
+                const uint32_t e2 = e - w1_, m = tab.lb[kk] - 1, i = e2 / m, j = e2 - (e2 / m) * m;
+                d2 = pt_seg_d2(sxv[fa_ + i], syv[fa_ + i], txv[fb_ + j], tyv[fb_ + j], txv[fb_ + j + 1],
+                               tyv[fb_ + j + 1]);
+            }
+            acc = fmin(acc, d2);
+        }
+        if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
+        wave_sync();
+        // min of the square roots = square root of the min (sqrt is monotone)
+        const double dist = ok ? sqrt(__longlong_as_double((long long)tab.mins[lane])) : INFINITY;
+        double run = dist;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double t = __shfl_up(run, o, 64);
+            if (lane >= o) run = fmin(run, t);
+        }
+        run = fmin(run, min_dist);
+        const unsigned long long hit = __ballot(ok && (run < d || run < ref));
+        if (hit) {
+            const int ks = uni(__ffsll(hit) - 1);
+            const double md = uni(__shfl(run, ks, 64));
+            r.evals += tab.pre[ks + 1];
+            min_dist = md;
+            r.min_dist = md;
+            return md < d ? 1 : 0;
+        }
+        r.evals += W;
+        if (__ballot(ok)) {
+            min_dist = uni(__shfl(run, 63, 64));
+            r.min_dist = min_dist;
+        }
+        wave_sync();
     }
-    return wave_min(best);
+    return 2;
 }
 
-// Argmin (tuple order) over the live candidates; returns its slot or -1.
-__device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lane, Cand &out) {
+// Argmin (tuple order) over the live candidates, and reference_dist: the
+// smallest low bound among the other live candidates (each lane keeps its
+// two smallest low bounds).  Returns the slot, or -1 when none is live.
+__device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lane, Cand &out, double &ref) {
     Cand best;
     best.apx = INFINITY;
     best.lo = INFINITY;
     best.hi = INFINITY;
     best.id1 = best.id2 = 0xffffffffu;
     int bi = -1;
+    double lo1 = INFINITY, lo2 = INFINITY;
+    int i1 = -1;
     for (uint32_t k = lane; k < n; k += 64) {
         const Cand c = cand[k];
         if (c.apx == INFINITY) continue;  // taken
         if (bi < 0 || tuple_less(c, best)) {
             best = c;
             bi = (int)k;
+        }
+        if (c.lo < lo1) {
+            lo2 = lo1;
+            lo1 = c.lo;
+            i1 = (int)k;
+        } else if (c.lo < lo2) {
+            lo2 = c.lo;
         }
     }
 #pragma unroll
@@ -341,35 +448,43 @@ __device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lan
             bi = ci;
         }
     }
-    out = best;
+    out.apx = uni(best.apx);
+    out.lo = uni(best.lo);
+    out.hi = uni(best.hi);
+    out.id1 = uni(best.id1);
+    out.id2 = uni(best.id2);
+    bi = uni(bi);
+    ref = fmin(kFar, wave_min(i1 == bi ? lo2 : lo1));
     return bi;
 }
 
 // Ideal::within for one pair; the wave's buffers: lists / bits (LDS or a
 // global slab) and the candidate slab.
-__device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uint32_t ib, double d, uint32_t *sl,
-                                               uint32_t *tl, uint32_t *sb, uint32_t *tb, Cand *cand, uint32_t pixcap,
-                                               uint32_t candcap, int lane) {
+template <int SH, class T>
+__device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uint32_t ib, double d, T *sl,
+                                               T *tl, uint32_t *sb, uint32_t *tb, Cand *cand, uint32_t pixcap,
+                                               uint32_t candcap, SeqTab &tab, int lane) {
     PairOut r;
     r.bit = 0;
     r.min_dist = kFar;
     r.ncand = 0;
     r.evals = 0;
-    const farm_raster_hdr hs = S.hdr[ia];
-    const farm_raster_hdr ht = S.hdr[ib];
+    r.boxes = 0;
+    const farm_raster_hdr &hs = S.hdr[ia];
+    const farm_raster_hdr &ht = S.hdr[ib];
     const int s_level = hs.nlayers - 1, t_level = ht.nlayers - 1;
     // every layer must fit the pixel buffers (the finest is the largest,
     // but check all: a coarse layer may be one pixel wider)
     for (int k = 0; k <= s_level; k++) {
         const LGrid g = lgrid(S, hs, k);
-        if ((uint64_t)g.dimx * g.dimy > pixcap || g.dimx > 0xffff || g.dimy > 0xffff) {
+        if ((uint64_t)g.dimx * g.dimy > pixcap || g.dimx > (1 << SH) || g.dimy > (1 << SH)) {
             r.bit = 3;
             return r;
         }
     }
     for (int k = 0; k <= t_level; k++) {
         const LGrid g = lgrid(S, ht, k);
-        if ((uint64_t)g.dimx * g.dimy > pixcap || g.dimx > 0xffff || g.dimy > 0xffff) {
+        if ((uint64_t)g.dimx * g.dimy > pixcap || g.dimx > (1 << SH) || g.dimy > (1 << SH)) {
             r.bit = 3;
             return r;
         }
@@ -380,8 +495,8 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         const LGrid gs = lgrid(S, hs, 0), gt = lgrid(S, ht, 0);
         ns = (uint32_t)(gs.dimx * gs.dimy);
         nt = (uint32_t)(gt.dimx * gt.dimy);
-        for (uint32_t k = lane; k < ns; k += 64) sl[k] = (k % gs.dimx) | ((k / gs.dimx) << 16);
-        for (uint32_t k = lane; k < nt; k += 64) tl[k] = (k % gt.dimx) | ((k / gt.dimx) << 16);
+        for (uint32_t k = lane; k < ns; k += 64) sl[k] = pack_xy<SH, T>(k % gs.dimx, k / gs.dimx);
+        for (uint32_t k = lane; k < nt; k += 64) tl[k] = pack_xy<SH, T>(k % gt.dimx, k / gt.dimx);
         wave_sync();
     }
     int i = 0, j = 0;
@@ -403,24 +518,25 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         const int npix_s = gs.dimx * gs.dimy, npix_t = gt.dimx * gt.dimy;
         clear_bits(sb, npix_s, lane);
         clear_bits(tb, npix_t, lane);
-        expand_side(sl, ns, s_next, s_next ? lgrid(S, hs, i - 1) : gs, gs, sb, lane);
-        expand_side(tl, nt, t_next, t_next ? lgrid(S, ht, j - 1) : gt, gt, tb, lane);
-        ns = compact(sb, npix_s, gs.dimx, sl, lane);
-        nt = compact(tb, npix_t, gt.dimx, tl, lane);
+        expand_side<SH>(sl, ns, s_next, s_next ? lgrid(S, hs, i - 1) : gs, gs, sb, lane);
+        expand_side<SH>(tl, nt, t_next, t_next ? lgrid(S, ht, j - 1) : gt, gt, tb, lane);
+        ns = compact<SH>(sb, npix_s, gs.dimx, sl, lane);
+        nt = compact<SH>(tb, npix_t, gt.dimx, tl, lane);
         if (ns == 0 || nt == 0) break;  // no BORDER pixel on a side: nothing to refine
         // pass 1: any BORDER box pair within d by its farthest corners
         // decides; else the smallest such maximum bounds the useful pairs
         double maxd = kFar;
         bool hit = false;
         const uint64_t total = (uint64_t)ns * nt;
+        r.boxes += total;
         {
             uint32_t a = lane / nt, b = lane - (lane / nt) * nt;
             const uint32_t q = 64 / nt, rr = 64 - q * nt;
             while (a < ns) {
-                const uint32_t va = sl[a], vb = tl[b];
+                const T va = sl[a], vb = tl[b];
                 double b1[4], b2[4];
-                pbox(gs, (int)(va & 0xffffu), (int)(va >> 16), b1);
-                pbox(gt, (int)(vb & 0xffffu), (int)(vb >> 16), b2);
+                pbox(gs, unpack_x<SH>(va), unpack_y<SH>(va), b1);
+                pbox(gt, unpack_x<SH>(vb), unpack_y<SH>(vb), b2);
                 const double mx = box_max_dist(b1, b2);
                 hit |= (mx <= d);
                 maxd = fmin(maxd, mx);
@@ -437,6 +553,7 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
             return r;
         }
         maxd = wave_min(maxd);
+        r.boxes += total;
         // pass 2: keep pairs with min distance <= d and below maxd; their
         // projections are the next level's sets, or (last level) they are
         // the candidates
@@ -453,9 +570,9 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                 bool keep = false;
                 Cand c;
                 if (a < ns) {
-                    const uint32_t va = sl[a], vb = tl[b];
-                    const int xa = (int)(va & 0xffffu), ya = (int)(va >> 16);
-                    const int xb = (int)(vb & 0xffffu), yb = (int)(vb >> 16);
+                    const T va = sl[a], vb = tl[b];
+                    const int xa = unpack_x<SH>(va), ya = unpack_y<SH>(va);
+                    const int xb = unpack_x<SH>(vb), yb = unpack_y<SH>(vb);
                     double b1[4], b2[4];
                     pbox(gs, xa, ya, b1);
                     pbox(gt, xb, yb, b2);
@@ -503,47 +620,25 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
             r.ncand = nc;
             break;
         }
-        ns = compact(sb, npix_s, gs.dimx, sl, lane);
-        nt = compact(tb, npix_t, gt.dimx, tl, lane);
+        ns = compact<SH>(sb, npix_s, gs.dimx, sl, lane);
+        nt = compact<SH>(tb, npix_t, gt.dimx, tl, lane);
     }
     if (!final_level || r.ncand == 0) return r;
 
     // refinement in tuple order
     const uint32_t nc = r.ncand;
     double min_dist = kFar;
-    const float *sxv = S.x + hs.vert_start, *syv = S.y + hs.vert_start;
-    const float *txv = S.x + ht.vert_start, *tyv = S.y + ht.vert_start;
     for (uint32_t step = 0; step < nc; step++) {
         Cand c;
-        const int slot = select_next(cand, nc, lane, c);
+        double ref;
+        const int slot = select_next(cand, nc, lane, c, ref);
         if (slot < 0) break;
         if (lane == 0) cand[slot].apx = INFINITY;
         wave_sync();
-        // reference_dist: smallest low bound among the candidates after it
-        double ref = kFar;
-        for (uint32_t k = lane; k < nc; k += 64) {
-            const Cand e = cand[k];
-            if (e.apx != INFINITY) ref = fmin(ref, e.lo);
-        }
-        ref = wave_min(ref);
-        const uint32_t s0 = S.pixoff[hs.pixoff_start + c.id1], s1 = S.pixoff[hs.pixoff_start + c.id1 + 1];
-        const uint32_t t0 = S.pixoff[ht.pixoff_start + c.id2], t1 = S.pixoff[ht.pixoff_start + c.id2 + 1];
-        for (uint32_t a = s0; a < s1; a++) {
-            const uint2 er1 = S.seqs[hs.seq_start + a];
-            for (uint32_t b = t0; b < t1; b++) {
-                const uint2 er2 = S.seqs[ht.seq_start + b];
-                if (er1.y < 2 || er2.y < 2) continue;
-                const double d2 = seq_pair_d2(txv + er2.x, tyv + er2.x, er2.y, sxv + er1.x, syv + er1.x, er1.y, lane);
-                r.evals += (uint64_t)er2.y * (er1.y - 1) + (uint64_t)er1.y * (er2.y - 1);
-                const double dist = sqrt(d2);
-                min_dist = min_dist < dist ? min_dist : dist;
-                r.min_dist = min_dist;
-                if (min_dist < d) {
-                    r.bit = 1;
-                    return r;
-                }
-                if (min_dist < ref) return r;
-            }
+        const int res = refine_cand(S, hs, ht, c.id1, c.id2, d, ref, min_dist, r, tab, lane);
+        if (res != 2) {
+            r.bit = (uint8_t)res;
+            return r;
         }
     }
     return r;
@@ -564,10 +659,14 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const uint64_t gw = (uint64_t)blockIdx.x * kWaves + wv;
-    uint32_t *sl, *tl, *sb, *tb;
+    __shared__ SeqTab tabs[kWaves];
+    using LT = typename std::conditional<BIG, uint32_t, uint16_t>::type;
+    constexpr int SH = BIG ? 16 : 8;
+    LT *sl, *tl;
+    uint32_t *sb, *tb;
     Cand *cand;
     if constexpr (!BIG) {
-        __shared__ uint32_t lists[kWaves][2][kPixMax];
+        __shared__ LT lists[kWaves][2][kPixMax];
         __shared__ uint32_t bits[kWaves][2][kBitWords];
         sl = lists[wv][0];
         tl = lists[wv][1];
@@ -577,17 +676,17 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
     } else {
         char *base = slab + gw * slab_stride;
         const uint64_t wbits = ((uint64_t)pixcap + 31) / 32;
-        sl = reinterpret_cast<uint32_t *>(base);
+        sl = reinterpret_cast<LT *>(base);
         tl = sl + pixcap;
-        sb = tl + pixcap;
+        sb = reinterpret_cast<uint32_t *>(tl + pixcap);
         tb = sb + wbits;
         cand = reinterpret_cast<Cand *>(base + (((2ull * pixcap + 2 * wbits) * 4 + 255) & ~255ull));
     }
-    unsigned long long ncand = 0, evals = 0, found = 0;
+    unsigned long long ncand = 0, evals = 0, found = 0, boxes = 0;
     for (;;) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(&ctr[0], kChunk);
-        base = __shfl(base, 0, 64);
+        base = uni(__shfl(base, 0, 64));
         if (base >= n) break;
         const uint32_t end = min(base + kChunk, n);
         for (uint32_t k = base; k < end; k++) {
@@ -602,7 +701,7 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
                 }
                 continue;
             }
-            const PairOut r = within_pair(S, ia, ib, d, sl, tl, sb, tb, cand, pixcap, candcap, lane);
+            const PairOut r = within_pair<SH>(S, ia, ib, d, sl, tl, sb, tb, cand, pixcap, candcap, tabs[wv], lane);
             if (r.bit == 3) {
                 if (lane == 0) {
                     if (BIG) {
@@ -616,6 +715,7 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
             }
             ncand += r.ncand;
             evals += r.evals;
+            boxes += r.boxes;
             found += r.bit;
             if (lane == 0) {
                 out[idx] = r.bit;
@@ -627,6 +727,7 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
         atomicAdd(&stats[0], ncand);
         atomicAdd(&stats[1], evals);
         atomicAdd(&stats[2], found);
+        atomicAdd(&stats[3], boxes);
     }
 }
 
@@ -712,7 +813,7 @@ int run_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t
         HIPCHK(h, hipGetLastError());
     }
     HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-    unsigned long long hst[3] = {0, 0, 0};
+    unsigned long long hst[4] = {0, 0, 0, 0};
     HIPCHK(h, hipMemcpyAsync(hst, wst, sizeof hst, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (novf) {
@@ -731,6 +832,7 @@ int run_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t
         st->bb_pixel_pairs = hst[0];
         st->segment_tests = hst[1];
         st->found = hst[2];
+        st->refine_items = hst[3];
         st->refine_ms = ms;
         st->total_ms = ms;
     }
@@ -755,6 +857,7 @@ int farm_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_
     int rc;
     if ((rc = ensure(h, h->wout, n + 16))) return rc;
     if ((rc = ensure(h, h->wmd, 8 * n + 16))) return rc;
+    // out / min_dist may be NULL: results stay on the device (counts only)
     if ((rc = run_within(h, begin, end, d_km, P<uint8_t>(h->wout), P<double>(h->wmd), st))) return rc;
     if (n && out) HIPCHK(h, hipMemcpy(out, h->wout.p, n, hipMemcpyDeviceToHost));
     if (n && min_dist) HIPCHK(h, hipMemcpy(min_dist, h->wmd.p, 8 * n, hipMemcpyDeviceToHost));

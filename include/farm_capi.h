@@ -193,8 +193,11 @@ int farm_intersect_costs(farm_handle h, uint64_t begin, uint64_t end, uint8_t *b
  * out[k - begin]: 1 within d_km, 0 not, 2 a polygon of the pair is
  * rejected by the reference raster builder.  min_dist (may be NULL): the
  * running minimum segment distance when the decision was taken (100000 when
- * no distance was computed).  The device-pointer variant takes device
- * buffers of end - begin entries. */
+ * no distance was computed).  out and min_dist may be NULL (results stay on
+ * the device).  The device-pointer variant takes device buffers of
+ * end - begin entries.  stats: bb_pixel_pairs = candidate pixel pairs of the
+ * last level, segment_tests = point-segment distances evaluated,
+ * refine_items = box-pair distances of the descent. */
 int farm_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out, double *min_dist,
                 farm_stats *stats);
 int farm_within_dev(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out_dev,
