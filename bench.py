@@ -211,6 +211,103 @@ def bench_within(args) -> dict:
     return out
 
 
+def bench_intersection(args) -> dict:
+    """Intersection (crossing points, boundary pieces, area of A n B):
+    5,000 complex x 100,000 lakes in [0,20]^2, -v 256."""
+    import oracle
+    from farm_amd import capi, datagen
+
+    side = 20.0 * np.sqrt(args.scale)
+    n_c = max(1, int(round(5000 * args.scale)))
+    n_l = max(1, int(round(100000 * args.scale)))
+    t = time.time()
+    cro, cxy = datagen.complex_set(args.seed + 5, n_c, 0.0, side)
+    lro, lxy = datagen.lake_set(args.seed + 6, n_l, 0.0, side)
+    log(f"intersection: generated {n_c} complex ({len(cxy)} vertices) + {n_l} lakes ({len(lxy)} vertices) "
+        f"in {time.time() - t:.1f}s")
+    A = capi.HostSet.from_rings(cro, cxy)
+    B = capi.HostSet.from_rings(lro, lxy)
+    pa, pb = capi.candidates(A, B, nthreads=args.threads)
+    both = capi.HostSet.concat(A, B)
+    t = time.time()
+    both.rasterize(args.vpr, args.threads)
+    log(f"intersection: {len(pa)} candidate pairs; rasterized in {time.time() - t:.1f}s")
+    dev = capi.Device(0)
+    dev.upload_polygons(both)
+    dev.upload_pairs(pa, pb + len(A))
+    nb = args.steps + args.warmup
+    cuts = np.linspace(0, len(pa), nb + 1).astype(np.int64)
+    for s in range(args.warmup):
+        st = dev.intersection_area(int(cuts[s]), int(cuts[s + 1]), want_results=False)[2]
+        log(f"intersection warmup step {s}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms")
+    per = []
+    t0 = time.time()
+    for s in range(args.warmup, nb):
+        st = dev.intersection_area(int(cuts[s]), int(cuts[s + 1]), want_results=False)[2]
+        per.append(st)
+        log(f"intersection step {s - args.warmup}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms, "
+            f"edge pairs {st['segment_tests']:.3e}, crossing records {st['refine_items']}")
+    wall = time.time() - t0
+    timed = int(cuts[nb] - cuts[args.warmup])
+    tests = sum(p["segment_tests"] for p in per)
+    kms = sum(p["total_ms"] for p in per)
+    # the edge-pair test: 3 f32 differences x 2 coordinates, 3 crosses (3 ops
+    # each), the screen (5 compares and 4 products): 24 f32 operations
+    ops = 24
+    achieved = tests * ops / (kms / 1000.0) / 1e12 if kms > 0 else 0.0
+    out = {
+        "metric": "intersection (area of A n B) candidate pairs/s",
+        "value": timed / wall,
+        "unit": "candidate pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (crossing tests), f64 (parameters, areas)",
+        "data": "synthetic, seeded",
+        "config": {"workload": f"intersection, {n_c} complex x {n_l} lakes, -v {args.vpr}",
+                   "candidate_pairs": int(len(pa)), "timed_pairs": timed,
+                   "found": int(sum(p["found"] for p in per)), "scale": args.scale},
+        "edge_pair_tests_per_s": tests / wall,
+        "roofline": {"bound": "valu", "kernel": "k_intersection", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "ops_per_edge_pair": ops},
+    }
+    if not args.no_cpu:
+        counts, area, _ = dev.intersection_area()
+        rng = np.random.default_rng(args.seed + 13)
+        n = len(pa)
+        sample = np.sort(rng.choice(n, size=min(n, 500), replace=False))
+        while True:
+            ua, ia = np.unique(pa[sample], return_inverse=True)
+            ub, ib = np.unique(pb[sample], return_inverse=True)
+            OA = oracle.OracleSet(*datagen.subset(cro, cxy, ua))
+            OB = oracle.OracleSet(*datagen.subset(lro, lxy, ub))
+            OA.rasterize(args.vpr, nthreads=args.threads)
+            OB.rasterize(args.vpr, nthreads=args.threads)
+            t = time.time()
+            wc, wa = oracle.intersection(OA, OB, ia.astype(np.uint32), ib.astype(np.uint32), args.threads)
+            dt = time.time() - t
+            if dt >= args.cpu_seconds or len(sample) >= n:
+                break
+            grow = int(len(sample) * min(8.0, max(1.5, args.cpu_seconds / max(dt, 1e-3))))
+            sample = np.sort(rng.choice(n, size=min(n, grow), replace=False))
+        cm = int((counts[sample] != wc).any(1).sum())
+        am = int((np.abs(area[sample] - wa) > 1e-9 * np.abs(wa) + 1e-15).sum())
+        out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "candidate pairs/s", "cores": args.threads,
+                               "kind": "port",
+                               "sample": f"{len(sample)} random candidate pairs of this workload, oracle "
+                                         f"intersection query only (rasterization untimed), {dt:.1f}s"}
+        out["parity_sample"] = {"pairs": int(len(sample)), "count_mismatches": cm, "area_mismatches_1e-9": am}
+        log(f"intersection cpu baseline: {len(sample)} pairs in {dt:.1f}s; count mismatches {cm}, "
+            f"area mismatches {am}")
+    dev.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,7 +325,7 @@ def main() -> int:
     ap.add_argument("--train-pairs", type=int, default=1_000_000)
     ap.add_argument("--label-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=8)
-    ap.add_argument("--workload", choices=["intersect", "within"], default="intersect")
+    ap.add_argument("--workload", choices=["intersect", "within", "intersection"], default="intersect")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the pair list would be sharded "
@@ -236,6 +333,9 @@ def main() -> int:
         return 2
     if args.workload == "within":
         print(json.dumps(bench_within(args)), flush=True)
+        return 0
+    if args.workload == "intersection":
+        print(json.dumps(bench_intersection(args)), flush=True)
         return 0
 
     from farm_amd import capi

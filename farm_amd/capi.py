@@ -181,6 +181,14 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_intersection_area": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
+    ),
+    "farm_intersection_area_dev": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
+    ),
     "farm_pair_features": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
     "farm_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "farm_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -468,6 +476,23 @@ class Device:
         md = np.zeros(max(n, 1), np.float64)
         self._check(lib.farm_within(self._h, begin, end, d_km, _ptr(bits), _ptr(md), C.byref(st)), "within")
         return bits[:n], md[:n], st.as_dict()
+
+    def intersection_area(self, begin: int = 0, end: int | None = None,
+                          want_results: bool = True) -> tuple[np.ndarray | None, np.ndarray | None, dict]:
+        """Intersection of pairs [begin, end): counts (n, 5) [crossing points
+        (-1 window all OUT, -2 all IN, -3 rejected polygon), pieces, IN, OUT,
+        BORDER] and the f64 area of A n B per pair."""
+        end = self.npairs if end is None else end
+        n = end - begin
+        st = Stats()
+        if not want_results:
+            self._check(lib.farm_intersection_area(self._h, begin, end, None, None, C.byref(st)), "intersection")
+            return None, None, st.as_dict()
+        counts = np.zeros((max(n, 1), 5), np.int32)
+        area = np.zeros(max(n, 1), np.float64)
+        self._check(lib.farm_intersection_area(self._h, begin, end, _ptr(counts), _ptr(area), C.byref(st)),
+                    "intersection")
+        return counts[:n], area[:n], st.as_dict()
 
     def pair_features(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray]:
         end = self.npairs if end is None else end

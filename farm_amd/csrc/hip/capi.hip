@@ -182,7 +182,7 @@ int farm_close(farm_handle h) {
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
                       &h->skipbuf, &h->layers, &h->lstatus, &h->areas, &h->wctr, &h->wovf, &h->wslab,
-                      &h->wslab_big, &h->wout, &h->wmd};
+                      &h->wslab_big, &h->wout, &h->wmd, &h->ring2a, &h->icounts, &h->iarea};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : h->ev)
@@ -210,6 +210,10 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
         for (uint64_t p = 0; p <= npix; p++)
             if (po[p] > r.nseqs || (p < npix && po[p] > po[p + 1]))
                 return fail(h, FARM_EINVAL, "polygon %lld: pixel offsets out of range", (long long)i);
+        const uint32_t *vo = ps->voff + r.vline_start;
+        for (int p = 0; p < r.dimx + 2; p++)
+            if (r.vnode_start + vo[p] > ps->nvnodes)
+                return fail(h, FARM_EINVAL, "polygon %lld: grid-line offsets out of range", (long long)i);
         const uint32_t *sq = ps->seqs + 2 * r.seq_start;
         for (uint32_t k = 0; k < r.nseqs; k++)
             if ((uint64_t)sq[2 * k] + sq[2 * k + 1] + 1 > r.nverts)
@@ -258,6 +262,7 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     if ((rc = upload(h, h->vnodes, ps->vnodes, 8 * ps->nvnodes))) return rc;
     if ((rc = ensure(h, h->pixedges, 4 * ps->nstatus))) return rc;
     h->has_layers = false;
+    h->ring_area_ready = false;
     if (layered) {
         if ((rc = upload(h, h->layers, ps->layers, sizeof(farm_layer) * ps->nlayers))) return rc;
         if ((rc = upload(h, h->lstatus, ps->lstatus, ps->nlstatus))) return rc;

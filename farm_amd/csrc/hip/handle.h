@@ -63,6 +63,9 @@ struct farm_handle_s {
     uint64_t max_border_pix = 0;  // most BORDER pixels of a finest layer
     farm::DevBuf layers, lstatus, areas;
     farm::DevBuf wctr, wovf, wslab, wslab_big, wout, wmd;
+    // intersection
+    bool ring_area_ready = false;
+    farm::DevBuf ring2a, icounts, iarea;
 };
 
 namespace farm {

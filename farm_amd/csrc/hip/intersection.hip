@@ -1,0 +1,642 @@
+// Intersection query on gfx950: boundary crossing points, classified
+// boundary pieces and the area of A n B.
+//
+// Follows Ideal::intersection (src/geometry/Ideal.cpp:1440-1683) up to the
+// piece statuses, where the reference returns: the crossing points of edges
+// in BORDER x BORDER pixel pairs (segment_intersect_batch,
+// src/include/geometry_computation.h:239-267), sorted and de-duplicated by
+// point, the boundary of each polygon split at them, and every piece
+// classified by Ideal::segment_contain (Ideal.cpp:1233-1327).  On top of
+// those pieces the area is integrated with Green's theorem:
+//   area = |sA * sum(A pieces IN B) + sB * sum(B pieces IN A)| / 2
+// (cross-product sums of the piece paths relative to the low corner of the
+// two MBBs' overlap; sX the orientation of ring X; a piece is IN when the
+// midpoint of its middle sub-segment is).  Without two distinct crossing
+// points the pair is contained (first vertex IN the other) or disjoint.
+//
+// One wavefront per pair:
+//  1. the source raster's window over the target MBB; all OUT / all IN
+//     windows end the pair (the reference's early return)
+//  2. every edge pair of every BORDER x BORDER pixel pair's sequence pairs,
+//     lanes over the flattened edge pairs; a float screen (margins far
+//     above its rounding) skips pairs whose parameters are clearly outside
+//     [0, 1]; the rest take the reference's exact f32 / f64 test; hits are
+//     appended to a per-wave record slab in global memory
+//  3. bitonic sorts of the slab (by point; by source edge and parameter;
+//     by target edge and parameter), de-duplication, pieces, statuses and
+//     the wave-cooperative path sums of the IN pieces
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "handle.h"
+#include "wave.h"
+
+namespace farm {
+
+namespace {
+
+struct Rec {
+    float px, py;
+    int32_t ei, ej;
+    double t, u;
+};
+
+constexpr int kWaves = 4;
+constexpr uint32_t kRecCap = 2048;     // records per wave slab (power of two)
+constexpr uint32_t kRecCapBig = 65536;  // second pass for pairs beyond kRecCap
+constexpr uint32_t kChunk = 2;          // pairs per queue grab
+
+// Point::operator== (Point.h:32)
+__device__ __forceinline__ bool pt_eq(float ax, float ay, float bx, float by) {
+    return fabsf(ax - bx) < 1e-9 && fabsf(ay - by) < 1e-9;
+}
+
+// key orders: 0 by point (then ei, ej, t, u); 1 by source edge and t (then
+// ej, u); 2 by target edge and u (then ei, t)
+template <int K>
+__device__ __forceinline__ bool rec_less(const Rec &a, const Rec &b) {
+    if constexpr (K == 0) {
+        if (a.px != b.px) return a.px < b.px;
+        if (a.py != b.py) return a.py < b.py;
+        if (a.ei != b.ei) return a.ei < b.ei;
+        if (a.ej != b.ej) return a.ej < b.ej;
+        if (a.t != b.t) return a.t < b.t;
+        return a.u < b.u;
+    } else if constexpr (K == 1) {
+        if (a.ei != b.ei) return a.ei < b.ei;
+        if (a.t != b.t) return a.t < b.t;
+        if (a.ej != b.ej) return a.ej < b.ej;
+        return a.u < b.u;
+    } else {
+        if (a.ej != b.ej) return a.ej < b.ej;
+        if (a.u != b.u) return a.u < b.u;
+        if (a.ei != b.ei) return a.ei < b.ei;
+        return a.t < b.t;
+    }
+}
+
+__device__ __forceinline__ Rec pad_rec() {
+    Rec r;
+    r.px = INFINITY;
+    r.py = INFINITY;
+    r.ei = INT_MAX;
+    r.ej = INT_MAX;
+    r.t = INFINITY;
+    r.u = INFINITY;
+    return r;
+}
+
+// Bitonic sort of R[0, n) in place; R has room for next_pow2(n) records.
+template <int K>
+__device__ void bitonic(Rec *R, uint32_t n, int lane) {
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    for (uint32_t i = n + lane; i < m; i += 64) R[i] = pad_rec();
+    wave_sync();
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < m; i += 64) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const Rec a = R[i], b = R[l];
+                    const bool up = (i & k) == 0;
+                    if (rec_less<K>(b, a) == up) {
+                        R[i] = b;
+                        R[l] = a;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+}
+
+// Ideal::segment_contain with count_intersection_nodes (Ideal.cpp:988-1006):
+// 0 OUT, 1 BORDER (on the boundary), 2 IN
+__device__ int seg_contain(const DevSet &S, const farm_raster_hdr &h, float px, float py) {
+    const int x = off_axis((double)px, h.mbr[0], h.step[0], h.dimx);
+    const int y = off_axis((double)py, h.mbr[1], h.step[1], h.dimy);
+    const uint32_t id = (uint32_t)(y * h.dimx + x);
+    const uint8_t st = S.status[h.pix_start + id];
+    if (st == kCC - 1) return 2;
+    if (st == 0) return 0;
+    const double bx_high = h.mbr[0] + (x + 1) * h.step[0];
+    bool ret = false;
+    const uint32_t s0 = S.pixoff[h.pixoff_start + id], s1 = S.pixoff[h.pixoff_start + id + 1];
+    const float *X = S.x + h.vert_start, *Y = S.y + h.vert_start;
+    for (uint32_t e = s0; e < s1; e++) {
+        const uint2 sq = S.seqs[h.seq_start + e];
+        for (uint32_t k = 0; k < sq.y; k++) {
+            const float v1x = X[sq.x + k], v1y = Y[sq.x + k];
+            const float v2x = X[sq.x + k + 1], v2y = Y[sq.x + k + 1];
+            if (pt_eq(px, py, v1x, v1y) || pt_eq(px, py, v2x, v2y)) return 1;
+            if ((v1y >= py) != (v2y >= py)) {
+                const double dx = (double)(v2x - v1x);
+                const double dy = (double)(v2y - v1y);
+                const double pyd = (double)(py - v1y);
+                if (fabs(dy) > 1e-9) {
+                    const double ix = dx * pyd / dy + (double)v1x;
+                    if (fabs((double)px - ix) < 1e-9) return 1;
+                    if ((double)px < ix && ix <= bx_high) ret = !ret;
+                }
+            } else if (v1y == py && v2y == py && ((v1x >= px) != (v2x >= px))) {
+                return 1;
+            }
+        }
+    }
+    const uint32_t *vo = S.voff + h.vline_start;
+    uint32_t i = vo[x + 1];
+    const uint32_t j = vo[x + 2];
+    int cnt = 0;
+    while (i < j && S.vnodes[h.vnode_start + i] < (double)py) {
+        cnt++;
+        i++;
+    }
+    if (cnt & 1) ret = !ret;
+    return ret ? 2 : 0;
+}
+
+struct PairRes {
+    int32_t c[5];  // [crossing points, pieces, IN, OUT, BORDER]
+    double area;
+    uint64_t bb, tests, raw;
+    bool over;
+};
+
+// Path point k of an area piece: 0 = a, 1..nvert = ring vertices from
+// `first` (cyclic over m), nvert + 1 = b.
+__device__ __forceinline__ void path_pt(const float *X, const float *Y, int m, int first, int nvert, float ax,
+                                        float ay, float bx, float by, int k, float &x, float &y) {
+    if (k == 0) {
+        x = ax;
+        y = ay;
+    } else if (k == nvert + 1) {
+        x = bx;
+        y = by;
+    } else {
+        int v = first + k - 1;
+        if (v >= m) v -= m;
+        x = X[v];
+        y = Y[v];
+    }
+}
+
+// One ring's pieces: D[0, n) sorted by this ring's (edge, parameter).
+// Counts the reference pieces by status into r and returns this lane's part
+// of the cross-product sum of the area pieces IN the other polygon.
+template <bool SRC>
+__device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const farm_raster_hdr &hq, const Rec *D,
+                              uint32_t n, double ox, double oy, int lane, PairRes &r, uint32_t &pieces) {
+    const float *X = S.x + hp.vert_start, *Y = S.y + hp.vert_start;
+    const int m = (int)hp.nverts - 1;
+    const double eps = (double)1e-6f;  // util.h:44
+    double acc = 0.0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        bool piece = false, in_area = false;
+        int st = 0, first = 0, nvert = 0;
+        float ax = 0, ay = 0, bx = 0, by = 0;
+        if (i < n) {
+            const Rec a = D[i], b = D[(i + 1) % n];
+            ax = a.px;
+            ay = a.py;
+            bx = b.px;
+            by = b.py;
+            if (!pt_eq(a.px, a.py, b.px, b.py)) {
+                piece = true;
+                // the reference's segment (Ideal.cpp:1559-1581, 1592-1614)
+                const int ea0 = SRC ? a.ei : a.ej, eb0 = SRC ? b.ei : b.ej;
+                const double ta0 = SRC ? a.t : a.u, tb0 = SRC ? b.t : b.u;
+                int ae = ea0, be = eb0;
+                double ap = ta0, bp = tb0;
+                if (fabs(ap - 1.0) < eps) {
+                    ae = (ae + 1) % m;
+                    ap = 0.0;
+                }
+                if (fabs(bp) < eps) {
+                    be--;
+                    bp = 1.0;
+                }
+                float rx, ry;
+                if (ae == be && ap < bp) {
+                    rx = (a.px + b.px) * 0.5f;
+                    ry = (a.py + b.py) * 0.5f;
+                } else {
+                    rx = X[ae + 1];
+                    ry = Y[ae + 1];
+                }
+                st = seg_contain(S, hq, rx, ry);
+                // the area piece: a -> ring vertices ea0 + 1 .. eb0 -> b
+                if (i + 1 < n) nvert = eb0 - ea0;
+                else nvert = (eb0 - ea0 + m) % m + ((eb0 == ea0 && tb0 <= ta0) ? m : 0);
+                if (nvert < 0) nvert = 0;
+                first = ea0 + 1;
+                const int hh = (nvert + 1) / 2;
+                float ux, uy, vx, vy;
+                path_pt(X, Y, m, first, nvert, ax, ay, bx, by, hh, ux, uy);
+                path_pt(X, Y, m, first, nvert, ax, ay, bx, by, hh + 1, vx, vy);
+                in_area = seg_contain(S, hq, (ux + vx) * 0.5f, (uy + vy) * 0.5f) == 2;
+            }
+        }
+        const uint64_t pm = __ballot(piece);
+        r.c[1] += __popcll(pm);
+        r.c[2] += __popcll(__ballot(piece && st == 2));
+        r.c[3] += __popcll(__ballot(piece && st == 0));
+        r.c[4] += __popcll(__ballot(piece && st == 1));
+        pieces += __popcll(pm);
+        // wave-cooperative path sums of this chunk's IN pieces
+        uint64_t inm = __ballot(in_area);
+        while (inm) {
+            const int src = __ffsll((unsigned long long)inm) - 1;
+            inm &= inm - 1;
+            const int f = __shfl(first, src, 64), nv = __shfl(nvert, src, 64);
+            const float pax = __shfl(ax, src, 64), pay = __shfl(ay, src, 64);
+            const float pbx = __shfl(bx, src, 64), pby = __shfl(by, src, 64);
+            for (int k = lane; k <= nv; k += 64) {
+                float x0, y0, x1, y1;
+                path_pt(X, Y, m, f, nv, pax, pay, pbx, pby, k, x0, y0);
+                path_pt(X, Y, m, f, nv, pax, pay, pbx, pby, k + 1, x1, y1);
+                acc += ((double)x0 - ox) * ((double)y1 - oy) - ((double)x1 - ox) * ((double)y0 - oy);
+            }
+        }
+    }
+    return acc;
+}
+
+// Ideal::intersection for one pair plus the area; R / D: the wave's two
+// record slabs of cap records each (cap a power of two).
+__device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const double *ring2a, uint32_t ia, uint32_t ib,
+                                                     Rec *R, Rec *D, uint32_t cap, int lane) {
+    PairRes r;
+    for (int k = 0; k < 5; k++) r.c[k] = 0;
+    r.area = 0.0;
+    r.bb = r.tests = r.raw = 0;
+    r.over = false;
+    const farm_raster_hdr &hA = S.hdr[ia];
+    const farm_raster_hdr &hB = S.hdr[ib];
+    // 1. source pixels over the target MBB (retrieve_pixels, +-1e-6)
+    const Window w = window_of(hA, hB.mbr[0], hB.mbr[1], hB.mbr[2], hB.mbr[3]);
+    const int wx = w.x1 - w.x0 + 1, wy = w.y1 - w.y0 + 1;
+    const uint32_t npx = (wx > 0 && wy > 0) ? (uint32_t)(wx * wy) : 0u;
+    uint32_t etn = 0, itn = 0;
+    for (uint32_t c0 = 0; c0 < npx; c0 += 64) {
+        const uint32_t k = c0 + lane;
+        uint8_t st = 1;
+        if (k < npx) {
+            const int x = w.x0 + (int)(k / wy), y = w.y0 + (int)(k % wy);
+            st = S.status[hA.pix_start + y * hA.dimx + x];
+        }
+        etn += __popcll(__ballot(k < npx && st == 0));
+        itn += __popcll(__ballot(k < npx && st == kCC - 1));
+    }
+    if (etn == npx) {
+        r.c[0] = -1;
+        return r;
+    }
+    if (itn == npx) {
+        r.c[0] = -2;
+        r.area = fabs(ring2a[ib]) / 2.0;
+        return r;
+    }
+    // 2. crossing points of the edges of BORDER x BORDER pixel pairs
+    const float *AX = S.x + hA.vert_start, *AY = S.y + hA.vert_start;
+    const float *BX = S.x + hB.vert_start, *BY = S.y + hB.vert_start;
+    uint32_t nrec = 0;
+    for (int x = w.x0; x <= w.x1; x++) {
+        for (int y = w.y0; y <= w.y1; y++) {
+            const uint32_t pa = (uint32_t)(y * hA.dimx + x);
+            if (show(S.status[hA.pix_start + pa]) != 1) continue;
+            double lx, ly, hx, hy;
+            pixel_box(hA, x, y, lx, ly, hx, hy);
+            const Window v = window_of(hB, lx, ly, hx, hy);
+            for (int x2 = v.x0; x2 <= v.x1; x2++) {
+                for (int y2 = v.y0; y2 <= v.y1; y2++) {
+                    const uint32_t pb = (uint32_t)(y2 * hB.dimx + x2);
+                    if (show(S.status[hB.pix_start + pb]) != 1) continue;
+                    r.bb++;
+                    const uint32_t sa0 = S.pixoff[hA.pixoff_start + pa], sa1 = S.pixoff[hA.pixoff_start + pa + 1];
+                    const uint32_t sb0 = S.pixoff[hB.pixoff_start + pb], sb1 = S.pixoff[hB.pixoff_start + pb + 1];
+                    for (uint32_t sa = sa0; sa < sa1; sa++) {
+                        const uint2 e1 = S.seqs[hA.seq_start + sa];
+                        for (uint32_t sb = sb0; sb < sb1; sb++) {
+                            const uint2 e2 = S.seqs[hB.seq_start + sb];
+                            const uint32_t la = e1.y, lb = e2.y;
+                            const uint64_t tot = (uint64_t)la * lb;
+                            r.tests += tot;
+                            if (tot == 0) continue;
+                            uint32_t i = lane / lb, j = lane - (lane / lb) * lb;
+                            const uint32_t q = 64 / lb, rr = 64 - q * lb;
+                            const uint64_t rounds = (tot + 63) / 64;
+                            for (uint64_t it = 0; it < rounds; it++) {
+                                bool hit = false;
+                                Rec rec;
+                                if (i < la) {
+                                    const uint32_t ea = e1.x + i, eb = e2.x + j;
+                                    const float ax = AX[ea], ay = AY[ea], a1x = AX[ea + 1], a1y = AY[ea + 1];
+                                    const float bx = BX[eb], by = BY[eb], b1x = BX[eb + 1], b1y = BY[eb + 1];
+                                    // segment_intersect_batch (geometry_computation.h:239-267)
+                                    const float d1x = a1x - ax, d1y = a1y - ay;
+                                    const float d2x = b1x - bx, d2y = b1y - by;
+                                    const float rx = bx - ax, ry = by - ay;
+                                    const float dn = d1x * d2y - d1y * d2x;
+                                    const float c2 = rx * d2y - ry * d2x;
+                                    const float c1 = rx * d1y - ry * d1x;
+                                    const float adn = fabsf(dn);
+                                    const float s2 = dn < 0.f ? -c2 : c2, s1 = dn < 0.f ? -c1 : c1;
+                                    // screen: t = c2 / dn and u = c1 / dn well inside
+                                    // [-1e-6, 1 + 1e-6] up to f32 rounding
+                                    if ((double)adn >= 1e-12 && s2 >= -1e-6f * adn && s2 <= 1.000001f * adn &&
+                                        s1 >= -1e-6f * adn && s1 <= 1.000001f * adn) {
+                                        const double denom = dn;
+                                        const double t = c2 / denom;
+                                        const double u = c1 / denom;
+                                        if (t >= -1e-9 && t <= 1 + 1e-9 && u >= -1e-9 && u <= 1 + 1e-9) {
+                                            const float tf = (float)t;
+                                            rec.px = ax + tf * d1x;
+                                            rec.py = ay + tf * d1y;
+                                            rec.ei = (int32_t)ea;
+                                            rec.ej = (int32_t)eb;
+                                            rec.t = t;
+                                            rec.u = u;
+                                            hit = true;
+                                        }
+                                    }
+                                }
+                                const uint64_t hm = __ballot(hit);
+                                if (hm) {
+                                    const uint32_t pos = nrec + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull));
+                                    if (hit) {
+                                        if (pos < cap) R[pos] = rec;
+                                    }
+                                    nrec += (uint32_t)__popcll(hm);
+                                }
+                                i += q;
+                                j += rr;
+                                if (j >= lb) {
+                                    j -= lb;
+                                    i++;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    r.raw = nrec;
+    if (nrec > cap) {
+        r.over = true;
+        return r;
+    }
+    wave_sync();
+    const double ox = fmax(hA.mbr[0], hB.mbr[0]), oy = fmax(hA.mbr[1], hB.mbr[1]);
+    uint32_t nk = 0;
+    if (nrec > 0) {
+        // 3. sort by point, keep the first of each run of equal points
+        bitonic<0>(R, nrec, lane);
+        bool small = false;
+        for (uint32_t k = lane; k < nrec; k += 64) {
+            const Rec e = R[k];
+            small |= fabsf(e.px) < 0.015625f || fabsf(e.py) < 0.015625f;
+        }
+        if (!wave_any(small)) {
+            // above 2^-6 the 1e-9 tolerance is float equality: a record is
+            // kept when it differs from its predecessor
+            for (uint32_t c0 = 0; c0 < nrec; c0 += 64) {
+                const uint32_t k = c0 + lane;
+                bool keep = false;
+                Rec e;
+                if (k < nrec) {
+                    e = R[k];
+                    keep = k == 0 || !pt_eq(R[k - 1].px, R[k - 1].py, e.px, e.py);
+                }
+                const uint64_t km = __ballot(keep);
+                if (keep) D[nk + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))] = e;
+                nk += (uint32_t)__popcll(km);
+            }
+        } else if (lane == 0) {
+            // std::unique: compare with the last kept record
+            for (uint32_t k = 0; k < nrec; k++) {
+                const Rec e = R[k];
+                if (nk == 0 || !pt_eq(D[nk - 1].px, D[nk - 1].py, e.px, e.py)) D[nk++] = e;
+            }
+        }
+        nk = uni(__shfl(nk, 0, 64));
+        wave_sync();
+    }
+    r.c[0] = (int32_t)nk;
+    uint32_t pieces = 0;
+    double acc_a = 0.0, acc_b = 0.0;
+    if (nk > 0) {
+        bitonic<1>(D, nk, lane);
+        acc_a = ring_pieces<true>(S, hA, hB, D, nk, ox, oy, lane, r, pieces);
+        wave_sync();
+        bitonic<2>(D, nk, lane);
+        acc_b = ring_pieces<false>(S, hB, hA, D, nk, ox, oy, lane, r, pieces);
+    }
+    if (pieces == 0) {
+        // containment or disjoint
+        if (seg_contain(S, hB, AX[0], AY[0]) == 2) r.area = fabs(ring2a[ia]) / 2.0;
+        else if (seg_contain(S, hA, BX[0], BY[0]) == 2) r.area = fabs(ring2a[ib]) / 2.0;
+        return r;
+    }
+    const double sa = ring2a[ia] < 0 ? -1.0 : 1.0, sb = ring2a[ib] < 0 ? -1.0 : 1.0;
+    const double side_a = wave_sum(acc_a), side_b = wave_sum(acc_b);
+    r.area = fabs(sa * side_a + sb * side_b) / 2.0;
+    return r;
+}
+
+}  // namespace
+
+// Twice the signed area of every ring, relative to its MBB's low corner
+// (one wave per polygon).
+__global__ __launch_bounds__(256) void k_ring_area(DevSet S, int64_t npolys, double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= npolys) return;
+    const farm_raster_hdr &h = S.hdr[p];
+    const float *X = S.x + h.vert_start, *Y = S.y + h.vert_start;
+    const double ox = h.mbr[0], oy = h.mbr[1];
+    double acc = 0.0;
+    for (uint32_t k = lane; k + 1 < h.nverts; k += 64)
+        acc += ((double)X[k] - ox) * ((double)Y[k + 1] - oy) - ((double)X[k + 1] - ox) * ((double)Y[k] - oy);
+    acc = wave_sum(acc);
+    if (lane == 0) out[p] = acc;
+}
+
+// Persistent waves over pairs; BIG: the pairs in `list` with larger slabs.
+template <bool BIG>
+__global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__restrict__ ring2a,
+                                                      const uint32_t *__restrict__ pa, const uint32_t *__restrict__ pb,
+                                                      uint32_t n, const uint32_t *__restrict__ list,
+                                                      int32_t *__restrict__ counts, double *__restrict__ area,
+                                                      Rec *__restrict__ slab, uint32_t cap,
+                                                      uint32_t *__restrict__ ctr, uint32_t *__restrict__ ovf,
+                                                      unsigned long long *__restrict__ stats) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t gw = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    Rec *R = slab + gw * 2ull * cap;
+    Rec *D = R + cap;
+    unsigned long long bb = 0, tests = 0, raw = 0, found = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ctr[0], kChunk);
+        base = uni(__shfl(base, 0, 64));
+        if (base >= n) break;
+        const uint32_t end = min(base + kChunk, n);
+        for (uint32_t k = base; k < end; k++) {
+            const uint32_t idx = BIG ? list[k] : k;
+            const uint32_t ia = pa[idx], ib = pb[idx];
+            if ((S.hdr[ia].flags | S.hdr[ib].flags) & FARM_HDR_REJECTED) {
+                if (lane < 5) counts[5ull * idx + lane] = lane == 0 ? -3 : 0;
+                if (lane == 0) area[idx] = 0.0;
+                continue;
+            }
+            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, lane);
+            if (r.over) {
+                if (lane == 0) {
+                    if (BIG) {
+                        counts[5ull * idx] = -4;
+                    } else {
+                        const uint32_t slot = atomicAdd(&ctr[1], 1u);
+                        ovf[slot] = idx;
+                    }
+                }
+                continue;
+            }
+            bb += r.bb;
+            tests += r.tests;
+            raw += r.raw;
+            found += r.area > 0.0;
+            if (lane < 5) counts[5ull * idx + lane] = r.c[lane];
+            if (lane == 0) area[idx] = r.area;
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[0], bb);
+        atomicAdd(&stats[1], tests);
+        atomicAdd(&stats[2], raw);
+        atomicAdd(&stats[3], found);
+    }
+}
+
+template __global__ void k_intersection<false>(DevSet, const double *, const uint32_t *, const uint32_t *, uint32_t,
+                                               const uint32_t *, int32_t *, double *, Rec *, uint32_t, uint32_t *,
+                                               uint32_t *, unsigned long long *);
+template __global__ void k_intersection<true>(DevSet, const double *, const uint32_t *, const uint32_t *, uint32_t,
+                                              const uint32_t *, int32_t *, double *, Rec *, uint32_t, uint32_t *,
+                                              uint32_t *, unsigned long long *);
+
+}  // namespace farm
+
+using namespace farm;
+
+namespace {
+
+int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,
+                     farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
+    if (h->has_layers) return fail(h, FARM_ESTATE, "intersection needs a set rasterized without -h");
+    if (begin > end || end > h->npairs)
+        return fail(h, FARM_EINVAL, "pair window [%llu,%llu) outside [0,%llu)", (unsigned long long)begin,
+                    (unsigned long long)end, (unsigned long long)h->npairs);
+    const uint64_t n64 = end - begin;
+    if (n64 > 0xffffffffull) return fail(h, FARM_ECAPACITY, "batch larger than 2^32 pairs");
+    const uint32_t n = (uint32_t)n64;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (st) memset(st, 0, sizeof *st);
+    if (st) st->pairs = n;
+    if (n == 0) return FARM_OK;
+    int rc;
+    DevSet S = devset(h);
+    if (!h->ring_area_ready) {
+        if ((rc = ensure(h, h->ring2a, 8ull * h->npolys))) return rc;
+        hipLaunchKernelGGL(k_ring_area, dim3((unsigned)((h->npolys + 3) / 4)), dim3(256), 0, h->stream, S, h->npolys,
+                           P<double>(h->ring2a));
+        HIPCHK(h, hipGetLastError());
+        h->ring_area_ready = true;
+    }
+    const uint32_t *pa = P<uint32_t>(h->pa) + begin;
+    const uint32_t *pb = P<uint32_t>(h->pb) + begin;
+    if ((rc = ensure(h, h->wctr, 64))) return rc;
+    if ((rc = ensure(h, h->wovf, 4ull * n))) return rc;
+    uint32_t *ctr = P<uint32_t>(h->wctr);
+    unsigned long long *ist = reinterpret_cast<unsigned long long *>(ctr + 8);
+    if ((rc = fill_u32(h, ctr, 16, 0))) return rc;
+    const unsigned nblocks = (unsigned)h->num_cus * 4;
+    const uint64_t nwaves = (uint64_t)nblocks * kWaves;
+    if ((rc = ensure(h, h->wslab, nwaves * 2ull * kRecCap * sizeof(Rec)))) return rc;
+    HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+    hipLaunchKernelGGL(k_intersection<false>, dim3(nblocks), dim3(64 * kWaves), 0, h->stream, S,
+                       (const double *)P<double>(h->ring2a), pa, pb, n, (const uint32_t *)nullptr, counts_dev, area_dev,
+                       P<Rec>(h->wslab), kRecCap, ctr, P<uint32_t>(h->wovf), ist);
+    HIPCHK(h, hipGetLastError());
+    uint32_t novf = 0;
+    HIPCHK(h, hipMemcpyAsync(&novf, ctr + 1, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (novf) {
+        // pairs with more raw crossing records than the first slabs hold
+        const unsigned bblocks = 16;
+        if ((rc = ensure(h, h->wslab_big, (uint64_t)bblocks * kWaves * 2ull * kRecCapBig * sizeof(Rec)))) return rc;
+        if ((rc = fill_u32(h, ctr, 1, 0))) return rc;
+        hipLaunchKernelGGL(k_intersection<true>, dim3(bblocks), dim3(64 * kWaves), 0, h->stream, S,
+                           (const double *)P<double>(h->ring2a), pa, pb, novf, (const uint32_t *)P<uint32_t>(h->wovf),
+                           counts_dev, area_dev, P<Rec>(h->wslab_big), kRecCapBig, ctr, P<uint32_t>(h->wovf), ist);
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+    unsigned long long hst[4] = {0, 0, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(hst, ist, sizeof hst, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (novf) {
+        std::vector<int32_t> tmp(5ull * n);
+        HIPCHK(h, hipMemcpy(tmp.data(), counts_dev, 20ull * n, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < n; k++)
+            if (tmp[5ull * k] == -4)
+                return fail(h, FARM_ECAPACITY, "pair %llu: more than %u crossing records",
+                            (unsigned long long)(begin + k), kRecCapBig);
+    }
+    if (st) {
+        float ms = 0;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+        st->bb_pixel_pairs = hst[0];
+        st->segment_tests = hst[1];
+        st->refine_items = hst[2];
+        st->found = hst[3];
+        st->refine_ms = ms;
+        st->total_ms = ms;
+    }
+    return FARM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int farm_intersection_area_dev(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,
+                               farm_stats *st) {
+    if (!counts_dev || !area_dev) return fail(h, FARM_EINVAL, "null output buffer");
+    return run_intersection(h, begin, end, counts_dev, area_dev, st);
+}
+
+int farm_intersection_area(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts, double *area,
+                           farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (begin > end) return fail(h, FARM_EINVAL, "bad pair window");
+    const uint64_t n = end - begin;
+    int rc;
+    if ((rc = ensure(h, h->icounts, 20 * n + 16))) return rc;
+    if ((rc = ensure(h, h->iarea, 8 * n + 16))) return rc;
+    if ((rc = run_intersection(h, begin, end, P<int32_t>(h->icounts), P<double>(h->iarea), st))) return rc;
+    if (n && counts) HIPCHK(h, hipMemcpy(counts, h->icounts.p, 20 * n, hipMemcpyDeviceToHost));
+    if (n && area) HIPCHK(h, hipMemcpy(area, h->iarea.p, 8 * n, hipMemcpyDeviceToHost));
+    return FARM_OK;
+}
+
+}  // extern "C"

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "handle.h"
+#include "wave.h"
 
 namespace farm {
 
@@ -170,30 +171,9 @@ __device__ __forceinline__ int classify(double area, double pixel_area, int coun
     return idx;
 }
 
-// Orders one wave's own LDS / global writes before its later reads by other
-// lanes (the candidate slab is global memory).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
 
-// Values every lane holds alike, moved to scalar registers.
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ double uni(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return uni(v);
-}
 
-__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
 __device__ __forceinline__ bool tuple_less(const Cand &a, const Cand &b) {
     if (a.apx != b.apx) return a.apx < b.apx;

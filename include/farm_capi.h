@@ -203,6 +203,23 @@ int farm_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_
 int farm_within_dev(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t *out_dev,
                     double *min_dist_dev, farm_stats *stats);
 
+/* Intersection over pairs [begin, end) of a set rasterized without -h
+ * (replaces cuda_intersection, src/cuda/query_intersection.cu:532-754, with
+ * the CPU semantics of Ideal::intersection, src/geometry/Ideal.cpp:1440-1683,
+ * which stops after classifying the boundary pieces).  counts (5 per pair,
+ * may be NULL): crossing points after de-duplication (-1: every source pixel
+ * over the target MBB is OUT, -2: every one IN, -3: a polygon is rejected by
+ * the raster builder), boundary pieces, pieces IN / OUT / on the boundary of
+ * the other polygon.  area (may be NULL): area of A n B, integrated over the
+ * boundary pieces IN the other polygon (Green's theorem, f64); no reference
+ * counterpart.  stats: bb_pixel_pairs, segment_tests (edge pairs tested),
+ * refine_items (crossing records before de-duplication), found (pairs with
+ * area > 0). */
+int farm_intersection_area(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts, double *area,
+                           farm_stats *stats);
+int farm_intersection_area_dev(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,
+                               farm_stats *stats);
+
 /* ---- learned part (no reference counterpart) ----
  * Pair features: FARM_NFEAT floats per candidate pair from the filter's
  * pixel windows: 64-bin histogram of BORDER x BORDER fill-ratio products,

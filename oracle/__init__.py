@@ -43,6 +43,10 @@ def _load() -> C.CDLL:
         "orc_candidates_within": (C.c_int64, [C.c_void_p, C.c_double, C.c_int64, C.c_void_p, C.c_void_p]),
         "orc_mbb": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
         "orc_failed": (C.c_int, [C.c_void_p, C.c_int64]),
+        "orc_intersection": (
+            C.c_int,
+            [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int],
+        ),
         "orc_layer": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_segment_intersect": (C.c_int, [C.c_void_p]),
         "orc_double_to_int": (C.c_int, [C.c_double]),
@@ -170,6 +174,22 @@ def within(a: OracleSet, b: OracleSet, pa: np.ndarray, pb: np.ndarray, d_km: flo
     if rc == ORC_EINVAL:
         raise ValueError("pair index out of range")
     return out, md, {"candidate_pixel_pairs": int(stats[0]), "distance_evals": int(stats[1])}
+
+
+def intersection(a: OracleSet, b: OracleSet, pa: np.ndarray, pb: np.ndarray,
+                 nthreads: int = 8) -> tuple[np.ndarray, np.ndarray]:
+    """Ideal::intersection per pair up to its boundary-piece statuses, and the
+    area of A n B.  Returns (counts (n, 5): [intersection points (-1 window
+    all OUT, -2 all IN, -3 the reference aborts), pieces, IN, OUT, BORDER],
+    areas (n,))."""
+    pa = np.ascontiguousarray(pa, dtype=np.uint32)
+    pb = np.ascontiguousarray(pb, dtype=np.uint32)
+    counts = np.zeros((len(pa), 5), dtype=np.int32)
+    area = np.zeros(len(pa), dtype=np.float64)
+    rc = lib.orc_intersection(a._h, b._h, len(pa), _p(pa), _p(pb), _p(counts), _p(area), nthreads)
+    if rc == ORC_EINVAL:
+        raise ValueError("pair index out of range")
+    return counts, area
 
 
 def candidates_within(s: OracleSet, d_km: float = 10.0) -> tuple[np.ndarray, np.ndarray]:

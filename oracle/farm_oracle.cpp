@@ -1212,6 +1212,268 @@ static bool within(const Poly &S, const Poly &T, double d, QStats &qs, double *m
     return false;
 }
 
+/* ---------------------------------------------------------------------- */
+/* Intersection (config 3)                                                 */
+/* ---------------------------------------------------------------------- */
+/* Intersection record (Point.h:132-139) */
+struct IxRec {
+    Pt p;
+    int ei, ej;
+    double t, u;
+};
+
+/* segment_intersect_batch(..., inters) (geometry_computation.h:239-267):
+ * f32 edge vectors and crosses, f64 parameters, f32 intersection point */
+static void segment_intersect_collect(const Pt *p1, const Pt *p2, int s1, int s2, int e1, int e2,
+                                      std::vector<IxRec> &out) {
+    for (int i = s1; i < e1; i++) {
+        for (int j = s2; j < e2; j++) {
+            const float d1x = p1[i + 1].x - p1[i].x, d1y = p1[i + 1].y - p1[i].y;
+            const float d2x = p2[j + 1].x - p2[j].x, d2y = p2[j + 1].y - p2[j].y;
+            const float rx = p2[j].x - p1[i].x, ry = p2[j].y - p1[i].y;
+            const float dn = d1x * d2y - d1y * d2x;
+            const double denom = dn;
+            if (std::abs(denom) < 1e-12) continue;
+            const float c2 = rx * d2y - ry * d2x;
+            const float c1 = rx * d1y - ry * d1x;
+            const double t = c2 / denom;
+            const double u = c1 / denom;
+            if (t >= -1e-9 && t <= 1 + 1e-9 && u >= -1e-9 && u <= 1 + 1e-9) {
+                const float tf = (float)t; /* Point::operator*(const float &) */
+                IxRec r;
+                r.p.x = p1[i].x + tf * d1x;
+                r.p.y = p1[i].y + tf * d1y;
+                r.ei = i;
+                r.ej = j;
+                r.t = t;
+                r.u = u;
+                out.push_back(r);
+            }
+        }
+    }
+}
+
+/* Ideal::segment_contain (Ideal.cpp:1233-1327) with count_intersection_nodes
+ * (Ideal.cpp:988-1006): 0 OUT, 1 BORDER (on the boundary), 2 IN */
+static int segment_contain(const Poly &P, const Pt &p) {
+    const Grid &g = P.g;
+    const int px = g.off_x(p.x), py = g.off_y(p.y);
+    const int id = g.id(px, py);
+    const Box bx = g.pixel_box(px, py);
+    const int st = g.show(id);
+    if (st == ST_IN) return 2;
+    if (st == ST_OUT) return 0;
+    bool ret = false;
+    const uint32_t ns = P.num_seqs(id);
+    for (uint32_t e = 0; e < ns; e++) {
+        const auto sq = P.seqs[P.offset[id] + e];
+        const uint32_t pos = sq.first;
+        for (uint32_t k = 0; k < sq.second; k++) {
+            const Pt &v1 = P.p[pos + k];
+            const Pt &v2 = P.p[pos + k + 1];
+            if (pt_eq(p, v1) || pt_eq(p, v2)) return 1;
+            if ((v1.y >= p.y) != (v2.y >= p.y)) {
+                const double dx = v2.x - v1.x; /* f32 differences, widened */
+                const double dy = v2.y - v1.y;
+                const double py_diff = p.y - v1.y;
+                if (std::abs(dy) > 1e-9) {
+                    const double int_x = dx * py_diff / dy + v1.x;
+                    if (std::fabs(p.x - int_x) < 1e-9) return 1;
+                    if (p.x < int_x && int_x <= bx.high[0]) ret = !ret;
+                }
+            } else if (v1.y == p.y && v2.y == p.y && (v1.x >= p.x) != (v2.x >= p.x)) {
+                return 1;
+            }
+        }
+    }
+    /* crossing nodes of the vertical grid line right of the pixel */
+    int count = 0;
+    const int x = px + 1;
+    uint32_t i = P.voff[x], j = P.voff[x + 1];
+    while (i < j && P.vnodes[i] < p.y) {
+        count++;
+        i++;
+    }
+    if (count % 2 == 1) ret = !ret;
+    return ret ? 2 : 0;
+}
+
+/* Cross-product sum of the path a -> ring vertices -> b relative to origin o,
+ * and its representative point: the midpoint (f32) of the middle sub-segment
+ * of the path.  first/last: ring vertex range (cyclic over nv - 1 vertices),
+ * nvert vertices in it. */
+static double path_cross(const Poly &P, const Pt &a, const Pt &b, int first, int nvert, double ox, double oy,
+                         Pt *rep) {
+    const int m = P.nv() - 1;
+    auto at = [&](int k) -> Pt { /* path point k: 0 = a, 1..nvert = vertices, nvert + 1 = b */
+        if (k == 0) return a;
+        if (k == nvert + 1) return b;
+        return P.p[(first + k - 1) % m];
+    };
+    double sum = 0.0;
+    for (int k = 0; k <= nvert; k++) {
+        const Pt u = at(k), v = at(k + 1);
+        sum += ((double)u.x - ox) * ((double)v.y - oy) - ((double)v.x - ox) * ((double)u.y - oy);
+    }
+    const int h = (nvert + 1) / 2;
+    const Pt u = at(h), v = at(h + 1);
+    rep->x = (u.x + v.x) * 0.5f;
+    rep->y = (u.y + v.y) * 0.5f;
+    return sum;
+}
+
+/* Shoelace sum of the whole ring relative to its own MBB's low corner
+ * (signed, twice the area) */
+static double ring_cross(const Poly &P) {
+    const double ox = P.mbb0.low[0], oy = P.mbb0.low[1];
+    double sum = 0.0;
+    for (int k = 0; k + 1 < P.nv(); k++)
+        sum += ((double)P.p[k].x - ox) * ((double)P.p[k + 1].y - oy) -
+               ((double)P.p[k + 1].x - ox) * ((double)P.p[k].y - oy);
+    return sum;
+}
+
+struct IxOut {
+    int32_t n_inters = 0; /* -1: every source pixel in the window OUT, -2: every one IN */
+    int32_t n_segments = 0, n_in = 0, n_out = 0, n_border = 0;
+    double area = 0.0;
+};
+
+/* Ideal::intersection (Ideal.cpp:1440-1683) up to the segment statuses (the
+ * reference returns there, before any polygon assembly), plus the area of
+ * A n B by Green's theorem over the classified boundary pieces:
+ *   area = |sA * sum(A pieces inside B) + sB * sum(B pieces inside A)| / 2
+ * where a piece runs along one ring between consecutive intersection points
+ * (ordered by edge, then parameter), its sum is the cross-product sum of its
+ * path relative to the low corner of the two MBBs' overlap, it counts when
+ * the midpoint of its middle sub-segment is IN the other polygon
+ * (segment_contain; BORDER pieces are not credited), and sX = sign of ring
+ * X's signed area.  With fewer than two distinct intersection points the
+ * area is area(A) if A's first vertex is IN B, area(B) if B's is IN A, else
+ * 0; a window of only IN source pixels gives area(B), only OUT 0.  Whole
+ * ring areas are shoelace sums relative to the ring's own MBB low corner. */
+static IxOut intersection(const Poly &A, const Poly &B) {
+    IxOut o;
+    const double ox = std::max(A.mbr.low[0], B.mbr.low[0]);
+    const double oy = std::max(A.mbr.low[1], B.mbr.low[1]);
+    std::vector<int> pxs, tpxs;
+    A.g.retrieve(B.mbr, pxs);
+    int etn = 0, itn = 0;
+    for (int p : pxs) {
+        if (A.g.show(p) == ST_OUT) etn++;
+        else if (A.g.show(p) == ST_IN) itn++;
+    }
+    if (etn == (int)pxs.size()) {
+        o.n_inters = -1;
+        return o;
+    }
+    if (itn == (int)pxs.size()) {
+        o.n_inters = -2;
+        o.area = std::fabs(ring_cross(B)) / 2.0;
+        return o;
+    }
+    std::vector<IxRec> in;
+    for (int p : pxs) {
+        const Box bx = A.g.pixel_box(A.g.gx(p), A.g.gy(p));
+        B.g.retrieve(bx, tpxs);
+        for (int p2 : tpxs) {
+            if (A.g.show(p) != ST_BORDER || B.g.show(p2) != ST_BORDER) continue;
+            for (uint32_t i = 0; i < A.num_seqs(p); i++) {
+                const auto r = A.seqs[A.offset[p] + i];
+                for (uint32_t j = 0; j < B.num_seqs(p2); j++) {
+                    const auto r2 = B.seqs[B.offset[p2] + j];
+                    REF_ASSERT(r.second != 0 && r2.second != 0);
+                    segment_intersect_collect(A.p.data(), B.p.data(), (int)r.first, (int)r2.first,
+                                              (int)(r.first + r.second), (int)(r2.first + r2.second), in);
+                }
+            }
+        }
+    }
+    /* sort by point, drop consecutive equal points (std::sort + std::unique;
+     * records with identical points are ordered by (ei, ej, t, u) here, which
+     * the reference leaves unspecified) */
+    auto key_pt = [](const IxRec &a, const IxRec &b) {
+        if (a.p.x != b.p.x) return a.p.x < b.p.x;
+        if (a.p.y != b.p.y) return a.p.y < b.p.y;
+        if (a.ei != b.ei) return a.ei < b.ei;
+        if (a.ej != b.ej) return a.ej < b.ej;
+        if (a.t != b.t) return a.t < b.t;
+        return a.u < b.u;
+    };
+    std::sort(in.begin(), in.end(), key_pt);
+    std::vector<IxRec> ix;
+    for (const IxRec &r : in)
+        if (ix.empty() || !pt_eq(ix.back().p, r.p)) ix.push_back(r);
+    const int n = (int)ix.size();
+    o.n_inters = n;
+    const double eps = (double)1e-6f; /* util.h:44 */
+    double side[2] = {0.0, 0.0};
+    int pieces = 0;
+    for (int s = 0; s < 2; s++) {
+        const bool src = s == 0;
+        const Poly &P = src ? A : B;
+        const Poly &Q = src ? B : A;
+        std::sort(ix.begin(), ix.end(), [src](const IxRec &a, const IxRec &b) {
+            const int ea = src ? a.ei : a.ej, eb = src ? b.ei : b.ej;
+            const double ta = src ? a.t : a.u, tb = src ? b.t : b.u;
+            if (ea != eb) return ea < eb;
+            if (ta != tb) return ta < tb;
+            return src ? (a.ej != b.ej ? a.ej < b.ej : a.u < b.u) : (a.ei != b.ei ? a.ei < b.ei : a.t < b.t);
+        });
+        const int m = P.nv() - 1;
+        for (int i = 0; i < n; i++) {
+            const IxRec &a = ix[i], &b = ix[(i + 1) % n];
+            if (pt_eq(a.p, b.p)) continue;
+            /* the reference's segment (Ideal.cpp:1559-1581 / 1592-1614) */
+            int ae = src ? a.ei : a.ej, be = src ? b.ei : b.ej;
+            double ap = src ? a.t : a.u, bp = src ? b.t : b.u;
+            if (std::fabs(ap - 1.0) < eps) {
+                ae = (ae + 1) % m;
+                ap = 0.0;
+            }
+            if (std::fabs(bp) < eps) {
+                be--;
+                bp = 1.0;
+            }
+            const bool same = ae == be && ap < bp;
+            const int edge_start = same ? -1 : ae + 1;
+            Pt rp;
+            if (edge_start == -1) {
+                rp.x = (a.p.x + b.p.x) * 0.5f;
+                rp.y = (a.p.y + b.p.y) * 0.5f;
+            } else {
+                rp = P.p[edge_start];
+            }
+            const int st = segment_contain(Q, rp);
+            o.n_segments++;
+            if (st == 2) o.n_in++;
+            else if (st == 0) o.n_out++;
+            else o.n_border++;
+            /* the area piece: a -> vertices (ea0 + 1 .. eb0) -> b along the ring */
+            const int ea0 = src ? a.ei : a.ej, eb0 = src ? b.ei : b.ej;
+            const double ta0 = src ? a.t : a.u, tb0 = src ? b.t : b.u;
+            int nvert;
+            if (i + 1 < n) nvert = eb0 - ea0;                      /* later in ring order */
+            else nvert = (eb0 - ea0 + m) % m + ((eb0 == ea0 && tb0 <= ta0) ? m : 0); /* wraps */
+            if (nvert < 0) nvert = 0;
+            Pt rep;
+            const double c = path_cross(P, a.p, b.p, ea0 + 1, nvert, ox, oy, &rep);
+            if (segment_contain(Q, rep) == 2) side[s] += c;
+            pieces++;
+        }
+    }
+    if (pieces == 0) {
+        /* no two distinct intersection points: containment or disjoint */
+        if (segment_contain(B, A.p[0]) == 2) o.area = std::fabs(ring_cross(A)) / 2.0;
+        else if (segment_contain(A, B.p[0]) == 2) o.area = std::fabs(ring_cross(B)) / 2.0;
+        return o;
+    }
+    const double sa = ring_cross(A) < 0 ? -1.0 : 1.0;
+    const double sb = ring_cross(B) < 0 ? -1.0 : 1.0;
+    o.area = std::fabs(sa * side[0] + sb * side[1]) / 2.0;
+    return o;
+}
+
 } // namespace
 
 /* ---------------------------------------------------------------------- */
@@ -1386,6 +1648,51 @@ int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint
         stats[1] = total.seg_tests;
         stats[2] = total.in_hits;
     }
+    return err.load();
+}
+
+int orc_intersection(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa, const uint32_t *pb,
+                     int32_t *counts, double *area, int nthreads) {
+    for (int64_t k = 0; k < npairs; k++)
+        if (pa[k] >= a->polys.size() || pb[k] >= b->polys.size()) return ORC_EINVAL;
+    std::atomic<int64_t> next(0);
+    std::atomic<int> err(ORC_OK);
+    auto work = [&]() {
+        for (;;) {
+            int64_t s = next.fetch_add(10);
+            if (s >= npairs) break;
+            for (int64_t k = s; k < std::min(npairs, s + 10); k++) {
+                IxOut r;
+                const Poly &A = a->polys[pa[k]], &B = b->polys[pb[k]];
+                bool bad = A.failed || B.failed;
+                if (!bad) {
+                    try {
+                        r = intersection(A, B);
+                    } catch (const RefAbort &) {
+                        bad = true;
+                    }
+                }
+                if (bad) {
+                    r = IxOut();
+                    r.n_inters = -3;
+                    r.area = 0.0;
+                    err = ORC_EREF_ABORT;
+                }
+                if (counts) {
+                    counts[5 * k + 0] = r.n_inters;
+                    counts[5 * k + 1] = r.n_segments;
+                    counts[5 * k + 2] = r.n_in;
+                    counts[5 * k + 3] = r.n_out;
+                    counts[5 * k + 4] = r.n_border;
+                }
+                if (area) area[k] = r.area;
+            }
+        }
+    };
+    if (nthreads < 1) nthreads = 1;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work);
+    for (auto &t : th) t.join();
     return err.load();
 }
 

@@ -72,6 +72,15 @@ int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint
 int orc_within(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa, const uint32_t *pb,
                double d_km, uint8_t *out, double *min_dist, int nthreads, int64_t *stats);
 
+/* Ideal::intersection per pair (source a[pa[k]], target b[pb[k]]) up to the
+ * reference's boundary-piece statuses, plus the intersection area by Green's
+ * theorem over the pieces (see farm_oracle.cpp).  counts (5 per pair, may be
+ * NULL): [intersection points after dedup (-1 window all OUT, -2 window all
+ * IN, -3 the reference aborts), pieces, IN, OUT, BORDER]; area (may be
+ * NULL): f64 area of A n B. */
+int orc_intersection(const orc_set *a, const orc_set *b, int64_t npairs, const uint32_t *pa, const uint32_t *pb,
+                     int32_t *counts, double *area, int nthreads);
+
 /* Within self-join candidates: (i, j), i != j, with i's MBB overlapping j's
  * MBB expanded by d_km (box::expand). */
 int64_t orc_candidates_within(const orc_set *s, double d_km, int64_t cap, uint32_t *pa, uint32_t *pb);

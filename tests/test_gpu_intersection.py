@@ -1,0 +1,88 @@
+"""GPU parity: intersection query (crossing points, boundary pieces and
+their statuses, and the area of A n B) through the C ABI against the oracle.
+Counts are bit-exact; areas agree to 1e-9 relative (the device sums the
+piece paths in another order)."""
+import numpy as np
+import pytest
+
+import oracle
+from farm_amd import capi, datagen
+from tests.helpers import Workload, rings_from_list
+
+pytestmark = pytest.mark.gpu
+
+AREA_RTOL = 1e-9
+AREA_ATOL = 1e-15  # deg^2: slivers whose area is pure rounding
+
+
+def _check(dev, w: Workload, window=None):
+    want_c, want_a = oracle.intersection(w.OA, w.OB, w.pa, w.pb)
+    ok = w.both.raster_ok()
+    assert np.array_equal(~ok, np.concatenate([w.OA.failed(), w.OB.failed()]))
+    w.upload(dev)
+    lo, hi = (0, len(w.pa)) if window is None else window
+    print(f"  intersection pairs={len(w.pa)} window=({lo},{hi})", flush=True)
+    c, a, st = dev.intersection_area(lo, hi)
+    want_c, want_a = want_c[lo:hi], want_a[lo:hi]
+    rows = np.nonzero((c != want_c).any(1))[0]
+    assert rows.size == 0, f"{rows.size} count rows differ, first {rows[:5]}: {c[rows[:5]]} vs {want_c[rows[:5]]}"
+    err = np.abs(a - want_a)
+    bad = np.nonzero(err > AREA_RTOL * np.abs(want_a) + AREA_ATOL)[0]
+    assert bad.size == 0, f"{bad.size} areas differ, first {bad[:5]}: {a[bad[:5]]} vs {want_a[bad[:5]]}"
+    assert st["found"] == int((want_a > 0).sum())
+    return want_c, want_a, st
+
+
+def test_complex_x_lakes_v256(device):
+    cro, cxy = datagen.complex_set(7, 12, 0.0, 2.0, nv_cap=20000)
+    lro, lxy = datagen.lake_set(8, 400, 0.0, 2.0)
+    w = Workload((cro, cxy), (lro, lxy), vpr=256)
+    c, a, st = _check(device, w)
+    assert (c[:, 0] > 0).sum() > 20 and (a > 0).sum() > 50
+
+
+def test_complex_x_lakes_v10_and_window(device):
+    cro, cxy = datagen.complex_set(17, 8, 0.0, 1.5, nv_cap=8000)
+    lro, lxy = datagen.lake_set(18, 300, 0.0, 1.5)
+    w = Workload((cro, cxy), (lro, lxy), vpr=10)
+    c, a, st = _check(device, w)
+    assert (c[:, 0] > 0).sum() > 20
+    _check(device, w, window=(len(w.pa) // 3, len(w.pa) // 3 + 200))
+
+
+def test_star_pairs(device):
+    src, tgt = datagen.star_pair_set(4, 300, 300, 0.0, 20.0)
+    w = Workload(src, tgt, vpr=10)
+    _check(device, w)
+
+
+def _rect(x0, y0, x1, y1, n=1):
+    t = np.arange(n) / n
+    e = [((x0, y0), (x1, y0)), ((x1, y0), (x1, y1)), ((x1, y1), (x0, y1)), ((x0, y1), (x0, y0))]
+    return np.vstack([np.stack([a[0] + (b[0] - a[0]) * t, a[1] + (b[1] - a[1]) * t], 1) for a, b in e]).astype(
+        np.float32)
+
+
+def test_hand_built_cases(device):
+    # crossing bars, containment, disjoint, shared edges, identical squares
+    a = [_rect(1.013, 1.401, 1.987, 1.611, 6), _rect(1.013, 1.017, 1.531, 1.329, 7),
+         _rect(3.1, 3.1, 3.2, 3.3, 3), _rect(10, 10, 20, 20), _rect(1, 1, 1.5, 1.5)]
+    b = [_rect(1.402, 1.017, 1.594, 1.993, 6), _rect(1.1, 1.1, 1.2, 1.2, 3), _rect(1.211, 0.893, 1.377, 1.611, 5),
+         _rect(20, 12, 25, 18), _rect(1, 1, 1.5, 1.5)]
+    n, m = len(a), len(b)
+    pa, pb = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    w = Workload(rings_from_list(a), rings_from_list(b), vpr=10, pairs=(pa.ravel(), pb.ravel()))
+    _check(device, w)
+
+
+def test_rejected_polygons(device):
+    # at one vertex per pixel this triangle puts a vertex on a grid corner
+    # inside one pixel row, where the reference raster builder never ends:
+    # its pairs answer -3
+    tri = np.array([[50, 50], [60, 50], [55, 58]], np.float32)
+    a = [tri, _rect(52, 49, 58, 52, 4)]
+    b = [_rect(51, 51, 59, 53, 4), tri + np.float32(0.5)]
+    pa, pb = np.meshgrid(np.arange(2), np.arange(2), indexing="ij")
+    w = Workload(rings_from_list(a), rings_from_list(b), vpr=1, pairs=(pa.ravel(), pb.ravel()))
+    c, _, _ = _check(device, w)
+    assert (c[:, 0] == -3).sum() >= 2

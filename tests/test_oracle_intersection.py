@@ -1,0 +1,168 @@
+"""Pins the oracle's intersection query (Ideal::intersection,
+src/geometry/Ideal.cpp:1440-1683, up to its boundary-piece statuses) and the
+area of A n B built on it, without the reference binary: closed forms for
+rectangles, Sutherland-Hodgman clipping for convex pairs (an independent
+algorithm), Monte Carlo estimates for star-shaped pairs, symmetry,
+containment and disjoint cases."""
+import numpy as np
+import pytest
+
+import oracle
+from farm_amd import datagen
+from tests.helpers import rings_from_list
+
+
+def _rect(x0, y0, x1, y1, n=1):
+    t = np.arange(n) / n
+    e = [((x0, y0), (x1, y0)), ((x1, y0), (x1, y1)), ((x1, y1), (x0, y1)), ((x0, y1), (x0, y0))]
+    return np.vstack([np.stack([a[0] + (b[0] - a[0]) * t, a[1] + (b[1] - a[1]) * t], 1) for a, b in e]).astype(
+        np.float32)
+
+
+def _run(polys, pairs, vpr=10):
+    O = oracle.OracleSet(*rings_from_list(polys))
+    assert O.rasterize(vpr) == -1
+    pa = np.array([p[0] for p in pairs], np.uint32)
+    pb = np.array([p[1] for p in pairs], np.uint32)
+    return oracle.intersection(O, O, pa, pb)
+
+
+def _shoelace(P):
+    P = np.asarray(P, np.float64)
+    Q = np.roll(P, -1, axis=0)
+    return 0.5 * abs(np.sum(P[:, 0] * Q[:, 1] - Q[:, 0] * P[:, 1]))
+
+
+def test_rectangles_closed_form():
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        x0, y0 = rng.uniform(1, 2, 2)
+        w0, h0 = rng.uniform(0.1, 0.6, 2)
+        x1, y1 = x0 + rng.uniform(-0.3, 0.3), y0 + rng.uniform(-0.3, 0.3)
+        w1, h1 = rng.uniform(0.1, 0.6, 2)
+        A = _rect(x0, y0, x0 + w0, y0 + h0, int(rng.integers(1, 9)))
+        B = _rect(x1, y1, x1 + w1, y1 + h1, int(rng.integers(1, 9)))
+        a32, b32 = A.astype(np.float64), B.astype(np.float64)
+        ox = max(0.0, min(a32[:, 0].max(), b32[:, 0].max()) - max(a32[:, 0].min(), b32[:, 0].min()))
+        oy = max(0.0, min(a32[:, 1].max(), b32[:, 1].max()) - max(a32[:, 1].min(), b32[:, 1].min()))
+        exact = ox * oy
+        counts, area = _run([A, B], [(0, 1), (1, 0)])
+        for k in range(2):
+            assert abs(area[k] - exact) <= 2e-6 * max(exact, 1e-3), (area[k], exact)
+
+
+def test_plus_sign_counts():
+    # two crossing bars: the 4 corners of the central square are the
+    # boundary crossings; each ring splits into 4 pieces, alternately inside
+    # and outside the other bar
+    A = _rect(1.013, 1.401, 1.987, 1.611, 6)
+    B = _rect(1.402, 1.017, 1.594, 1.993, 6)
+    counts, area = _run([A, B], [(0, 1), (1, 0)])
+    for k in range(2):
+        assert counts[k].tolist() == [4, 8, 4, 4, 0]
+    exact = (1.594 - 1.402) * (1.611 - 1.401)
+    assert np.allclose(area, exact, rtol=2e-6)
+
+
+def _regular(cx, cy, r, n, rot):
+    a = rot + 2 * np.pi * np.arange(n) / n
+    return np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], 1).astype(np.float32)
+
+
+def _sutherland_hodgman(subject, clip):
+    out = [tuple(p) for p in np.asarray(subject, np.float64)]
+    C = np.asarray(clip, np.float64)
+    for i in range(len(C)):
+        a, b = C[i], C[(i + 1) % len(C)]
+        inp, out = out, []
+        if not inp:
+            break
+
+        def inside(p):
+            return (b[0] - a[0]) * (p[1] - a[1]) - (b[1] - a[1]) * (p[0] - a[0]) >= 0
+
+        def cut(p, q):
+            x1, y1, x2, y2 = p[0], p[1], q[0], q[1]
+            x3, y3, x4, y4 = a[0], a[1], b[0], b[1]
+            den = (x1 - x2) * (y3 - y4) - (y1 - y2) * (x3 - x4)
+            t = ((x1 - x3) * (y3 - y4) - (y1 - y3) * (x3 - x4)) / den
+            return (x1 + t * (x2 - x1), y1 + t * (y2 - y1))
+
+        for j in range(len(inp)):
+            p, q = inp[j], inp[(j + 1) % len(inp)]
+            if inside(q):
+                if not inside(p):
+                    out.append(cut(p, q))
+                out.append(q)
+            elif inside(p):
+                out.append(cut(p, q))
+    return _shoelace(out) if len(out) >= 3 else 0.0
+
+
+def test_convex_pairs_match_sutherland_hodgman():
+    rng = np.random.default_rng(5)
+    polys, pairs, want = [], [], []
+    for k in range(60):
+        A = _regular(*rng.uniform(1, 2, 2), rng.uniform(0.1, 0.4), int(rng.integers(5, 40)), rng.uniform(0, 6.3))
+        B = _regular(*rng.uniform(1, 2, 2), rng.uniform(0.1, 0.4), int(rng.integers(5, 40)), rng.uniform(0, 6.3))
+        polys += [A, B]
+        pairs += [(2 * k, 2 * k + 1), (2 * k + 1, 2 * k)]
+        w = _sutherland_hodgman(A, B)
+        want += [w, w]
+    counts, area = _run(polys, pairs)
+    want = np.array(want)
+    scale = np.array([min(_shoelace(polys[p[0]]), _shoelace(polys[p[1]])) for p in pairs])
+    assert (np.abs(area - want) <= 1e-5 * scale).all(), np.max(np.abs(area - want) / scale)
+    assert (want > 0).sum() > 40 and (want == 0).sum() > 5
+
+
+def _inside(P, x, y):
+    # even-odd crossing number, vectorised over query points
+    P = np.asarray(P, np.float64)
+    Q = np.roll(P, -1, axis=0)
+    ins = np.zeros(len(x), bool)
+    for (x1, y1), (x2, y2) in zip(P, Q):
+        c = (y1 > y) != (y2 > y)
+        xi = x1 + (y - y1) * (x2 - x1) / np.where(y2 != y1, y2 - y1, 1.0)
+        ins ^= c & (x < xi)
+    return ins
+
+
+def test_star_pairs_against_monte_carlo_and_symmetry():
+    ro, xy = datagen.star_polygons(np.random.default_rng(9), 40, 1.0, 3.0, np.full(40, 120), np.log(0.5), 0.3)
+    rings = [xy[int(ro[i]):int(ro[i + 1]) - 1] for i in range(len(ro) - 1)]
+    O = oracle.OracleSet(ro, xy)
+    assert O.rasterize(10) == -1
+    pa, pb = oracle.candidates(O, O)
+    keep = pa < pb
+    pa, pb = pa[keep], pb[keep]
+    assert len(pa) > 20
+    c1, a1 = oracle.intersection(O, O, pa, pb)
+    c2, a2 = oracle.intersection(O, O, pb, pa)
+    sizes = np.array([min(_shoelace(rings[i]), _shoelace(rings[j])) for i, j in zip(pa, pb)])
+    assert (np.abs(a1 - a2) <= 1e-6 * sizes).all()
+    assert (a1 <= sizes * (1 + 1e-6)).all()
+    rng = np.random.default_rng(1)
+    for k in range(len(pa)):
+        P, Q = rings[pa[k]], rings[pb[k]]
+        lo = np.maximum(P.min(0), Q.min(0)).astype(np.float64)
+        hi = np.minimum(P.max(0), Q.max(0)).astype(np.float64)
+        if (hi <= lo).any():
+            assert a1[k] == 0
+            continue
+        n = 20000
+        x = rng.uniform(lo[0], hi[0], n)
+        y = rng.uniform(lo[1], hi[1], n)
+        frac = (_inside(P, x, y) & _inside(Q, x, y)).mean()
+        box = (hi - lo).prod()
+        est, sd = frac * box, np.sqrt(max(frac * (1 - frac), 1.0 / n) / n) * box
+        assert abs(a1[k] - est) <= 5 * sd + 1e-9, (k, a1[k], est, sd)
+
+
+def test_containment_and_disjoint():
+    big = _rect(1.013, 1.017, 1.531, 1.329, 7)
+    small = _rect(1.1, 1.1, 1.2, 1.2, 3)
+    far = _rect(3.1, 3.1, 3.2, 3.3, 3)
+    counts, area = _run([big, small, far], [(0, 1), (1, 0), (0, 2), (2, 0)])
+    assert np.allclose(area[:2], _shoelace(small.astype(np.float64)), rtol=1e-12)
+    assert area[2] == 0 and area[3] == 0
