@@ -1,0 +1,68 @@
+"""The drop-in executables (intersect, intersection, within_polygon) on .dt
+files written by dump_polygons: their count-found lines against the oracle
+on the same inputs."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from farm_amd import capi, datagen
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "farm_amd", "bin")
+
+
+def _dt(tmp_path, name, ro, xy):
+    wkt = tmp_path / f"{name}.wkt"
+    wkt.write_text("\n".join(datagen.to_wkt_lines(ro, xy)) + "\n")
+    dt = tmp_path / f"{name}.dt"
+    subprocess.run([os.path.join(BIN, "dump_polygons"), str(wkt), str(dt)], check=True, capture_output=True,
+                   timeout=60)
+    return str(dt)
+
+
+def _found(args):
+    print("  run", " ".join(os.path.basename(a) for a in args), flush=True)
+    p = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr[-2000:]
+    m = re.search(r"count-found:\s+(\d+)", p.stderr)
+    assert m, p.stderr[-2000:]
+    return int(m.group(1))
+
+
+def test_intersect_and_intersection_executables(tmp_path):
+    (ra, xa), (rb, xb) = datagen.star_pair_set(21, 300, 300, 0.0, 30.0)
+    s, t = _dt(tmp_path, "src", ra, xa), _dt(tmp_path, "tgt", rb, xb)
+    A, B = capi.HostSet.load_dt(s), capi.HostSet.load_dt(t)
+    pa, pb = capi.candidates(A, B)
+    OA = oracle.OracleSet(ra, xa)
+    OB = oracle.OracleSet(rb, xb)
+    assert OA.rasterize(10) == -1 and OB.rasterize(10) == -1
+    bits, _ = oracle.intersect(OA, OB, pa, pb)
+    _, area = oracle.intersection(OA, OB, pa, pb)
+    for extra in ([], ["-b", "1000"]):
+        assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
+            (bits == 1).sum())
+        assert _found([os.path.join(BIN, "intersection"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
+            (area > 0).sum())
+
+
+def test_within_polygon_executable(tmp_path):
+    ro, xy = datagen.lake_set(23, 600, 0.0, 1.0)
+    s = _dt(tmp_path, "lakes", ro, xy)
+    O = oracle.OracleSet(ro, xy)
+    assert O.rasterize(10, hierarchy=True) == -1
+    pa, pb = oracle.candidates_within(O, 10.0)
+    bits, _, _ = oracle.within(O, O, pa, pb, 10.0)
+    want = int((bits == 1).sum())
+    exe = os.path.join(BIN, "within_polygon")
+    assert _found([exe, "-s", s, "-r", "-g", "-h", "-n", "8"]) == want
+    assert _found([exe, "-s", s, "-r", "-g", "-h", "-n", "8", "-b", "5000"]) == want
+    # without -h the reference dereferences an empty layer list; here it is an error
+    p = subprocess.run([exe, "-s", s, "-r", "-g"], capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0 and "-h" in p.stderr
