@@ -322,11 +322,22 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the within and intersection sub-benchmarks")
     ap.add_argument("--train-pairs", type=int, default=1_000_000)
     ap.add_argument("--label-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--workload", choices=["intersect", "within", "intersection"], default="intersect")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only time a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        from farm_amd import capi
+
+        dev = capi.Device(0)
+        gbps = dev.copy_probe(args.copy_probe, 10)
+        dev.close()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 11, "gbps_read_plus_write": gbps}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the pair list would be sharded "
                                    "across GPUs by predicted cost (see DESIGN.md)", "n_gpus": args.gpus}))
@@ -376,11 +387,15 @@ def main() -> int:
         feat, dec = dev.pair_features(w0, w1)
         und = np.nonzero(dec == -1)[0]
         rng = np.random.default_rng(args.seed + 11)
-        take = np.sort(rng.choice(und, size=min(len(und), args.train_pairs), replace=False))
+        # training pairs: undecided pairs of the first 80% of the window;
+        # the last 20% (all pairs) calibrate the -a threshold
+        split = int(len(dec) * 0.8)
+        take = np.sort(rng.choice(und[und < split], size=min(int((und < split).sum()), args.train_pairs),
+                                  replace=False))
         lab = learned.Labeled(feat[take], dec[take], lbits[take], ltests[take])
         norm, losses = learned.train(dev, lab, seed=args.seed, epochs=args.epochs)
-        z_train = dev.mlp_forward(learned.apply_norm(feat, norm))[:, 0]
-        thr = learned.threshold_for_precision(dec, z_train, lbits.astype(bool), 0.99)
+        z_cal = dev.mlp_forward(learned.apply_norm(feat[split:], norm))[:, 0]
+        thr = learned.threshold_for_precision(dec[split:], z_cal, lbits[split:].astype(bool), 0.99)
         order = 1
         learned_info = {"train_pairs": int(len(take)), "epochs": args.epochs, "loss_first": float(losses[0]),
                         "loss_last": float(losses[-1]), "threshold": thr, "seconds": time.time() - t}
@@ -516,6 +531,17 @@ def main() -> int:
         line["parity_sample"] = {"pairs": int(timed.sum()), "mismatches": mism}
         log(f"cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; sample mismatches vs GPU: {mism}")
     dev.close()
+    if not args.no_extra:
+        # the other two queries at their own full sizes, each with its own
+        # throughput, roofline, CPU baseline and parity sample
+        import copy
+
+        sub = copy.copy(args)
+        sub.steps, sub.warmup, sub.cpu_seconds, sub.scale = 2, 1, 10.0, args.scale
+        for name, fn in (("within", bench_within), ("intersection", bench_intersection)):
+            r = fn(sub)
+            line[name] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
+                                              "cpu_baseline", "parity_sample") if k in r}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -205,6 +205,7 @@ EXPORTS = {
     "farm_mlp_predict_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]),
     "farm_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_int)]),
     "farm_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "farm_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "farm_hostset_from_rings": (C.c_void_p, [C.c_int64, C.c_void_p, C.c_void_p]),
     "farm_hostset_load_dt": (C.c_void_p, [C.c_char_p]),
     "farm_hostset_load_wkt": (C.c_void_p, [C.c_char_p]),
@@ -570,6 +571,12 @@ class Device:
             with open(cache, "w") as f:
                 json.dump(known, f, indent=1, sort_keys=True)
         return best.value
+
+    def copy_probe(self, nbytes: int, iters: int = 10) -> float:
+        """GB/s (read + write) of a plain streaming copy of nbytes."""
+        g = C.c_double()
+        self._check(lib.farm_copy_probe(self._h, nbytes, iters, C.byref(g)), "copy_probe")
+        return g.value
 
     def close(self) -> None:
         if getattr(self, "_h", None):

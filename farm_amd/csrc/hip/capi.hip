@@ -2,6 +2,7 @@
 // Declarations and their reference counterparts: include/farm_capi.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -36,6 +37,14 @@ namespace farm {
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Plain streaming copy (16 B per lane per step): the reference point for
+// HBM byte counters and bandwidth.
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                    uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // learned -a decision: an IN overlap decides true, no BORDER pair decides
@@ -103,6 +112,32 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
                          farm_stats *st, uint32_t *pair_tests = nullptr);
 
 const char *farm_version(void) { return "farm-mi355x 0.1 (gfx950)"; }
+
+int farm_copy_probe(farm_handle h, uint64_t bytes, int iters, double *gbps) {
+    if (!h || iters < 1 || bytes < 16) return fail(h, FARM_EINVAL, "bad copy probe arguments");
+    HIPCHK(h, hipSetDevice(h->device));
+    const uint64_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return fail(h, FARM_ENOMEM, "copy probe: hipMalloc failed");
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(h, FARM_ENOMEM, "copy probe: hipMalloc failed");
+    }
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)h->num_cus * 32);
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, h->stream, (const uint4 *)a, (uint4 *)b, n);
+    (void)hipEventRecord(h->ev[0], h->stream);
+    for (int k = 0; k < iters; k++)
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, h->stream, (const uint4 *)a, (uint4 *)b, n);
+    (void)hipEventRecord(h->ev[1], h->stream);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(h, FARM_EDEVICE, "copy probe: %s", hipGetErrorString(e));
+    if (gbps) *gbps = 2.0 * n * 16 * iters / (ms / 1000.0) / 1e9;
+    return FARM_OK;
+}
 
 int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile) {
     if (!h) return FARM_EINVAL;

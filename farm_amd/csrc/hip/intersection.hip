@@ -268,10 +268,80 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
     return acc;
 }
 
+// The reference's crossing test of source edge ea and target edge eb
+// (segment_intersect_batch, geometry_computation.h:239-267: f32 edge vectors
+// and crosses, f64 parameters, f32 point), behind a screen that drops pairs
+// whose parameters c2/dn, c1/dn lie clearly outside [-1e-6, 1 + 1e-6]
+// (the f32 products in the screen are off by far less than the margins).
+// Hits are appended to R through the wave's LDS counter.
+__device__ __forceinline__ void edge_test(float ax, float ay, float d1x, float d1y, float bx, float by, float d2x,
+                                          float d2y, uint32_t ea, uint32_t eb, Rec *R, uint32_t cap, uint32_t *cnt) {
+    const float rx = bx - ax, ry = by - ay;
+    const float dn = d1x * d2y - d1y * d2x;
+    const float c2 = rx * d2y - ry * d2x;
+    const float c1 = rx * d1y - ry * d1x;
+    const float adn = fabsf(dn);
+    const float s2 = dn < 0.f ? -c2 : c2, s1 = dn < 0.f ? -c1 : c1;
+    const float lo = -1e-6f * adn, hi = 1.000001f * adn;
+    if (!(s2 >= lo && s2 <= hi && s1 >= lo && s1 <= hi && (double)adn >= 1e-12)) return;
+    const double denom = dn;
+    const double t = c2 / denom;
+    const double u = c1 / denom;
+    if (t >= -1e-9 && t <= 1 + 1e-9 && u >= -1e-9 && u <= 1 + 1e-9) {
+        const float tf = (float)t;
+        Rec rec;
+        rec.px = ax + tf * d1x;
+        rec.py = ay + tf * d1y;
+        rec.ei = (int32_t)ea;
+        rec.ej = (int32_t)eb;
+        rec.t = t;
+        rec.u = u;
+        const uint32_t pos = atomicAdd(cnt, 1u);
+        if (pos < cap) R[pos] = rec;
+    }
+}
+
+// All edge pairs of one sequence pair.  Source edges go to lanes and stay in
+// registers; target edges stream uniformly (scalar loads) when the source
+// sequence fills the wave, else lanes split into phases over the target
+// edges.
+__device__ __forceinline__ void collect_seq_pair(const float *AX, const float *AY, const float *BX, const float *BY,
+                                                 uint2 e1, uint2 e2, Rec *R, uint32_t cap, uint32_t *cnt, int lane) {
+    const uint32_t la = e1.x, na = e1.y, lb = e2.x, nb = e2.y;  // first edge, edge count
+    if (na == 0 || nb == 0) return;
+    if (na >= 48) {
+        for (uint32_t i0 = 0; i0 < na; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const bool on = i < na;
+            const uint32_t ea = la + (on ? i : 0);
+            const float ax = AX[ea], ay = AY[ea];
+            const float d1x = AX[ea + 1] - ax, d1y = AY[ea + 1] - ay;
+            float bx = BX[lb], by = BY[lb];
+            for (uint32_t j = 0; j < nb; j++) {
+                const float b1x = BX[lb + j + 1], b1y = BY[lb + j + 1];
+                if (on) edge_test(ax, ay, d1x, d1y, bx, by, b1x - bx, b1y - by, ea, lb + j, R, cap, cnt);
+                bx = b1x;
+                by = b1y;
+            }
+        }
+    } else {
+        const uint32_t phases = 64 / na;  // >= 1
+        const uint32_t i = lane % na, ph = lane / na;
+        if (ph >= phases) return;
+        const uint32_t ea = la + i;
+        const float ax = AX[ea], ay = AY[ea];
+        const float d1x = AX[ea + 1] - ax, d1y = AY[ea + 1] - ay;
+        for (uint32_t j = ph; j < nb; j += phases) {
+            const float bx = BX[lb + j], by = BY[lb + j];
+            edge_test(ax, ay, d1x, d1y, bx, by, BX[lb + j + 1] - bx, BY[lb + j + 1] - by, ea, lb + j, R, cap, cnt);
+        }
+    }
+}
+
 // Ideal::intersection for one pair plus the area; R / D: the wave's two
 // record slabs of cap records each (cap a power of two).
 __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const double *ring2a, uint32_t ia, uint32_t ib,
-                                                     Rec *R, Rec *D, uint32_t cap, int lane) {
+                                                     Rec *R, Rec *D, uint32_t cap, uint32_t *cnt, int lane) {
     PairRes r;
     for (int k = 0; k < 5; k++) r.c[k] = 0;
     r.area = 0.0;
@@ -306,7 +376,8 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
     // 2. crossing points of the edges of BORDER x BORDER pixel pairs
     const float *AX = S.x + hA.vert_start, *AY = S.y + hA.vert_start;
     const float *BX = S.x + hB.vert_start, *BY = S.y + hB.vert_start;
-    uint32_t nrec = 0;
+    if (lane == 0) *cnt = 0u;
+    wave_sync();
     for (int x = w.x0; x <= w.x1; x++) {
         for (int y = w.y0; y <= w.y1; y++) {
             const uint32_t pa = (uint32_t)(y * hA.dimx + x);
@@ -325,69 +396,16 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
                         const uint2 e1 = S.seqs[hA.seq_start + sa];
                         for (uint32_t sb = sb0; sb < sb1; sb++) {
                             const uint2 e2 = S.seqs[hB.seq_start + sb];
-                            const uint32_t la = e1.y, lb = e2.y;
-                            const uint64_t tot = (uint64_t)la * lb;
-                            r.tests += tot;
-                            if (tot == 0) continue;
-                            uint32_t i = lane / lb, j = lane - (lane / lb) * lb;
-                            const uint32_t q = 64 / lb, rr = 64 - q * lb;
-                            const uint64_t rounds = (tot + 63) / 64;
-                            for (uint64_t it = 0; it < rounds; it++) {
-                                bool hit = false;
-                                Rec rec;
-                                if (i < la) {
-                                    const uint32_t ea = e1.x + i, eb = e2.x + j;
-                                    const float ax = AX[ea], ay = AY[ea], a1x = AX[ea + 1], a1y = AY[ea + 1];
-                                    const float bx = BX[eb], by = BY[eb], b1x = BX[eb + 1], b1y = BY[eb + 1];
-                                    // segment_intersect_batch (geometry_computation.h:239-267)
-                                    const float d1x = a1x - ax, d1y = a1y - ay;
-                                    const float d2x = b1x - bx, d2y = b1y - by;
-                                    const float rx = bx - ax, ry = by - ay;
-                                    const float dn = d1x * d2y - d1y * d2x;
-                                    const float c2 = rx * d2y - ry * d2x;
-                                    const float c1 = rx * d1y - ry * d1x;
-                                    const float adn = fabsf(dn);
-                                    const float s2 = dn < 0.f ? -c2 : c2, s1 = dn < 0.f ? -c1 : c1;
-                                    // screen: t = c2 / dn and u = c1 / dn well inside
-                                    // [-1e-6, 1 + 1e-6] up to f32 rounding
-                                    if ((double)adn >= 1e-12 && s2 >= -1e-6f * adn && s2 <= 1.000001f * adn &&
-                                        s1 >= -1e-6f * adn && s1 <= 1.000001f * adn) {
-                                        const double denom = dn;
-                                        const double t = c2 / denom;
-                                        const double u = c1 / denom;
-                                        if (t >= -1e-9 && t <= 1 + 1e-9 && u >= -1e-9 && u <= 1 + 1e-9) {
-                                            const float tf = (float)t;
-                                            rec.px = ax + tf * d1x;
-                                            rec.py = ay + tf * d1y;
-                                            rec.ei = (int32_t)ea;
-                                            rec.ej = (int32_t)eb;
-                                            rec.t = t;
-                                            rec.u = u;
-                                            hit = true;
-                                        }
-                                    }
-                                }
-                                const uint64_t hm = __ballot(hit);
-                                if (hm) {
-                                    const uint32_t pos = nrec + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull));
-                                    if (hit) {
-                                        if (pos < cap) R[pos] = rec;
-                                    }
-                                    nrec += (uint32_t)__popcll(hm);
-                                }
-                                i += q;
-                                j += rr;
-                                if (j >= lb) {
-                                    j -= lb;
-                                    i++;
-                                }
-                            }
+                            r.tests += (uint64_t)e1.y * e2.y;
+                            collect_seq_pair(AX, AY, BX, BY, e1, e2, R, cap, cnt, lane);
                         }
                     }
                 }
             }
         }
     }
+    wave_sync();
+    const uint32_t nrec = uni(*cnt);
     r.raw = nrec;
     if (nrec > cap) {
         r.over = true;
@@ -482,6 +500,8 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
     const uint64_t gw = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     Rec *R = slab + gw * 2ull * cap;
     Rec *D = R + cap;
+    __shared__ uint32_t rec_cnt[kWaves];
+    uint32_t *cnt = &rec_cnt[threadIdx.x >> 6];
     unsigned long long bb = 0, tests = 0, raw = 0, found = 0;
     for (;;) {
         uint32_t base = 0;
@@ -497,7 +517,7 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
                 if (lane == 0) area[idx] = 0.0;
                 continue;
             }
-            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, lane);
+            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, cnt, lane);
             if (r.over) {
                 if (lane == 0) {
                     if (BIG) {

@@ -150,10 +150,11 @@ def precision_recall(pred: np.ndarray, truth: np.ndarray) -> tuple[float, float]
 
 def evaluate(train_pairs: int = 500_000, eval_pairs: int = 2_000_000, target: float = 0.99, epochs: int = 12,
              seed: int = 1) -> dict:
-    """Learned -a on the star-pair distribution: train on `train_pairs`
-    pairs of seeded instances, pick the threshold for `target` precision on
-    them, evaluate on `eval_pairs` pairs of fresh seeds (exact labels, the
-    reference heuristic and the filter alone on the same pairs)."""
+    """Learned -a on the star-pair distribution: train on 80% of
+    `train_pairs` labeled pairs of seeded instances, pick the threshold for
+    `target` precision (with a 2-sigma margin) on the held-out 20%, evaluate
+    on `eval_pairs` pairs of fresh seeds (exact labels, the reference
+    heuristic and the filter alone on the same pairs)."""
     import time
 
     per_instance = 4200  # candidate pairs of one 2,000 x 2,000 instance, about
@@ -162,10 +163,10 @@ def evaluate(train_pairs: int = 500_000, eval_pairs: int = 2_000_000, target: fl
     dev = capi.Device(0)
     t = time.time()
     tr_ps = build_pairsets([10_000 + seed * 100_000 + i for i in range(n_tr)])
-    tr = label(dev, tr_ps)
+    tr, cal = split_calibration(label(dev, tr_ps), 0.2, seed)
     norm, losses = train(dev, tr, seed=seed, epochs=epochs)
-    z_tr = dev.mlp_forward(apply_norm(tr.feat, norm))[:, 0]
-    thr = threshold_for_precision(tr.decided, z_tr, tr.bits.astype(bool), target)
+    z_cal = dev.mlp_forward(apply_norm(cal.feat, norm))[:, 0]
+    thr = threshold_for_precision(cal.decided, z_cal, cal.bits.astype(bool), target)
     t_train = time.time() - t
     ev_ps = build_pairsets([50_000 + seed * 100_000 + i for i in range(n_ev)])
     ev = label(dev, ev_ps)
@@ -193,8 +194,19 @@ def evaluate(train_pairs: int = 500_000, eval_pairs: int = 2_000_000, target: fl
     return res
 
 
-def threshold_for_precision(decided, logit, truth, target: float = 0.99) -> float:
-    """Smallest logit threshold whose overall precision reaches target."""
+def split_calibration(lab: Labeled, frac: float = 0.2, seed: int = 0) -> tuple[Labeled, Labeled]:
+    """Random split of labeled pairs into a training part and a held-out
+    calibration part (for the -a threshold)."""
+    rng = np.random.default_rng(seed)
+    cal = rng.random(len(lab.bits)) < frac
+    take = lambda m: Labeled(lab.feat[m], lab.decided[m], lab.bits[m], lab.tests[m])  # noqa: E731
+    return take(~cal), take(cal)
+
+
+def threshold_for_precision(decided, logit, truth, target: float = 0.99, zconf: float = 2.0) -> float:
+    """Smallest logit threshold whose overall precision, less zconf standard
+    errors of its binomial estimate, reaches target (pick it on pairs the
+    network was not trained on)."""
     und = decided == -1
     base_tp = int(((decided == 1) & truth).sum())
     base_fp = int(((decided == 1) & ~truth).sum())
@@ -203,8 +215,10 @@ def threshold_for_precision(decided, logit, truth, target: float = 0.99) -> floa
     order = np.argsort(-z)
     tp = base_tp + np.cumsum(t[order])
     fp = base_fp + np.cumsum(~t[order])
-    prec = tp / np.maximum(tp + fp, 1)
-    ok = np.nonzero(prec >= target)[0]
+    n = np.maximum(tp + fp, 1)
+    prec = tp / n
+    lower = prec - zconf * np.sqrt(prec * (1 - prec) / n)
+    ok = np.nonzero(lower >= target)[0]
     if len(ok) == 0:
         return float(np.inf)
     k = ok[-1]  # accept the top k+1 logits

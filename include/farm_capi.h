@@ -259,6 +259,12 @@ int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile);
 /* Set the tile width used when opts->tile_a is 0 (e.g. a persisted choice). */
 int farm_set_tile(farm_handle h, int tile);
 
+/* HBM calibration: times `iters` launches of a plain streaming copy of
+ * `bytes` bytes between two scratch buffers; *gbps = bytes read + written
+ * per second.  Its byte counters are the reference point for the traffic
+ * figures of the other kernels.  No reference counterpart. */
+int farm_copy_probe(farm_handle h, uint64_t bytes, int iters, double *gbps);
+
 /* Library version string. */
 const char *farm_version(void);
 

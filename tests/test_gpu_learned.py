@@ -66,27 +66,27 @@ def test_training_reduces_loss(device):
 @pytest.fixture(scope="module")
 def star_model(device):
     train_ps = learned.build_pairsets(list(range(500, 516)))
-    lab = learned.label(device, train_ps)
+    lab, cal = learned.split_calibration(learned.label(device, train_ps), 0.2, 0)
     norm, losses = learned.train(device, lab, seed=0, epochs=15)
     params = device.mlp_get_params()
     test_ps = learned.build_pairsets(list(range(900, 908)))
     test_lab = learned.label(device, test_ps)
-    return {"norm": norm, "params": params, "train": lab, "test": test_lab, "test_ps": test_ps, "losses": losses}
+    return {"norm": norm, "params": params, "train": lab, "cal": cal, "test": test_lab, "test_ps": test_ps,
+            "losses": losses}
 
 
 def test_learned_approx_beats_heuristic_at_99_precision(device, star_model):
     M = star_model
     assert M["losses"][-1] < M["losses"][0]
-    tr, te = M["train"], M["test"]
+    cal, te = M["cal"], M["test"]
     device.mlp_set_params(M["params"], M["norm"])
-    # threshold chosen on the training pairs, evaluated on fresh seeds
+    # threshold chosen on held-out calibration pairs, evaluated on fresh seeds
     device.upload_polygons(M["test_ps"].both)  # (label() left the test set uploaded)
     device.upload_pairs(M["test_ps"].pa, M["test_ps"].pb)
     z_te = device.mlp_predict_pairs()
     truth = te.bits.astype(bool)
-    # training-set threshold: reload train predictions on the host copy of features
-    z_tr = device.mlp_forward(learned.apply_norm(tr.feat, M["norm"]))
-    thr = learned.threshold_for_precision(tr.decided, z_tr[:, 0], tr.bits.astype(bool), 0.995)
+    z_cal = device.mlp_forward(learned.apply_norm(cal.feat, M["norm"]))
+    thr = learned.threshold_for_precision(cal.decided, z_cal[:, 0], cal.bits.astype(bool), 0.99)
     pred = learned.decisions(te.decided, z_te[:, 0], thr)
     p_l, r_l = learned.precision_recall(pred, truth)
     # the reference heuristic on the same pairs
