@@ -38,6 +38,22 @@ HBM_PEAK_GBS = 8000.0
 OPS_PER_TEST = 42
 
 
+def measured_bytes_per_test() -> float | None:
+    """HBM bytes per executed segment test of k_refine on this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled
+    by the copy-kernel calibration)."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "k_refine_hbm_traffic_complex_x_lakes.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    f = re.search(r"fetch_bytes_per_segment_test: ([0-9.eE+-]+)", text)
+    w = re.search(r"write_bytes_per_segment_test: ([0-9.eE+-]+)", text)
+    return float(f.group(1)) + float(w.group(1)) if f and w else None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -421,6 +437,8 @@ def main() -> int:
     filter_ms = sum(s["filter_ms"] for s in per_step)
     achieved = tests * OPS_PER_TEST / (refine_ms / 1000.0) / 1e12 if refine_ms > 0 else 0.0
     found = sum(s["found"] for s in per_step)
+    bpt = measured_bytes_per_test() if (args.scale == 1.0 and args.vpr == 256) else None
+    traffic = tests / args.steps * bpt if bpt else None
 
     line = {
         "metric": "exact intersect pairs/s plain vs batched+learned; approx-mode recall at 99% prec",
@@ -454,7 +472,10 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/k_refine_hbm_traffic_complex_x_lakes.md (bytes per test x tests per launch)"
+            if traffic else None,
+            "algorithmic_bytes": sum(s["refine_bytes"] for s in per_step) / args.steps,
             "ops_per_segment_test": OPS_PER_TEST,
         },
     }
