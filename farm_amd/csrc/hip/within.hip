@@ -198,36 +198,50 @@ __device__ __forceinline__ int unpack_y(T v) {
     return (int)((uint32_t)v >> SH);
 }
 
-// Pixel set -> compact list in id order.  bits has ceil(npix / 32) words;
-// lanes take one word each.
+// Pixel set -> compact list (ascending ids).  bits has ceil(npix / 32)
+// words; lanes take one word each.  The exclusive prefix sum of the per-lane
+// counts (0..32) is assembled from six bit-plane ballots, with no
+// cross-lane data movement.
 template <int SH, class T>
 __device__ __forceinline__ uint32_t compact(const uint32_t *bits, int npix, int dimx, T *list, int lane) {
     const int nw = (npix + 31) >> 5;
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t total = 0;
     for (int w0 = 0; w0 < nw; w0 += 64) {
         const int w = w0 + lane;
-        const uint32_t word = w < nw ? bits[w] : 0u;
-        const uint32_t c = __popc(word);
-        // inclusive scan of c over the wave
-        uint32_t s = c;
+        uint32_t word = w < nw ? bits[w] : 0u;
+        const uint32_t c = (uint32_t)__popc(word);
+        uint32_t pos = 0, sum = 0;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(s, o, 64);
-            if (lane >= o) s += t;
+        for (int k = 0; k < 6; k++) {
+            const uint64_t m = __ballot((c >> k) & 1u);
+            pos += (uint32_t)__popcll(m & below) << k;
+            sum += (uint32_t)__popcll(m) << k;
         }
-        uint32_t pos = total + s - c;
-        uint32_t m = word;
-        while (m) {
-            const int b = __ffs(m) - 1;
-            m &= m - 1;
-            const uint32_t id = (uint32_t)(w * 32 + b);
-            const uint32_t y = id / (uint32_t)dimx, x = id - y * (uint32_t)dimx;
-            list[pos++] = pack_xy<SH, T>(x, y);
+        pos += total;
+        if (word) {
+            const uint32_t id0 = (uint32_t)w * 32u;
+            const uint32_t y0 = id0 / (uint32_t)dimx, x0 = id0 - y0 * (uint32_t)dimx;
+            while (word) {
+                const uint32_t b = (uint32_t)(__ffs(word) - 1);
+                word &= word - 1;
+                uint32_t x = x0 + b, y = y0;
+                if (dimx >= 32) {  // b < 32 <= dimx: at most one row further
+                    if (x >= (uint32_t)dimx) {
+                        x -= (uint32_t)dimx;
+                        y++;
+                    }
+                } else {
+                    y = (id0 + b) / (uint32_t)dimx;
+                    x = id0 + b - y * (uint32_t)dimx;
+                }
+                list[pos++] = pack_xy<SH, T>(x, y);
+            }
         }
-        total += uni(__shfl(s, 63, 64));
+        total += sum;
     }
     wave_sync();
-    return total;
+    return uni(total);
 }
 
 __device__ __forceinline__ void clear_bits(uint32_t *bits, int npix, int lane) {
@@ -250,13 +264,13 @@ struct PairOut {
 // pg), or (same layer) the members themselves; BORDER ones go to bits.
 template <int SH, class T>
 __device__ __forceinline__ void expand_side(const T *list, uint32_t n, bool next, const LGrid &pg,
-                                            const LGrid &g, uint32_t *bits, int lane) {
+                                            const LGrid &g, const uint8_t *st, uint32_t *bits, int lane) {
     for (uint32_t k = lane; k < n; k += 64) {
         const T v = list[k];
         const int px = unpack_x<SH>(v), py = unpack_y<SH>(v);
         if (!next) {
             const uint32_t id = (uint32_t)(py * g.dimx + px);
-            if (border(g.st[id])) set_bit(bits, id);
+            if (border(st[id])) set_bit(bits, id);
             continue;
         }
         // the parent box shrunk by 1e-6, then retrieve_pixels'
@@ -274,7 +288,7 @@ __device__ __forceinline__ void expand_side(const T *list, uint32_t n, bool next
         for (int x = sx; x <= ex; x++)
             for (int y = sy; y <= ey; y++) {
                 const uint32_t id = (uint32_t)(y * g.dimx + x);
-                if (border(g.st[id])) set_bit(bits, id);
+                if (border(st[id])) set_bit(bits, id);
             }
     }
     wave_sync();
@@ -286,27 +300,48 @@ struct SeqTab {
     unsigned long long mins[64];  // squared distance bits (non-negative doubles order as integers)
 };
 
-// Refines one candidate pixel pair (id1, id2): the truncated distances of
-// all its edge-sequence pairs (segment_to_segment_within_batch with vs1 = the
-// target sequence, vs2 = the source one), the evaluations of up to 64
-// sequence pairs flattened over the lanes, each pair's minimum gathered in
-// LDS; then the reference's checks after every valid sequence pair
-// (running minimum < d: within; < ref: not within) become a running-minimum
-// scan and the first lane that passes one.  Returns 1 / 0 when decided, 2 to
-// go on with the next candidate.
-__device__ __forceinline__ int refine_cand(const DevSet &S, const farm_raster_hdr &hs, const farm_raster_hdr &ht,
-                                           uint32_t id1, uint32_t id2, double d, double ref, double &min_dist,
-                                           PairOut &r, SeqTab &tab, int lane) {
-    const uint32_t s0 = S.pixoff[hs.pixoff_start + id1], n1 = S.pixoff[hs.pixoff_start + id1 + 1] - s0;
-    const uint32_t t0 = S.pixoff[ht.pixoff_start + id2], n2 = S.pixoff[ht.pixoff_start + id2 + 1] - t0;
-    const uint32_t np = n1 * n2;
+constexpr int kBatch = 4;  // candidates refined together (their sequence pairs share one table)
+
+// A run of candidates taken in tuple order: pixel-pair sequence ranges and
+// each one's reference_dist.  All values are uniform across the wave.
+struct CandBatch {
+    int n = 0;
+    uint32_t nrows = 0;
+    uint32_t s0[kBatch], t0[kBatch], n2[kBatch], row0[kBatch];
+    double ref[kBatch];
+};
+
+// Refines a batch of candidate pixel pairs: the truncated distances of all
+// their edge-sequence pairs (segment_to_segment_within_batch with vs1 = the
+// target sequence, vs2 = the source one), rows = the candidates' sequence
+// pairs in order, 64 rows at a time, their evaluations flattened over the
+// lanes and each row's minimum gathered in LDS; then the reference's checks
+// after every valid sequence pair (running minimum < d: within; < the row's
+// candidate's reference_dist: not within) become a running-minimum scan and
+// the first lane that passes one.  Returns 1 / 0 when decided, 2 to go on.
+__device__ __forceinline__ int refine_batch(const DevSet &S, const farm_raster_hdr &hs, const farm_raster_hdr &ht,
+                                            const CandBatch &cb, double d, double &min_dist, PairOut &r,
+                                            SeqTab &tab, int lane) {
+    const uint32_t nrows = cb.nrows;
     const float *sxv = S.x + hs.vert_start, *syv = S.y + hs.vert_start;
     const float *txv = S.x + ht.vert_start, *tyv = S.y + ht.vert_start;
-    for (uint32_t c0 = 0; c0 < np; c0 += 64) {
+    for (uint32_t c0 = 0; c0 < nrows; c0 += 64) {
         const uint32_t k = c0 + lane;
         uint32_t la = 0, lb = 0, fa = 0, fb = 0;
-        if (k < np) {
-            const uint32_t a = k / n2, b = k - (k / n2) * n2;
+        double ref = kFar;
+        if (k < nrows) {
+            uint32_t s0 = 0, t0 = 0, n2 = 1, rr = 0;
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) {
+                if (q < cb.n && k >= cb.row0[q]) {
+                    s0 = cb.s0[q];
+                    t0 = cb.t0[q];
+                    n2 = cb.n2[q];
+                    rr = cb.row0[q];
+                    ref = cb.ref[q];
+                }
+            }
+            const uint32_t loc = k - rr, a = loc / n2, b = loc - (loc / n2) * n2;
             const uint2 e1 = S.seqs[hs.seq_start + s0 + a];
             const uint2 e2 = S.seqs[ht.seq_start + t0 + b];
             fa = e1.x;
@@ -315,22 +350,17 @@ __device__ __forceinline__ int refine_cand(const DevSet &S, const farm_raster_hd
             lb = e2.y;
         }
         // sequences of fewer than 2 edges are skipped (Ideal.cpp:2630)
-        const bool ok = k < np && la >= 2 && lb >= 2;
+        const bool ok = k < nrows && la >= 2 && lb >= 2;
         const uint32_t w1 = ok ? lb * (la - 1) : 0u;
         const uint32_t w = ok ? w1 + la * (lb - 1) : 0u;
-        uint32_t inc = w;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += t;
-        }
-        const uint32_t W = uni(__shfl(inc, 63, 64));
+        uint32_t W;
+        const uint32_t pre = wave_excl_scan(w, lane, &W);
         tab.fa[lane] = fa;
         tab.la[lane] = la;
         tab.fb[lane] = fb;
         tab.lb[lane] = lb;
         tab.w1[lane] = w1;
-        tab.pre[lane] = inc - w;
+        tab.pre[lane] = pre;
         if (lane == 63) tab.pre[64] = W;
         tab.mins[lane] = ~0ull;
         wave_sync();
@@ -442,8 +472,8 @@ __device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lan
 // global slab) and the candidate slab.
 template <int SH, class T>
 __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uint32_t ib, double d, T *sl,
-                                               T *tl, uint32_t *sb, uint32_t *tb, Cand *cand, uint32_t pixcap,
-                                               uint32_t candcap, SeqTab &tab, int lane) {
+                                               T *tl, uint32_t *sb, uint32_t *tb, uint8_t *stage, Cand *cand,
+                                               uint32_t pixcap, uint32_t candcap, SeqTab &tab, int lane) {
     PairOut r;
     r.bit = 0;
     r.min_dist = kFar;
@@ -498,8 +528,16 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         const int npix_s = gs.dimx * gs.dimy, npix_t = gt.dimx * gt.dimy;
         clear_bits(sb, npix_s, lane);
         clear_bits(tb, npix_t, lane);
-        expand_side<SH>(sl, ns, s_next, s_next ? lgrid(S, hs, i - 1) : gs, gs, sb, lane);
-        expand_side<SH>(tl, nt, t_next, t_next ? lgrid(S, ht, j - 1) : gt, gt, tb, lane);
+        const uint8_t *sts = gs.st, *stt = gt.st;
+        if (stage) {  // the two layers' fill classes, staged in LDS
+            for (int k = lane; k < npix_s; k += 64) stage[k] = gs.st[k];
+            for (int k = lane; k < npix_t; k += 64) stage[pixcap + k] = gt.st[k];
+            wave_sync();
+            sts = stage;
+            stt = stage + pixcap;
+        }
+        expand_side<SH>(sl, ns, s_next, s_next ? lgrid(S, hs, i - 1) : gs, gs, sts, sb, lane);
+        expand_side<SH>(tl, nt, t_next, t_next ? lgrid(S, ht, j - 1) : gt, gt, stt, tb, lane);
         ns = compact<SH>(sb, npix_s, gs.dimx, sl, lane);
         nt = compact<SH>(tb, npix_t, gt.dimx, tl, lane);
         if (ns == 0 || nt == 0) break;  // no BORDER pixel on a side: nothing to refine
@@ -606,16 +644,41 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
     if (!final_level || r.ncand == 0) return r;
 
     // refinement in tuple order
+    // candidates are taken one by one in tuple order and refined in runs of
+    // up to kBatch whose sequence pairs fit one 64-row table (a candidate
+    // with more rows runs alone, 64 rows at a time)
     const uint32_t nc = r.ncand;
     double min_dist = kFar;
-    for (uint32_t step = 0; step < nc; step++) {
-        Cand c;
-        double ref;
-        const int slot = select_next(cand, nc, lane, c, ref);
-        if (slot < 0) break;
-        if (lane == 0) cand[slot].apx = INFINITY;
-        wave_sync();
-        const int res = refine_cand(S, hs, ht, c.id1, c.id2, d, ref, min_dist, r, tab, lane);
+    Cand c;
+    double ref;
+    int slot = select_next(cand, nc, lane, c, ref);
+    while (slot >= 0) {
+        CandBatch cb;
+        bool open = true;
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            if (open && slot >= 0) {
+                const uint32_t s0 = S.pixoff[hs.pixoff_start + c.id1], s1 = S.pixoff[hs.pixoff_start + c.id1 + 1];
+                const uint32_t t0 = S.pixoff[ht.pixoff_start + c.id2], t1 = S.pixoff[ht.pixoff_start + c.id2 + 1];
+                const uint32_t rows = (s1 - s0) * (t1 - t0);
+                if (q > 0 && cb.nrows + rows > 64) {
+                    open = false;  // c starts the next run
+                } else {
+                    cb.s0[q] = s0;
+                    cb.t0[q] = t0;
+                    cb.n2[q] = t1 - t0;
+                    cb.ref[q] = ref;
+                    cb.row0[q] = cb.nrows;
+                    cb.nrows += rows;
+                    cb.n = q + 1;
+                    if (lane == 0) cand[slot].apx = INFINITY;
+                    wave_sync();
+                    slot = select_next(cand, nc, lane, c, ref);
+                    if (cb.nrows >= 64) open = false;
+                }
+            }
+        }
+        const int res = refine_batch(S, hs, ht, cb, d, min_dist, r, tab, lane);
         if (res != 2) {
             r.bit = (uint8_t)res;
             return r;
@@ -644,14 +707,17 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
     constexpr int SH = BIG ? 16 : 8;
     LT *sl, *tl;
     uint32_t *sb, *tb;
+    uint8_t *stage = nullptr;
     Cand *cand;
     if constexpr (!BIG) {
         __shared__ LT lists[kWaves][2][kPixMax];
         __shared__ uint32_t bits[kWaves][2][kBitWords];
+        __shared__ uint8_t stages[kWaves][2 * kPixMax];
         sl = lists[wv][0];
         tl = lists[wv][1];
         sb = bits[wv][0];
         tb = bits[wv][1];
+        stage = stages[wv];
         cand = reinterpret_cast<Cand *>(slab + gw * slab_stride);
     } else {
         char *base = slab + gw * slab_stride;
@@ -681,7 +747,7 @@ __global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__rest
                 }
                 continue;
             }
-            const PairOut r = within_pair<SH>(S, ia, ib, d, sl, tl, sb, tb, cand, pixcap, candcap, tabs[wv], lane);
+            const PairOut r = within_pair<SH>(S, ia, ib, d, sl, tl, sb, tb, stage, cand, pixcap, candcap, tabs[wv], lane);
             if (r.bit == 3) {
                 if (lane == 0) {
                     if (BIG) {
