@@ -69,7 +69,45 @@ struct RasterScratch {
     std::vector<int> strip_start;
     std::vector<P2> strips;
     std::vector<int> strip_col;
+    std::vector<int> bcnt, bidx, brng;  // edges bucketed by column / row
 };
+
+namespace {
+
+// Edges k = 0..n-2 of the polyline V, bucketed by the cells of one axis they
+// can touch (cell c spans [lo + c*step, lo + (c+1)*step]); one cell of margin
+// on each side absorbs the accumulated cell bounds and the float roundings
+// of the clip tests, so every edge that can add a point to a cell is in its
+// bucket, in polyline order.
+void bucket_edges(const float *V, int n, int axis, double lo, double step, int ncell, std::vector<int> &cnt,
+                  std::vector<int> &idx, std::vector<int> &rng) {
+    cnt.assign((size_t)ncell + 1, 0);
+    rng.resize(2 * (size_t)std::max(n - 1, 0));
+    const double inv = 1.0 / step;
+    float a = V[axis];
+    for (int k = 0; k + 1 < n; k++) {
+        const float b = V[2 * (k + 1) + axis];
+        const double mn = std::min(a, b), mx = std::max(a, b);
+        int c0 = (int)std::floor((mn - lo) * inv) - 1;
+        int c1 = (int)std::floor((mx - lo) * inv) + 1;
+        c0 = std::max(c0, 0);
+        c1 = std::min(c1, ncell - 1);
+        rng[2 * k] = c0;
+        rng[2 * k + 1] = c1;
+        for (int c = c0; c <= c1; c++) cnt[c + 1]++;
+        a = b;
+    }
+    for (int c = 0; c < ncell; c++) cnt[c + 1] += cnt[c];
+    idx.resize(cnt[ncell]);
+    std::vector<int> &pos = rng;  // reuse: cursors live past the ranges
+    const size_t base = rng.size();
+    pos.resize(base + ncell);
+    for (int c = 0; c < ncell; c++) pos[base + c] = cnt[c];
+    for (int k = 0; k + 1 < n; k++)
+        for (int c = rng[2 * k]; c <= rng[2 * k + 1]; c++) idx[pos[base + c]++] = k;
+}
+
+}  // namespace
 
 RasterScratch *raster_scratch_new() { return new RasterScratch(); }
 void raster_scratch_free(RasterScratch *s) { delete s; }
@@ -427,12 +465,18 @@ static void build_raster_impl(const float *ring, int nv, RasterOut &out, RasterS
         double Xi = x0, Xi1 = Xi + sx;
         const double kx = Xi + dimx * sx;
         int col = 0;
+        const int ncol = dimx + 2;
+        bucket_edges(ring, nv, 0, x0, sx, ncol, S->bcnt, S->bidx, S->brng);
         while (Xi1 < kx + 1e-9) {
             const size_t base = strips.size();
             S->strip.clear();
             dd.reset(64);
             const float fXi = (float)Xi, fXi1 = (float)Xi1;
-            for (int e = 0; e < nv - 1; e++) {
+            // only the edges that can reach this column, in ring order
+            const bool bucketed = col < ncol;
+            const int kb = bucketed ? S->bcnt[col] : 0, ke = bucketed ? S->bcnt[col + 1] : nv - 1;
+            for (int kk = kb; kk < ke; kk++) {
+                const int e = bucketed ? S->bidx[kk] : kk;
                 const P2 A = p[e], B = p[e + 1];
                 double yi, yi1;
                 if (A.x == B.x) {
@@ -474,15 +518,22 @@ static void build_raster_impl(const float *ring, int nv, RasterOut &out, RasterS
 
     const double ky = y0 + dimy * sy;
     auto &cell = S->cell;
+    const int nrow = dimy + 2;
     for (size_t si = 0; si + 1 < sstart.size(); si++) {
         const P2 *V = strips.data() + sstart[si];
         const int nvs = sstart[si + 1] - sstart[si];
         const int col = scol[si];
+        // small strips scan all their edges per row; large ones bucket them
+        const bool use_rows = (int64_t)nvs * nrow > 4096;
+        if (use_rows) bucket_edges(reinterpret_cast<const float *>(V), nvs, 1, y0, sy, nrow, S->bcnt, S->bidx, S->brng);
         double Yi = y0, Yi1 = Yi + sy;
         int row = 0;
         while (Yi1 < ky + 1e-9) {
             cell.clear();
-            for (int k = 0; k + 1 < nvs; k++) {
+            const bool bucketed = use_rows && row < nrow;
+            const int kb = bucketed ? S->bcnt[row] : 0, ke = bucketed ? S->bcnt[row + 1] : nvs - 1;
+            for (int kk = kb; kk < ke; kk++) {
+                const int k = bucketed ? S->bidx[kk] : kk;
                 const P2 A = V[k], B = V[k + 1];
                 bool skip = false, ok0 = true, ok1 = true;
                 double xi = 0, xi1 = 0;
