@@ -48,6 +48,10 @@ def test_intersect_and_intersection_executables(tmp_path):
     for extra in ([], ["-b", "1000"]):
         assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (bits == 1).sum())
+    # -a: the reference's heuristic (IN overlap or a BORDER pair with prob > 1)
+    abits, _ = oracle.intersect(OA, OB, pa, pb, 1)
+    assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-a"]) == int((abits == 1).sum())
+    for extra in ([], ["-b", "1000"]):
         assert _found([os.path.join(BIN, "intersection"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (area > 0).sum())
 
