@@ -75,21 +75,28 @@ struct RasterScratch {
 namespace {
 
 // Edges k = 0..n-2 of the polyline V, bucketed by the cells of one axis they
-// can touch (cell c spans [lo + c*step, lo + (c+1)*step]); one cell of margin
-// on each side absorbs the accumulated cell bounds and the float roundings
-// of the clip tests, so every edge that can add a point to a cell is in its
-// bucket, in polyline order.
+// can touch (cell c spans [lo + c*step, lo + (c+1)*step]), in polyline order.
+// An edge spanning [mn, mx] on the axis adds a point to cell c only if a cell
+// bound, rounded to float, lies in [mn, mx] or an end point lies in the cell:
+// the lower bound of cell c + 1 counts for cell c, and a float rounding moves
+// a bound by at most |x| * 2^-24.  The margin m covers that rounding, the
+// accumulated cell bounds (one double rounding per cell) and the rounding of
+// the bucket arithmetic, so the cells [floor((mn - m - lo) / step),
+// floor((mx + m - lo) / step)] hold every cell the clip tests can reach.
 void bucket_edges(const float *V, int n, int axis, double lo, double step, int ncell, std::vector<int> &cnt,
                   std::vector<int> &idx, std::vector<int> &rng) {
     cnt.assign((size_t)ncell + 1, 0);
     rng.resize(2 * (size_t)std::max(n - 1, 0));
     const double inv = 1.0 / step;
+    double amax = std::max(std::fabs(lo), std::fabs(lo + (ncell + 1) * step));
+    for (int k = 0; k < n; k++) amax = std::max(amax, (double)std::fabs(V[2 * k + axis]));
+    const double m = amax * (0x1p-22 + (ncell + 8) * 0x1p-50) + step * 0x1p-20;
     float a = V[axis];
     for (int k = 0; k + 1 < n; k++) {
         const float b = V[2 * (k + 1) + axis];
         const double mn = std::min(a, b), mx = std::max(a, b);
-        int c0 = (int)std::floor((mn - lo) * inv) - 1;
-        int c1 = (int)std::floor((mx - lo) * inv) + 1;
+        int c0 = (int)std::floor((mn - m - lo) * inv);
+        int c1 = (int)std::floor((mx + m - lo) * inv);
         c0 = std::max(c0, 0);
         c1 = std::min(c1, ncell - 1);
         rng[2 * k] = c0;
@@ -140,9 +147,10 @@ RasterGrid size_grid(const double mbb[4], int num_pixels) {
 namespace {
 
 // Point dedup of TempPolygon::addPoint (MyPolygon.h:49-57).  When every
-// coordinate the strip can hold is >= 2^-6 (and all share a sign), two
-// different floats differ by at least 2^-29 > 1e-9, so tolerant equality is
-// exact equality and a hash set answers it.
+// coordinate the strip can hold has |v| >= 2^-6, two different floats on one
+// axis differ by at least 2^-29 > 1e-9 (same sign: one ulp at 2^-6; opposite
+// signs: 2^-5), so tolerant equality is exact equality and a hash set answers
+// it.
 struct Dedup {
     bool exact;
     std::vector<uint64_t> *tab;
@@ -450,9 +458,11 @@ static void build_raster_impl(const float *ring, int nv, RasterOut &out, RasterS
 
     // ---- fill classes: clip to each column strip, then to each pixel ----
     const double pix_area = sx * sy;
-    // every clip point lies in the grid box; above 0.016 distinct floats are
-    // more than 1e-9 apart, so the tolerant dedup is an exact one there
-    const bool exact_dedup = (x0 > 0.0160 && y0 > 0.0160);
+    // every clip point lies in the grid box (one cell of slack past its high
+    // side); when neither axis of it reaches into [-0.016, 0.016], distinct
+    // floats are more than 1e-9 apart and the tolerant dedup is an exact one
+    auto off_zero = [](double lo, double hi) { return lo > 0.0160 || hi < -0.0160; };
+    const bool exact_dedup = off_zero(x0, x0 + (dimx + 1) * sx) && off_zero(y0, y0 + (dimy + 1) * sy);
     Dedup dd{exact_dedup, &S->hash};
     auto &strips = S->strips;
     auto &sstart = S->strip_start;
@@ -523,14 +533,12 @@ static void build_raster_impl(const float *ring, int nv, RasterOut &out, RasterS
         const P2 *V = strips.data() + sstart[si];
         const int nvs = sstart[si + 1] - sstart[si];
         const int col = scol[si];
-        // small strips scan all their edges per row; large ones bucket them
-        const bool use_rows = (int64_t)nvs * nrow > 4096;
-        if (use_rows) bucket_edges(reinterpret_cast<const float *>(V), nvs, 1, y0, sy, nrow, S->bcnt, S->bidx, S->brng);
+        bucket_edges(reinterpret_cast<const float *>(V), nvs, 1, y0, sy, nrow, S->bcnt, S->bidx, S->brng);
         double Yi = y0, Yi1 = Yi + sy;
         int row = 0;
         while (Yi1 < ky + 1e-9) {
             cell.clear();
-            const bool bucketed = use_rows && row < nrow;
+            const bool bucketed = row < nrow;
             const int kb = bucketed ? S->bcnt[row] : 0, ke = bucketed ? S->bcnt[row + 1] : nvs - 1;
             for (int kk = kb; kk < ke; kk++) {
                 const int k = bucketed ? S->bidx[kk] : kk;

@@ -222,3 +222,22 @@ def test_product_raster_matches_oracle_near_origin():
     (ro, xy), _ = datagen.star_pair_set(14, 100, 1, lo=0.0, hi=0.02)
     xy = (xy * np.float32(0.01)).astype(np.float32)
     _compare_rasters(ro, xy, 10)
+
+
+def test_product_raster_matches_oracle_negative_coordinates():
+    # western/southern hemisphere: the exact (hashed) dedup holds on both
+    # sides of zero
+    ro, xy = datagen.lake_set(15, 150, lo=-123.0, hi=-120.0)
+    xy = (xy + np.float32([0.0, 80.0])).astype(np.float32)
+    _compare_rasters(ro, xy, 10)
+    _compare_rasters(ro, xy, 1)
+
+
+def test_product_raster_matches_oracle_fine_pixels_far_from_origin():
+    # pixels a few float spacings wide (x near 178, where one float step is
+    # 2^-16): the edge buckets must still hold every edge a rounded cell bound
+    # reaches
+    ro, xy = datagen.lake_set(16, 60, lo=0.0, hi=1.0)
+    xy = (xy.astype(np.float64) * 0.01 + [178.0, -60.0]).astype(np.float32)
+    _compare_rasters(ro, xy, 1)
+    _compare_rasters(ro, xy, 4)
