@@ -301,39 +301,162 @@ __device__ __forceinline__ void edge_test(float ax, float ay, float d1x, float d
     }
 }
 
+// Edge pairs that passed the line screen below, waiting for the exact test:
+// vertex indexes of the source and target edge's first vertex.
+constexpr uint32_t kQueue = 128;  // flushed once 64 are queued, so one push always fits
+struct EdgeQueue {
+    uint32_t ea[kQueue];
+    uint32_t eb[kQueue];
+};
+
+// The exact test (edge_test) of every queued edge pair, one per lane.
+__device__ __forceinline__ void flush_edges(const float *AX, const float *AY, const float *BX, const float *BY,
+                                            EdgeQueue &Q, uint32_t &qn, Rec *R, uint32_t cap, uint32_t *cnt,
+                                            int lane) {
+    wave_sync();
+    for (uint32_t k = lane; k < qn; k += 64) {
+        const uint32_t ea = Q.ea[k], eb = Q.eb[k];
+        const float ax = AX[ea], ay = AY[ea], bx = BX[eb], by = BY[eb];
+        edge_test(ax, ay, AX[ea + 1] - ax, AY[ea + 1] - ay, bx, by, BX[eb + 1] - bx, BY[eb + 1] - by, ea, eb, R,
+                  cap, cnt);
+    }
+    qn = 0;
+    wave_sync();
+}
+
+// Appends the lanes with cand set (ballot prefix); qn is wave-uniform.
+__device__ __forceinline__ void push_edge(bool cand, uint32_t ea, uint32_t eb, const float *AX, const float *AY,
+                                          const float *BX, const float *BY, EdgeQueue &Q, uint32_t &qn, Rec *R,
+                                          uint32_t cap, uint32_t *cnt, int lane) {
+    const uint64_t m = __ballot(cand);
+    if (m == 0) return;
+    if (cand) {
+        const uint32_t pos = qn + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        Q.ea[pos] = ea;
+        Q.eb[pos] = eb;
+    }
+    qn = uni(qn + (uint32_t)__popcll(m));
+    if (qn >= 64) flush_edges(AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+}
+
+// Line screen.  For the source edge a + s*d1 of a lane, o_j = (b_j - a) x d1
+// is computed with edge_test's own c1 expression, once per target vertex, and
+// serves the two target edges that meet at b_j.  edge_test accepts edge j only
+// if u = o_j / dn lies in [-1e-9, 1 + 1e-9]; in exact arithmetic o_{j+1} =
+// o_j - dn, and the rounding of o_j, o_{j+1} and dn (each product and
+// difference rounded once, f32) moves o_{j+1} - (o_j - dn) by at most
+// 2^-21 * |d1|max * (R_j + R_{j+1}), R = |b - a|_1, while 1e-9 * |dn| stays
+// below 2^-29 of the same scale.  So an accepted edge has min(o_j, o_{j+1})
+// <= m and max(o_j, o_{j+1}) >= -m for m = 2^-19 * |d1|_1 * (R_j + R_{j+1})
+// (four times the bound, which absorbs the rounding of m itself); edges that
+// fail the screen cannot produce a record, and the rest go to the exact test.
+struct LineProbe {
+    float o, r;
+};
+__device__ __forceinline__ LineProbe line_probe(float ax, float ay, float d1x, float d1y, float bx, float by) {
+    const float rx = bx - ax, ry = by - ay;
+    return {rx * d1y - ry * d1x, fabsf(rx) + fabsf(ry)};
+}
+__device__ __forceinline__ bool line_pass(LineProbe p0, LineProbe p1, float k) {
+    const float m = k * (p0.r + p1.r);
+    return (p0.o <= m || p1.o <= m) && (p0.o >= -m || p1.o >= -m);
+}
+// The same test for the whole wave as a lane mask: four compares whose masks
+// combine in scalar registers (llvm.amdgcn.fcmp; 3 = ordered >=, 5 = ordered <=).
+__device__ __forceinline__ uint64_t line_pass_mask(LineProbe p0, LineProbe p1, float k) {
+    const float m = k * (p0.r + p1.r);
+    const uint64_t lo = __builtin_amdgcn_fcmpf(p0.o, m, 5) | __builtin_amdgcn_fcmpf(p1.o, m, 5);
+    const uint64_t hi = __builtin_amdgcn_fcmpf(p0.o, -m, 3) | __builtin_amdgcn_fcmpf(p1.o, -m, 3);
+    return lo & hi;
+}
+
+// Target vertices read through the constant address space: the vertex arrays
+// do not change during a launch, and uniform addresses then become scalar
+// loads.
+typedef __attribute__((address_space(4))) const float cfloat;
+constexpr int kScreenChunk = 16;  // target vertices per chunk (<= the vertex padding)
+
+// Queues the lanes of mask (vertex indexes ea per lane, eb uniform).
+__device__ __forceinline__ void push_mask(uint64_t mask, uint32_t ea, uint32_t eb, const float *AX, const float *AY,
+                                          const float *BX, const float *BY, EdgeQueue &Q, uint32_t &qn, Rec *R,
+                                          uint32_t cap, uint32_t *cnt, int lane) {
+    if ((mask >> lane) & 1ull) {
+        const uint32_t pos = qn + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        Q.ea[pos] = ea;
+        Q.eb[pos] = eb;
+    }
+    qn = uni(qn + (uint32_t)__popcll(mask));
+    if (qn >= 64) flush_edges(AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+}
+
 // All edge pairs of one sequence pair.  Source edges go to lanes and stay in
-// registers; target edges stream uniformly (scalar loads) when the source
-// sequence fills the wave, else lanes split into phases over the target
-// edges.
+// registers; when the source sequence fills the wave, the target vertices
+// stream in uniform chunks (scalar loads), else lanes split into phases, each
+// over a contiguous block of the target edges.  Pairs passing the line screen
+// are queued for the exact test.
 __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *AY, const float *BX, const float *BY,
-                                                 uint2 e1, uint2 e2, Rec *R, uint32_t cap, uint32_t *cnt, int lane) {
+                                                 uint2 e1, uint2 e2, EdgeQueue &Q, uint32_t &qn, Rec *R,
+                                                 uint32_t cap, uint32_t *cnt, int lane) {
     const uint32_t la = e1.x, na = e1.y, lb = e2.x, nb = e2.y;  // first edge, edge count
     if (na == 0 || nb == 0) return;
     if (na >= 48) {
+        // a chunk reads up to kScreenChunk - 1 vertices past the sequence,
+        // inside the zeroed padding that follows the vertex arrays at worst
+        const cfloat *bxc = (const cfloat *)(BX + lb), *byc = (const cfloat *)(BY + lb);
         for (uint32_t i0 = 0; i0 < na; i0 += 64) {
             const uint32_t i = i0 + lane;
             const bool on = i < na;
             const uint32_t ea = la + (on ? i : 0);
             const float ax = AX[ea], ay = AY[ea];
             const float d1x = AX[ea + 1] - ax, d1y = AY[ea + 1] - ay;
-            float bx = BX[lb], by = BY[lb];
-            for (uint32_t j = 0; j < nb; j++) {
-                const float b1x = BX[lb + j + 1], b1y = BY[lb + j + 1];
-                if (on) edge_test(ax, ay, d1x, d1y, bx, by, b1x - bx, b1y - by, ea, lb + j, R, cap, cnt);
-                bx = b1x;
-                by = b1y;
+            const float k = 0x1p-19f * (fabsf(d1x) + fabsf(d1y));
+            const uint64_t on_mask = __ballot(on);
+            LineProbe p0 = line_probe(ax, ay, d1x, d1y, bxc[0], byc[0]);
+            for (uint32_t j0 = 0; j0 < nb; j0 += kScreenChunk) {
+                // every step of the chunk is evaluated (the loads stay inside
+                // the padding); steps past the sequence are masked off, and
+                // the carried probe only matters when a full chunk follows
+                const uint32_t nstep = min((uint32_t)kScreenChunk, nb - j0);
+                float qx[kScreenChunk], qy[kScreenChunk];
+#pragma unroll
+                for (int m = 0; m < kScreenChunk; m++) {
+                    qx[m] = bxc[j0 + 1 + m];
+                    qy[m] = byc[j0 + 1 + m];
+                }
+                uint64_t mk[kScreenChunk];
+                uint64_t any = 0;
+#pragma unroll
+                for (int m = 0; m < kScreenChunk; m++) {
+                    const LineProbe p1 = line_probe(ax, ay, d1x, d1y, qx[m], qy[m]);
+                    mk[m] = (uint32_t)m < nstep ? line_pass_mask(p0, p1, k) & on_mask : 0ull;
+                    any |= mk[m];
+                    p0 = p1;
+                }
+                if (any) {
+#pragma unroll
+                    for (int m = 0; m < kScreenChunk; m++)
+                        if (mk[m]) push_mask(mk[m], ea, lb + j0 + m, AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+                }
             }
         }
     } else {
         const uint32_t phases = 64 / na;  // >= 1
+        const uint32_t blk = (nb + phases - 1) / phases;
         const uint32_t i = lane % na, ph = lane / na;
-        if (ph >= phases) return;
+        const bool on = ph < phases;
+        const uint32_t j0 = on ? min(ph * blk, nb) : nb, j1 = on ? min(j0 + blk, nb) : nb;
         const uint32_t ea = la + i;
         const float ax = AX[ea], ay = AY[ea];
         const float d1x = AX[ea + 1] - ax, d1y = AY[ea + 1] - ay;
-        for (uint32_t j = ph; j < nb; j += phases) {
-            const float bx = BX[lb + j], by = BY[lb + j];
-            edge_test(ax, ay, d1x, d1y, bx, by, BX[lb + j + 1] - bx, BY[lb + j + 1] - by, ea, lb + j, R, cap, cnt);
+        const float k = 0x1p-19f * (fabsf(d1x) + fabsf(d1y));
+        LineProbe p0 = line_probe(ax, ay, d1x, d1y, BX[lb + j0], BY[lb + j0]);
+        for (uint32_t t = 0; t < blk; t++) {
+            const uint32_t j = j0 + t;
+            const bool act = j < j1;
+            const uint32_t v = act ? j + 1 : j0;  // vertex lb + v is within the sequence
+            const LineProbe p1 = line_probe(ax, ay, d1x, d1y, BX[lb + v], BY[lb + v]);
+            push_edge(act && line_pass(p0, p1, k), ea, lb + j, AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+            p0 = p1;
         }
     }
 }
@@ -341,7 +464,8 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
 // Ideal::intersection for one pair plus the area; R / D: the wave's two
 // record slabs of cap records each (cap a power of two).
 __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const double *ring2a, uint32_t ia, uint32_t ib,
-                                                     Rec *R, Rec *D, uint32_t cap, uint32_t *cnt, int lane) {
+                                                     Rec *R, Rec *D, uint32_t cap, uint32_t *cnt, EdgeQueue &Q,
+                                                     int lane) {
     PairRes r;
     for (int k = 0; k < 5; k++) r.c[k] = 0;
     r.area = 0.0;
@@ -378,6 +502,7 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
     const float *BX = S.x + hB.vert_start, *BY = S.y + hB.vert_start;
     if (lane == 0) *cnt = 0u;
     wave_sync();
+    uint32_t qn = 0;
     for (int x = w.x0; x <= w.x1; x++) {
         for (int y = w.y0; y <= w.y1; y++) {
             const uint32_t pa = (uint32_t)(y * hA.dimx + x);
@@ -397,13 +522,14 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
                         for (uint32_t sb = sb0; sb < sb1; sb++) {
                             const uint2 e2 = S.seqs[hB.seq_start + sb];
                             r.tests += (uint64_t)e1.y * e2.y;
-                            collect_seq_pair(AX, AY, BX, BY, e1, e2, R, cap, cnt, lane);
+                            collect_seq_pair(AX, AY, BX, BY, e1, e2, Q, qn, R, cap, cnt, lane);
                         }
                     }
                 }
             }
         }
     }
+    if (qn) flush_edges(AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
     wave_sync();
     const uint32_t nrec = uni(*cnt);
     r.raw = nrec;
@@ -501,7 +627,9 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
     Rec *R = slab + gw * 2ull * cap;
     Rec *D = R + cap;
     __shared__ uint32_t rec_cnt[kWaves];
+    __shared__ EdgeQueue queues[kWaves];
     uint32_t *cnt = &rec_cnt[threadIdx.x >> 6];
+    EdgeQueue &Q = queues[threadIdx.x >> 6];
     unsigned long long bb = 0, tests = 0, raw = 0, found = 0;
     for (;;) {
         uint32_t base = 0;
@@ -517,7 +645,7 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
                 if (lane == 0) area[idx] = 0.0;
                 continue;
             }
-            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, cnt, lane);
+            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, cnt, Q, lane);
             if (r.over) {
                 if (lane == 0) {
                     if (BIG) {

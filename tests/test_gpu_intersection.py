@@ -86,3 +86,30 @@ def test_rejected_polygons(device):
     w = Workload(rings_from_list(a), rings_from_list(b), vpr=1, pairs=(pa.ravel(), pb.ravel()))
     c, _, _ = _check(device, w)
     assert (c[:, 0] == -3).sum() >= 2
+
+
+def _diamond(cx, cy, r, n):
+    t = np.arange(n) / n
+    c = [(cx, cy - r), (cx + r, cy), (cx, cy + r), (cx - r, cy)]
+    return np.vstack([np.stack([a[0] + (b[0] - a[0]) * t, a[1] + (b[1] - a[1]) * t], 1)
+                      for a, b in zip(c, c[1:] + c[:1])]).astype(np.float32)
+
+
+def test_touching_long_sequences(device):
+    # shared vertices (t, u exactly 0 or 1), vertices on the other ring's
+    # edges, collinear overlaps, a 1e-7 near miss and a sawtooth whose teeth
+    # touch an edge: at -v 256 every ring is one long sequence (the chunked
+    # screen), at -v 10 the sequences are short (the phased screen)
+    a = [_rect(1, 1, 2, 2, 60), _rect(4, 1, 5, 2, 64)]
+    xs = np.linspace(4.0, 5.0, 41)
+    ys = np.where(np.arange(41) % 4 == 0, 1.0, np.where(np.arange(41) % 2 == 1, 0.9, 1.1))
+    saw = np.vstack([np.stack([xs, ys], 1), [[5.0, 0.5], [4.0, 0.5]]]).astype(np.float32)
+    b = [_rect(1.5, 0.5, 2.5, 1.5, 60), _rect(2.0, 1.25, 3.0, 1.75, 60), _rect(1.25, 2.0, 1.75, 3.0, 50),
+         _rect(0.5, 1.3, 0.9999999, 1.7, 60), _diamond(4.5, 2.5, 0.5, 40), _diamond(4.0 + 1 / 128, 2.3, 0.3, 40),
+         saw]
+    n, m = len(a), len(b)
+    pa, pb = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    for vpr in (256, 10):
+        w = Workload(rings_from_list(a), rings_from_list(b), vpr=vpr, pairs=(pa.ravel(), pb.ravel()))
+        c, _, _ = _check(device, w)
+        assert (c[:, 0] > 0).sum() >= 4  # crossings were found, not just screened away
