@@ -17,14 +17,17 @@
 // One wavefront per pair:
 //  1. the source raster's window over the target MBB; all OUT / all IN
 //     windows end the pair (the reference's early return)
-//  2. every edge pair of every BORDER x BORDER pixel pair's sequence pairs,
-//     lanes over the flattened edge pairs; a float screen (margins far
-//     above its rounding) skips pairs whose parameters are clearly outside
-//     [0, 1]; the rest take the reference's exact f32 / f64 test; hits are
-//     appended to a per-wave record slab in global memory
+//  2. every edge pair of every BORDER x BORDER pixel pair's sequence pairs:
+//     source edges on lanes, target vertices streamed; a line screen (the
+//     side of each target vertex against the source edge's line, with a
+//     margin above the f32 rounding) drops edge pairs that cannot cross;
+//     the rest are queued in LDS and take the reference's exact f32 / f64
+//     test a wave at a time; hits are appended to a per-wave record slab in
+//     global memory
 //  3. bitonic sorts of the slab (by point; by source edge and parameter;
-//     by target edge and parameter), de-duplication, pieces, statuses and
-//     the wave-cooperative path sums of the IN pieces
+//     by target edge and parameter), de-duplication, pieces, their statuses
+//     (point-in-polygon tests, wave-cooperative when a chunk holds few
+//     pieces) and the wave-cooperative path sums of the IN pieces
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -161,6 +164,54 @@ __device__ int seg_contain(const DevSet &S, const farm_raster_hdr &h, float px, 
     return ret ? 2 : 0;
 }
 
+// seg_contain for one point (the same for every lane), the whole wave
+// testing the pixel's edges together.  The serial form returns 1 at the first
+// edge that puts the point on the boundary, else the parity of the crossings
+// to the right within the pixel and of the sorted grid-line nodes below it:
+// neither depends on the order the edges are visited in, so "on" is a wave
+// any, the parity a ballot count, and the node prefix a count of nodes < py.
+__device__ int seg_contain_wave(const DevSet &S, const farm_raster_hdr &h, float px, float py, int lane) {
+    const int x = off_axis((double)px, h.mbr[0], h.step[0], h.dimx);
+    const int y = off_axis((double)py, h.mbr[1], h.step[1], h.dimy);
+    const uint32_t id = (uint32_t)(y * h.dimx + x);
+    const uint8_t st = S.status[h.pix_start + id];
+    if (st == kCC - 1) return 2;
+    if (st == 0) return 0;
+    const double bx_high = h.mbr[0] + (x + 1) * h.step[0];
+    bool on = false, par = false;
+    const uint32_t s0 = S.pixoff[h.pixoff_start + id], s1 = S.pixoff[h.pixoff_start + id + 1];
+    const float *X = S.x + h.vert_start, *Y = S.y + h.vert_start;
+    for (uint32_t e = s0; e < s1; e++) {
+        const uint2 sq = S.seqs[h.seq_start + e];
+        for (uint32_t k = lane; k < sq.y; k += 64) {
+            const float v1x = X[sq.x + k], v1y = Y[sq.x + k];
+            const float v2x = X[sq.x + k + 1], v2y = Y[sq.x + k + 1];
+            if (pt_eq(px, py, v1x, v1y) || pt_eq(px, py, v2x, v2y)) on = true;
+            if ((v1y >= py) != (v2y >= py)) {
+                const double dx = (double)(v2x - v1x);
+                const double dy = (double)(v2y - v1y);
+                const double pyd = (double)(py - v1y);
+                if (fabs(dy) > 1e-9) {
+                    const double ix = dx * pyd / dy + (double)v1x;
+                    if (fabs((double)px - ix) < 1e-9) on = true;
+                    if ((double)px < ix && ix <= bx_high) par = !par;
+                }
+            } else if (v1y == py && v2y == py && ((v1x >= px) != (v2x >= px))) {
+                on = true;
+            }
+        }
+    }
+    if (wave_any(on)) return 1;
+    uint32_t cnt = (uint32_t)__popcll(__ballot(par));
+    const uint32_t *vo = S.voff + h.vline_start;
+    const uint32_t i = vo[x + 1], j = vo[x + 2];
+    for (uint32_t k0 = i; k0 < j; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        cnt += (uint32_t)__popcll(__ballot(k < j && S.vnodes[h.vnode_start + k] < (double)py));
+    }
+    return (cnt & 1) ? 2 : 0;
+}
+
 struct PairRes {
     int32_t c[5];  // [crossing points, pieces, IN, OUT, BORDER]
     double area;
@@ -201,6 +252,7 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
         bool piece = false, in_area = false;
         int st = 0, first = 0, nvert = 0;
         float ax = 0, ay = 0, bx = 0, by = 0;
+        float rx = 0, ry = 0, mx = 0, my = 0;  // the two points tested against the other polygon
         if (i < n) {
             const Rec a = D[i], b = D[(i + 1) % n];
             ax = a.px;
@@ -222,7 +274,6 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
                     be--;
                     bp = 1.0;
                 }
-                float rx, ry;
                 if (ae == be && ap < bp) {
                     rx = (a.px + b.px) * 0.5f;
                     ry = (a.py + b.py) * 0.5f;
@@ -230,7 +281,6 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
                     rx = X[ae + 1];
                     ry = Y[ae + 1];
                 }
-                st = seg_contain(S, hq, rx, ry);
                 // the area piece: a -> ring vertices ea0 + 1 .. eb0 -> b
                 if (i + 1 < n) nvert = eb0 - ea0;
                 else nvert = (eb0 - ea0 + m) % m + ((eb0 == ea0 && tb0 <= ta0) ? m : 0);
@@ -240,10 +290,28 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
                 float ux, uy, vx, vy;
                 path_pt(X, Y, m, first, nvert, ax, ay, bx, by, hh, ux, uy);
                 path_pt(X, Y, m, first, nvert, ax, ay, bx, by, hh + 1, vx, vy);
-                in_area = seg_contain(S, hq, (ux + vx) * 0.5f, (uy + vy) * 0.5f) == 2;
+                mx = (ux + vx) * 0.5f;
+                my = (uy + vy) * 0.5f;
             }
         }
         const uint64_t pm = __ballot(piece);
+        if (__popcll(pm) <= 32) {
+            // few pieces: the wave tests each point's pixel edges together
+            uint64_t q = pm;
+            while (q) {
+                const int src = __ffsll((unsigned long long)q) - 1;
+                q &= q - 1;
+                const int s1 = seg_contain_wave(S, hq, uni(__shfl(rx, src, 64)), uni(__shfl(ry, src, 64)), lane);
+                const int s2 = seg_contain_wave(S, hq, uni(__shfl(mx, src, 64)), uni(__shfl(my, src, 64)), lane);
+                if (lane == src) {
+                    st = s1;
+                    in_area = s2 == 2;
+                }
+            }
+        } else if (piece) {
+            st = seg_contain(S, hq, rx, ry);
+            in_area = seg_contain(S, hq, mx, my) == 2;
+        }
         r.c[1] += __popcll(pm);
         r.c[2] += __popcll(__ballot(piece && st == 2));
         r.c[3] += __popcll(__ballot(piece && st == 0));
@@ -400,6 +468,22 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
     const uint32_t la = e1.x, na = e1.y, lb = e2.x, nb = e2.y;  // first edge, edge count
     if (na == 0 || nb == 0) return;
     if (na >= 48) {
+        // the target sequence's box bounds every R_j of the screen: with
+        // R_j, R_j+1 <= Rmax = max|bx - ax| + max|by - ay| over the box, one
+        // margin M = 2^-19 * |d1|_1 * 2 * Rmax >= m serves the whole
+        // sequence, and each vertex's side of the line (o > M, o < -M) is a
+        // lane mask shared by its two edges
+        float bx0 = INFINITY, by0 = INFINITY, nbx1 = INFINITY, nby1 = INFINITY;
+        for (uint32_t v = lane; v <= nb; v += 64) {
+            const float x = BX[lb + v], y = BY[lb + v];
+            bx0 = fminf(bx0, x);
+            by0 = fminf(by0, y);
+            nbx1 = fminf(nbx1, -x);
+            nby1 = fminf(nby1, -y);
+        }
+        bx0 = wave_min(bx0);
+        by0 = wave_min(by0);
+        const float bx1 = -wave_min(nbx1), by1 = -wave_min(nby1);
         // a chunk reads up to kScreenChunk - 1 vertices past the sequence,
         // inside the zeroed padding that follows the vertex arrays at worst
         const cfloat *bxc = (const cfloat *)(BX + lb), *byc = (const cfloat *)(BY + lb);
@@ -409,13 +493,15 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
             const uint32_t ea = la + (on ? i : 0);
             const float ax = AX[ea], ay = AY[ea];
             const float d1x = AX[ea + 1] - ax, d1y = AY[ea + 1] - ay;
-            const float k = 0x1p-19f * (fabsf(d1x) + fabsf(d1y));
+            const float rmax = fmaxf(fabsf(bx0 - ax), fabsf(bx1 - ax)) + fmaxf(fabsf(by0 - ay), fabsf(by1 - ay));
+            const float M = 0x1p-18f * (fabsf(d1x) + fabsf(d1y)) * rmax;
             const uint64_t on_mask = __ballot(on);
-            LineProbe p0 = line_probe(ax, ay, d1x, d1y, bxc[0], byc[0]);
+            float o = line_probe(ax, ay, d1x, d1y, bxc[0], byc[0]).o;
+            uint64_t gt = __builtin_amdgcn_fcmpf(o, M, 2), lt = __builtin_amdgcn_fcmpf(o, -M, 4);
             for (uint32_t j0 = 0; j0 < nb; j0 += kScreenChunk) {
                 // every step of the chunk is evaluated (the loads stay inside
                 // the padding); steps past the sequence are masked off, and
-                // the carried probe only matters when a full chunk follows
+                // the carried masks only matter when a full chunk follows
                 const uint32_t nstep = min((uint32_t)kScreenChunk, nb - j0);
                 float qx[kScreenChunk], qy[kScreenChunk];
 #pragma unroll
@@ -427,10 +513,14 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
                 uint64_t any = 0;
 #pragma unroll
                 for (int m = 0; m < kScreenChunk; m++) {
-                    const LineProbe p1 = line_probe(ax, ay, d1x, d1y, qx[m], qy[m]);
-                    mk[m] = (uint32_t)m < nstep ? line_pass_mask(p0, p1, k) & on_mask : 0ull;
+                    o = line_probe(ax, ay, d1x, d1y, qx[m], qy[m]).o;
+                    // fcmp predicates: 2 = ordered >, 4 = ordered <
+                    const uint64_t gt1 = __builtin_amdgcn_fcmpf(o, M, 2), lt1 = __builtin_amdgcn_fcmpf(o, -M, 4);
+                    const uint64_t fail = (gt & gt1) | (lt & lt1);
+                    mk[m] = (uint32_t)m < nstep ? ~fail & on_mask : 0ull;
                     any |= mk[m];
-                    p0 = p1;
+                    gt = gt1;
+                    lt = lt1;
                 }
                 if (any) {
 #pragma unroll
@@ -585,8 +675,8 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
     }
     if (pieces == 0) {
         // containment or disjoint
-        if (seg_contain(S, hB, AX[0], AY[0]) == 2) r.area = fabs(ring2a[ia]) / 2.0;
-        else if (seg_contain(S, hA, BX[0], BY[0]) == 2) r.area = fabs(ring2a[ib]) / 2.0;
+        if (seg_contain_wave(S, hB, AX[0], AY[0], lane) == 2) r.area = fabs(ring2a[ia]) / 2.0;
+        else if (seg_contain_wave(S, hA, BX[0], BY[0], lane) == 2) r.area = fabs(ring2a[ib]) / 2.0;
         return r;
     }
     const double sa = ring2a[ia] < 0 ? -1.0 : 1.0, sb = ring2a[ib] < 0 ? -1.0 : 1.0;

@@ -3,7 +3,9 @@ edge pair that the reference's crossing test accepts.
 
 The kernel skips an edge pair (a + s*d1, b_j .. b_j+1) when both
 o_j = (b_j - a) x d1 and o_j+1 exceed m = 2^-19 * |d1|_1 * (R_j + R_j+1)
-(R = |b - a|_1), or both fall below -m.  The crossing test it guards
+(R = |b - a|_1), or both fall below -m; for long sequences it uses one
+margin per source edge, M = 2^-18 * |d1|_1 * Rmax >= m, with Rmax bounding
+R over the target sequence's box.  The crossing test it guards
 (segment_intersect_batch with intersection records,
 geometry_computation.h:239-267) keeps the pair when |dn| >= 1e-12 and
 t = c2 / dn, u = c1 / dn both lie in [-1e-9, 1 + 1e-9], where c1 = o_j and
@@ -45,6 +47,20 @@ def _screen(ax, ay, a1x, a1y, bx, by, b1x, b1y):
     return ((o0 <= m) | (o1 <= m)) & ((o0 >= -m) | (o1 >= -m))
 
 
+def _screen_box(ax, ay, a1x, a1y, bx, by, b1x, b1y):
+    # the kernel's form for long sequences: one margin per source edge from
+    # the target sequence's box (here the smallest possible box, the edge's
+    # own two vertices), each vertex classified against it once
+    d1x, d1y = a1x - ax, a1y - ay
+    bx0, bx1 = np.minimum(bx, b1x), np.maximum(bx, b1x)
+    by0, by1 = np.minimum(by, b1y), np.maximum(by, b1y)
+    rmax = np.maximum(np.abs(bx0 - ax), np.abs(bx1 - ax)) + np.maximum(np.abs(by0 - ay), np.abs(by1 - ay))
+    M = F(2.0 ** -18) * (np.abs(d1x) + np.abs(d1y)) * rmax
+    o0 = (bx - ax) * d1y - (by - ay) * d1x
+    o1 = (b1x - ax) * d1y - (b1y - ay) * d1x
+    return ~(((o0 > M) & (o1 > M)) | ((o0 < -M) & (o1 < -M)))
+
+
 def _near_degenerate(rng, n, scale, offset):
     # end points on a coarse lattice (exact touches and collinear runs are
     # common), half of them nudged by a few ulps
@@ -63,8 +79,9 @@ def test_screen_never_rejects_an_accepted_pair():
         for _ in range(4):
             args = _near_degenerate(rng, 250_000, scale, offset)
             acc = _accepts(*args)
-            scr = _screen(*args)
-            assert not (acc & ~scr).any(), (scale, offset, np.nonzero(acc & ~scr)[0][:5])
+            for screen in (_screen, _screen_box):
+                scr = screen(*args)
+                assert not (acc & ~scr).any(), (screen.__name__, scale, offset, np.nonzero(acc & ~scr)[0][:5])
             accepted += int(acc.sum())
     assert accepted > 100_000  # the cases do reach the accept branch
 
@@ -80,6 +97,8 @@ def test_screen_rejects_most_pairs_of_a_chain():
     a1x, a1y = (ax + F(0.01)).astype(F), (ay + F(0.004)).astype(F)
     A = [np.repeat(v, 2000) for v in (ax, ay, a1x, a1y)]
     B = [np.tile(v, 500) for v in (cx[:-1], cy[:-1], cx[1:], cy[1:])]
-    scr = _screen(*A, *B)
-    assert not (_accepts(*A, *B) & ~scr).any()
-    assert scr.mean() < 0.05
+    acc = _accepts(*A, *B)
+    for screen in (_screen, _screen_box):
+        scr = screen(*A, *B)
+        assert not (acc & ~scr).any()
+        assert scr.mean() < 0.05
