@@ -457,6 +457,55 @@ __device__ __forceinline__ void push_mask(uint64_t mask, uint32_t ea, uint32_t e
     if (qn >= 64) flush_edges(AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
 }
 
+// One lane's source edge for the chunked screen, and the side masks of a
+// target vertex: o = (q - a) x d1 (fused: the screen's own value, within the
+// same error bound as the reference's c1, which the margin M covers); bits of
+// gt / lt set where o > M / o < -M (fcmp predicates 2 and 4, ordered).
+struct ScreenLane {
+    float ax, ay, d1x, d1y, M;
+};
+__device__ __forceinline__ void screen_side(const ScreenLane &L, float qx, float qy, uint64_t &gt, uint64_t &lt) {
+    const float o = fmaf(qx - L.ax, L.d1y, -((qy - L.ay) * L.d1x));
+    gt = __builtin_amdgcn_fcmpf(o, L.M, 2);
+    lt = __builtin_amdgcn_fcmpf(o, -L.M, 4);
+}
+
+// Target edges j0 .. j0 + nstep - 1 (nstep == kScreenChunk unless TAIL): the
+// vertices of the chunk are uniform scalar loads, which may read up to
+// kScreenChunk - 1 vertices past the sequence, inside the zeroed padding that
+// follows the vertex arrays at worst.  An edge passes unless both its
+// vertices lie beyond the margin on one side.
+template <bool TAIL>
+__device__ __forceinline__ void screen_chunk(const ScreenLane &L, const cfloat *bxc, const cfloat *byc, uint32_t j0,
+                                             uint32_t nstep, uint64_t on_mask, uint64_t &gt, uint64_t &lt,
+                                             uint32_t ea, uint32_t lb, const float *AX, const float *AY,
+                                             const float *BX, const float *BY, EdgeQueue &Q, uint32_t &qn, Rec *R,
+                                             uint32_t cap, uint32_t *cnt, int lane) {
+    float qx[kScreenChunk], qy[kScreenChunk];
+#pragma unroll
+    for (int m = 0; m < kScreenChunk; m++) {
+        qx[m] = bxc[j0 + 1 + m];
+        qy[m] = byc[j0 + 1 + m];
+    }
+    uint64_t mk[kScreenChunk];
+    uint64_t any = 0;
+#pragma unroll
+    for (int m = 0; m < kScreenChunk; m++) {
+        uint64_t gt1, lt1;
+        screen_side(L, qx[m], qy[m], gt1, lt1);
+        mk[m] = ~((gt & gt1) | (lt & lt1)) & on_mask;
+        if (TAIL && (uint32_t)m >= nstep) mk[m] = 0;
+        any |= mk[m];
+        gt = gt1;
+        lt = lt1;
+    }
+    if (any) {
+#pragma unroll
+        for (int m = 0; m < kScreenChunk; m++)
+            if (mk[m]) push_mask(mk[m], ea, lb + j0 + m, AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+    }
+}
+
 // All edge pairs of one sequence pair.  Source edges go to lanes and stay in
 // registers; when the source sequence fills the wave, the target vertices
 // stream in uniform chunks (scalar loads), else lanes split into phases, each
@@ -484,8 +533,6 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
         bx0 = wave_min(bx0);
         by0 = wave_min(by0);
         const float bx1 = -wave_min(nbx1), by1 = -wave_min(nby1);
-        // a chunk reads up to kScreenChunk - 1 vertices past the sequence,
-        // inside the zeroed padding that follows the vertex arrays at worst
         const cfloat *bxc = (const cfloat *)(BX + lb), *byc = (const cfloat *)(BY + lb);
         for (uint32_t i0 = 0; i0 < na; i0 += 64) {
             const uint32_t i = i0 + lane;
@@ -496,38 +543,16 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
             const float rmax = fmaxf(fabsf(bx0 - ax), fabsf(bx1 - ax)) + fmaxf(fabsf(by0 - ay), fabsf(by1 - ay));
             const float M = 0x1p-18f * (fabsf(d1x) + fabsf(d1y)) * rmax;
             const uint64_t on_mask = __ballot(on);
-            float o = line_probe(ax, ay, d1x, d1y, bxc[0], byc[0]).o;
-            uint64_t gt = __builtin_amdgcn_fcmpf(o, M, 2), lt = __builtin_amdgcn_fcmpf(o, -M, 4);
-            for (uint32_t j0 = 0; j0 < nb; j0 += kScreenChunk) {
-                // every step of the chunk is evaluated (the loads stay inside
-                // the padding); steps past the sequence are masked off, and
-                // the carried masks only matter when a full chunk follows
-                const uint32_t nstep = min((uint32_t)kScreenChunk, nb - j0);
-                float qx[kScreenChunk], qy[kScreenChunk];
-#pragma unroll
-                for (int m = 0; m < kScreenChunk; m++) {
-                    qx[m] = bxc[j0 + 1 + m];
-                    qy[m] = byc[j0 + 1 + m];
-                }
-                uint64_t mk[kScreenChunk];
-                uint64_t any = 0;
-#pragma unroll
-                for (int m = 0; m < kScreenChunk; m++) {
-                    o = line_probe(ax, ay, d1x, d1y, qx[m], qy[m]).o;
-                    // fcmp predicates: 2 = ordered >, 4 = ordered <
-                    const uint64_t gt1 = __builtin_amdgcn_fcmpf(o, M, 2), lt1 = __builtin_amdgcn_fcmpf(o, -M, 4);
-                    const uint64_t fail = (gt & gt1) | (lt & lt1);
-                    mk[m] = (uint32_t)m < nstep ? ~fail & on_mask : 0ull;
-                    any |= mk[m];
-                    gt = gt1;
-                    lt = lt1;
-                }
-                if (any) {
-#pragma unroll
-                    for (int m = 0; m < kScreenChunk; m++)
-                        if (mk[m]) push_mask(mk[m], ea, lb + j0 + m, AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
-                }
-            }
+            ScreenLane L{ax, ay, d1x, d1y, M};
+            uint64_t gt, lt;
+            screen_side(L, bxc[0], byc[0], gt, lt);
+            const uint32_t nfull = nb / kScreenChunk * kScreenChunk;
+            for (uint32_t j0 = 0; j0 < nfull; j0 += kScreenChunk)
+                screen_chunk<false>(L, bxc, byc, j0, kScreenChunk, on_mask, gt, lt, ea, lb, AX, AY, BX, BY, Q, qn, R,
+                                    cap, cnt, lane);
+            if (nfull < nb)
+                screen_chunk<true>(L, bxc, byc, nfull, nb - nfull, on_mask, gt, lt, ea, lb, AX, AY, BX, BY, Q, qn, R,
+                                   cap, cnt, lane);
         }
     } else {
         const uint32_t phases = 64 / na;  // >= 1

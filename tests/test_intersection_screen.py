@@ -5,7 +5,8 @@ The kernel skips an edge pair (a + s*d1, b_j .. b_j+1) when both
 o_j = (b_j - a) x d1 and o_j+1 exceed m = 2^-19 * |d1|_1 * (R_j + R_j+1)
 (R = |b - a|_1), or both fall below -m; for long sequences it uses one
 margin per source edge, M = 2^-18 * |d1|_1 * Rmax >= m, with Rmax bounding
-R over the target sequence's box.  The crossing test it guards
+R over the target sequence's box, and o computed with one fused
+multiply-add (one rounding fewer than the reference's c1).  The crossing test it guards
 (segment_intersect_batch with intersection records,
 geometry_computation.h:239-267) keeps the pair when |dn| >= 1e-12 and
 t = c2 / dn, u = c1 / dn both lie in [-1e-9, 1 + 1e-9], where c1 = o_j and
@@ -56,8 +57,14 @@ def _screen_box(ax, ay, a1x, a1y, bx, by, b1x, b1y):
     by0, by1 = np.minimum(by, b1y), np.maximum(by, b1y)
     rmax = np.maximum(np.abs(bx0 - ax), np.abs(bx1 - ax)) + np.maximum(np.abs(by0 - ay), np.abs(by1 - ay))
     M = F(2.0 ** -18) * (np.abs(d1x) + np.abs(d1y)) * rmax
-    o0 = (bx - ax) * d1y - (by - ay) * d1x
-    o1 = (b1x - ax) * d1y - (b1y - ay) * d1x
+
+    def fused(qx, qy):
+        # fmaf(qx - ax, d1y, -((qy - ay) * d1x)): one rounding of the exact
+        # f32 x f32 product minus the rounded one (f64 holds the product)
+        p = ((qy - ay) * d1x).astype(np.float64)
+        return ((qx - ax).astype(np.float64) * d1y.astype(np.float64) - p).astype(F)
+
+    o0, o1 = fused(bx, by), fused(b1x, b1y)
     return ~(((o0 > M) & (o1 > M)) | ((o0 < -M) & (o1 < -M)))
 
 
