@@ -58,6 +58,24 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+CAND_TIMES: dict = {}
+
+
+def device_candidates(dev, mbb_a, mbb_b, pa, pb, b_base: int, **kw) -> dict:
+    """The same candidate pairs built on the device (farm_generate_pairs),
+    timed and compared with the host R-tree's; they stay as the pair list."""
+    dev.generate_pairs(mbb_a, mbb_b, b_base=b_base, **kw)  # first call allocates
+    t = time.time()
+    n = dev.generate_pairs(mbb_a, mbb_b, b_base=b_base, **kw)
+    dt = time.time() - t
+    ga, gb = dev.download_pairs()
+    same = bool(n == len(pa) and np.array_equal(ga, pa) and np.array_equal(gb, pb.astype(np.uint32) + np.uint32(b_base)))
+    log(f"device candidate pairs: {n} in {dt * 1000:.1f} ms, identical to the host R-tree's: {same}")
+    if not same:
+        raise RuntimeError("device candidate pairs differ from the host R-tree's")
+    return {"pairs": int(n), "host_rtree_s": CAND_TIMES.get("host_rtree_s"), "device_s": dt, "identical": same}
+
+
 def build_workload(scale: float, seed: int, vpr: int, threads: int):
     from farm_amd import capi, datagen
 
@@ -68,11 +86,12 @@ def build_workload(scale: float, seed: int, vpr: int, threads: int):
     cro, cxy = datagen.complex_set(seed, n_c, 0.0, side)
     lro, lxy = datagen.lake_set(seed + 1, n_l, 0.0, side)
     log(f"generated {n_c} complex ({len(cxy)} vertices) + {n_l} lakes ({len(lxy)} vertices) in {time.time() - t:.1f}s")
-    t = time.time()
     A = capi.HostSet.from_rings(cro, cxy)
     B = capi.HostSet.from_rings(lro, lxy)
+    t = time.time()
     pa, pb = capi.candidates(A, B, nthreads=threads)
-    log(f"{len(pa)} candidate pairs in {time.time() - t:.1f}s")
+    CAND_TIMES["host_rtree_s"] = time.time() - t
+    log(f"{len(pa)} candidate pairs (host R-tree, {threads} threads) in {CAND_TIMES['host_rtree_s']:.1f}s")
     t = time.time()
     both = capi.HostSet.concat(A, B)
     both.rasterize(vpr, threads)
@@ -135,7 +154,9 @@ def within_workload(scale: float, seed: int, threads: int):
     log(f"within: -h rasterized (-v 10, --NLow 1) in {time.time() - t:.1f}s")
     t = time.time()
     pa, pb = capi.candidates(H, H, expand_km=10.0, skip_self=True, nthreads=threads)
-    log(f"within: {len(pa)} ordered candidate pairs (MBB expanded by 10 km) in {time.time() - t:.1f}s")
+    CAND_TIMES["host_rtree_s"] = time.time() - t
+    log(f"within: {len(pa)} ordered candidate pairs (MBB expanded by 10 km, host R-tree) in "
+        f"{CAND_TIMES['host_rtree_s']:.1f}s")
     return (ro, xy), H, pa, pb
 
 
@@ -169,7 +190,8 @@ def bench_within(args) -> dict:
     (ro, xy), H, pa, pb = within_workload(args.scale, args.seed, args.threads)
     dev = capi.Device(0)
     dev.upload_polygons(H)
-    dev.upload_pairs(pa, pb)
+    m = H.mbbs()
+    cand_info = device_candidates(dev, m, m, pa, pb, 0, expand_km=10.0, skip_self=True)
     nb = args.steps + args.warmup
     cuts = np.linspace(0, len(pa), nb + 1).astype(np.int64)
     for s in range(args.warmup):
@@ -208,6 +230,7 @@ def bench_within(args) -> dict:
         "roofline": {"bound": "valu", "kernel": "k_within", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "ops_per_point_segment": OPS_PER_PT_SEG, "ops_per_box_pair": OPS_PER_BOX_PAIR},
+        "candidates": cand_info,
     }
     if not args.no_cpu:
         bits, md, _ = dev.within(10.0)
@@ -371,8 +394,9 @@ def main() -> int:
     dev = capi.Device(0)
     t = time.time()
     dev.upload_polygons(both)
-    dev.upload_pairs(pa, pb + len(A))
     log(f"upload {time.time() - t:.1f}s")
+    # the pair list is built on the device (and checked against the host's)
+    cand_info = device_candidates(dev, A.mbbs(), B.mbbs(), pa, pb, len(A))
 
     # Every libfarm call returns after its stream has drained (the counters are
     # read back), so host wall-clock brackets exactly the device work.  The
@@ -535,6 +559,8 @@ def main() -> int:
         log(f"-a learned P={p_l:.4f} R={r_l:.4f} ({t_l * 1000:.1f} ms) | heuristic P={p_h:.4f} R={r_h:.4f} "
             f"({t_h * 1000:.1f} ms) | filter-only R={r_f:.4f}")
 
+    line["candidates"] = cand_info
+
     if not args.no_cpu:
         rng = np.random.default_rng(args.seed + 7)
         res = cpu_baseline(rings, pa, pb, args.vpr, args.threads, args.cpu_seconds, rng, None)
@@ -562,7 +588,7 @@ def main() -> int:
         for name, fn in (("within", bench_within), ("intersection", bench_intersection)):
             r = fn(sub)
             line[name] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline",
-                                              "cpu_baseline", "parity_sample") if k in r}
+                                              "cpu_baseline", "parity_sample", "candidates") if k in r}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -164,6 +164,12 @@ EXPORTS = {
     "farm_last_error": (C.c_char_p, [C.c_void_p]),
     "farm_upload_polygons": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
     "farm_upload_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "farm_generate_pairs": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_double, C.c_int, C.c_uint32, C.c_uint32,
+         C.POINTER(C.c_uint64)],
+    ),
+    "farm_download_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
     "farm_intersect": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)]),
     "farm_intersect_dev": (
         C.c_int,
@@ -425,6 +431,26 @@ class Device:
         b = np.ascontiguousarray(b, dtype=np.uint32)
         self._check(lib.farm_upload_pairs(self._h, _ptr(a), _ptr(b), len(a)), "upload_pairs")
         self.npairs = len(a)
+
+    def generate_pairs(self, mbb_a: np.ndarray, mbb_b: np.ndarray, expand_km: float = 0.0, skip_self: bool = False,
+                       a_base: int = 0, b_base: int = 0) -> int:
+        """Candidate pairs built on the device from (n, 4) boxes (low x, low
+        y, high x, high y): the same pairs, in the same order, as
+        candidates(), kept as the pair list; returns their count."""
+        a = np.ascontiguousarray(mbb_a, dtype=np.float64).reshape(-1, 4)
+        b = np.ascontiguousarray(mbb_b, dtype=np.float64).reshape(-1, 4)
+        n = C.c_uint64(0)
+        self._check(lib.farm_generate_pairs(self._h, _ptr(a), len(a), _ptr(b), len(b), float(expand_km),
+                                            int(bool(skip_self)), a_base, b_base, C.byref(n)), "generate_pairs")
+        self.npairs = int(n.value)
+        return self.npairs
+
+    def download_pairs(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        end = self.npairs if end is None else end
+        a = np.zeros(max(end - begin, 1), np.uint32)
+        b = np.zeros(max(end - begin, 1), np.uint32)
+        self._check(lib.farm_download_pairs(self._h, begin, end, _ptr(a), _ptr(b)), "download_pairs")
+        return a[:end - begin], b[:end - begin]
 
     def intersect(self, begin: int = 0, end: int | None = None, mode: int = MODE_EXACT_BATCHED,
                   tile_a: int = 0, want_bits: bool = True, order: int = 0, accept: float = 0.0,

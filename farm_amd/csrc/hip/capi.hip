@@ -217,7 +217,9 @@ int farm_close(farm_handle h) {
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
                       &h->skipbuf, &h->layers, &h->lstatus, &h->areas, &h->wctr, &h->wovf, &h->wslab,
-                      &h->wslab_big, &h->wout, &h->wmd, &h->ring2a, &h->icounts, &h->iarea};
+                      &h->wslab_big, &h->wout, &h->wmd, &h->ring2a, &h->icounts, &h->iarea, &h->cg_boxa,
+                      &h->cg_boxq, &h->cg_cellcnt, &h->cg_celloff, &h->cg_items, &h->cg_cnt, &h->cg_off,
+                      &h->cg_keys, &h->cg_keys2};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : h->ev)

@@ -664,6 +664,17 @@ const Dpk kDpk;
 
 namespace farm {
 const double *km_lon_table() { return kDpk.v; }
+
+// box::expand(d, geography) (Box.cpp:200-214): the query box of a within
+// join grows by d km, in longitude by the table entry of its low latitude
+void expand_box_km(double q[4], double km) {
+    const double sx = km * kDpk.lon(q[1]);
+    const double sy = km * kDegPerKmLat;
+    q[0] -= sx;
+    q[2] += sx;
+    q[1] -= sy;
+    q[3] += sy;
+}
 }  // namespace farm
 
 extern "C" int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b, double expand_km, int skip_self,
@@ -684,14 +695,7 @@ extern "C" int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b,
             for (int64_t j = c * chunk; j < std::min(nb, (c + 1) * chunk); j++) {
                 double q[4];
                 memcpy(q, &b->mbb[4 * j], sizeof q);
-                if (expand_km > 0) {  // box::expand (Box.cpp:200-214)
-                    double sx = expand_km * kDpk.lon(q[1]);
-                    double sy = expand_km * kDegPerKmLat;
-                    q[0] -= sx;
-                    q[2] += sx;
-                    q[1] -= sy;
-                    q[3] += sy;
-                }
+                if (expand_km > 0) farm::expand_box_km(q, expand_km);
                 hits.clear();
                 rt.search(q, a->mbb.data(), [&](int32_t i) {
                     if (skip_self && i == j) return;

@@ -166,6 +166,24 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps);
  * upload at src/cuda/preprocessing.cu:352-358. */
 int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint64_t n);
 
+/* Candidate pairs generated on the device, replacing the host R-tree probe
+ * (rtree_query + MySearchCallback, src/query/intersect_cpu.cpp:7-36 and
+ * src/query/within_polygon_cpu.cpp:8-48, RTree::Search
+ * src/index/RTree.h:581-600) and the pair upload after it: every (i, j),
+ * i < na, j < nb, whose boxes overlap inclusively, box = {low x, low y,
+ * high x, high y} as loaded (mbb_a[4i..4i+3], mbb_b[4j..4j+3]); with
+ * expand_km > 0 box j is first expanded as box::expand does
+ * (Box.cpp:200-214); with skip_self, i == j is skipped.  The pairs become
+ * the handle's pair list as (a_base + i, b_base + j), ordered by j then i
+ * (the order farm_candidates returns); *npairs gets their count.  Needs the
+ * polygons uploaded (a_base + na and b_base + nb within the set); fewer than
+ * 2^31 boxes per side and 2^31 pairs (else FARM_ECAPACITY). */
+int farm_generate_pairs(farm_handle h, const double *mbb_a, int64_t na, const double *mbb_b, int64_t nb,
+                        double expand_km, int skip_self, uint32_t a_base, uint32_t b_base, uint64_t *npairs);
+
+/* Copies pairs [begin, end) of the handle's pair list to host arrays. */
+int farm_download_pairs(farm_handle h, uint64_t begin, uint64_t end, uint32_t *a, uint32_t *b);
+
 /* Exact / approximate intersect over pairs [begin, end) (replaces
  * cuda_intersect, src/cuda/query_intersect.cu:335-464, whose batch window is
  * query_context::index/index_end).  out[k - begin] = 1 if pair k intersects.
