@@ -375,16 +375,16 @@ __device__ __forceinline__ int refine_batch(const DevSet &S, const farm_raster_h
             }
             const uint32_t e = u - tab.pre[kk];
             const uint32_t fa_ = tab.fa[kk], fb_ = tab.fb[kk], w1_ = tab.w1[kk];
-            double d2;
-            if (e < w1_) {  // point i of the target sequence, segment j of the source one
-                const uint32_t m = tab.la[kk] - 1, i = e / m, j = e - (e / m) * m;
-                d2 = pt_seg_d2(txv[fb_ + i], tyv[fb_ + i], sxv[fa_ + j], syv[fa_ + j], sxv[fa_ + j + 1],
-                               syv[fa_ + j + 1]);
-            } else {  // point i of the source sequence, segment j of the target one
-                const uint32_t e2 = e - w1_, m = tab.lb[kk] - 1, i = e2 / m, j = e2 - (e2 / m) * m;
-                d2 = pt_seg_d2(sxv[fa_ + i], syv[fa_ + i], txv[fb_ + j], tyv[fb_ + j], txv[fb_ + j + 1],
-                               tyv[fb_ + j + 1]);
-            }
+            // e < w1: point i of the target sequence, segment j of the source
+            // one; else point i of the source sequence, segment j of the
+            // target one.  One distance per lane either way: the operands are
+            // selected, so lanes on both sides of w1 share one evaluation.
+            const bool tgt_pt = e < w1_;
+            const uint32_t e2 = tgt_pt ? e : e - w1_, m = (tgt_pt ? tab.la[kk] : tab.lb[kk]) - 1;
+            const uint32_t i = e2 / m, j = e2 - i * m;
+            const float *ptx = tgt_pt ? txv + fb_ : sxv + fa_, *pty = tgt_pt ? tyv + fb_ : syv + fa_;
+            const float *sgx = tgt_pt ? sxv + fa_ : txv + fb_, *sgy = tgt_pt ? syv + fa_ : tyv + fb_;
+            const double d2 = pt_seg_d2(ptx[i], pty[i], sgx[j], sgy[j], sgx[j + 1], sgy[j + 1]);
             acc = fmin(acc, d2);
         }
         if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
