@@ -219,7 +219,7 @@ int farm_close(farm_handle h) {
                       &h->skipbuf, &h->layers, &h->lstatus, &h->areas, &h->wctr, &h->wovf, &h->wslab,
                       &h->wslab_big, &h->wout, &h->wmd, &h->ring2a, &h->icounts, &h->iarea, &h->cg_boxa,
                       &h->cg_boxq, &h->cg_cellcnt, &h->cg_celloff, &h->cg_items, &h->cg_cnt, &h->cg_off,
-                      &h->cg_keys, &h->cg_keys2};
+                      &h->cg_keys, &h->cg_keys2, &h->cg_uoff};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : h->ev)

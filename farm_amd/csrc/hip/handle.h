@@ -67,7 +67,7 @@ struct farm_handle_s {
     bool ring_area_ready = false;
     farm::DevBuf ring2a, icounts, iarea;
     // candidate generation
-    farm::DevBuf cg_boxa, cg_boxq, cg_cellcnt, cg_celloff, cg_items, cg_cnt, cg_off, cg_keys, cg_keys2;
+    farm::DevBuf cg_boxa, cg_boxq, cg_cellcnt, cg_celloff, cg_items, cg_uoff, cg_cnt, cg_off, cg_keys, cg_keys2;
 };
 
 namespace farm {

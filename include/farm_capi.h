@@ -176,8 +176,9 @@ int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint6
  * (Box.cpp:200-214); with skip_self, i == j is skipped.  The pairs become
  * the handle's pair list as (a_base + i, b_base + j), ordered by j then i
  * (the order farm_candidates returns); *npairs gets their count.  Needs the
- * polygons uploaded (a_base + na and b_base + nb within the set); fewer than
- * 2^31 boxes per side and 2^31 pairs (else FARM_ECAPACITY). */
+ * polygons uploaded (a_base + na and b_base + nb within the set) and finite,
+ * non-inverted boxes (else FARM_EINVAL); fewer than 2^31 boxes per side and
+ * 2^31 pairs (else FARM_ECAPACITY). */
 int farm_generate_pairs(farm_handle h, const double *mbb_a, int64_t na, const double *mbb_b, int64_t nb,
                         double expand_km, int skip_self, uint32_t a_base, uint32_t b_base, uint64_t *npairs);
 

@@ -80,7 +80,9 @@ def test_errors(device):
     m = H.mbbs()
     bad = m.copy()
     bad[3, 2] = np.nan
-    for args, kw in (((m, bad), {}), ((m, m), {"b_base": 1}), ((m, m), {"expand_km": -1.0})):
+    inverted = m.copy()
+    inverted[7, [0, 2]] = inverted[7, [2, 0]]
+    for args, kw in (((m, bad), {}), ((inverted, m), {}), ((m, m), {"b_base": 1}), ((m, m), {"expand_km": -1.0})):
         with pytest.raises(capi.FarmError) as e:
             device.generate_pairs(*args, **kw)
         assert e.value.code == capi.FARM_EINVAL
