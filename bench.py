@@ -250,6 +250,55 @@ def bench_within(args) -> dict:
     return out
 
 
+def bench_candidates(args) -> dict:
+    """Candidate pairs of the within self-join (300,000 lakes in [0,20]^2,
+    boxes expanded by 10 km) built on the device, against the host R-tree."""
+    from farm_amd import capi, datagen
+
+    side = 20.0 * np.sqrt(args.scale)
+    n = max(1, int(round(300000 * args.scale)))
+    ro, xy = datagen.lake_set(args.seed + 3, n, 0.0, side)
+    H = capi.HostSet.from_rings(ro, xy)
+    t = time.time()
+    pa, pb = capi.candidates(H, H, expand_km=10.0, skip_self=True, nthreads=args.threads)
+    host_s = time.time() - t
+    H.rasterize(1 << 30, args.threads)  # one pixel per polygon: the upload needs a raster, the join does not
+    dev = capi.Device(0)
+    dev.upload_polygons(H)
+    m = H.mbbs()
+    for _ in range(args.warmup):
+        dev.generate_pairs(m, m, expand_km=10.0, skip_self=True)
+    ts = []
+    for s in range(args.steps):
+        t = time.time()
+        npairs = dev.generate_pairs(m, m, expand_km=10.0, skip_self=True)
+        ts.append(time.time() - t)
+        log(f"candidates step {s}: {npairs} pairs in {ts[-1] * 1000:.1f} ms")
+    ga, gb = dev.download_pairs()
+    same = bool(np.array_equal(ga, pa) and np.array_equal(gb, pb))
+    dev.close()
+    wall = sum(ts)
+    return {
+        "metric": "candidate pairs generated/s (within self-join, 10 km)",
+        "value": npairs * args.steps / wall,
+        "unit": "candidate pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (box comparisons)",
+        "data": "synthetic, seeded",
+        "config": {"workload": f"MBB join, {n} lakes, boxes expanded by 10 km, self pairs skipped",
+                   "candidate_pairs": int(npairs), "scale": args.scale},
+        "host_rtree": {"value": len(pa) / host_s, "unit": "candidate pairs/s", "threads": args.threads,
+                       "what": "the whole join on the host R-tree (farm_candidates)"},
+        "parity_sample": {"pairs": int(len(pa)), "identical": same},
+    }
+
+
 def bench_intersection(args) -> dict:
     """Intersection (crossing points, boundary pieces, area of A n B):
     5,000 complex x 100,000 lakes in [0,20]^2, -v 256."""
@@ -365,7 +414,7 @@ def main() -> int:
     ap.add_argument("--train-pairs", type=int, default=1_000_000)
     ap.add_argument("--label-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=8)
-    ap.add_argument("--workload", choices=["intersect", "within", "intersection"], default="intersect")
+    ap.add_argument("--workload", choices=["intersect", "within", "intersection", "candidates"], default="intersect")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only time a plain device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -386,6 +435,9 @@ def main() -> int:
         return 0
     if args.workload == "intersection":
         print(json.dumps(bench_intersection(args)), flush=True)
+        return 0
+    if args.workload == "candidates":
+        print(json.dumps(bench_candidates(args)), flush=True)
         return 0
 
     from farm_amd import capi
