@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "../../../include/farm_capi.h"
 #include "../../../include/farm_host.h"
@@ -36,6 +37,7 @@ struct DriverArgs {
     int device = 0;
     int tile = 0;
     bool plain = false;
+    bool gpu_candidates = false;  // build the candidate pairs on the device instead of the host R-tree
 };
 
 inline double now_ms() {
@@ -67,6 +69,7 @@ inline void usage(const char *prog) {
            "  -r, -g, -a, -h          rasterize / gpu / approximation / hierarchy\n"
            "  --plain                 one-pair-per-workgroup kernel instead of the batched one\n"
            "  --tile E                source edges per lane in the batched kernel (1, 2, 4)\n"
+           "  --gpu-candidates        build the candidate pairs on the device (same pairs as the R-tree)\n"
            "  --device N              HIP device\n",
            prog);
 }
@@ -108,6 +111,7 @@ inline bool parse_args(int argc, char **argv, DriverArgs &a) {
         else if (k == "-d" || k == "--within_distance") a.within_km = atoi(val());
         else if (k == "--device") a.device = atoi(val());
         else if (k == "--tile") a.tile = atoi(val());
+        else if (k == "--gpu-candidates") a.gpu_candidates = true;
         else {
             // boost::program_options rejection of the reference (query_context.cpp:250-255)
             printf("Undefined Option!\n");
@@ -142,3 +146,19 @@ inline farm_hostset *load_or_die(const std::string &path) {
             exit(1);                                                                        \
         }                                                                                   \
     } while (0)
+
+// --gpu-candidates: the pairs of farm_candidates(src, tgt, km, skip_self),
+// built on the device after the polygon upload (target ids shifted by
+// b_base) and kept there as the pair list; returns their count.
+inline uint64_t device_candidates(farm_handle h, const farm_hostset *src, const farm_hostset *tgt, double km,
+                                  int skip_self, uint32_t b_base) {
+    const int64_t ns = farm_hostset_size(src), nt = farm_hostset_size(tgt);
+    std::vector<double> ma(4 * (size_t)ns), mb(4 * (size_t)nt);
+    DIE_IF(h, farm_hostset_mbbs(src, ma.data()), "source boxes");
+    DIE_IF(h, farm_hostset_mbbs(tgt, mb.data()), "target boxes");
+    const double t = now_ms();
+    uint64_t n = 0;
+    DIE_IF(h, farm_generate_pairs(h, ma.data(), ns, mb.data(), nt, km, skip_self, 0, b_base, &n), "generate pairs");
+    logf("device candidate pairs: %llu takes %.3f ms", (unsigned long long)n, now_ms() - t);
+    return n;
+}

@@ -1,7 +1,8 @@
 // intersect: drop-in for the reference's intersect / intersect_cpu drivers
 // (src/query/intersect.cpp:39-97, intersect_cpu.cpp:65-126): load two .dt
-// sets, generate candidate pairs with the host R-tree, rasterize on the host,
-// then run the exact (or -a approximate) intersect on the GPU in -b batches.
+// sets, generate candidate pairs with the host R-tree (or on the device with
+// --gpu-candidates), rasterize on the host, then run the exact (or -a
+// approximate) intersect on the GPU in -b batches.
 // Prints count-found like query_context::print_stats (query_context.cpp:173).
 #include <vector>
 
@@ -20,13 +21,16 @@ int main(int argc, char **argv) {
 
     double t = now_ms();
     uint32_t *pa = nullptr, *pb = nullptr;
-    int64_t npairs = farm_candidates(src, tgt, 0.0, 0, a.threads, &pa, &pb);
-    if (npairs < 0) {
-        logf("candidate generation failed");
-        return 1;
+    int64_t npairs = 0;
+    if (!a.gpu_candidates) {
+        npairs = farm_candidates(src, tgt, 0.0, 0, a.threads, &pa, &pb);
+        if (npairs < 0) {
+            logf("candidate generation failed");
+            return 1;
+        }
+        logf("rtree query finished: %lld candidate pairs takes %.3f ms", (long long)npairs, now_ms() - t);
+        for (int64_t i = 0; i < npairs; i++) pb[i] += (uint32_t)ns;  // sources then targets in one set
     }
-    logf("rtree query finished: %lld candidate pairs takes %.3f ms", (long long)npairs, now_ms() - t);
-    for (int64_t i = 0; i < npairs; i++) pb[i] += (uint32_t)ns;  // sources then targets in one set
 
     t = now_ms();
     farm_hostset *all = farm_hostset_concat(src, tgt);
@@ -52,7 +56,8 @@ int main(int argc, char **argv) {
     farm_polyset ps;
     DIE_IF(h, farm_hostset_view(all, &ps), "view");
     DIE_IF(h, farm_upload_polygons(h, &ps), "upload polygons");
-    DIE_IF(h, farm_upload_pairs(h, pa, pb, (uint64_t)npairs), "upload pairs");
+    if (a.gpu_candidates) npairs = (int64_t)device_candidates(h, src, tgt, 0.0, 0, (uint32_t)ns);
+    else DIE_IF(h, farm_upload_pairs(h, pa, pb, (uint64_t)npairs), "upload pairs");
     logf("preprocess for gpu finished takes %.3f ms", now_ms() - t);
 
     farm_opts o;

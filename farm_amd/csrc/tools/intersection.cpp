@@ -1,6 +1,7 @@
 // intersection: drop-in for the reference's intersection / intersection_cpu
 // drivers (src/query/intersection.cpp:40-97, intersection_cpu.cpp:67-124):
-// load two .dt sets, candidate pairs from the host R-tree, host rasters, then
+// load two .dt sets, candidate pairs from the host R-tree (on the device with
+// --gpu-candidates), host rasters, then
 // the intersection query on the GPU in -b batches: crossing points, boundary
 // pieces and their statuses (where the reference stops) and the area of each
 // pair's intersection.  Prints count-found (pairs with a positive area) and
@@ -22,13 +23,16 @@ int main(int argc, char **argv) {
 
     double t = now_ms();
     uint32_t *pa = nullptr, *pb = nullptr;
-    int64_t npairs = farm_candidates(src, tgt, 0.0, 0, a.threads, &pa, &pb);
-    if (npairs < 0) {
-        logf("candidate generation failed");
-        return 1;
+    int64_t npairs = 0;
+    if (!a.gpu_candidates) {
+        npairs = farm_candidates(src, tgt, 0.0, 0, a.threads, &pa, &pb);
+        if (npairs < 0) {
+            logf("candidate generation failed");
+            return 1;
+        }
+        logf("rtree query finished: %lld candidate pairs takes %.3f ms", (long long)npairs, now_ms() - t);
+        for (int64_t i = 0; i < npairs; i++) pb[i] += (uint32_t)ns;
     }
-    logf("rtree query finished: %lld candidate pairs takes %.3f ms", (long long)npairs, now_ms() - t);
-    for (int64_t i = 0; i < npairs; i++) pb[i] += (uint32_t)ns;
 
     t = now_ms();
     farm_hostset *all = farm_hostset_concat(src, tgt);
@@ -53,7 +57,8 @@ int main(int argc, char **argv) {
     farm_polyset ps;
     DIE_IF(h, farm_hostset_view(all, &ps), "view");
     DIE_IF(h, farm_upload_polygons(h, &ps), "upload polygons");
-    DIE_IF(h, farm_upload_pairs(h, pa, pb, (uint64_t)npairs), "upload pairs");
+    if (a.gpu_candidates) npairs = (int64_t)device_candidates(h, src, tgt, 0.0, 0, (uint32_t)ns);
+    else DIE_IF(h, farm_upload_pairs(h, pa, pb, (uint64_t)npairs), "upload pairs");
     logf("preprocess for gpu finished takes %.3f ms", now_ms() - t);
 
     const uint64_t batch = a.batch ? a.batch : (uint64_t)npairs;

@@ -2,7 +2,8 @@
 // within_polygon_cpu drivers (src/query/within_polygon.cpp:26-116,
 // within_polygon_cpu.cpp:20-146): a self-join of the -s set; candidate pairs
 // are every (i, j), i != j, whose MBBs overlap after j's MBB is expanded by
-// the within distance (both orders kept); the -h hierarchy is built on the
+// the within distance (both orders kept; built on the device with
+// --gpu-candidates); the -h hierarchy is built on the
 // host and the within decision runs on the GPU in -b batches.  Prints
 // count-found like query_context::print_stats (query_context.cpp:173).
 #include <vector>
@@ -27,12 +28,15 @@ int main(int argc, char **argv) {
 
     double t = now_ms();
     uint32_t *pa = nullptr, *pb = nullptr;
-    int64_t npairs = farm_candidates(src, src, (double)a.within_km, 1, a.threads, &pa, &pb);
-    if (npairs < 0) {
-        logf("candidate generation failed");
-        return 1;
+    int64_t npairs = 0;
+    if (!a.gpu_candidates) {
+        npairs = farm_candidates(src, src, (double)a.within_km, 1, a.threads, &pa, &pb);
+        if (npairs < 0) {
+            logf("candidate generation failed");
+            return 1;
+        }
+        logf("rtree query finished: %lld candidate pairs takes %.3f ms", (long long)npairs, now_ms() - t);
     }
-    logf("rtree query finished: %lld candidate pairs takes %.3f ms", (long long)npairs, now_ms() - t);
 
     t = now_ms();
     if (farm_hostset_rasterize_h(src, a.vpr, a.nlow, a.threads) != FARM_OK) {
@@ -55,7 +59,8 @@ int main(int argc, char **argv) {
     farm_polyset ps;
     DIE_IF(h, farm_hostset_view(src, &ps), "view");
     DIE_IF(h, farm_upload_polygons(h, &ps), "upload polygons");
-    DIE_IF(h, farm_upload_pairs(h, pa, pb, (uint64_t)npairs), "upload pairs");
+    if (a.gpu_candidates) npairs = (int64_t)device_candidates(h, src, src, (double)a.within_km, 1, 0);
+    else DIE_IF(h, farm_upload_pairs(h, pa, pb, (uint64_t)npairs), "upload pairs");
     logf("preprocess for gpu finished takes %.3f ms", now_ms() - t);
 
     const uint64_t batch = a.batch ? a.batch : (uint64_t)npairs;

@@ -45,13 +45,13 @@ def test_intersect_and_intersection_executables(tmp_path):
     assert OA.rasterize(10) == -1 and OB.rasterize(10) == -1
     bits, _ = oracle.intersect(OA, OB, pa, pb)
     _, area = oracle.intersection(OA, OB, pa, pb)
-    for extra in ([], ["-b", "1000"]):
+    for extra in ([], ["-b", "1000"], ["--gpu-candidates"], ["--gpu-candidates", "-b", "777"]):
         assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (bits == 1).sum())
     # -a: the reference's heuristic (IN overlap or a BORDER pair with prob > 1)
     abits, _ = oracle.intersect(OA, OB, pa, pb, 1)
     assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-a"]) == int((abits == 1).sum())
-    for extra in ([], ["-b", "1000"]):
+    for extra in ([], ["-b", "1000"], ["--gpu-candidates"]):
         assert _found([os.path.join(BIN, "intersection"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (area > 0).sum())
 
@@ -67,6 +67,7 @@ def test_within_polygon_executable(tmp_path):
     exe = os.path.join(BIN, "within_polygon")
     assert _found([exe, "-s", s, "-r", "-g", "-h", "-n", "8"]) == want
     assert _found([exe, "-s", s, "-r", "-g", "-h", "-n", "8", "-b", "5000"]) == want
+    assert _found([exe, "-s", s, "-r", "-g", "-h", "-n", "8", "--gpu-candidates"]) == want
     # without -h the reference dereferences an empty layer list; here it is an error
     p = subprocess.run([exe, "-s", s, "-r", "-g"], capture_output=True, text=True, timeout=60)
     assert p.returncode != 0 and "-h" in p.stderr
