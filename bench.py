@@ -54,11 +54,28 @@ def measured_bytes_per_test() -> float | None:
     return float(f.group(1)) + float(w.group(1)) if f and w else None
 
 
+def measured_bytes_per_pair(name: str) -> float | None:
+    """HBM bytes per candidate pair of a query kernel, from its committed
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
+    calibration)."""
+    import re
+
+    try:
+        text = open(os.path.join(ROOT, "profiles", name)).read()
+    except OSError:
+        return None
+    f = re.search(r"fetch_bytes_per_pair: ([0-9.eE+-]+)", text)
+    w = re.search(r"write_bytes_per_pair: ([0-9.eE+-]+)", text)
+    return float(f.group(1)) + float(w.group(1)) if f and w else None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 CAND_TIMES: dict = {}
+WITHIN_TRAFFIC = "k_within_hbm_traffic_lakes_300k_10km.md"
+INTERSECTION_TRAFFIC = "k_intersection_hbm_traffic_complex_x_lakes.md"
 
 
 def device_candidates(dev, mbb_a, mbb_b, pa, pb, b_base: int, **kw) -> dict:
@@ -209,6 +226,8 @@ def bench_within(args) -> dict:
     kms = sum(p["total_ms"] for p in per)
     ops = sum(p["segment_tests"] * OPS_PER_PT_SEG + p["refine_items"] * OPS_PER_BOX_PAIR for p in per)
     achieved = ops / (kms / 1000.0) / 1e12 if kms > 0 else 0.0
+    bpp = measured_bytes_per_pair(WITHIN_TRAFFIC) if args.scale == 1.0 else None
+    traffic = bpp * timed / args.steps if bpp else None  # HBM bytes per launch
     out = {
         "metric": "within-distance candidate pairs/s",
         "value": timed / wall,
@@ -228,7 +247,8 @@ def bench_within(args) -> dict:
         "point_segment_distances_per_s": sum(p["segment_tests"] for p in per) / wall,
         "box_pair_distances_per_s": sum(p["refine_items"] for p in per) / wall,
         "roofline": {"bound": "valu", "kernel": "k_within", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_source": WITHIN_TRAFFIC if traffic else None,
                      "ops_per_point_segment": OPS_PER_PT_SEG, "ops_per_box_pair": OPS_PER_BOX_PAIR},
         "candidates": cand_info,
     }
@@ -343,6 +363,8 @@ def bench_intersection(args) -> dict:
     # each), the screen (5 compares and 4 products): 24 f32 operations
     ops = 24
     achieved = tests * ops / (kms / 1000.0) / 1e12 if kms > 0 else 0.0
+    bpp = measured_bytes_per_pair(INTERSECTION_TRAFFIC) if (args.scale == 1.0 and args.vpr == 256) else None
+    traffic = bpp * timed / args.steps if bpp else None  # HBM bytes per launch
     out = {
         "metric": "intersection (area of A n B) candidate pairs/s",
         "value": timed / wall,
@@ -361,8 +383,8 @@ def bench_intersection(args) -> dict:
                    "found": int(sum(p["found"] for p in per)), "scale": args.scale},
         "edge_pair_tests_per_s": tests / wall,
         "roofline": {"bound": "valu", "kernel": "k_intersection", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "ops_per_edge_pair": ops},
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_source": INTERSECTION_TRAFFIC if traffic else None, "ops_per_edge_pair": ops},
     }
     if not args.no_cpu:
         counts, area, _ = dev.intersection_area()
