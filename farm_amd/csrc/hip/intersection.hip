@@ -429,14 +429,6 @@ __device__ __forceinline__ bool line_pass(LineProbe p0, LineProbe p1, float k) {
     const float m = k * (p0.r + p1.r);
     return (p0.o <= m || p1.o <= m) && (p0.o >= -m || p1.o >= -m);
 }
-// The same test for the whole wave as a lane mask: four compares whose masks
-// combine in scalar registers (llvm.amdgcn.fcmp; 3 = ordered >=, 5 = ordered <=).
-__device__ __forceinline__ uint64_t line_pass_mask(LineProbe p0, LineProbe p1, float k) {
-    const float m = k * (p0.r + p1.r);
-    const uint64_t lo = __builtin_amdgcn_fcmpf(p0.o, m, 5) | __builtin_amdgcn_fcmpf(p1.o, m, 5);
-    const uint64_t hi = __builtin_amdgcn_fcmpf(p0.o, -m, 3) | __builtin_amdgcn_fcmpf(p1.o, -m, 3);
-    return lo & hi;
-}
 
 // Target vertices read through the constant address space: the vertex arrays
 // do not change during a launch, and uniform addresses then become scalar
