@@ -51,7 +51,7 @@ struct Rec {
 
 constexpr int kWaves = 4;
 constexpr uint32_t kRecCap = 2048;     // records per wave slab (power of two)
-constexpr uint32_t kRecCapBig = 65536;  // second pass for pairs beyond kRecCap
+constexpr uint32_t kRecCapBig = 65536;  // second pass for pairs beyond kRecCap (a third sizes to the largest)
 constexpr uint32_t kChunk = 2;          // pairs per queue grab
 
 // Point::operator== (Point.h:32)
@@ -755,8 +755,9 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
             const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, cnt, Q, lane);
             if (r.over) {
                 if (lane == 0) {
-                    if (BIG) {
+                    if (BIG) {  // still too many: the host reruns it with larger slabs
                         counts[5ull * idx] = -4;
+                        counts[5ull * idx + 1] = (int32_t)min(r.raw, (uint32_t)INT_MAX);
                     } else {
                         const uint32_t slot = atomicAdd(&ctr[1], 1u);
                         ovf[slot] = idx;
@@ -850,12 +851,47 @@ int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *count
     HIPCHK(h, hipMemcpyAsync(hst, ist, sizeof hst, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (novf) {
-        std::vector<int32_t> tmp(5ull * n);
-        HIPCHK(h, hipMemcpy(tmp.data(), counts_dev, 20ull * n, hipMemcpyDeviceToHost));
-        for (uint32_t k = 0; k < n; k++)
-            if (tmp[5ull * k] == -4)
-                return fail(h, FARM_ECAPACITY, "pair %llu: more than %u crossing records",
-                            (unsigned long long)(begin + k), kRecCapBig);
+        // pairs beyond the second slabs: once more, with slabs sized to the
+        // largest of them (fewer waves, at most 8 GiB of slabs)
+        std::vector<uint32_t> list(novf), big;
+        HIPCHK(h, hipMemcpy(list.data(), P<uint32_t>(h->wovf), 4ull * novf, hipMemcpyDeviceToHost));
+        uint32_t most = 0;
+        for (uint32_t idx : list) {
+            int32_t c[2];
+            HIPCHK(h, hipMemcpy(c, counts_dev + 5ull * idx, 8, hipMemcpyDeviceToHost));
+            if (c[0] == -4) {
+                big.push_back(idx);
+                most = std::max(most, (uint32_t)c[1]);
+            }
+        }
+        if (!big.empty()) {
+            uint64_t cap = 2ull * kRecCapBig;
+            while (cap < most) cap <<= 1;
+            if (cap > (1ull << 26))
+                return fail(h, FARM_ECAPACITY, "a pair has %u crossing records (limit 2^26)", most);
+            const uint64_t per_wave = 2ull * cap * sizeof(Rec);
+            const uint64_t waves = std::max<uint64_t>(kWaves, std::min<uint64_t>(64, (8ull << 30) / per_wave));
+            const unsigned blocks = (unsigned)(waves / kWaves);
+            if ((rc = ensure(h, h->wslab_big, (uint64_t)blocks * kWaves * per_wave))) return rc;
+            HIPCHK(h, hipMemcpyAsync(P<uint32_t>(h->wovf), big.data(), 4ull * big.size(), hipMemcpyHostToDevice,
+                                     h->stream));
+            if ((rc = fill_u32(h, ctr, 1, 0))) return rc;
+            hipLaunchKernelGGL(k_intersection<true>, dim3(blocks), dim3(64 * kWaves), 0, h->stream, S,
+                               (const double *)P<double>(h->ring2a), pa, pb, (uint32_t)big.size(),
+                               (const uint32_t *)P<uint32_t>(h->wovf), counts_dev, area_dev, P<Rec>(h->wslab_big),
+                               (uint32_t)cap, ctr, P<uint32_t>(h->wovf), ist);
+            HIPCHK(h, hipGetLastError());
+            HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+            HIPCHK(h, hipMemcpyAsync(hst, ist, sizeof hst, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            for (uint32_t idx : big) {
+                int32_t c0 = 0;
+                HIPCHK(h, hipMemcpy(&c0, counts_dev + 5ull * idx, 4, hipMemcpyDeviceToHost));
+                if (c0 == -4)
+                    return fail(h, FARM_ECAPACITY, "pair %llu: more than %llu crossing records",
+                                (unsigned long long)(begin + idx), (unsigned long long)cap);
+            }
+        }
     }
     if (st) {
         float ms = 0;

@@ -233,7 +233,8 @@ int farm_within_dev(farm_handle h, uint64_t begin, uint64_t end, double d_km, ui
  * boundary pieces IN the other polygon (Green's theorem, f64); no reference
  * counterpart.  stats: bb_pixel_pairs, segment_tests (edge pairs tested),
  * refine_items (crossing records before de-duplication), found (pairs with
- * area > 0). */
+ * area > 0).  A pair may have up to 2^26 crossing records (else
+ * FARM_ECAPACITY). */
 int farm_intersection_area(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts, double *area,
                            farm_stats *stats);
 int farm_intersection_area_dev(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,

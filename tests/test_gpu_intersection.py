@@ -113,3 +113,27 @@ def test_touching_long_sequences(device):
         w = Workload(rings_from_list(a), rings_from_list(b), vpr=vpr, pairs=(pa.ravel(), pb.ravel()))
         c, _, _ = _check(device, w)
         assert (c[:, 0] > 0).sum() >= 4  # crossings were found, not just screened away
+
+
+def _combs(n):
+    # interleaved combs: every tooth of A crosses every tooth of B (n^2 points)
+    xs = np.arange(n + 1) / n
+    ya = np.where(np.arange(n + 1) % 2 == 0, 0.0, 1.0)
+    a = np.vstack([np.stack([xs + 1.0, ya + 1.0], 1), [[2.0, 0.8], [1.0, 0.8]]])
+    ys = (np.arange(n + 1) / n) * 0.98 + 0.0103
+    xb = np.where(np.arange(n + 1) % 2 == 0, 0.0, 1.0) * 1.02 - 0.0107
+    b = np.vstack([np.stack([xb + 1.0, ys + 1.0], 1), [[0.8, 1.9903], [0.8, 1.0103]]])
+    return a.astype(np.float32), b.astype(np.float32)
+
+
+def test_pairs_beyond_the_record_slabs(device):
+    # 40,000 crossing records rerun with the 65,536-record slabs; 90,000 go
+    # to the third pass, whose slabs are sized to the largest pair
+    a2, b2 = _combs(200)
+    a3, b3 = _combs(300)
+    a = [a2, a3, _rect(1.2, 1.2, 1.4, 1.4, 9)]
+    b = [b2, b3, _rect(1.3, 1.3, 1.5, 1.5, 9)]
+    pa, pb = np.arange(3), np.arange(3)
+    w = Workload(rings_from_list(a), rings_from_list(b), vpr=10, pairs=(pa, pb))
+    c, _, st = _check(device, w)
+    assert c[0, 0] == 40000 and c[1, 0] == 90000
