@@ -117,6 +117,33 @@ def build_workload(scale: float, seed: int, vpr: int, threads: int):
     return (cro, cxy, lro, lxy), A, B, both, pa, pb
 
 
+ONE_THREAD_SECONDS = 4.0
+
+
+def host_cpu() -> dict:
+    """CPU model and core count of this host (the CPU baselines run here)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count()}
+
+
+def one_thread(run, n_sample: int, dt: float, threads: int) -> dict:
+    """The oracle on one thread over the first m pairs of the timed sample, m
+    chosen to take about ONE_THREAD_SECONDS; run(m) returns (seconds, whether
+    the answers equal the multi-thread run's)."""
+    m = int(min(n_sample, max(20, n_sample * ONE_THREAD_SECONDS / max(dt * threads, 1e-3))))
+    t1, same = run(m)
+    return {"value": m / t1, "unit": "candidate pairs/s", "cores": 1, "pairs": m, "seconds": t1,
+            "same_answers": bool(same)}
+
+
 def cpu_baseline(rings, pa, pb, vpr, threads, target_s, rng, plain_bits_fn):
     """Oracle query-phase throughput on a random sample of the pairs."""
     import oracle
@@ -142,6 +169,13 @@ def cpu_baseline(rings, pa, pb, vpr, threads, target_s, rng, plain_bits_fn):
             break
         grow = int(len(sample) * min(8.0, max(1.5, target_s / max(dt, 1e-3))))
         sample = np.sort(rng.choice(n, size=min(n, grow), replace=False))
+
+    def run1(m):
+        t1 = time.time()
+        b1, _ = oracle.intersect(OA, OB, ia[:m].astype(np.uint32), ib[:m].astype(np.uint32), 0, 1)
+        return time.time() - t1, np.array_equal(b1, results["bits"][:m])
+
+    results["one_thread"] = one_thread(run1, len(sample), results["seconds"], threads)
     return results
 
 
@@ -195,6 +229,14 @@ def within_cpu_baseline(ro, xy, pa, pb, threads, target_s, rng):
         dt = time.time() - t
         res = {"sample": sample, "bits": bits, "md": md, "seconds": dt}
         if dt >= target_s or len(sample) >= n:
+
+            def run1(m):
+                t1 = time.time()
+                b1, md1, _ = oracle.within(O, O, ia[:m], ib[:m], 10.0, 1)
+                return time.time() - t1, (np.array_equal(b1, bits[:m])
+                                          and np.array_equal(md1.view(np.uint64), md[:m].view(np.uint64)))
+
+            res["one_thread"] = one_thread(run1, len(sample), dt, threads)
             return res
         grow = int(len(sample) * min(8.0, max(1.5, target_s / max(dt, 1e-3))))
         sample = np.sort(rng.choice(n, size=min(n, grow), replace=False))
@@ -262,7 +304,8 @@ def bench_within(args) -> dict:
         out["cpu_baseline"] = {"value": len(samp) / res["seconds"], "unit": "candidate pairs/s",
                                "cores": args.threads, "kind": "port",
                                "sample": f"{len(samp)} random candidate pairs of this workload, oracle within "
-                                         f"query only (-h rasterization untimed), {res['seconds']:.1f}s"}
+                                         f"query only (-h rasterization untimed), {res['seconds']:.1f}s",
+                               "one_thread": res["one_thread"], "host": host_cpu()}
         out["parity_sample"] = {"pairs": int(len(samp)), "mismatches": mism, "min_dist_mismatches": mdm}
         log(f"within cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; mismatches {mism}, "
             f"min-distance mismatches {mdm}")
@@ -405,12 +448,20 @@ def bench_intersection(args) -> dict:
                 break
             grow = int(len(sample) * min(8.0, max(1.5, args.cpu_seconds / max(dt, 1e-3))))
             sample = np.sort(rng.choice(n, size=min(n, grow), replace=False))
+        def run1(m):
+            t1 = time.time()
+            c1, a1 = oracle.intersection(OA, OB, ia[:m].astype(np.uint32), ib[:m].astype(np.uint32), 1)
+            return time.time() - t1, np.array_equal(c1, wc[:m]) and np.array_equal(a1.view(np.uint64),
+                                                                                    wa[:m].view(np.uint64))
+
+        single = one_thread(run1, len(sample), dt, args.threads)
         cm = int((counts[sample] != wc).any(1).sum())
         am = int((np.abs(area[sample] - wa) > 1e-9 * np.abs(wa) + 1e-15).sum())
         out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "candidate pairs/s", "cores": args.threads,
                                "kind": "port",
                                "sample": f"{len(sample)} random candidate pairs of this workload, oracle "
-                                         f"intersection query only (rasterization untimed), {dt:.1f}s"}
+                                         f"intersection query only (rasterization untimed), {dt:.1f}s",
+                               "one_thread": single, "host": host_cpu()}
         out["parity_sample"] = {"pairs": int(len(sample)), "count_mismatches": cm, "area_mismatches_1e-9": am}
         log(f"intersection cpu baseline: {len(sample)} pairs in {dt:.1f}s; count mismatches {cm}, "
             f"area mismatches {am}")
@@ -648,6 +699,8 @@ def main() -> int:
             "kind": "port",
             "sample": f"{len(samp)} random candidate pairs of this workload, oracle query phase only "
                       f"(rasterization untimed), {res['seconds']:.1f}s",
+            "one_thread": res["one_thread"],
+            "host": host_cpu(),
         }
         line["parity_sample"] = {"pairs": int(timed.sum()), "mismatches": mism}
         log(f"cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; sample mismatches vs GPU: {mism}")
