@@ -6,10 +6,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <unistd.h>
 
 #include "../../../include/farm_host.h"
 #include "raster.h"
@@ -126,24 +130,58 @@ farm_hostset *farm_hostset_from_rings(int64_t n, const uint64_t *ring_off, const
     return s;
 }
 
+// load_binary_file (storage.cpp:361-449): the meta table at the end gives
+// the first record; records are decoded back to back (load_unit
+// storage.cpp:320-360, MyPolygon::decode MyPolygon.cpp:747-759) and rings with
+// fewer than 3 stored vertices are dropped (storage.cpp:341).  Ids follow
+// file order.  The file is read with parallel preads, one pass over the
+// record headers finds every kept ring and hole, and the vertex copies and
+// boxes then run in parallel over ranges of polygons.
 farm_hostset *farm_hostset_load_dt(const char *path) {
     FILE *f = fopen(path, "rb");
     if (!f) return nullptr;
     fseek(f, 0, SEEK_END);
-    long fsz = ftell(f);
-    if (fsz < 8) {
+    const long fsz_l = ftell(f);
+    if (fsz_l < 8) {
         fclose(f);
         return nullptr;
     }
-    std::vector<char> buf((size_t)fsz);
-    fseek(f, 0, SEEK_SET);
-    size_t rd = fread(buf.data(), 1, (size_t)fsz, f);
-    fclose(f);
-    if (rd != (size_t)fsz) return nullptr;
+    const uint64_t fsz = (uint64_t)fsz_l;
+    std::unique_ptr<char[]> storage(new (std::nothrow) char[fsz]);  // no zero fill: pread writes all of it
+    if (!storage) {
+        fclose(f);
+        return nullptr;
+    }
+    struct {
+        char *p;
+        char *data() const { return p; }
+    } buf{storage.get()};
+    const int fd = fileno(f);
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    {
+        const uint64_t piece = (fsz + nth - 1) / nth;
+        std::atomic<bool> ok(true);
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&, t]() {
+                uint64_t o = t * piece, e = std::min(fsz, o + piece);
+                while (o < e) {
+                    const ssize_t r = pread(fd, buf.data() + o, e - o, (off_t)o);
+                    if (r <= 0) {
+                        ok = false;
+                        return;
+                    }
+                    o += (uint64_t)r;
+                }
+            });
+        for (auto &t : th) t.join();
+        fclose(f);
+        if (!ok) return nullptr;
+    }
     uint64_t n;
     memcpy(&n, buf.data() + fsz - 8, 8);
     const uint64_t meta_bytes = 48;  // PolygonMeta (MyPolygon.h:29-35)
-    if (n == 0 || n * meta_bytes + 8 > (uint64_t)fsz) return nullptr;
+    if (n == 0 || n > (fsz - 8) / meta_bytes) return nullptr;
     const char *meta = buf.data() + fsz - 8 - n * meta_bytes;
     uint64_t first_off, last_off;
     uint32_t last_size;
@@ -151,10 +189,13 @@ farm_hostset *farm_hostset_load_dt(const char *path) {
     memcpy(&last_off, meta + (n - 1) * meta_bytes + 8, 8);
     memcpy(&last_size, meta + (n - 1) * meta_bytes, 4);
     const uint64_t data_end = last_off + last_size;
-    if (data_end > (uint64_t)fsz - 8 - n * meta_bytes) return nullptr;
-    farm_hostset *s = new farm_hostset();
-    // polygons are decoded back to back (storage.cpp:320-360,
-    // MyPolygon::decode MyPolygon.cpp:747-759)
+    if (data_end > fsz - 8 - n * meta_bytes || first_off > data_end) return nullptr;
+    // one pass over the record headers: the kept rings and their holes
+    struct Rec {
+        uint64_t ring, nv, hole0;  // ring offset, stored vertices, first entry in holes
+    };
+    std::vector<Rec> recs;
+    std::vector<std::pair<uint64_t, uint64_t>> holes;  // (offset, vertices)
     uint64_t off = first_off;
     auto rd64 = [&](uint64_t &v) -> bool {
         if (off + 8 > data_end) return false;
@@ -164,29 +205,59 @@ farm_hostset *farm_hostset_load_dt(const char *path) {
     };
     while (off < data_end) {
         uint64_t nh, nv;
-        if (!rd64(nh) || !rd64(nv) || nv == 0 || off + nv * 8 > data_end) {
-            delete s;
-            return nullptr;
-        }
-        const float *ring = reinterpret_cast<const float *>(buf.data() + off);
+        if (!rd64(nh) || !rd64(nv) || nv == 0 || nv > (data_end - off) / 8) return nullptr;
+        const uint64_t ring = off;
         off += nv * 8;
-        std::vector<std::pair<const float *, uint64_t>> holes;
+        const size_t h0 = holes.size();
         for (uint64_t h = 0; h < nh; h++) {
             uint64_t hn;
-            if (!rd64(hn) || off + hn * 8 > data_end) {
-                delete s;
-                return nullptr;
-            }
-            holes.push_back({reinterpret_cast<const float *>(buf.data() + off), hn});
+            if (!rd64(hn) || hn > (data_end - off) / 8) return nullptr;
+            holes.push_back({off, hn});
             off += hn * 8;
         }
-        if (nv + 1 < 3) continue;
-        push_polygon(s, ring, nv);
-        for (auto &hh : holes) {
-            s->hole_xy.insert(s->hole_xy.end(), hh.first, hh.first + 2 * hh.second);
-            s->hole_off.push_back(s->hole_xy.size() / 2);
+        if (nv + 1 < 3) {
+            holes.resize(h0);
+            continue;
         }
-        s->hole_first.back() = s->hole_off.size() - 1;
+        recs.push_back({ring, nv, (uint64_t)h0});
+    }
+    // sizes from prefix sums, then the copies in parallel
+    farm_hostset *s = new farm_hostset();
+    const size_t np = recs.size();
+    s->ring_off.resize(np + 1);
+    s->hole_first.resize(np + 1);
+    s->hole_off.resize(holes.size() + 1);
+    s->ring_off[0] = 0;
+    s->hole_off[0] = 0;
+    for (size_t i = 0; i < np; i++) s->ring_off[i + 1] = s->ring_off[i] + recs[i].nv;
+    for (size_t k = 0; k < holes.size(); k++) s->hole_off[k + 1] = s->hole_off[k] + holes[k].second;
+    // hole_first[i + 1]: the hole_off entry that ends polygon i's holes
+    s->hole_first[0] = 0;
+    for (size_t i = 0; i < np; i++)
+        s->hole_first[i + 1] = (i + 1 < np ? recs[i + 1].hole0 : holes.size());
+    s->xy.resize(2 * s->ring_off[np]);
+    s->hole_xy.resize(2 * s->hole_off[holes.size()]);
+    s->mbb.resize(4 * np);
+    {
+        std::atomic<size_t> next(0);
+        const size_t chunk = 1024;
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&]() {
+                for (;;) {
+                    const size_t c0 = next.fetch_add(chunk);
+                    if (c0 >= np) break;
+                    for (size_t i = c0; i < std::min(np, c0 + chunk); i++) {
+                        const float *ring = reinterpret_cast<const float *>(buf.data() + recs[i].ring);
+                        memcpy(&s->xy[2 * s->ring_off[i]], ring, 8 * recs[i].nv);
+                        ring_mbb(ring, recs[i].nv, &s->mbb[4 * i]);
+                        const uint64_t hend = i + 1 < np ? recs[i + 1].hole0 : holes.size();
+                        for (uint64_t k = recs[i].hole0; k < hend; k++)
+                            memcpy(&s->hole_xy[2 * s->hole_off[k]], buf.data() + holes[k].first, 8 * holes[k].second);
+                    }
+                }
+            });
+        for (auto &t : th) t.join();
     }
     return s;
 }

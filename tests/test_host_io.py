@@ -130,3 +130,47 @@ def test_device_calls_fail_cleanly_without_gpu():
     h = C.c_void_p()
     rc = capi.lib.farm_open(99, C.byref(h))
     assert rc == capi.FARM_EDEVICE and not h.value
+
+
+def _write_dt(path, polys):
+    # .dt bytes: records (u64 holes, u64 nv, f32x2 ring, then each hole), the
+    # 48-byte meta per polygon (size, vertices, offset, biased box), u64 count
+    import struct
+
+    body, meta = bytearray(), bytearray()
+    bias = np.float64(np.float32(1e-9))
+    for ring, holes in polys:
+        off = len(body)
+        rec = struct.pack("<QQ", len(holes), len(ring)) + np.asarray(ring, np.float32).tobytes()
+        for h in holes:
+            rec += struct.pack("<Q", len(h)) + np.asarray(h, np.float32).tobytes()
+        body += rec
+        r = np.asarray(ring, np.float32).astype(np.float64)
+        meta += struct.pack("<IIQdddd", len(rec), len(ring), off, r[:, 0].min() - bias, r[:, 1].min() - bias,
+                            r[:, 0].max() + bias, r[:, 1].max() + bias)
+    path.write_bytes(bytes(body + meta + struct.pack("<Q", len(polys))))
+
+
+def test_dt_load_holes_and_dropped_rings(tmp_path):
+    rng = np.random.default_rng(5)
+
+    def ring(n, c):
+        t = np.sort(rng.random(n)) * 2 * np.pi
+        p = np.stack([c[0] + np.cos(t), c[1] + np.sin(t)], 1)
+        return np.vstack([p, p[:1]]).astype(np.float32)
+
+    polys = []
+    for i in range(40):
+        holes = [ring(5 + k, (3.0 * i, 0.5 * k)) for k in range(i % 4)]
+        polys.append((ring(20 + i, (3.0 * i, 0.0)), holes))
+    tiny = (np.array([[9.0, 9.0]], np.float32), [ring(6, (9.0, 9.0)), ring(7, (9.0, 9.5))])  # dropped, holes too
+    full, want = tmp_path / "full.dt", tmp_path / "want.dt"
+    _write_dt(full, polys[:17] + [tiny] + polys[17:])
+    _write_dt(want, polys)
+    H = capi.HostSet.load_dt(str(full))
+    assert len(H) == 40
+    for i in (0, 16, 17, 39):
+        assert np.array_equal(H.ring(i), polys[i][0])
+    again = tmp_path / "again.dt"
+    H.save_dt(str(again))
+    assert again.read_bytes() == want.read_bytes()
