@@ -479,22 +479,26 @@ __device__ __forceinline__ void screen_chunk(const ScreenLane &L, const cfloat *
         qx[m] = bxc[j0 + 1 + m];
         qy[m] = byc[j0 + 1 + m];
     }
-    uint64_t mk[kScreenChunk];
-    uint64_t any = 0;
+    // the failing lanes of each step; the active-lane mask is applied once,
+    // to the chunk's summary and to the steps that push
+    uint64_t fail[kScreenChunk];
+    uint64_t all_fail = ~0ull;
 #pragma unroll
     for (int m = 0; m < kScreenChunk; m++) {
         uint64_t gt1, lt1;
         screen_side(L, qx[m], qy[m], gt1, lt1);
-        mk[m] = ~((gt & gt1) | (lt & lt1)) & on_mask;
-        if (TAIL && (uint32_t)m >= nstep) mk[m] = 0;
-        any |= mk[m];
+        fail[m] = (gt & gt1) | (lt & lt1);
+        if (TAIL && (uint32_t)m >= nstep) fail[m] = ~0ull;
+        all_fail &= fail[m];
         gt = gt1;
         lt = lt1;
     }
-    if (any) {
+    if (~all_fail & on_mask) {
 #pragma unroll
-        for (int m = 0; m < kScreenChunk; m++)
-            if (mk[m]) push_mask(mk[m], ea, lb + j0 + m, AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+        for (int m = 0; m < kScreenChunk; m++) {
+            const uint64_t pass = ~fail[m] & on_mask;
+            if (pass) push_mask(pass, ea, lb + j0 + m, AX, AY, BX, BY, Q, qn, R, cap, cnt, lane);
+        }
     }
 }
 
