@@ -8,6 +8,8 @@
 //   (segment_intersect, geometry_computation.h:215-237, binary32).
 // The answer is an OR, so any evaluation order (and any early exit) returns
 // the reference's bit.
+#include <type_traits>
+
 #include "device_layout.h"
 
 namespace farm {
@@ -308,21 +310,25 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 float r = bax[e] * ty[e];
                 Op[e] = l - r;  // a.cross(b, c), bit-identical to the reference form
             }
-            for (uint32_t base = 1; base <= L && !stop; base += kChunk) {
-                const uint32_t nstep = min(kChunk, L - base + 1);
+            // one chunk of target vertices base .. base + nstep - 1; full
+            // chunks (TAIL false) evaluate every step unconditionally, so the
+            // chunk's vertices arrive as one batch of scalar loads
+            auto chunk = [&](auto tail_tag, uint32_t base, uint32_t nstep) {
+                constexpr bool TAIL = decltype(tail_tag)::value;
                 // the target vertices of this chunk, uniform: scalar loads
                 // (the vertex arrays carry padding, so the chunk may run past L)
+                const float *cxp = bx_ + (v0 + base), *cyp = by_ + (v0 + base);  // contiguous: one batch
                 float qx[kChunk], qy[kChunk];
 #pragma unroll
                 for (int m = 0; m < (int)kChunk; m++) {
-                    qx[m] = bx_[v0 + base + m];
-                    qy[m] = by_[v0 + base + m];
+                    qx[m] = cxp[m];
+                    qy[m] = cyp[m];
                 }
                 const float cx0 = cx, cy0 = cy;
                 bool maybe = false;
 #pragma unroll
                 for (int m = 0; m < (int)kChunk; m++) {
-                    if ((uint32_t)m < nstep) {
+                    if (!TAIL || (uint32_t)m < nstep) {
                         const float ddx = qx[m] - cx, ddy = qy[m] - cy;
                         float o[E + 1];
 #pragma unroll
@@ -369,7 +375,12 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 if (!stop && (++chunks & 1u) == 0 &&
                     __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                     stop = true;
-            }
+            };
+            const uint32_t nfull = L / kChunk;
+            for (uint32_t c = 0; c < nfull && !stop; c++)
+                chunk(std::integral_constant<bool, false>(), 1 + c * kChunk, kChunk);
+            if (!stop && nfull * kChunk < L)
+                chunk(std::integral_constant<bool, true>(), 1 + nfull * kChunk, L - nfull * kChunk);
         }
         if (__any(hit)) {
             if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
