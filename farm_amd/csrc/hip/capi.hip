@@ -26,6 +26,7 @@ __global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const PixPa
                          const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *, uint32_t *);
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
+__global__ void k_vdiff(const float *, const float *, uint64_t, float *, float *);
 __global__ void k_finalize(const uint32_t *, uint32_t, uint8_t *, unsigned long long *);
 }  // namespace farm
 
@@ -210,7 +211,7 @@ int farm_close(farm_handle h) {
     if (!h) return FARM_EINVAL;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->voff,
+    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->dx,       &h->dy,     &h->voff,
                       &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->result, &h->npp,      &h->pp_off,
                       &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->ctr,    &h->hist,
                       &h->cursor, &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
@@ -295,6 +296,13 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     }
     if ((rc = upload(h, h->x, ps->x, 4 * ps->nverts))) return rc;
     if ((rc = upload(h, h->y, ps->y, 4 * ps->nverts))) return rc;
+    // successive vertex differences over the padded arrays (the last one
+    // reads no vertex past the padding and is 0)
+    for (DevBuf *b : {&h->dx, &h->dy})
+        if ((rc = ensure(h, *b, 4 * (ps->nverts + kVertexPad)))) return rc;
+    hipLaunchKernelGGL(k_vdiff, dim3(blocks_for(ps->nverts + kVertexPad)), dim3(256), 0, h->stream,
+                       P<float>(h->x), P<float>(h->y), ps->nverts + kVertexPad, P<float>(h->dx), P<float>(h->dy));
+    HIPCHK(h, hipGetLastError());
     if ((rc = upload(h, h->voff, ps->voff, 4 * ps->nvoff))) return rc;
     if ((rc = upload(h, h->vnodes, ps->vnodes, 8 * ps->nvnodes))) return rc;
     if ((rc = ensure(h, h->pixedges, 4 * ps->nstatus))) return rc;

@@ -17,6 +17,9 @@ struct DevSet {
     const uint2 *seqs;
     const float *x;
     const float *y;
+    // dx[v] = x[v + 1] - x[v] and dy likewise, rounded to f32 (refinement only)
+    const float *dx;
+    const float *dy;
     const uint32_t *pixedges;  // per pixel: sum of sequence lengths (BORDER), else 0
     const uint32_t *voff;
     const double *vnodes;

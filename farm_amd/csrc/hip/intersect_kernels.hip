@@ -252,6 +252,8 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         const float *ay_ = S.y + A.vert_start;
         const float *bx_ = S.x + B.vert_start;
         const float *by_ = S.y + B.vert_start;
+        const float *bdx_ = S.dx + B.vert_start;
+        const float *bdy_ = S.dy + B.vert_start;
         const uint32_t a_s0 = pofa[P.pa], a_s1 = pofa[P.pa + 1];
         const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
         const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
@@ -318,18 +320,22 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 // the target vertices of this chunk, uniform: scalar loads
                 // (the vertex arrays carry padding, so the chunk may run past L)
                 const float *cxp = bx_ + (v0 + base), *cyp = by_ + (v0 + base);  // contiguous: one batch
-                float qx[kChunk], qy[kChunk];
+                // step m's direction (q_m - q_{m-1}), precomputed with the same rounding
+                const float *dxp = bdx_ + (v0 + base - 1), *dyp = bdy_ + (v0 + base - 1);
+                float qx[kChunk], qy[kChunk], qdx[kChunk], qdy[kChunk];
 #pragma unroll
                 for (int m = 0; m < (int)kChunk; m++) {
                     qx[m] = cxp[m];
                     qy[m] = cyp[m];
+                    qdx[m] = dxp[m];
+                    qdy[m] = dyp[m];
                 }
                 const float cx0 = cx, cy0 = cy;
                 bool maybe = false;
 #pragma unroll
                 for (int m = 0; m < (int)kChunk; m++) {
                     if (!TAIL || (uint32_t)m < nstep) {
-                        const float ddx = qx[m] - cx, ddy = qy[m] - cy;
+                        const float ddx = qdx[m], ddy = qdy[m];  // == qx[m] - cx, qy[m] - cy
                         float o[E + 1];
 #pragma unroll
                         for (int v = 0; v <= E; v++) {
@@ -524,6 +530,17 @@ __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restr
     for (int o = 32; o > 0; o >>= 1) tests += __shfl_xor(tests, o);
     if ((tid & 63) == 0) atomicAdd(&ctr[2], tests);
     if (tid == 0 && hit) result[k] = 1;
+}
+
+// dx[v] = x[v + 1] - x[v], dy[v] = y[v + 1] - y[v] over n padded vertices;
+// the same rounded f32 differences the refinement would form per step
+__global__ __launch_bounds__(256) void k_vdiff(const float *__restrict__ x, const float *__restrict__ y, uint64_t n,
+                                               float *__restrict__ dx, float *__restrict__ dy) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const bool last = v + 1 == n;
+    dx[v] = last ? 0.f : x[v + 1] - x[v];
+    dy[v] = last ? 0.f : y[v + 1] - y[v];
 }
 
 // per-pixel edge count of every BORDER pixel (sum of its sequence lengths)
