@@ -202,8 +202,9 @@ __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, 
 
 // ------------------------------------------------------------------------
 // Persistent refinement.  Each wave takes the next tile from a global
-// cursor.  A tile is 64 lanes x E consecutive edge slots of the source
-// pixel's sequences (consecutive lanes share one vertex); the target pixel's
+// cursor.  A tile is 64 lanes x W consecutive edge slots of the source
+// pixel's sequences (consecutive lanes share one vertex; the pixel's last
+// tile narrows to the fewest slots per lane that cover it); the target pixel's
 // edges stream through scalar registers (every lane needs the same target
 // vertex), so no LDS is used.  Per target vertex q and source edge (a, b):
 //   O(q)   = a.cross(b, q)      -- reused as o1/o2 of consecutive edges
@@ -213,7 +214,18 @@ __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, 
 // ------------------------------------------------------------------------
 constexpr uint32_t kChunk = 16;  // target vertices per scalar-load chunk
 
-template <int E>
+// f(integral_constant<int, E>) for the E in [E0, W] equal to the wave-uniform width ew
+template <int E0, int W, class F>
+__device__ __forceinline__ void with_width(uint32_t ew, F &f) {
+    if constexpr (E0 < W) {
+        if (ew == (uint32_t)E0) f(std::integral_constant<int, E0>());
+        else with_width<E0 + 1, W>(ew, f);
+    } else {
+        f(std::integral_constant<int, W>());
+    }
+}
+
+template <int W>
 __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                  const uint32_t *__restrict__ pb_ids,
                                                  const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
@@ -258,151 +270,159 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
         const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
 
-        // source vertices of this lane: slots first .. first + E
-        const uint32_t first = tile * 64u * E + (uint32_t)lane * E;
-        float px[E + 1], py[E + 1];
-        bool ev[E];
-        {
-            uint32_t k = a_s0, sbase = 0;
-            uint2 sq = sqa[k];
-            for (int m = 0; m <= E; m++) {
-                const uint32_t s = first + m;
-                while (k + 1 < a_s1 && s >= sbase + sq.y + 1) {
-                    sbase += sq.y + 1;
-                    k++;
-                    sq = sqa[k];
+        // width of this tile: W slots per lane, except the pixel's last tile,
+        // which takes the fewest slots per lane that still cover its rest
+        const uint32_t rest = P.slots - 1 - tile * 64u * W;  // edge-starting slots from this tile on
+        const uint32_t ew = __builtin_amdgcn_readfirstlane(rest >= 64u * W ? (uint32_t)W : (rest + 63) / 64);
+        auto run = [&](auto width_tag) {
+            constexpr int E = decltype(width_tag)::value;
+            // source vertices of this lane: slots first .. first + E
+            const uint32_t first = tile * 64u * W + (uint32_t)lane * E;
+            float px[E + 1], py[E + 1];
+            bool ev[E];
+            {
+                uint32_t k = a_s0, sbase = 0;
+                uint2 sq = sqa[k];
+                for (int m = 0; m <= E; m++) {
+                    const uint32_t s = first + m;
+                    while (k + 1 < a_s1 && s >= sbase + sq.y + 1) {
+                        sbase += sq.y + 1;
+                        k++;
+                        sq = sqa[k];
+                    }
+                    uint32_t o = s - sbase;
+                    const bool inside = s < sbase + sq.y + 1;
+                    if (!inside) o = sq.y;  // past the last slot: clamp (no edge)
+                    const uint32_t v = sq.x + o;
+                    px[m] = ax_[v];
+                    py[m] = ay_[v];
+                    if (m < E) ev[m] = inside && o < sq.y;
                 }
-                uint32_t o = s - sbase;
-                const bool inside = s < sbase + sq.y + 1;
-                if (!inside) o = sq.y;  // past the last slot: clamp (no edge)
-                const uint32_t v = sq.x + o;
-                px[m] = ax_[v];
-                py[m] = ay_[v];
-                if (m < E) ev[m] = inside && o < sq.y;
             }
-        }
-        float bax[E], bay[E];
-        int nvalid = 0;
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            bax[e] = px[e + 1] - px[e];
-            bay[e] = py[e + 1] - py[e];
-            nvalid += ev[e] ? 1 : 0;
-        }
-        bool hit = false;
-        bool stop = false;
-        uint32_t chunks = 0;
-        for (uint32_t j = b_s0; j < b_s1 && !stop; j++) {
-            const uint2 sq = sqb[j];
-            const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
-            const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
-            float cx = bx_[v0], cy = by_[v0];
-            // tx/ty: source vertex minus the current target vertex.  By the
-            // antisymmetry of rounded subtraction, (q - a) == -(a - q), so one
-            // difference serves O(q) at this step and o(p) at the next one.
-            float tx[E + 1], ty[E + 1], Op[E];
-#pragma unroll
-            for (int v = 0; v <= E; v++) {
-                tx[v] = px[v] - cx;
-                ty[v] = py[v] - cy;
-            }
+            float bax[E], bay[E];
+            int nvalid = 0;
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                float l = bay[e] * tx[e];
-                float r = bax[e] * ty[e];
-                Op[e] = l - r;  // a.cross(b, c), bit-identical to the reference form
+                bax[e] = px[e + 1] - px[e];
+                bay[e] = py[e + 1] - py[e];
+                nvalid += ev[e] ? 1 : 0;
             }
-            // one chunk of target vertices base .. base + nstep - 1; full
-            // chunks (TAIL false) evaluate every step unconditionally, so the
-            // chunk's vertices arrive as one batch of scalar loads
-            auto chunk = [&](auto tail_tag, uint32_t base, uint32_t nstep) {
-                constexpr bool TAIL = decltype(tail_tag)::value;
-                // the target vertices of this chunk, uniform: scalar loads
-                // (the vertex arrays carry padding, so the chunk may run past L)
-                const float *cxp = bx_ + (v0 + base), *cyp = by_ + (v0 + base);  // contiguous: one batch
-                // step m's direction (q_m - q_{m-1}), precomputed with the same rounding
-                const float *dxp = bdx_ + (v0 + base - 1), *dyp = bdy_ + (v0 + base - 1);
-                float qx[kChunk], qy[kChunk], qdx[kChunk], qdy[kChunk];
+            bool hit = false;
+            bool stop = false;
+            uint32_t chunks = 0;
+            for (uint32_t j = b_s0; j < b_s1 && !stop; j++) {
+                const uint2 sq = sqb[j];
+                const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
+                const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
+                float cx = bx_[v0], cy = by_[v0];
+                // tx/ty: source vertex minus the current target vertex.  By the
+                // antisymmetry of rounded subtraction, (q - a) == -(a - q), so one
+                // difference serves O(q) at this step and o(p) at the next one.
+                float tx[E + 1], ty[E + 1], Op[E];
 #pragma unroll
-                for (int m = 0; m < (int)kChunk; m++) {
-                    qx[m] = cxp[m];
-                    qy[m] = cyp[m];
-                    qdx[m] = dxp[m];
-                    qdy[m] = dyp[m];
+                for (int v = 0; v <= E; v++) {
+                    tx[v] = px[v] - cx;
+                    ty[v] = py[v] - cy;
                 }
-                const float cx0 = cx, cy0 = cy;
-                bool maybe = false;
 #pragma unroll
-                for (int m = 0; m < (int)kChunk; m++) {
-                    if (!TAIL || (uint32_t)m < nstep) {
-                        const float ddx = qdx[m], ddy = qdy[m];  // == qx[m] - cx, qy[m] - cy
-                        float o[E + 1];
+                for (int e = 0; e < E; e++) {
+                    float l = bay[e] * tx[e];
+                    float r = bax[e] * ty[e];
+                    Op[e] = l - r;  // a.cross(b, c), bit-identical to the reference form
+                }
+                // one chunk of target vertices base .. base + nstep - 1; full
+                // chunks (TAIL false) evaluate every step unconditionally, so the
+                // chunk's vertices arrive as one batch of scalar loads
+                auto chunk = [&](auto tail_tag, uint32_t base, uint32_t nstep) {
+                    constexpr bool TAIL = decltype(tail_tag)::value;
+                    // the target vertices of this chunk, uniform: scalar loads
+                    // (the vertex arrays carry padding, so the chunk may run past L)
+                    const float *cxp = bx_ + (v0 + base), *cyp = by_ + (v0 + base);  // contiguous: one batch
+                    // step m's direction (q_m - q_{m-1}), precomputed with the same rounding
+                    const float *dxp = bdx_ + (v0 + base - 1), *dyp = bdy_ + (v0 + base - 1);
+                    float qx[kChunk], qy[kChunk], qdx[kChunk], qdy[kChunk];
 #pragma unroll
-                        for (int v = 0; v <= E; v++) {
-                            float l = tx[v] * ddy;
-                            float r = ty[v] * ddx;
-                            o[v] = l - r;  // c.cross(p_v, d)
-                            tx[v] = px[v] - qx[m];
-                            ty[v] = py[v] - qy[m];
-                        }
-#pragma unroll
-                        for (int e = 0; e < E; e++) {
-                            float l = bay[e] * tx[e];
-                            float r = bax[e] * ty[e];
-                            const float On = l - r;  // a.cross(b, d)
-                            // superset of segment_intersect: both pairs of
-                            // orientations have opposite (or zero) signs, or
-                            // the collinear branch may apply (o3 == 0)
-                            const float g = fminf(fmaxf(o[e] * o[e + 1], Op[e] * On), fabsf(o[e]));
-                            maybe |= ev[e] && g <= 0.f;
-                            Op[e] = On;
-                        }
-                        cx = qx[m];
-                        cy = qy[m];
+                    for (int m = 0; m < (int)kChunk; m++) {
+                        qx[m] = cxp[m];
+                        qy[m] = cyp[m];
+                        qdx[m] = dxp[m];
+                        qdy[m] = dyp[m];
                     }
-                }
-                if (__any(maybe)) {
-                    // rare: re-evaluate the chunk with the reference predicate
-                    if (maybe) {
-                        float ccx = cx0, ccy = cy0;
-                        for (uint32_t m = 0; m < nstep; m++) {
-                            const float dx = bx_[v0 + base + m], dy = by_[v0 + base + m];
+                    const float cx0 = cx, cy0 = cy;
+                    bool maybe = false;
 #pragma unroll
-                            for (int e = 0; e < E; e++)
-                                if (ev[e] && seg_intersect(px[e], py[e], px[e + 1], py[e + 1], ccx, ccy, dx, dy))
-                                    hit = true;
-                            ccx = dx;
-                            ccy = dy;
+                    for (int m = 0; m < (int)kChunk; m++) {
+                        if (!TAIL || (uint32_t)m < nstep) {
+                            const float ddx = qdx[m], ddy = qdy[m];  // == qx[m] - cx, qy[m] - cy
+                            float o[E + 1];
+#pragma unroll
+                            for (int v = 0; v <= E; v++) {
+                                float l = tx[v] * ddy;
+                                float r = ty[v] * ddx;
+                                o[v] = l - r;  // c.cross(p_v, d)
+                                tx[v] = px[v] - qx[m];
+                                ty[v] = py[v] - qy[m];
+                            }
+#pragma unroll
+                            for (int e = 0; e < E; e++) {
+                                float l = bay[e] * tx[e];
+                                float r = bax[e] * ty[e];
+                                const float On = l - r;  // a.cross(b, d)
+                                // superset of segment_intersect: both pairs of
+                                // orientations have opposite (or zero) signs, or
+                                // the collinear branch may apply (o3 == 0)
+                                const float g = fminf(fmaxf(o[e] * o[e + 1], Op[e] * On), fabsf(o[e]));
+                                maybe |= ev[e] && g <= 0.f;
+                                Op[e] = On;
+                            }
+                            cx = qx[m];
+                            cy = qy[m];
                         }
                     }
-                    if (__any(hit)) stop = true;
-                }
-                tests += (unsigned long long)nvalid * nstep;
-                if (!stop && (++chunks & 1u) == 0 &&
-                    __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    stop = true;
-            };
-            const uint32_t nfull = L / kChunk;
-            for (uint32_t c = 0; c < nfull && !stop; c++)
-                chunk(std::integral_constant<bool, false>(), 1 + c * kChunk, kChunk);
-            if (!stop && nfull * kChunk < L)
-                chunk(std::integral_constant<bool, true>(), 1 + nfull * kChunk, L - nfull * kChunk);
-        }
-        if (__any(hit)) {
-            if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (pair_tests) {  // executed tests of this tile, for the cost labels
-            unsigned long long tt = tests - tests_before;
-            for (int o = 32; o > 0; o >>= 1) tt += __shfl_xor(tt, o);
-            if (lane == 0) atomicAdd(&pair_tests[pair], (uint32_t)min(tt, 0xffffffffull));
-        }
-        if (lane == 0) {
-            // algorithmic refine bytes of this tile: its source vertices plus
-            // the target pixel's sequences once
-            const uint32_t nb = b_s1 - b_s0;
-            rbytes += 16 + 8ull * nb + 8ull * (S.pixedges[B.pix_start + P.pb] + nb);
-        }
-        rbytes += (unsigned long long)nvalid * 8;
+                    if (__any(maybe)) {
+                        // rare: re-evaluate the chunk with the reference predicate
+                        if (maybe) {
+                            float ccx = cx0, ccy = cy0;
+                            for (uint32_t m = 0; m < nstep; m++) {
+                                const float dx = bx_[v0 + base + m], dy = by_[v0 + base + m];
+#pragma unroll
+                                for (int e = 0; e < E; e++)
+                                    if (ev[e] && seg_intersect(px[e], py[e], px[e + 1], py[e + 1], ccx, ccy, dx, dy))
+                                        hit = true;
+                                ccx = dx;
+                                ccy = dy;
+                            }
+                        }
+                        if (__any(hit)) stop = true;
+                    }
+                    tests += (unsigned long long)nvalid * nstep;
+                    if (!stop && (++chunks & 1u) == 0 &&
+                        __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        stop = true;
+                };
+                const uint32_t nfull = L / kChunk;
+                for (uint32_t c = 0; c < nfull && !stop; c++)
+                    chunk(std::integral_constant<bool, false>(), 1 + c * kChunk, kChunk);
+                if (!stop && nfull * kChunk < L)
+                    chunk(std::integral_constant<bool, true>(), 1 + nfull * kChunk, L - nfull * kChunk);
+            }
+            if (__any(hit)) {
+                if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (pair_tests) {  // executed tests of this tile, for the cost labels
+                unsigned long long tt = tests - tests_before;
+                for (int o = 32; o > 0; o >>= 1) tt += __shfl_xor(tt, o);
+                if (lane == 0) atomicAdd(&pair_tests[pair], (uint32_t)min(tt, 0xffffffffull));
+            }
+            if (lane == 0) {
+                // algorithmic refine bytes of this tile: its source vertices plus
+                // the target pixel's sequences once
+                const uint32_t nb = b_s1 - b_s0;
+                rbytes += 16 + 8ull * nb + 8ull * (S.pixedges[B.pix_start + P.pb] + nb);
+            }
+            rbytes += (unsigned long long)nvalid * 8;
+        };
+        with_width<1, W>(ew, run);
     }
     for (int o = 32; o > 0; o >>= 1) {
         tests += __shfl_xor(tests, o);
