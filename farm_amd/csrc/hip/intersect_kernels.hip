@@ -299,12 +299,13 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                     if (m < E) ev[m] = inside && o < sq.y;
                 }
             }
-            float bax[E], bay[E];
+            float bax[E], bay[E], inv[E];
             int nvalid = 0;
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 bax[e] = px[e + 1] - px[e];
                 bay[e] = py[e + 1] - py[e];
+                inv[e] = ev[e] ? -INFINITY : INFINITY;  // lifts a slot without an edge out of the screen
                 nvalid += ev[e] ? 1 : 0;
             }
             bool hit = false;
@@ -349,7 +350,9 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                         qdy[m] = dyp[m];
                     }
                     const float cx0 = cx, cy0 = cy;
-                    bool maybe = false;
+                    // running minimum of the screen over the chunk, even and odd
+                    // edges apart (two short dependency chains)
+                    float gmin[2] = {INFINITY, INFINITY};
 #pragma unroll
                     for (int m = 0; m < (int)kChunk; m++) {
                         if (!TAIL || (uint32_t)m < nstep) {
@@ -370,15 +373,18 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                                 const float On = l - r;  // a.cross(b, d)
                                 // superset of segment_intersect: both pairs of
                                 // orientations have opposite (or zero) signs, or
-                                // the collinear branch may apply (o3 == 0)
-                                const float g = fminf(fmaxf(o[e] * o[e + 1], Op[e] * On), fabsf(o[e]));
-                                maybe |= ev[e] && g <= 0.f;
+                                // the collinear branch may apply (o3 == 0).  A slot
+                                // without an edge enters only through o3 == 0, a
+                                // false candidate the exact replay discards.
+                                const float h = fmaxf(fmaxf(o[e] * o[e + 1], Op[e] * On), inv[e]);
+                                gmin[e & 1] = fminf(fminf(gmin[e & 1], h), fabsf(o[e]));
                                 Op[e] = On;
                             }
                             cx = qx[m];
                             cy = qy[m];
                         }
                     }
+                    const bool maybe = fminf(gmin[0], gmin[1]) <= 0.f;
                     if (__any(maybe)) {
                         // rare: re-evaluate the chunk with the reference predicate
                         if (maybe) {
