@@ -15,7 +15,9 @@ Also reported:
   plain     the one-pair-per-workgroup kernel on the first timed batch
   roofline  the refinement kernel: 42 FP32 ops per executed segment test
             (6 orientations x 7 ops) / its event-timed duration, against the
-            MI355X FP32 vector peak
+            MI355X FP32 vector peak.  The screen executes fewer ops than that,
+            so roofline.valu_issue also gives the VALU lane-ops the kernel
+            issues per test (rocprofv3 SQ_INSTS_VALU) against the issue peak
   cpu_baseline  the CPU oracle (restatement of the reference intersect_cpu
             query phase) on a bounded random sample of the same pairs,
             all host threads; the sample is also checked bit for bit
@@ -52,6 +54,24 @@ def measured_bytes_per_test() -> float | None:
     f = re.search(r"fetch_bytes_per_segment_test: ([0-9.eE+-]+)", text)
     w = re.search(r"write_bytes_per_segment_test: ([0-9.eE+-]+)", text)
     return float(f.group(1)) + float(w.group(1)) if f and w else None
+
+
+# one VALU lane-op per lane slot: 256 CUs x 4 SIMDs x 32 lanes per cycle x 2.4 GHz
+# (the 157.3 TFLOP/s FP32 vector peak counts each FMA as 2)
+VALU_ISSUE_PEAK_TOPS = 256 * 4 * 32 * 2.4e9 / 1e12
+
+
+def measured_valu_ops_per_test() -> float | None:
+    """VALU lane-ops k_refine issues per executed segment test on this
+    workload, from the committed rocprofv3 SQ_INSTS_VALU summary."""
+    import re
+
+    try:
+        text = open(os.path.join(ROOT, "profiles", "k_refine_valu_complex_x_lakes.md")).read()
+    except OSError:
+        return None
+    m = re.search(r"valu_lane_ops_per_segment_test: ([0-9.eE+-]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def measured_bytes_per_pair(name: str) -> float | None:
@@ -586,8 +606,11 @@ def main() -> int:
     filter_ms = sum(s["filter_ms"] for s in per_step)
     achieved = tests * OPS_PER_TEST / (refine_ms / 1000.0) / 1e12 if refine_ms > 0 else 0.0
     found = sum(s["found"] for s in per_step)
-    bpt = measured_bytes_per_test() if (args.scale == 1.0 and args.vpr == 256) else None
+    full_size = args.scale == 1.0 and args.vpr == 256
+    bpt = measured_bytes_per_test() if full_size else None
     traffic = tests / args.steps * bpt if bpt else None
+    opt = measured_valu_ops_per_test() if full_size else None
+    issued = tests * opt / (refine_ms / 1000.0) / 1e12 if opt and refine_ms > 0 else None
 
     line = {
         "metric": "exact intersect pairs/s plain vs batched+learned; approx-mode recall at 99% prec",
@@ -626,6 +649,10 @@ def main() -> int:
             if traffic else None,
             "algorithmic_bytes": sum(s["refine_bytes"] for s in per_step) / args.steps,
             "ops_per_segment_test": OPS_PER_TEST,
+            "valu_issue": {"lane_ops_per_test": opt, "achieved": issued, "peak": VALU_ISSUE_PEAK_TOPS,
+                           "unit": "tera lane-ops/s", "frac": issued / VALU_ISSUE_PEAK_TOPS,
+                           "source": "profiles/k_refine_valu_complex_x_lakes.md (SQ_INSTS_VALU x 64 / tests)"}
+            if issued else None,
         },
     }
 
