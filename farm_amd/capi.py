@@ -150,6 +150,7 @@ class Stats(C.Structure):
         ("filter_ms", C.c_double),
         ("refine_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("pruned_tests", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

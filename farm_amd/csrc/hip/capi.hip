@@ -27,6 +27,8 @@ __global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const PixPa
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
 __global__ void k_vdiff(const float *, const float *, uint64_t, float *, float *);
+__global__ void k_chunk_count(DevSet, int64_t, uint32_t *);
+__global__ void k_chunk_box(DevSet, int64_t, const uint32_t *, float4 *);
 __global__ void k_finalize(const uint32_t *, uint32_t, uint8_t *, unsigned long long *);
 }  // namespace farm
 
@@ -211,7 +213,7 @@ int farm_close(farm_handle h) {
     if (!h) return FARM_EINVAL;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->dx,       &h->dy,     &h->voff,
+    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->dx,       &h->dy,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
                       &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->result, &h->npp,      &h->pp_off,
                       &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->ctr,    &h->hist,
                       &h->cursor, &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
@@ -321,6 +323,25 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     if (ps->npolys > 0) {
         hipLaunchKernelGGL(k_pixedges, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h), ps->npolys,
                            P<uint32_t>(h->pixedges));
+        HIPCHK(h, hipGetLastError());
+    }
+    // boxes of the refinement's 16-step target chunks, per edge sequence
+    if ((rc = ensure(h, h->chcnt, 4 * (ps->nseqs + 1)))) return rc;
+    if ((rc = ensure(h, h->choff, 4 * (ps->nseqs + 1)))) return rc;
+    if ((rc = fill_u32(h, h->chcnt.p, ps->nseqs + 1, 0))) return rc;
+    if (ps->npolys > 0) {
+        hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h),
+                           ps->npolys, P<uint32_t>(h->chcnt));
+        HIPCHK(h, hipGetLastError());
+    }
+    if ((rc = exclusive_scan(h, P<uint32_t>(h->chcnt), P<uint32_t>(h->choff), ps->nseqs + 1))) return rc;
+    uint32_t nchunks = 0;
+    HIPCHK(h, hipMemcpyAsync(&nchunks, P<uint32_t>(h->choff) + ps->nseqs, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if ((rc = ensure(h, h->chbox, 16ull * (nchunks + kRefScreen)))) return rc;
+    if (ps->npolys > 0) {
+        hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h), ps->npolys,
+                           (const uint32_t *)P<uint32_t>(h->choff), P<float4>(h->chbox));
         HIPCHK(h, hipGetLastError());
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -470,6 +491,7 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
         st->refine_items = c[3];
         st->refine_bytes = c[4];
         st->found = c[5];
+        st->pruned_tests = c[8];
         if (n) {
             float t01 = 0, t23 = 0, t03 = 0;
             HIPCHK(h, hipEventElapsedTime(&t01, h->ev[0], h->ev[1]));

@@ -20,6 +20,10 @@ struct DevSet {
     // dx[v] = x[v + 1] - x[v] and dy likewise, rounded to f32 (refinement only)
     const float *dx;
     const float *dy;
+    // per edge sequence: index of its first 16-step chunk box; the boxes
+    // (centre x, y, half extents x, y), refinement only
+    const uint32_t *choff;
+    const float4 *chbox;
     const uint32_t *pixedges;  // per pixel: sum of sequence lengths (BORDER), else 0
     const uint32_t *voff;
     const double *vnodes;
@@ -28,6 +32,12 @@ struct DevSet {
     const uint8_t *lstatus;
     const double *areas;
 };
+
+// The refinement walks a target edge sequence in chunks of kRefChunk steps and
+// screens kRefScreen chunk boxes per scalar load (the box array keeps kRefScreen
+// spare entries past its end).
+constexpr uint32_t kRefChunk = 16;
+constexpr uint32_t kRefScreen = 4;
 
 // A BORDER x BORDER pixel pair of one candidate pair.
 struct PixPair {

@@ -39,7 +39,7 @@ struct farm_handle_s {
     std::string err;
     // polygon set
     int64_t npolys = 0;
-    farm::DevBuf hdr, status, pixoff, seqs, x, y, dx, dy, voff, vnodes, pixedges;
+    farm::DevBuf hdr, status, pixoff, seqs, x, y, dx, dy, choff, chcnt, chbox, voff, vnodes, pixedges;
     uint64_t nstatus = 0;
     // pairs
     uint64_t npairs = 0;
@@ -127,6 +127,8 @@ inline DevSet devset(farm_handle h) {
     S.y = P<float>(h->y);
     S.dx = P<float>(h->dx);
     S.dy = P<float>(h->dy);
+    S.choff = P<uint32_t>(h->choff);
+    S.chbox = P<float4>(h->chbox);
     S.pixedges = P<uint32_t>(h->pixedges);
     S.voff = P<uint32_t>(h->voff);
     S.vnodes = P<double>(h->vnodes);

@@ -212,7 +212,60 @@ __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, 
 // and a superset filter (o1*o2 <= 0 or o3 == 0) and o3*o4 <= 0 sends the
 // rare candidates to the exact reference predicate.
 // ------------------------------------------------------------------------
-constexpr uint32_t kChunk = 16;  // target vertices per scalar-load chunk
+
+// Boxes of the refinement's target chunks.  A sequence (first, len) is walked
+// in chunks c = 0, 1, ..: steps base = 1 + 16c .. min(len, base + 15), whose
+// segments span vertices first + base - 1 .. first + base + n - 1.  The box is
+// stored as a centre and half extents rounded up, so [c - h, c + h] holds
+// every vertex of the chunk in exact arithmetic.  Infinite extents (the prune
+// never applies) go to a box that reaches past 2^40, holds a non-finite
+// vertex, or holds a segment of length 0: for c == d the reference's
+// c.cross(a, d) is 0 for every a, so segment_intersect takes its collinear
+// branch and answers with the bounding-box overlap alone, on either side of
+// line ab.
+__global__ __launch_bounds__(256) void k_chunk_count(DevSet S, int64_t npolys, uint32_t *__restrict__ cnt) {
+    const int64_t p = blockIdx.x;
+    if (p >= npolys) return;
+    const farm_raster_hdr H = S.hdr[p];
+    for (uint32_t s = threadIdx.x; s < H.nseqs; s += blockDim.x)
+        cnt[H.seq_start + s] = (S.seqs[H.seq_start + s].y + kRefChunk - 1) / kRefChunk;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, const uint32_t *__restrict__ choff,
+                                                   float4 *__restrict__ box) {
+    const int64_t p = blockIdx.x;
+    if (p >= npolys) return;
+    const farm_raster_hdr H = S.hdr[p];
+    const float *x = S.x + H.vert_start, *y = S.y + H.vert_start;
+    for (uint32_t s = threadIdx.x; s < H.nseqs; s += blockDim.x) {
+        const uint2 sq = S.seqs[H.seq_start + s];
+        uint32_t o = choff[H.seq_start + s];
+        for (uint32_t base = 1; base <= sq.y; base += kRefChunk, o++) {
+            const uint32_t n = min(kRefChunk, sq.y - base + 1);
+            float x0 = x[sq.x + base - 1], x1 = x0, y0 = y[sq.x + base - 1], y1 = y0;
+            bool open = isfinite(x0) && isfinite(y0);  // false: never prune this chunk
+            float lx = x0, ly = y0;
+            for (uint32_t m = 0; m < n; m++) {
+                const float vx = x[sq.x + base + m], vy = y[sq.x + base + m];
+                open = open && isfinite(vx) && isfinite(vy) && !(vx == lx && vy == ly);
+                lx = vx;
+                ly = vy;
+                x0 = fminf(x0, vx);
+                x1 = fmaxf(x1, vx);
+                y0 = fminf(y0, vy);
+                y1 = fmaxf(y1, vy);
+            }
+            const float cx = 0.5f * x0 + 0.5f * x1, cy = 0.5f * y0 + 0.5f * y1;
+            // a rounded difference is within 2^-24 of the exact one; the
+            // factor and the smallest normal cover it and the product's rounding
+            float hx = fmaxf(x1 - cx, cx - x0) * 1.0000003f + 1.1754944e-38f;
+            float hy = fmaxf(y1 - cy, cy - y0) * 1.0000003f + 1.1754944e-38f;
+            const float big = 0x1p40f;
+            if (!(open && fabsf(cx) + hx < big && fabsf(cy) + hy < big)) hx = hy = INFINITY;
+            box[o] = make_float4(cx, cy, hx, hy);
+        }
+    }
+}
 
 // f(integral_constant<int, E>) for the E in [E0, W] equal to the wave-uniform width ew
 template <int E0, int W, class F>
@@ -225,6 +278,8 @@ __device__ __forceinline__ void with_width(uint32_t ew, F &f) {
     }
 }
 
+constexpr uint32_t kRefQueue = 128;  // queued items per wave: a power of 2 above 63 + 64
+
 template <int W>
 __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                  const uint32_t *__restrict__ pb_ids,
@@ -233,9 +288,19 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                                                  uint32_t *__restrict__ cursor, uint32_t *__restrict__ result,
                                                  unsigned long long *__restrict__ ctr,
                                                  uint32_t *__restrict__ pair_tests) {
+    // per wave: the tile's source slots (x then y, W + 1 each) and edge bits
+    // of every lane, and the ring of queued items (first vertex of the target
+    // chunk, source lane | steps << 8)
+    __shared__ float s_src[4][64][2 * (W + 1)];
+    __shared__ uint32_t s_ev[4][64];
+    __shared__ uint2 s_ring[4][kRefQueue];
     const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    float(*src)[2 * (W + 1)] = s_src[wv];
+    uint32_t *sev = s_ev[wv];
+    uint2 *ring = s_ring[wv];
     const uint32_t total = tile_off[npp];
-    unsigned long long tests = 0, items = 0, rbytes = 0;
+    unsigned long long tests = 0, items = 0, rbytes = 0, pruned = 0;
     for (;;) {
         const unsigned long long tests_before = tests;
         uint32_t t = 0;
@@ -266,6 +331,7 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         const float *by_ = S.y + B.vert_start;
         const float *bdx_ = S.dx + B.vert_start;
         const float *bdy_ = S.dy + B.vert_start;
+        const uint32_t *bchoff = S.choff + B.seq_start;
         const uint32_t a_s0 = pofa[P.pa], a_s1 = pofa[P.pa + 1];
         const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
         const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
@@ -299,118 +365,188 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                     if (m < E) ev[m] = inside && o < sq.y;
                 }
             }
-            float bax[E], bay[E], inv[E];
+            float bax[E], bay[E];
             int nvalid = 0;
+            uint32_t evb = 0;
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 bax[e] = px[e + 1] - px[e];
                 bay[e] = py[e + 1] - py[e];
-                inv[e] = ev[e] ? -INFINITY : INFINITY;  // lifts a slot without an edge out of the screen
                 nvalid += ev[e] ? 1 : 0;
+                evb |= (ev[e] ? 1u : 0u) << e;
             }
+            // publish this lane's slots for the lanes that evaluate its items
+#pragma unroll
+            for (int v = 0; v <= E; v++) {
+                src[lane][v] = px[v];
+                src[lane][E + 1 + v] = py[v];
+            }
+            sev[lane] = evb;
+            __builtin_amdgcn_wave_barrier();
             bool hit = false;
             bool stop = false;
-            uint32_t chunks = 0;
+            uint32_t qhead = 0, qn = 0, screened = 0;  // wave-uniform
+
+            // Evaluate the n <= 64 oldest queued items, one per lane: the source
+            // lane's E edges against the chunk's nstep target segments.  Per
+            // target vertex q and source edge (a, b):
+            //   O(q) = a.cross(b, q)   -- o1/o2 of consecutive steps
+            //   o(p) = c.cross(p, d)   -- per source vertex p, for the segment (c, d)
+            // carried from step to step: tx/ty = p - c serves O(c) and, by the
+            // antisymmetry of rounded subtraction, o(p) of the next step.  A
+            // superset filter (o1*o2 <= 0 or o3 == 0, and o3*o4 <= 0) sends
+            // the rare candidates to the exact reference predicate.
+            auto drain = [&](uint32_t n) {
+                const bool has = (uint32_t)lane < n;
+                const uint2 it = ring[(qhead + (uint32_t)lane) & (kRefQueue - 1)];
+                const uint32_t sl = has ? (it.y & 63u) : (uint32_t)lane;
+                const uint32_t ns = has ? (it.y >> 8) : 0u;
+                const uint32_t vs = has ? it.x : 0u;  // the chunk's first vertex c
+                float qpx[E + 1], qpy[E + 1];
+#pragma unroll
+                for (int v = 0; v <= E; v++) {
+                    qpx[v] = src[sl][v];
+                    qpy[v] = src[sl][E + 1 + v];
+                }
+                const uint32_t eb = sev[sl];
+                float qbax[E], qbay[E], qinv[E];
+                int qnv = 0;
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    qbax[e] = qpx[e + 1] - qpx[e];
+                    qbay[e] = qpy[e + 1] - qpy[e];
+                    const bool ve = (eb >> e) & 1u;
+                    qinv[e] = ve ? -INFINITY : INFINITY;  // lifts a slot without an edge out of the screen
+                    qnv += ve ? 1 : 0;
+                }
+                const float c0x = bx_[vs], c0y = by_[vs];
+                float tx[E + 1], ty[E + 1], Op[E];
+#pragma unroll
+                for (int v = 0; v <= E; v++) {
+                    tx[v] = qpx[v] - c0x;
+                    ty[v] = qpy[v] - c0y;
+                }
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    float l = qbay[e] * tx[e];
+                    float r = qbax[e] * ty[e];
+                    Op[e] = l - r;  // a.cross(b, c), bit-identical to the reference form
+                }
+                // running minimum of the screen, even and odd edges apart
+                float gmin[2] = {INFINITY, INFINITY};
+#pragma unroll
+                for (int m = 0; m < (int)kRefChunk; m++) {
+                    if ((uint32_t)m < ns) {
+                        const float qx = bx_[vs + 1 + m], qy = by_[vs + 1 + m];
+                        // the step's direction (q - c), precomputed with the same rounding
+                        const float ddx = bdx_[vs + m], ddy = bdy_[vs + m];
+                        float o[E + 1];
+#pragma unroll
+                        for (int v = 0; v <= E; v++) {
+                            float l = tx[v] * ddy;
+                            float r = ty[v] * ddx;
+                            o[v] = l - r;  // c.cross(p_v, d)
+                            tx[v] = qpx[v] - qx;
+                            ty[v] = qpy[v] - qy;
+                        }
+#pragma unroll
+                        for (int e = 0; e < E; e++) {
+                            float l = qbay[e] * tx[e];
+                            float r = qbax[e] * ty[e];
+                            const float On = l - r;  // a.cross(b, d)
+                            // superset of segment_intersect: both pairs of
+                            // orientations have opposite (or zero) signs, or the
+                            // collinear branch may apply (o3 == 0).  A slot
+                            // without an edge enters only through o3 == 0, a
+                            // false candidate the exact replay discards.
+                            const float h = fmaxf(fmaxf(o[e] * o[e + 1], Op[e] * On), qinv[e]);
+                            gmin[e & 1] = fminf(fminf(gmin[e & 1], h), fabsf(o[e]));
+                            Op[e] = On;
+                        }
+                    }
+                }
+                const bool maybe = fminf(gmin[0], gmin[1]) <= 0.f;
+                if (__any(maybe)) {
+                    // rare: re-evaluate the item with the reference predicate
+                    if (maybe) {
+                        float ccx = c0x, ccy = c0y;
+                        for (uint32_t m = 0; m < ns; m++) {
+                            const float dx = bx_[vs + 1 + m], dy = by_[vs + 1 + m];
+#pragma unroll
+                            for (int e = 0; e < E; e++)
+                                if (((eb >> e) & 1u) &&
+                                    seg_intersect(qpx[e], qpy[e], qpx[e + 1], qpy[e + 1], ccx, ccy, dx, dy))
+                                    hit = true;
+                            ccx = dx;
+                            ccy = dy;
+                        }
+                    }
+                }
+                tests += (unsigned long long)qnv * ns;
+                qhead = (qhead + n) & (kRefQueue - 1);
+                qn -= n;
+                if (__any(hit) || __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    stop = true;
+            };
+
             for (uint32_t j = b_s0; j < b_s1 && !stop; j++) {
                 const uint2 sq = sqb[j];
                 const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
                 const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
-                float cx = bx_[v0], cy = by_[v0];
-                // tx/ty: source vertex minus the current target vertex.  By the
-                // antisymmetry of rounded subtraction, (q - a) == -(a - q), so one
-                // difference serves O(q) at this step and o(p) at the next one.
-                float tx[E + 1], ty[E + 1], Op[E];
+                const float4 *cbox = S.chbox + __builtin_amdgcn_readfirstlane(bchoff[j]);
+                const uint32_t nch = (L + kRefChunk - 1) / kRefChunk;
+                for (uint32_t c = 0; c < nch && !stop; c++) {
+                    // Exact prune of this lane's edges against the chunk.  With
+                    // M = 2^-19 Q (|b - a|_1 + 2Q), Q the L1 reach from a to the
+                    // chunk box: if every box point q has |a.cross(b, q)| > M in
+                    // exact arithmetic, the reference's f32 a.cross(b, q) has one
+                    // nonzero sign on the whole chunk (rounding <= 3 2^-24 Q
+                    // |b - a|_1), and its collinear branch (c.cross(a, d) ==
+                    // c.cross(b, d) == 0) needs a and b within rounding of line
+                    // cd, which puts c within 2^-20.7 Q (|b - a|_1 + 2Q) of line
+                    // ab: segment_intersect is false for every segment of the
+                    // chunk.  The f32 evaluation below errs by a few 2^-24 Q
+                    // |b - a|_1, covered by M.  (A chunk with a segment of
+                    // length 0 has an infinite box and is never pruned.)
+                    const float4 bb = cbox[c];
+                    bool need = false;
 #pragma unroll
-                for (int v = 0; v <= E; v++) {
-                    tx[v] = px[v] - cx;
-                    ty[v] = py[v] - cy;
-                }
-#pragma unroll
-                for (int e = 0; e < E; e++) {
-                    float l = bay[e] * tx[e];
-                    float r = bax[e] * ty[e];
-                    Op[e] = l - r;  // a.cross(b, c), bit-identical to the reference form
-                }
-                // one chunk of target vertices base .. base + nstep - 1; full
-                // chunks (TAIL false) evaluate every step unconditionally, so the
-                // chunk's vertices arrive as one batch of scalar loads
-                auto chunk = [&](auto tail_tag, uint32_t base, uint32_t nstep) {
-                    constexpr bool TAIL = decltype(tail_tag)::value;
-                    // the target vertices of this chunk, uniform: scalar loads
-                    // (the vertex arrays carry padding, so the chunk may run past L)
-                    const float *cxp = bx_ + (v0 + base), *cyp = by_ + (v0 + base);  // contiguous: one batch
-                    // step m's direction (q_m - q_{m-1}), precomputed with the same rounding
-                    const float *dxp = bdx_ + (v0 + base - 1), *dyp = bdy_ + (v0 + base - 1);
-                    float qx[kChunk], qy[kChunk], qdx[kChunk], qdy[kChunk];
-#pragma unroll
-                    for (int m = 0; m < (int)kChunk; m++) {
-                        qx[m] = cxp[m];
-                        qy[m] = cyp[m];
-                        qdx[m] = dxp[m];
-                        qdy[m] = dyp[m];
+                    for (int e = 0; e < E; e++) {
+                        const float dxc = bb.x - px[e], dyc = bb.y - py[e];
+                        const float l = bax[e] * dyc;
+                        const float r = bay[e] * dxc;
+                        const float oc = l - r;                                            // at the box centre
+                        const float rr = fmaf(fabsf(bay[e]), bb.z, fabsf(bax[e]) * bb.w);  // its spread
+                        const float q = (fabsf(dxc) + fabsf(dyc)) + (bb.z + bb.w);
+                        const float sab = fabsf(bax[e]) + fabsf(bay[e]);
+                        const float m19 = fmaxf(q * fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, at least 2^-80
+                        const bool prune = (fabsf(oc) - rr) * 0x1p19f > m19;
+                        need |= ev[e] && !prune;
                     }
-                    const float cx0 = cx, cy0 = cy;
-                    // running minimum of the screen over the chunk, even and odd
-                    // edges apart (two short dependency chains)
-                    float gmin[2] = {INFINITY, INFINITY};
-#pragma unroll
-                    for (int m = 0; m < (int)kChunk; m++) {
-                        if (!TAIL || (uint32_t)m < nstep) {
-                            const float ddx = qdx[m], ddy = qdy[m];  // == qx[m] - cx, qy[m] - cy
-                            float o[E + 1];
-#pragma unroll
-                            for (int v = 0; v <= E; v++) {
-                                float l = tx[v] * ddy;
-                                float r = ty[v] * ddx;
-                                o[v] = l - r;  // c.cross(p_v, d)
-                                tx[v] = px[v] - qx[m];
-                                ty[v] = py[v] - qy[m];
-                            }
-#pragma unroll
-                            for (int e = 0; e < E; e++) {
-                                float l = bay[e] * tx[e];
-                                float r = bax[e] * ty[e];
-                                const float On = l - r;  // a.cross(b, d)
-                                // superset of segment_intersect: both pairs of
-                                // orientations have opposite (or zero) signs, or
-                                // the collinear branch may apply (o3 == 0).  A slot
-                                // without an edge enters only through o3 == 0, a
-                                // false candidate the exact replay discards.
-                                const float h = fmaxf(fmaxf(o[e] * o[e + 1], Op[e] * On), inv[e]);
-                                gmin[e & 1] = fminf(fminf(gmin[e & 1], h), fabsf(o[e]));
-                                Op[e] = On;
-                            }
-                            cx = qx[m];
-                            cy = qy[m];
-                        }
+                    const uint32_t nstep = min(kRefChunk, L - c * kRefChunk);
+                    const unsigned long long bm = __ballot(need);
+                    if (bm) {
+                        // append this chunk for every lane that needs it
+                        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                        if (need)
+                            ring[(qhead + qn + below) & (kRefQueue - 1)] =
+                                make_uint2(v0 + c * kRefChunk, (uint32_t)lane | (nstep << 8));
+                        qn += (uint32_t)__popcll(bm);
                     }
-                    const bool maybe = fminf(gmin[0], gmin[1]) <= 0.f;
-                    if (__any(maybe)) {
-                        // rare: re-evaluate the chunk with the reference predicate
-                        if (maybe) {
-                            float ccx = cx0, ccy = cy0;
-                            for (uint32_t m = 0; m < nstep; m++) {
-                                const float dx = bx_[v0 + base + m], dy = by_[v0 + base + m];
-#pragma unroll
-                                for (int e = 0; e < E; e++)
-                                    if (ev[e] && seg_intersect(px[e], py[e], px[e + 1], py[e + 1], ccx, ccy, dx, dy))
-                                        hit = true;
-                                ccx = dx;
-                                ccy = dy;
-                            }
-                        }
-                        if (__any(hit)) stop = true;
-                    }
-                    tests += (unsigned long long)nvalid * nstep;
-                    if (!stop && (++chunks & 1u) == 0 &&
-                        __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    if (!need) pruned += (unsigned long long)nvalid * nstep;
+                    if (qn >= 64) {
+                        __builtin_amdgcn_wave_barrier();
+                        drain(64);
+                    } else if ((++screened & 31u) == 0 &&
+                               __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                         stop = true;
-                };
-                const uint32_t nfull = L / kChunk;
-                for (uint32_t c = 0; c < nfull && !stop; c++)
-                    chunk(std::integral_constant<bool, false>(), 1 + c * kChunk, kChunk);
-                if (!stop && nfull * kChunk < L)
-                    chunk(std::integral_constant<bool, true>(), 1 + nfull * kChunk, L - nfull * kChunk);
+                    }
+                }
+            }
+            if (!stop && qn) {
+                __builtin_amdgcn_wave_barrier();
+                drain(qn);
             }
             if (__any(hit)) {
                 if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -433,11 +569,13 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
     for (int o = 32; o > 0; o >>= 1) {
         tests += __shfl_xor(tests, o);
         rbytes += __shfl_xor(rbytes, o);
+        pruned += __shfl_xor(pruned, o);
     }
     if (lane == 0) {
         atomicAdd(&ctr[2], tests);
         atomicAdd(&ctr[3], items);
         atomicAdd(&ctr[4], rbytes);
+        atomicAdd(&ctr[8], pruned);
     }
 }
 

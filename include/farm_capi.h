@@ -148,6 +148,7 @@ typedef struct farm_stats {
     double filter_ms;          /* device time of the filter kernel(s) */
     double refine_ms;          /* device time of the refinement kernel(s) */
     double total_ms;           /* device time of the whole call */
+    uint64_t pruned_tests;     /* segment tests skipped whole by the exact chunk-box prune */
 } farm_stats;
 
 typedef struct farm_handle_s *farm_handle;
