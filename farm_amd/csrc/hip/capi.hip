@@ -23,7 +23,8 @@ template <int E>
 __global__ void k_tiles(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 template <int E>
 __global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                         const uint32_t *, uint32_t, uint32_t *, uint32_t *, unsigned long long *, uint32_t *);
+                         const uint2 *, uint32_t, uint32_t *, uint32_t *, unsigned long long *, uint32_t *);
+__global__ void k_tile_map(const uint32_t *, uint32_t, uint32_t, uint2 *);
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
 __global__ void k_vdiff(const float *, const float *, uint64_t, float *, float *);
@@ -96,12 +97,26 @@ int launch_tiles_refine(farm_handle h, const DevSet &S, const uint32_t *pa, cons
     HIPCHK(h, hipGetLastError());
     int rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->tiles), P<uint32_t>(h->tile_off), npp_total + 1);
     if (rc) return rc;
+    // the tile total (32-bit offsets: checked against the 64-bit count first)
+    unsigned long long ntiles64 = 0;
+    uint32_t ntiles = 0;
+    HIPCHK(h, hipMemcpyAsync(&ntiles64, ctr + 7, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&ntiles, P<uint32_t>(h->tile_off) + npp_total, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (ntiles64 > 0xffffffffull)
+        return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed 32-bit offsets; use smaller batches", ntiles64);
+    if ((rc = ensure(h, h->tile_map, 8ull * ntiles + 8))) return rc;
+    if (npp_total) {
+        hipLaunchKernelGGL(k_tile_map, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
+                           (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, ntiles, P<uint2>(h->tile_map));
+        HIPCHK(h, hipGetLastError());
+    }
     if ((rc = fill_u32(h, h->cursor.p, 1, 0))) return rc;
     const unsigned grid = (unsigned)(h->num_cus * 8);
     HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
     hipLaunchKernelGGL(k_refine<E>, dim3(grid), dim3(256), 0, h->stream, S, pa, pb,
                        (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
-                       (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, P<uint32_t>(h->cursor),
+                       (const uint2 *)P<uint2>(h->tile_map), ntiles, P<uint32_t>(h->cursor),
                        P<uint32_t>(h->result), ctr, pair_tests);
     HIPCHK(h, hipGetLastError());
     return FARM_OK;
@@ -215,7 +230,7 @@ int farm_close(farm_handle h) {
     (void)hipStreamSynchronize(h->stream);
     DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->dx,       &h->dy,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
                       &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->result, &h->npp,      &h->pp_off,
-                      &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->ctr,    &h->hist,
+                      &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->tile_map, &h->ctr,    &h->hist,
                       &h->cursor, &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,

@@ -45,7 +45,7 @@ struct farm_handle_s {
     uint64_t npairs = 0;
     farm::DevBuf pa, pb;
     // per-call scratch of the intersect driver
-    farm::DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, ctr, hist, cursor, scan_tmp;
+    farm::DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, tile_map, ctr, hist, cursor, scan_tmp;
     farm::DevBuf pair_tests;  // per-pair executed segment tests (cost labels)
     int tuned_tile = 4;       // source edges per lane of the refinement (farm_autotune)
     // learned part

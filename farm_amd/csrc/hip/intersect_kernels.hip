@@ -200,17 +200,27 @@ __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, 
     if ((threadIdx.x & 63) == 0 && t) atomicAdd(&ctr[7], t);
 }
 
+// (ordered pixel pair, tile within it) of every tile, from the scanned counts
+__global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ tile_off, uint32_t npp, uint32_t total,
+                                                  uint2 *__restrict__ map) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npp) return;
+    const uint32_t t0 = tile_off[q], t1 = min(tile_off[q + 1], total);
+    for (uint32_t t = t0; t < t1; t++) map[t] = make_uint2(q, t - t0);
+}
+
 // ------------------------------------------------------------------------
 // Persistent refinement.  Each wave takes the next tile from a global
-// cursor.  A tile is 64 lanes x W consecutive edge slots of the source
-// pixel's sequences (consecutive lanes share one vertex; the pixel's last
-// tile narrows to the fewest slots per lane that cover it); the target pixel's
-// edges stream through scalar registers (every lane needs the same target
-// vertex), so no LDS is used.  Per target vertex q and source edge (a, b):
-//   O(q)   = a.cross(b, q)      -- reused as o1/o2 of consecutive edges
-//   o(p)   = c.cross(p, d)      -- per source vertex p, for target edge (c,d)
-// and a superset filter (o1*o2 <= 0 or o3 == 0) and o3*o4 <= 0 sends the
-// rare candidates to the exact reference predicate.
+// cursor.  A tile is 64 lanes x W consecutive edge slots of one source
+// pixel's sequences (consecutive lanes share one vertex), refined against
+// the edge sequences of one target pixel, walked in chunks of 16 segments.
+// Per chunk every lane first tests its W edges against the chunk's box (an
+// exact prune: no edge whose line keeps the whole box at a safe distance can
+// meet a segment of the chunk under the reference predicate); the (lane,
+// chunk) pairs that survive are queued in LDS, and every 64 queued items are
+// evaluated one per lane: the lane's edges (read from LDS) against the
+// chunk's segments (its own loads), with the screen below and, for the rare
+// candidates, the exact reference predicate.
 // ------------------------------------------------------------------------
 
 // Boxes of the refinement's target chunks.  A sequence (first, len) is walked
@@ -267,24 +277,13 @@ __global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, con
     }
 }
 
-// f(integral_constant<int, E>) for the E in [E0, W] equal to the wave-uniform width ew
-template <int E0, int W, class F>
-__device__ __forceinline__ void with_width(uint32_t ew, F &f) {
-    if constexpr (E0 < W) {
-        if (ew == (uint32_t)E0) f(std::integral_constant<int, E0>());
-        else with_width<E0 + 1, W>(ew, f);
-    } else {
-        f(std::integral_constant<int, W>());
-    }
-}
-
-constexpr uint32_t kRefQueue = 128;  // queued items per wave: a power of 2 above 63 + 64
+constexpr uint32_t kRefQueue = 512;  // queued items per wave: a power of 2 above 63 + 64 kRefScreen
 
 template <int W>
 __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                  const uint32_t *__restrict__ pb_ids,
                                                  const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
-                                                 const uint32_t *__restrict__ tile_off, uint32_t npp,
+                                                 const uint2 *__restrict__ tile_map, uint32_t total,
                                                  uint32_t *__restrict__ cursor, uint32_t *__restrict__ result,
                                                  unsigned long long *__restrict__ ctr,
                                                  uint32_t *__restrict__ pair_tests) {
@@ -299,7 +298,6 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
     float(*src)[2 * (W + 1)] = s_src[wv];
     uint32_t *sev = s_ev[wv];
     uint2 *ring = s_ring[wv];
-    const uint32_t total = tile_off[npp];
     unsigned long long tests = 0, items = 0, rbytes = 0, pruned = 0;
     for (;;) {
         const unsigned long long tests_before = tests;
@@ -307,14 +305,10 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         if (lane == 0) t = atomicAdd(cursor, 1u);
         t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
         if (t >= total) break;
-        // ordered pixel pair owning tile t: largest q with tile_off[q] <= t
-        uint32_t lo = 0, hi = npp;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (tile_off[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t tile = t - tile_off[lo];
+        // ordered pixel pair owning tile t, and the tile's place in it
+        const uint2 tm = tile_map[t];
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(tm.x);
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(tm.y);
         const PixPair P = pps[__builtin_amdgcn_readfirstlane(order[lo])];
         const uint32_t pair = __builtin_amdgcn_readfirstlane(P.pair);
         if (__hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
@@ -335,11 +329,9 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         const uint32_t a_s0 = pofa[P.pa], a_s1 = pofa[P.pa + 1];
         const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
         const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
-
-        // width of this tile: W slots per lane, except the pixel's last tile,
-        // which takes the fewest slots per lane that still cover its rest
-        const uint32_t rest = P.slots - 1 - tile * 64u * W;  // edge-starting slots from this tile on
-        const uint32_t ew = __builtin_amdgcn_readfirstlane(rest >= 64u * W ? (uint32_t)W : (rest + 63) / 64);
+        // the tile's body, invoked once below (as a lambda it is allocated
+        // to fewer registers than as a plain block: 167 against 176 VGPRs at
+        // W = 8, three waves per SIMD instead of two)
         auto run = [&](auto width_tag) {
             constexpr int E = decltype(width_tag)::value;
             // source vertices of this lane: slots first .. first + E
@@ -434,7 +426,7 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 }
                 // running minimum of the screen, even and odd edges apart
                 float gmin[2] = {INFINITY, INFINITY};
-#pragma unroll
+#pragma unroll 4
                 for (int m = 0; m < (int)kRefChunk; m++) {
                     if ((uint32_t)m < ns) {
                         const float qx = bx_[vs + 1 + m], qy = by_[vs + 1 + m];
@@ -495,53 +487,65 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
                 const float4 *cbox = S.chbox + __builtin_amdgcn_readfirstlane(bchoff[j]);
                 const uint32_t nch = (L + kRefChunk - 1) / kRefChunk;
-                for (uint32_t c = 0; c < nch && !stop; c++) {
-                    // Exact prune of this lane's edges against the chunk.  With
-                    // M = 2^-19 Q (|b - a|_1 + 2Q), Q the L1 reach from a to the
-                    // chunk box: if every box point q has |a.cross(b, q)| > M in
-                    // exact arithmetic, the reference's f32 a.cross(b, q) has one
-                    // nonzero sign on the whole chunk (rounding <= 3 2^-24 Q
-                    // |b - a|_1), and its collinear branch (c.cross(a, d) ==
-                    // c.cross(b, d) == 0) needs a and b within rounding of line
-                    // cd, which puts c within 2^-20.7 Q (|b - a|_1 + 2Q) of line
-                    // ab: segment_intersect is false for every segment of the
-                    // chunk.  The f32 evaluation below errs by a few 2^-24 Q
-                    // |b - a|_1, covered by M.  (A chunk with a segment of
-                    // length 0 has an infinite box and is never pruned.)
-                    const float4 bb = cbox[c];
-                    bool need = false;
+                for (uint32_t g = 0; g < nch && !stop; g += kRefScreen) {
+                    // Exact prune of this lane's edges against kRefScreen chunks
+                    // (their boxes are one scalar load; boxes past the sequence's
+                    // last chunk are read, the array keeps spare entries, but not
+                    // used).  With M = 2^-19 Q (|b - a|_1 + 2Q), Q the L1 reach
+                    // from a to the chunk box, at least 2^-80: if every box point
+                    // q has |a.cross(b, q)| > M in exact arithmetic, the
+                    // reference's f32 a.cross(b, q) has one nonzero sign on the
+                    // whole chunk (rounding <= 3 2^-24 Q |b - a|_1), and its
+                    // collinear branch (c.cross(a, d) == c.cross(b, d) == 0)
+                    // needs a and b within rounding of line cd, which puts c
+                    // within 2^-20.7 Q (|b - a|_1 + 2Q) of line ab:
+                    // segment_intersect is false for every segment of the chunk.
+                    // The f32 evaluation below errs by a few 2^-24 Q |b - a|_1,
+                    // covered by M.  (A chunk with a segment of length 0 has an
+                    // infinite box and is never pruned.)
+                    bool need[kRefScreen];
 #pragma unroll
-                    for (int e = 0; e < E; e++) {
-                        const float dxc = bb.x - px[e], dyc = bb.y - py[e];
-                        const float l = bax[e] * dyc;
-                        const float r = bay[e] * dxc;
-                        const float oc = l - r;                                            // at the box centre
-                        const float rr = fmaf(fabsf(bay[e]), bb.z, fabsf(bax[e]) * bb.w);  // its spread
-                        const float q = (fabsf(dxc) + fabsf(dyc)) + (bb.z + bb.w);
-                        const float sab = fabsf(bax[e]) + fabsf(bay[e]);
-                        const float m19 = fmaxf(q * fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, at least 2^-80
-                        const bool prune = (fabsf(oc) - rr) * 0x1p19f > m19;
-                        need |= ev[e] && !prune;
+                    for (int i = 0; i < (int)kRefScreen; i++) {
+                        const float4 bb = cbox[g + i];
+                        need[i] = false;
+#pragma unroll
+                        for (int e = 0; e < E; e++) {
+                            const float dxc = bb.x - px[e], dyc = bb.y - py[e];
+                            const float l = bax[e] * dyc;
+                            const float r = bay[e] * dxc;
+                            const float oc = l - r;                                            // at the box centre
+                            const float rr = fmaf(fabsf(bay[e]), bb.z, fabsf(bax[e]) * bb.w);  // its spread
+                            const float q = (fabsf(dxc) + fabsf(dyc)) + (bb.z + bb.w);
+                            const float sab = fabsf(bax[e]) + fabsf(bay[e]);
+                            const float m19 = fmaxf(q * fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, at least 2^-80
+                            const bool prune = (fabsf(oc) - rr) * 0x1p19f > m19;
+                            need[i] = need[i] || (ev[e] && !prune);
+                        }
                     }
-                    const uint32_t nstep = min(kRefChunk, L - c * kRefChunk);
-                    const unsigned long long bm = __ballot(need);
-                    if (bm) {
-                        // append this chunk for every lane that needs it
-                        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-                            (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-                        if (need)
-                            ring[(qhead + qn + below) & (kRefQueue - 1)] =
-                                make_uint2(v0 + c * kRefChunk, (uint32_t)lane | (nstep << 8));
-                        qn += (uint32_t)__popcll(bm);
+#pragma unroll
+                    for (int i = 0; i < (int)kRefScreen; i++) {
+                        const uint32_t c = g + i;
+                        if (c >= nch) break;
+                        const uint32_t nstep = min(kRefChunk, L - c * kRefChunk);
+                        const unsigned long long bm = __ballot(need[i]);
+                        if (bm) {
+                            // append this chunk for every lane that needs it
+                            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                            if (need[i])
+                                ring[(qhead + qn + below) & (kRefQueue - 1)] =
+                                    make_uint2(v0 + c * kRefChunk, (uint32_t)lane | (nstep << 8));
+                            qn += (uint32_t)__popcll(bm);
+                        }
+                        if (!need[i]) pruned += (unsigned long long)nvalid * nstep;
                     }
-                    if (!need) pruned += (unsigned long long)nvalid * nstep;
-                    if (qn >= 64) {
+                    while (qn >= 64 && !stop) {
                         __builtin_amdgcn_wave_barrier();
                         drain(64);
-                    } else if ((++screened & 31u) == 0 &&
-                               __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        stop = true;
                     }
+                    if (!stop && (++screened & 7u) == 0 &&
+                        __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        stop = true;
                 }
             }
             if (!stop && qn) {
@@ -564,7 +568,7 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
             }
             rbytes += (unsigned long long)nvalid * 8;
         };
-        with_width<1, W>(ew, run);
+        run(std::integral_constant<int, W>());
     }
     for (int o = 32; o > 0; o >>= 1) {
         tests += __shfl_xor(tests, o);
@@ -741,7 +745,7 @@ template __global__ void k_tiles<2>(const PixPair *, const uint32_t *, uint32_t,
 template __global__ void k_tiles<4>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 #define FARM_REFINE_INST(EE)                                                                                   \
     template __global__ void k_refine<EE>(DevSet, const uint32_t *, const uint32_t *, const PixPair *,          \
-                                          const uint32_t *, const uint32_t *, uint32_t, uint32_t *, uint32_t *, \
+                                          const uint32_t *, const uint2 *, uint32_t, uint32_t *, uint32_t *,    \
                                           unsigned long long *, uint32_t *);
 FARM_REFINE_INST(1)
 FARM_REFINE_INST(2)

@@ -2,7 +2,8 @@
 
 k_refine (farm_amd/csrc/hip/intersect_kernels.hip) skips a source edge (a, b)
 against a chunk of up to 16 target segments when the chunk's box lies on one
-side of line ab by more than M = 2^-19 Q (|b - a|_1 + 2Q).  The skip is only
+side of line ab by more than M = 2^-19 Q (|b - a|_1 + 2Q), Q the L1 reach
+from a to the box.  The skip is only
 allowed if the reference predicate (segment_intersect,
 src/include/geometry_computation.h:215-221) is false for every segment of
 the chunk.  These tests restate the box (k_chunk_box) and the test in f32
@@ -45,13 +46,13 @@ def pruned(a, b, box) -> bool:
     ax, ay = f32(a[0]), f32(a[1])
     bax, bay = f32(f32(b[0]) - ax), f32(f32(b[1]) - ay)
     with np.errstate(all="ignore"):
+        sab19 = f32(f32(2.0 ** -19) * f32(abs(bax) + abs(bay)))
         dxc, dyc = f32(cx - ax), f32(cy - ay)
-        oc = f32(f32(bax * dyc) - f32(bay * dxc))
+        oc = fmaf(bax, dyc, -f32(bay * dxc))
         rr = fmaf(abs(bay), hx, f32(abs(bax) * hy))
         q = f32(f32(abs(dxc) + abs(dyc)) + f32(hx + hy))
-        sab = f32(abs(bax) + abs(bay))
-        m19 = max(f32(q * fmaf(2.0, q, sab)), f32(2.0 ** -61)) if not np.isnan(q) else f32(np.nan)
-        return bool(f32(f32(abs(oc) - rr) * f32(2.0 ** 19)) > m19)
+        m = max(f32(q * fmaf(q, 2.0 ** -18, sab19)), f32(2.0 ** -80)) if not np.isnan(q) else f32(np.nan)
+        return bool(f32(abs(oc) - rr) > m)
 
 
 def check(a, b, chunk: np.ndarray) -> bool:
