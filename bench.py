@@ -15,9 +15,11 @@ Also reported:
   plain     the one-pair-per-workgroup kernel on the first timed batch
   roofline  the refinement kernel: 42 FP32 ops per executed segment test
             (6 orientations x 7 ops) / its event-timed duration, against the
-            MI355X FP32 vector peak.  The screen executes fewer ops than that,
-            so roofline.valu_issue also gives the VALU lane-ops the kernel
-            issues per test (rocprofv3 SQ_INSTS_VALU) against the issue peak
+            MI355X FP32 vector peak.  The kernel's exact prune skips most
+            tests whole, so roofline.reference_equivalent also counts the
+            pruned ones (the reference's own work), and roofline.valu_issue
+            gives the VALU lane-ops the kernel issues (rocprofv3
+            SQ_INSTS_VALU, per executed test) against the issue peak
   cpu_baseline  the CPU oracle (restatement of the reference intersect_cpu
             query phase) on a bounded random sample of the same pairs,
             all host threads; the sample is also checked bit for bit
@@ -40,10 +42,11 @@ HBM_PEAK_GBS = 8000.0
 OPS_PER_TEST = 42
 
 
-def measured_bytes_per_test() -> float | None:
-    """HBM bytes per executed segment test of k_refine on this workload, from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled
-    by the copy-kernel calibration)."""
+def measured_bytes_per_test(tile: int) -> float | None:
+    """HBM bytes per executed segment test of k_refine<tile> on this workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already
+    scaled by the copy-kernel calibration); None for a width it has no
+    figures for."""
     import re
 
     path = os.path.join(ROOT, "profiles", "k_refine_hbm_traffic_complex_x_lakes.md")
@@ -51,8 +54,8 @@ def measured_bytes_per_test() -> float | None:
         text = open(path).read()
     except OSError:
         return None
-    f = re.search(r"fetch_bytes_per_segment_test: ([0-9.eE+-]+)", text)
-    w = re.search(r"write_bytes_per_segment_test: ([0-9.eE+-]+)", text)
+    f = re.search(rf"fetch_bytes_per_segment_test W{tile}: ([0-9.eE+-]+)", text)
+    w = re.search(rf"write_bytes_per_segment_test W{tile}: ([0-9.eE+-]+)", text)
     return float(f.group(1)) + float(w.group(1)) if f and w else None
 
 
@@ -61,16 +64,17 @@ def measured_bytes_per_test() -> float | None:
 VALU_ISSUE_PEAK_TOPS = 256 * 4 * 32 * 2.4e9 / 1e12
 
 
-def measured_valu_ops_per_test() -> float | None:
-    """VALU lane-ops k_refine issues per executed segment test on this
-    workload, from the committed rocprofv3 SQ_INSTS_VALU summary."""
+def measured_valu_ops_per_test(tile: int) -> float | None:
+    """VALU lane-ops k_refine<tile> issues per executed segment test on this
+    workload, from the committed rocprofv3 SQ_INSTS_VALU summary; None for a
+    width it has no figure for."""
     import re
 
     try:
         text = open(os.path.join(ROOT, "profiles", "k_refine_valu_complex_x_lakes.md")).read()
     except OSError:
         return None
-    m = re.search(r"valu_lane_ops_per_segment_test: ([0-9.eE+-]+)", text)
+    m = re.search(rf"valu_lane_ops_per_segment_test W{tile}: ([0-9.eE+-]+)", text)
     return float(m.group(1)) if m else None
 
 
@@ -553,7 +557,7 @@ def main() -> int:
     if not tile:
         # autotune the refinement tile width on part of the warmup window
         t = time.time()
-        tile = dev.autotune(0, int(min(cuts[1], 1_000_000)))
+        tile = dev.autotune(0, int(min(cuts[1], 3_000_000)))
         log(f"autotuned tile width {tile} in {time.time() - t:.1f}s")
     args.tile = tile
 
@@ -597,19 +601,21 @@ def main() -> int:
                            order=order).stats
         per_step.append(st)
         log(f"step {s - args.warmup}: {cuts[s + 1] - cuts[s]} pairs, filter {st['filter_ms']:.1f} ms, "
-            f"refine {st['refine_ms']:.1f} ms, total {st['total_ms']:.1f} ms, tests {st['segment_tests']:.3e}")
+            f"refine {st['refine_ms']:.1f} ms, total {st['total_ms']:.1f} ms, tests {st['segment_tests']:.3e} "
+            f"(pruned {st['pruned_tests']:.3e})")
     wall = time.time() - t0
     timed_pairs = int(cuts[nb] - cuts[args.warmup])
     value = timed_pairs / wall
     tests = sum(s["segment_tests"] for s in per_step)
+    pruned = sum(s["pruned_tests"] for s in per_step)
     refine_ms = sum(s["refine_ms"] for s in per_step)
     filter_ms = sum(s["filter_ms"] for s in per_step)
     achieved = tests * OPS_PER_TEST / (refine_ms / 1000.0) / 1e12 if refine_ms > 0 else 0.0
     found = sum(s["found"] for s in per_step)
     full_size = args.scale == 1.0 and args.vpr == 256
-    bpt = measured_bytes_per_test() if full_size else None
+    bpt = measured_bytes_per_test(args.tile) if full_size else None
     traffic = tests / args.steps * bpt if bpt else None
-    opt = measured_valu_ops_per_test() if full_size else None
+    opt = measured_valu_ops_per_test(args.tile) if full_size else None
     issued = tests * opt / (refine_ms / 1000.0) / 1e12 if opt and refine_ms > 0 else None
 
     line = {
@@ -649,6 +655,11 @@ def main() -> int:
             if traffic else None,
             "algorithmic_bytes": sum(s["refine_bytes"] for s in per_step) / args.steps,
             "ops_per_segment_test": OPS_PER_TEST,
+            "pruned_tests_per_step": pruned / args.steps,
+            "reference_equivalent": {
+                "tests_per_s": (tests + pruned) / (refine_ms / 1000.0) if refine_ms > 0 else 0.0,
+                "achieved": (tests + pruned) * OPS_PER_TEST / (refine_ms / 1000.0) / 1e12 if refine_ms > 0 else 0.0,
+                "unit": "TFLOP/s", "note": "42 ops per test the reference would run (executed + pruned)"},
             "valu_issue": {"lane_ops_per_test": opt, "achieved": issued, "peak": VALU_ISSUE_PEAK_TOPS,
                            "unit": "tera lane-ops/s", "frac": issued / VALU_ISSUE_PEAK_TOPS,
                            "source": "profiles/k_refine_valu_complex_x_lakes.md (SQ_INSTS_VALU x 64 / tests)"}
