@@ -330,8 +330,8 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
         const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
         // the tile's body, invoked once below (as a lambda it is allocated
-        // to fewer registers than as a plain block: 167 against 176 VGPRs at
-        // W = 8, three waves per SIMD instead of two)
+        // to fewer registers than as a plain block: 128 against 155 VGPRs at
+        // W = 8, four waves per SIMD instead of three)
         auto run = [&](auto width_tag) {
             constexpr int E = decltype(width_tag)::value;
             // source vertices of this lane: slots first .. first + E
@@ -357,13 +357,10 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                     if (m < E) ev[m] = inside && o < sq.y;
                 }
             }
-            float bax[E], bay[E];
             int nvalid = 0;
             uint32_t evb = 0;
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                bax[e] = px[e + 1] - px[e];
-                bay[e] = py[e + 1] - py[e];
                 nvalid += ev[e] ? 1 : 0;
                 evb |= (ev[e] ? 1u : 0u) << e;
             }
@@ -503,6 +500,19 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                     // The f32 evaluation below errs by a few 2^-24 Q |b - a|_1,
                     // covered by M.  (A chunk with a segment of length 0 has an
                     // infinite box and is never pruned.)
+                    // this lane's slots again, from LDS (not kept in registers
+                    // across the evaluation of queued items)
+                    float spx[E + 1], spy[E + 1], sbx[E], sby[E];
+#pragma unroll
+                    for (int v = 0; v <= E; v++) {
+                        spx[v] = src[lane][v];
+                        spy[v] = src[lane][E + 1 + v];
+                    }
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        sbx[e] = spx[e + 1] - spx[e];
+                        sby[e] = spy[e + 1] - spy[e];
+                    }
                     bool need[kRefScreen];
 #pragma unroll
                     for (int i = 0; i < (int)kRefScreen; i++) {
@@ -510,16 +520,16 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                         need[i] = false;
 #pragma unroll
                         for (int e = 0; e < E; e++) {
-                            const float dxc = bb.x - px[e], dyc = bb.y - py[e];
-                            const float l = bax[e] * dyc;
-                            const float r = bay[e] * dxc;
+                            const float dxc = bb.x - spx[e], dyc = bb.y - spy[e];
+                            const float l = sbx[e] * dyc;
+                            const float r = sby[e] * dxc;
                             const float oc = l - r;                                            // at the box centre
-                            const float rr = fmaf(fabsf(bay[e]), bb.z, fabsf(bax[e]) * bb.w);  // its spread
+                            const float rr = fmaf(fabsf(sby[e]), bb.z, fabsf(sbx[e]) * bb.w);  // its spread
                             const float q = (fabsf(dxc) + fabsf(dyc)) + (bb.z + bb.w);
-                            const float sab = fabsf(bax[e]) + fabsf(bay[e]);
+                            const float sab = fabsf(sbx[e]) + fabsf(sby[e]);
                             const float m19 = fmaxf(q * fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, at least 2^-80
                             const bool prune = (fabsf(oc) - rr) * 0x1p19f > m19;
-                            need[i] = need[i] || (ev[e] && !prune);
+                            need[i] = need[i] || (((evb >> e) & 1u) && !prune);
                         }
                     }
 #pragma unroll
