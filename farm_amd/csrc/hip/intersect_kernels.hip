@@ -521,13 +521,11 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
 #pragma unroll
                         for (int e = 0; e < E; e++) {
                             const float dxc = bb.x - spx[e], dyc = bb.y - spy[e];
-                            const float l = sbx[e] * dyc;
-                            const float r = sby[e] * dxc;
-                            const float oc = l - r;                                            // at the box centre
+                            const float oc = fmaf(sbx[e], dyc, -(sby[e] * dxc));               // at the box centre
                             const float rr = fmaf(fabsf(sby[e]), bb.z, fabsf(sbx[e]) * bb.w);  // its spread
                             const float q = (fabsf(dxc) + fabsf(dyc)) + (bb.z + bb.w);
                             const float sab = fabsf(sbx[e]) + fabsf(sby[e]);
-                            const float m19 = fmaxf(q * fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, at least 2^-80
+                            const float m19 = fmaf(q, fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, plus 2^-61
                             const bool prune = (fabsf(oc) - rr) * 0x1p19f > m19;
                             need[i] = need[i] || (((evb >> e) & 1u) && !prune);
                         }

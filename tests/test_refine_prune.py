@@ -46,13 +46,13 @@ def pruned(a, b, box) -> bool:
     ax, ay = f32(a[0]), f32(a[1])
     bax, bay = f32(f32(b[0]) - ax), f32(f32(b[1]) - ay)
     with np.errstate(all="ignore"):
-        sab19 = f32(f32(2.0 ** -19) * f32(abs(bax) + abs(bay)))
         dxc, dyc = f32(cx - ax), f32(cy - ay)
         oc = fmaf(bax, dyc, -f32(bay * dxc))
         rr = fmaf(abs(bay), hx, f32(abs(bax) * hy))
         q = f32(f32(abs(dxc) + abs(dyc)) + f32(hx + hy))
-        m = max(f32(q * fmaf(q, 2.0 ** -18, sab19)), f32(2.0 ** -80)) if not np.isnan(q) else f32(np.nan)
-        return bool(f32(abs(oc) - rr) > m)
+        sab = f32(abs(bax) + abs(bay))
+        m19 = fmaf(q, fmaf(2.0, q, sab), 2.0 ** -61)  # 2^19 M, plus 2^-61
+        return bool(f32(f32(abs(oc) - rr) * f32(2.0 ** 19)) > m19)
 
 
 def check(a, b, chunk: np.ndarray) -> bool:
