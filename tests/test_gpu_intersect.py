@@ -115,3 +115,19 @@ def test_degenerate_shapes(device):
         w = Workload(rings_from_list(a), rings_from_list(b), vpr=vpr)
         want, _ = _check(device, w, allow_ref_failures=True)
         assert (want != 2).sum() >= 5
+
+
+def test_zero_length_target_segments(device):
+    # Target rings strictly on one side of the line of a source edge, inside
+    # that edge's box, each with a segment of length 0: a repeated vertex, or
+    # (one-pixel raster) the reference's extra edge onto the repeated closing
+    # vertex.  For c == d the reference's segment_intersect takes its
+    # collinear branch and answers with the box overlap alone, so both pairs
+    # are true; a prune that skipped the chunk by the line side would miss
+    # them.
+    a = [np.array([[0, 0], [1, 0], [1, 1]], np.float32)]
+    b = [np.array([[0.45, 0.5], [0.45, 0.5], [0.40, 0.55], [0.35, 0.50]], np.float32),
+         np.array([[0.45, 0.5], [0.40, 0.55], [0.35, 0.50]], np.float32)]
+    w = Workload(rings_from_list(a), rings_from_list(b), pairs=([0, 0], [0, 1]))
+    want, _ = _check(device, w)
+    assert list(want) == [1, 1]
