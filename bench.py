@@ -525,7 +525,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the pair list would be sharded "
-                                   "across GPUs by predicted cost (see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "across GPUs by predicted cost, one process per GPU", "n_gpus": args.gpus}))
         return 2
     if args.workload == "within":
         print(json.dumps(bench_within(args)), flush=True)

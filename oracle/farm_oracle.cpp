@@ -13,7 +13,7 @@
  * for Point arithmetic, binary64 for boxes, steps and distances, no FMA.
  *
  * How each function is pinned (the reference has no tests or golden data and
- * cannot be built here: boost is absent):
+ * cannot be built here: it needs boost, which is absent):
  *   double_to_int, get_offset_*, retrieve_pixels  -> closed forms / exhaustive
  *       edge values (tests/test_oracle_primitives.py)
  *   segment_intersect -> exact rational orientation on integer grids where
