@@ -1213,7 +1213,7 @@ static bool within(const Poly &S, const Poly &T, double d, QStats &qs, double *m
 }
 
 /* ---------------------------------------------------------------------- */
-/* Intersection (config 3)                                                 */
+/* Intersection                                                            */
 /* ---------------------------------------------------------------------- */
 /* Intersection record (Point.h:132-139) */
 struct IxRec {

@@ -49,6 +49,7 @@ class Workload:
         dev.upload_pairs(self.pa, self.pb + len(self.A))
 
 
-def config1(seed: int, n: int = 2000) -> Workload:
+def star_pairs(seed: int, n: int = 2000) -> Workload:
+    """n x n seeded star polygons (50..500 vertices) and their candidate pairs."""
     a, b = datagen.star_pair_set(seed, n, n)
     return Workload(a, b)

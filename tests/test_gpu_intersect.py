@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from farm_amd import capi
-from tests.helpers import Workload, config1, rings_from_list
+from tests.helpers import Workload, rings_from_list, star_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -39,22 +39,22 @@ def _check(dev, w: Workload, modes=MODES, allow_ref_failures=False):
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_config1_full(device, seed):
-    print(f"config1 seed={seed}", flush=True)
-    w = config1(seed)
+def test_star_pairs_2k_full(device, seed):
+    print(f"star pairs 2000 x 2000 seed={seed}", flush=True)
+    w = star_pairs(seed)
     want, _ = _check(device, w)
     assert 0 < want.sum() < len(want)
 
 
-def test_config1_many_seeds(device):
+def test_star_pairs_many_seeds(device):
     # 20 smaller seeded instances, every kernel variant
     for seed in range(100, 120):
-        print(f"config1-small seed={seed}", flush=True)
-        _check(device, config1(seed, n=400))
+        print(f"star pairs 400 x 400 seed={seed}", flush=True)
+        _check(device, star_pairs(seed, n=400))
 
 
 def test_heuristic_mode_matches_oracle(device):
-    w = config1(5)
+    w = star_pairs(5)
     want, _ = w.oracle(1)
     w.upload(device)
     r = device.intersect(mode=capi.MODE_APPROX_HEURISTIC)
@@ -62,19 +62,19 @@ def test_heuristic_mode_matches_oracle(device):
 
 
 def test_autotuned_tile_is_exact(device, tmp_path):
-    w = config1(8, n=800)
+    w = star_pairs(8, n=800)
     want, _ = w.oracle(0)
     w.upload(device)
     cache = str(tmp_path / "tune.json")
-    tile = device.autotune(cache=cache, key="config1-800")
+    tile = device.autotune(cache=cache, key="star-pairs-800")
     assert tile in (1, 2, 4, 8)
-    assert device.autotune(cache=cache, key="config1-800") == tile  # served from the cache
+    assert device.autotune(cache=cache, key="star-pairs-800") == tile  # served from the cache
     r = device.intersect()  # tile_a = 0: the tuned width
     assert np.array_equal(r.bits, want)
 
 
 def test_batch_windows_concatenate(device):
-    w = config1(9, n=600)
+    w = star_pairs(9, n=600)
     want, _ = w.oracle(0)
     w.upload(device)
     n = len(w.pa)
