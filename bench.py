@@ -594,11 +594,17 @@ def main() -> int:
         st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False,
                            order=order).stats
         log(f"warmup step {s}: {cuts[s + 1] - cuts[s]} pairs, {st['total_ms']:.1f} ms")
+    # the timed steps leave their bits in device memory (read back after the
+    # timed region): they are the ones the parity sample checks
+    from farm_amd.hipmem import DeviceBuffer
+
+    t_lo = int(cuts[args.warmup])
+    tbuf = DeviceBuffer(int(cuts[nb]) - t_lo)
     per_step = []
     t0 = time.time()
     for s in range(args.warmup, nb):
-        st = dev.intersect(int(cuts[s]), int(cuts[s + 1]), capi.MODE_EXACT_BATCHED, args.tile, want_bits=False,
-                           order=order).stats
+        st = dev.intersect_dev(tbuf.ptr + int(cuts[s]) - t_lo, int(cuts[s]), int(cuts[s + 1]),
+                               capi.MODE_EXACT_BATCHED, args.tile, order=order)
         per_step.append(st)
         log(f"step {s - args.warmup}: {cuts[s + 1] - cuts[s]} pairs, filter {st['filter_ms']:.1f} ms, "
             f"refine {st['refine_ms']:.1f} ms, total {st['total_ms']:.1f} ms, tests {st['segment_tests']:.3e} "
@@ -606,6 +612,8 @@ def main() -> int:
     wall = time.time() - t0
     timed_pairs = int(cuts[nb] - cuts[args.warmup])
     value = timed_pairs / wall
+    timed_bits = tbuf.to_host(np.uint8, timed_pairs)
+    tbuf.free()
     tests = sum(s["segment_tests"] for s in per_step)
     pruned = sum(s["pruned_tests"] for s in per_step)
     refine_ms = sum(s["refine_ms"] for s in per_step)
@@ -689,8 +697,12 @@ def main() -> int:
                          "speedup_batched_vs_plain": plain_s / batched_first if batched_first > 0 else None}
         log(f"plain kernel on step-0 batch: {plain_s * 1000:.1f} ms ({plain_rate:.3e} pairs/s)")
 
-    # exact bits of every pair (untimed): parity sample and approximate-mode truth
+    # exact bits of every pair (untimed, cost-bound queue order): the
+    # approximate modes' truth, and a check that the timed steps' bits (the
+    # learned queue order) are the same
     bits = dev.intersect(0, len(pa), capi.MODE_EXACT_BATCHED, args.tile).bits
+    order_mismatch = int((bits[t_lo:int(cuts[nb])] != timed_bits).sum())
+    log(f"timed steps' bits vs an untimed cost-bound-order run: {order_mismatch} mismatches")
     if learned_info is not None:
         from farm_amd import learned
 
@@ -728,8 +740,8 @@ def main() -> int:
         rng = np.random.default_rng(args.seed + 7)
         res = cpu_baseline(rings, pa, pb, args.vpr, args.threads, args.cpu_seconds, rng, None)
         samp = res["sample"]
-        timed = (samp >= cuts[0]) & (samp < cuts[nb])
-        mism = int((bits[samp][timed] != res["bits"][timed]).sum())
+        timed = (samp >= t_lo) & (samp < cuts[nb])
+        mism = int((timed_bits[samp[timed] - t_lo] != res["bits"][timed]).sum())
         line["cpu_baseline"] = {
             "value": len(samp) / res["seconds"],
             "unit": "candidate pairs/s",
@@ -740,7 +752,10 @@ def main() -> int:
             "one_thread": res["one_thread"],
             "host": host_cpu(),
         }
-        line["parity_sample"] = {"pairs": int(timed.sum()), "mismatches": mism}
+        line["parity_sample"] = {"pairs": int(timed.sum()), "mismatches": mism,
+                                 "bits_from": f"the {args.steps} timed steps ({'learned' if order else 'cost-bound'} "
+                                              f"queue order, tile {args.tile})",
+                                 "timed_vs_cost_bound_order_mismatches": order_mismatch}
         log(f"cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; sample mismatches vs GPU: {mism}")
     dev.close()
     if not args.no_extra:

@@ -176,6 +176,8 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_set_pixel_pair_limit": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "farm_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "farm_intersect_costs": (
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
@@ -468,14 +470,24 @@ class Device:
         self._check(rc, "intersect")
         return IntersectResult(out[: end - begin] if want_bits else None, st.as_dict())
 
-    def intersect_dev(self, out_dev_ptr: int, begin: int = 0, end: int | None = None,
-                      mode: int = MODE_EXACT_BATCHED, tile_a: int = 0) -> dict:
+    def intersect_dev(self, out_dev_ptr: int | None, begin: int = 0, end: int | None = None,
+                      mode: int = MODE_EXACT_BATCHED, tile_a: int = 0, order: int = 0, accept: float = 0.0,
+                      reject: float | None = None) -> dict:
+        """intersect() with the 0/1 results left in device memory at
+        out_dev_ptr (end - begin bytes; None: counts only)."""
         end = self.npairs if end is None else end
-        o = Opts(mode=mode, tile_a=tile_a)
+        o = Opts(mode=mode, tile_a=tile_a, order=order, approx_threshold=accept,
+                 approx_reject=accept if reject is None else reject)
         st = Stats()
-        rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr), C.byref(st))
+        rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr or 0), C.byref(st))
         self._check(rc, "intersect_dev")
         return st.as_dict()
+
+    def set_pixel_pair_limit(self, limit: int) -> None:
+        self._check(lib.farm_set_pixel_pair_limit(self._h, int(limit)), "set_pixel_pair_limit")
+
+    def set_graphs(self, on: bool) -> None:
+        self._check(lib.farm_set_graphs(self._h, int(bool(on))), "set_graphs")
 
     def intersect_costs(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray, dict]:
         """Exact bits plus the segment tests executed per pair."""

@@ -11,26 +11,32 @@
 #include <vector>
 
 #include "handle.h"
+#include "isect_work.h"
 
 namespace farm {
-__global__ void k_filter(DevSet, const uint32_t *, const uint32_t *, uint32_t, int, int, const uint8_t *, uint32_t *,
-                         const uint32_t *, PixPair *, uint32_t *, unsigned long long *);
-__global__ void k_bucket_hist(DevSet, const uint32_t *, const uint32_t *, const PixPair *, uint32_t, const float *,
-                              uint8_t *, uint32_t *);
+__global__ void k_filter(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, int, int, const uint8_t *,
+                         uint32_t *, const uint32_t *, PixPair *, uint64_t, uint32_t *, unsigned long long *);
+__global__ void k_scan_sums(const uint32_t *, const unsigned long long *, uint64_t, uint32_t *);
+__global__ void k_scan_parts(uint32_t *, const unsigned long long *, uint64_t, uint32_t *);
+__global__ void k_scan_apply(const uint32_t *, const unsigned long long *, uint64_t, const uint32_t *, uint32_t *);
+__global__ void k_set_desc(IsxDesc *, unsigned long long, unsigned long long, uint8_t *, uint32_t *);
+__global__ void k_bucket_hist(DevSet, const uint32_t *, const IsxDesc *, const PixPair *, const unsigned long long *,
+                              uint64_t, const float *, uint8_t *, uint32_t *);
 __global__ void k_bucket_scan(uint32_t *, uint32_t *);
-__global__ void k_bucket_scatter(const uint8_t *, uint32_t, uint32_t *, uint32_t *);
+__global__ void k_bucket_scatter(const uint8_t *, const unsigned long long *, uint64_t, uint32_t *, uint32_t *);
 template <int E>
-__global__ void k_tiles(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
+__global__ void k_tiles(const PixPair *, const uint32_t *, unsigned long long *, uint64_t, uint32_t *);
 template <int E>
-__global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const PixPair *, const uint32_t *,
-                         const uint2 *, uint32_t, uint32_t *, uint32_t *, unsigned long long *, uint32_t *);
-__global__ void k_tile_map(const uint32_t *, uint32_t, uint32_t, uint2 *);
+__global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, const PixPair *,
+                         const uint32_t *, const uint2 *, uint64_t, uint64_t, uint32_t *, uint32_t *,
+                         unsigned long long *);
+__global__ void k_tile_map(const uint32_t *, const unsigned long long *, uint64_t, uint64_t, uint2 *);
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
 __global__ void k_vdiff(const float *, const float *, uint64_t, float *, float *);
 __global__ void k_chunk_count(DevSet, int64_t, uint32_t *);
 __global__ void k_chunk_box(DevSet, int64_t, const uint32_t *, float4 *);
-__global__ void k_finalize(const uint32_t *, uint32_t, uint8_t *, unsigned long long *);
+__global__ void k_finalize(const uint32_t *, const IsxDesc *, unsigned long long *);
 }  // namespace farm
 
 extern "C" int farm_mlp_predict_pairs(farm_handle h, uint64_t begin, uint64_t end, float *out);
@@ -86,48 +92,184 @@ int exclusive_scan(farm_handle h, const T *in, T *out, uint64_t n) {
     return FARM_OK;
 }
 
-// tiles per ordered pixel pair, their scan, and the persistent refinement
-// with E source edges per lane (events ev[2]..ev[3] bracket the refinement)
-template <int E>
-int launch_tiles_refine(farm_handle h, const DevSet &S, const uint32_t *pa, const uint32_t *pb, uint32_t npp_total,
-                        unsigned long long *ctr, uint32_t *pair_tests) {
-    hipLaunchKernelGGL(k_tiles<E>, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
-                       (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order), npp_total,
-                       P<uint32_t>(h->tiles), ctr);
+// Exclusive scan on the device of n = bounded_count(nsrc, cap) values,
+// out[n] = total (no host round trip, no temporaries beyond W.part).
+int scan_dev(farm_handle h, const IsxWork &W, const uint32_t *in, const unsigned long long *nsrc, uint64_t cap,
+             uint32_t *out) {
+    hipLaunchKernelGGL(k_scan_sums, dim3(kScanParts), dim3(256), 0, h->stream, in, nsrc, cap, W.part);
     HIPCHK(h, hipGetLastError());
-    int rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->tiles), P<uint32_t>(h->tile_off), npp_total + 1);
-    if (rc) return rc;
-    // the tile total (32-bit offsets: checked against the 64-bit count first)
-    unsigned long long ntiles64 = 0;
-    uint32_t ntiles = 0;
-    HIPCHK(h, hipMemcpyAsync(&ntiles64, ctr + 7, 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(&ntiles, P<uint32_t>(h->tile_off) + npp_total, 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (ntiles64 > 0xffffffffull)
-        return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed 32-bit offsets; use smaller batches", ntiles64);
-    if ((rc = ensure(h, h->tile_map, 8ull * ntiles + 8))) return rc;
-    if (npp_total) {
-        hipLaunchKernelGGL(k_tile_map, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
-                           (const uint32_t *)P<uint32_t>(h->tile_off), npp_total, ntiles, P<uint2>(h->tile_map));
-        HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanParts), 0, h->stream, W.part, nsrc, cap, out);
+    HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_scan_apply, dim3(kScanParts), dim3(256), 0, h->stream, in, nsrc, cap,
+                       (const uint32_t *)W.part, out);
+    HIPCHK(h, hipGetLastError());
+    return FARM_OK;
+}
+
+// What one launch sequence of the batched intersect depends on (the key of
+// its captured graph).  Everything else -- the pair window, the output and
+// cost-label pointers -- is read from the workspace's IsxDesc.
+struct IsxCfg {
+    int32_t approx;           // reference -a: filter only
+    int32_t tile;             // source edges per lane of the refinement
+    const float *pred_cost;   // learned queue order: the cost head's predictions
+    const uint8_t *skip;      // pairs decided before the filter (learned -a)
+    const void *ws;           // workspace base and capacities
+    uint64_t cap_pairs, cap_pp, cap_tiles;
+    uint64_t set_gen;         // device arrays of the uploaded set and pairs
+    const uint32_t *pa, *pb;  // the pair list
+    int32_t grid;             // blocks of the grid-stride kernels
+    int32_t pad;
+};
+static_assert(sizeof(IsxCfg) <= sizeof(((farm_handle_s::IsxGraph *)nullptr)->key), "graph key too large");
+
+// The batched pipeline up to the refinement: filter count pass, scan, emit
+// pass, cost order, tiles, tile map (exact), or the filter alone (-a).
+int isx_front(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
+    int rc;
+    const unsigned g = (unsigned)c.grid;
+    if ((rc = fill_u32(h, W.ctr, 2 * kCtrSlots, 0))) return rc;
+    hipLaunchKernelGGL(k_filter, dim3(g), dim3(256), 0, h->stream, S, c.pa, c.pb, (const IsxDesc *)W.desc, 0,
+                       c.approx, c.skip, W.npp, (const uint32_t *)nullptr, (PixPair *)nullptr, W.cap_pp, W.result,
+                       W.ctr);
+    HIPCHK(h, hipGetLastError());
+    if (c.approx) return FARM_OK;
+    if ((rc = scan_dev(h, W, W.npp, &W.desc->n, W.cap_pairs, W.pp_off))) return rc;
+    hipLaunchKernelGGL(k_filter, dim3(g), dim3(256), 0, h->stream, S, c.pa, c.pb, (const IsxDesc *)W.desc, 1, 0,
+                       c.skip, W.npp, (const uint32_t *)W.pp_off, W.pps, W.cap_pp, W.result, W.ctr);
+    HIPCHK(h, hipGetLastError());
+    if ((rc = fill_u32(h, W.hist, 64, 0))) return rc;
+    hipLaunchKernelGGL(k_bucket_hist, dim3(g), dim3(256), 0, h->stream, S, c.pb, (const IsxDesc *)W.desc,
+                       (const PixPair *)W.pps, (const unsigned long long *)W.ctr, W.cap_pp, c.pred_cost, W.bucket,
+                       W.hist);
+    HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, h->stream, W.hist, W.cursor);
+    HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(g), dim3(256), 0, h->stream, (const uint8_t *)W.bucket,
+                       (const unsigned long long *)W.ctr, W.cap_pp, W.cursor, W.order);
+    HIPCHK(h, hipGetLastError());
+    switch (c.tile) {
+        case 1:
+            hipLaunchKernelGGL(k_tiles<1>, dim3(g), dim3(256), 0, h->stream, (const PixPair *)W.pps,
+                               (const uint32_t *)W.order, W.ctr, W.cap_pp, W.tiles);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_tiles<2>, dim3(g), dim3(256), 0, h->stream, (const PixPair *)W.pps,
+                               (const uint32_t *)W.order, W.ctr, W.cap_pp, W.tiles);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_tiles<4>, dim3(g), dim3(256), 0, h->stream, (const PixPair *)W.pps,
+                               (const uint32_t *)W.order, W.ctr, W.cap_pp, W.tiles);
+            break;
+        default:
+            hipLaunchKernelGGL(k_tiles<8>, dim3(g), dim3(256), 0, h->stream, (const PixPair *)W.pps,
+                               (const uint32_t *)W.order, W.ctr, W.cap_pp, W.tiles);
+            break;
     }
-    if ((rc = fill_u32(h, h->cursor.p, 1, 0))) return rc;
-    const unsigned grid = (unsigned)(h->num_cus * 8);
-    HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-    hipLaunchKernelGGL(k_refine<E>, dim3(grid), dim3(256), 0, h->stream, S, pa, pb,
-                       (const PixPair *)P<PixPair>(h->pps), (const uint32_t *)P<uint32_t>(h->order),
-                       (const uint2 *)P<uint2>(h->tile_map), ntiles, P<uint32_t>(h->cursor),
-                       P<uint32_t>(h->result), ctr, pair_tests);
     HIPCHK(h, hipGetLastError());
+    if ((rc = scan_dev(h, W, W.tiles, &W.ctr[kCtrPixPairs], W.cap_pp, W.tile_off))) return rc;
+    hipLaunchKernelGGL(k_tile_map, dim3(g), dim3(256), 0, h->stream, (const uint32_t *)W.tile_off,
+                       (const unsigned long long *)W.ctr, W.cap_pp, W.cap_tiles, W.tile_map);
+    HIPCHK(h, hipGetLastError());
+    return fill_u32(h, W.cursor + 63, 1, 0);  // the refinement's tile cursor
+}
+
+// The persistent refinement with the tile width of the call.
+int isx_refine(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
+    const dim3 grid((unsigned)(h->num_cus * 8)), blk(256);
+    const IsxDesc *d = W.desc;
+    const PixPair *pps = W.pps;
+    const uint32_t *ord = W.order;
+    const uint2 *map = W.tile_map;
+    switch (c.tile) {
+        case 1:
+            hipLaunchKernelGGL(k_refine<1>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_refine<2>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_refine<4>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+            break;
+        default:
+            hipLaunchKernelGGL(k_refine<8>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+            break;
+    }
+    HIPCHK(h, hipGetLastError());
+    return FARM_OK;
+}
+
+void drop_graphs(farm_handle h) {
+    for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
+// The front of the pipeline as one graph launch: captured on the first
+// call with this configuration, replayed afterwards.
+int isx_front_graph(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
+    unsigned char key[sizeof(((farm_handle_s::IsxGraph *)nullptr)->key)];
+    memset(key, 0, sizeof key);
+    memcpy(key, &c, sizeof c);
+    for (auto &g : h->graphs)
+        if (!memcmp(g.key, key, sizeof key)) {
+            HIPCHK(h, hipGraphLaunch(g.exec, h->stream));
+            return FARM_OK;
+        }
+    if (h->graphs.size() >= 16) drop_graphs(h);
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = isx_front(h, S, W, c);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(h, FARM_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    farm_handle_s::IsxGraph g;
+    memcpy(g.key, key, sizeof key);
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(h, FARM_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+    h->graphs.push_back(g);
+    HIPCHK(h, hipGraphLaunch(g.exec, h->stream));
+    return FARM_OK;
+}
+
+// A workspace holding at least these capacities (the handle's own, grown:
+// every captured graph refers to the old block and is dropped with it).
+int isx_reserve(farm_handle h, uint64_t pairs, IsxWork *W) {
+    const uint64_t cp = std::max<uint64_t>(h->isx_cap_pairs, std::max<uint64_t>(pairs, 1024));
+    const uint64_t cq = std::min<uint64_t>(0xffffffffull,
+                                           std::max<uint64_t>(h->isx_cap_pp, std::max(h->isx_want_pp, 4 * cp)));
+    const uint64_t ct = std::min<uint64_t>(0xffffffffull,
+                                           std::max<uint64_t>(h->isx_cap_tiles, std::max(h->isx_want_tiles, cq)));
+    if (!h->isx.p || cp > h->isx_cap_pairs || cq > h->isx_cap_pp || ct > h->isx_cap_tiles) {
+        drop_graphs(h);
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        free_buf(h->isx);
+        h->isx_cap_pairs = h->isx_cap_pp = h->isx_cap_tiles = 0;
+        int rc = ensure(h, h->isx, isx_bytes(cp, cq, ct));
+        if (rc) return rc;
+        h->isx_cap_pairs = cp;
+        h->isx_cap_pp = cq;
+        h->isx_cap_tiles = ct;
+    }
+    *W = isx_carve(h->isx.p, h->isx_cap_pairs, h->isx_cap_pp, h->isx_cap_tiles);
     return FARM_OK;
 }
 
 }  // namespace
 
+
 extern "C" {
 
-static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
-                         farm_stats *st, uint32_t *pair_tests = nullptr);
+enum { OUT_NONE = 0, OUT_DEV = 1, OUT_HOST = 2 };
+static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, int out_kind,
+                         uint8_t *out, farm_stats *st, uint32_t *pair_tests = nullptr);
 
 const char *farm_version(void) { return "farm-mi355x 0.1 (gfx950)"; }
 
@@ -170,9 +312,9 @@ int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile) {
         o.mode = FARM_MODE_EXACT_BATCHED;
         o.tile_a = E;
         farm_stats st;
-        int rc = run_intersect(h, begin, end, &o, nullptr, &st);
+        int rc = run_intersect(h, begin, end, &o, OUT_NONE, nullptr, &st);
         if (rc) return rc;
-        rc = run_intersect(h, begin, end, &o, nullptr, &st);  // second run: warm caches
+        rc = run_intersect(h, begin, end, &o, OUT_NONE, nullptr, &st);  // second run: warm caches
         if (rc) return rc;
         if (st.total_ms < best_ms) {
             best_ms = st.total_ms;
@@ -228,10 +370,10 @@ int farm_close(farm_handle h) {
     if (!h) return FARM_EINVAL;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->dx,       &h->dy,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
-                      &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->result, &h->npp,      &h->pp_off,
-                      &h->pps,    &h->bucket,  &h->order,  &h->tiles,  &h->tile_off, &h->tile_map, &h->ctr,    &h->hist,
-                      &h->cursor, &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
+                      &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->isx,
+                      &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
                       &h->skipbuf, &h->layers, &h->lstatus, &h->areas, &h->wctr, &h->wovf, &h->wslab,
@@ -335,6 +477,7 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     }
     h->npolys = ps->npolys;
     h->nstatus = ps->nstatus;
+    h->set_gen++;
     if (ps->npolys > 0) {
         hipLaunchKernelGGL(k_pixedges, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h), ps->npolys,
                            P<uint32_t>(h->pixedges));
@@ -374,174 +517,149 @@ int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint6
     if ((rc = upload(h, h->pa, a, 4 * n))) return rc;
     if ((rc = upload(h, h->pb, b, 4 * n))) return rc;
     h->npairs = n;
+    h->set_gen++;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return FARM_OK;
 }
 
-static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
-                         farm_stats *st, uint32_t *pair_tests) {
+static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, int out_kind,
+                         uint8_t *out, farm_stats *st, uint32_t *pair_tests) {
     if (!h) return FARM_EINVAL;
     if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
     if (begin > end || end > h->npairs) return fail(h, FARM_EINVAL, "pair window [%llu,%llu) outside [0,%llu)",
                                                     (unsigned long long)begin, (unsigned long long)end,
                                                     (unsigned long long)h->npairs);
     const uint64_t n64 = end - begin;
-    if (n64 > 0xffffffffull) return fail(h, FARM_ECAPACITY, "batch larger than 2^32 pairs");
+    if (n64 > 0x7fffffffull) return fail(h, FARM_ECAPACITY, "batch of %llu pairs: at most 2^31 - 1 per call",
+                                         (unsigned long long)n64);
     const uint32_t n = (uint32_t)n64;
     const int mode = opts ? opts->mode : FARM_MODE_EXACT_BATCHED;
     int E = opts && opts->tile_a ? opts->tile_a : h->tuned_tile;
     if (E != 1 && E != 2 && E != 4 && E != 8) return fail(h, FARM_EINVAL, "tile_a must be 1, 2, 4 or 8");
-    HIPCHK(h, hipSetDevice(h->device));
-    int rc;
-    const uint32_t *pa = P<uint32_t>(h->pa) + begin;
-    const uint32_t *pb = P<uint32_t>(h->pb) + begin;
-    if ((rc = ensure(h, h->result, 4ull * n + 4))) return rc;
-    if ((rc = ensure(h, h->ctr, 8 * 16))) return rc;
-    if ((rc = fill_u32(h, h->ctr.p, 32, 0))) return rc;
-    DevSet S = devset(h);
-    unsigned long long *ctr = P<unsigned long long>(h->ctr);
-    HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
-    float filter_ms = 0, refine_ms = 0;
-    if (n == 0) {
-        // nothing to do
-    } else if (mode == FARM_MODE_EXACT_PLAIN) {
-        if ((rc = fill_u32(h, h->result.p, n, 0))) return rc;
-        HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-        hipLaunchKernelGGL(k_plain, dim3(n), dim3(256), 0, h->stream, S, pa, pb, n, P<uint32_t>(h->result), ctr);
-        HIPCHK(h, hipGetLastError());
-        HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-    } else if (mode == FARM_MODE_EXACT_BATCHED || mode == FARM_MODE_APPROX_HEURISTIC ||
-               mode == FARM_MODE_APPROX_LEARNED) {
-        const int approx = mode == FARM_MODE_APPROX_HEURISTIC;
-        const bool learned = mode == FARM_MODE_APPROX_LEARNED;
-        const bool learned_order = opts && opts->order == 1 && !approx;
-        const uint8_t *skip = nullptr;
-        const float *pred_cost = nullptr;
-        if (learned || learned_order) {
-            // pair features + MLP on the device (counted in the filter phase)
-            if ((rc = farm_mlp_predict_pairs(h, begin, end, nullptr))) return rc;
-            pred_cost = P<float>(h->pred) + 1;
-        }
-        if (learned) {
-            if ((rc = ensure(h, h->skipbuf, n))) return rc;
-            const float accept = opts->approx_threshold;
-            const float reject = opts->approx_reject < accept ? opts->approx_reject : accept;
-            hipLaunchKernelGGL(k_learned_decide, dim3(blocks_for(n)), dim3(256), 0, h->stream,
-                               (const int8_t *)P<int8_t>(h->decided), (const float *)P<float>(h->pred), n, accept,
-                               reject, P<uint32_t>(h->result), P<uint8_t>(h->skipbuf));
-            HIPCHK(h, hipGetLastError());
-            skip = P<uint8_t>(h->skipbuf);
-        }
-        if ((rc = ensure(h, h->npp, 4ull * (n + 1)))) return rc;
-        if ((rc = ensure(h, h->pp_off, 4ull * (n + 1)))) return rc;
-        if ((rc = fill_u32(h, h->npp.p, n + 1, 0))) return rc;
-        if (pair_tests && (rc = fill_u32(h, pair_tests, n, 0))) return rc;
-        hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 0, approx, skip,
-                           P<uint32_t>(h->npp), (const uint32_t *)nullptr, (PixPair *)nullptr,
-                           P<uint32_t>(h->result), ctr);
-        HIPCHK(h, hipGetLastError());
-        uint32_t npp_total = 0;
-        if (!approx) {
-            if ((rc = exclusive_scan<uint32_t>(h, P<uint32_t>(h->npp), P<uint32_t>(h->pp_off), n + 1))) return rc;
-            HIPCHK(h, hipMemcpyAsync(&npp_total, P<uint32_t>(h->pp_off) + n, 4, hipMemcpyDeviceToHost, h->stream));
-            HIPCHK(h, hipStreamSynchronize(h->stream));
-        }
-        HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-        if (npp_total > 0) {
-            if ((rc = ensure(h, h->pps, sizeof(PixPair) * (size_t)npp_total))) return rc;
-            if ((rc = ensure(h, h->bucket, npp_total))) return rc;
-            if ((rc = ensure(h, h->order, 4ull * npp_total))) return rc;
-            if ((rc = ensure(h, h->tiles, 4ull * (npp_total + 1)))) return rc;
-            if ((rc = ensure(h, h->tile_off, 4ull * (npp_total + 1)))) return rc;
-            if ((rc = ensure(h, h->hist, 4 * 64))) return rc;
-            if ((rc = ensure(h, h->cursor, 4 * 64))) return rc;
-            hipLaunchKernelGGL(k_filter, dim3(blocks_for(n)), dim3(256), 0, h->stream, S, pa, pb, n, 1, 0, skip,
-                               P<uint32_t>(h->npp), (const uint32_t *)P<uint32_t>(h->pp_off), P<PixPair>(h->pps),
-                               P<uint32_t>(h->result), ctr);
-            HIPCHK(h, hipGetLastError());
-            if ((rc = fill_u32(h, h->hist.p, 64, 0))) return rc;
-            hipLaunchKernelGGL(k_bucket_hist, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream, S, pa, pb,
-                               (const PixPair *)P<PixPair>(h->pps), npp_total,
-                               learned_order ? pred_cost : (const float *)nullptr, P<uint8_t>(h->bucket),
-                               P<uint32_t>(h->hist));
-            HIPCHK(h, hipGetLastError());
-            hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(64), 0, h->stream, P<uint32_t>(h->hist),
-                               P<uint32_t>(h->cursor));
-            HIPCHK(h, hipGetLastError());
-            hipLaunchKernelGGL(k_bucket_scatter, dim3(blocks_for(npp_total)), dim3(256), 0, h->stream,
-                               (const uint8_t *)P<uint8_t>(h->bucket), npp_total, P<uint32_t>(h->cursor),
-                               P<uint32_t>(h->order));
-            HIPCHK(h, hipGetLastError());
-            if ((rc = fill_u32(h, P<uint32_t>(h->tiles) + npp_total, 1, 0))) return rc;
-            switch (E) {
-                case 1: rc = launch_tiles_refine<1>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
-                case 2: rc = launch_tiles_refine<2>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
-                case 4: rc = launch_tiles_refine<4>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
-                default: rc = launch_tiles_refine<8>(h, S, pa, pb, npp_total, ctr, pair_tests); break;
-            }
-            if (rc) return rc;
-        } else {
-            HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-        }
-    } else {
+    if (mode < FARM_MODE_EXACT_PLAIN || mode > FARM_MODE_APPROX_LEARNED)
         return fail(h, FARM_EINVAL, "mode %d not supported by farm_intersect", mode);
-    }
-    if (n) {
-        HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
-        hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n)), dim3(256), 0, h->stream,
-                           (const uint32_t *)P<uint32_t>(h->result), n, out_dev, ctr);
+    const bool approx = mode == FARM_MODE_APPROX_HEURISTIC;
+    const bool learned = mode == FARM_MODE_APPROX_LEARNED;
+    const bool learned_order = opts && opts->order == 1 && mode == FARM_MODE_EXACT_BATCHED;
+    HIPCHK(h, hipSetDevice(h->device));
+    const DevSet S = devset(h);
+    unsigned long long c[kCtrSlots];
+    IsxWork W;
+    for (int attempt = 0;; attempt++) {
+        int rc;
+        if ((rc = isx_reserve(h, n, &W))) return rc;
+        uint8_t *outp = out_kind == OUT_DEV ? out : (out_kind == OUT_HOST ? W.bits : nullptr);
+        hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(64), 0, h->stream, W.desc, (unsigned long long)begin,
+                           (unsigned long long)n, outp, pair_tests);
         HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+        if (mode == FARM_MODE_EXACT_PLAIN) {
+            if ((rc = fill_u32(h, W.ctr, 2 * kCtrSlots, 0))) return rc;
+            if ((rc = fill_u32(h, W.result, n, 0))) return rc;
+            if (pair_tests && (rc = fill_u32(h, pair_tests, n, 0))) return rc;
+            HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+            HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+            if (n) {
+                hipLaunchKernelGGL(k_plain, dim3(n), dim3(256), 0, h->stream, S,
+                                   (const uint32_t *)P<uint32_t>(h->pa) + begin,
+                                   (const uint32_t *)P<uint32_t>(h->pb) + begin, n, W.result, W.ctr);
+                HIPCHK(h, hipGetLastError());
+            }
+        } else {
+            IsxCfg cfg;
+            memset(&cfg, 0, sizeof cfg);
+            cfg.approx = approx ? 1 : 0;
+            cfg.tile = approx ? 0 : E;
+            if (learned || learned_order) {
+                // pair features + MLP on the device (timed with the filter)
+                if ((rc = farm_mlp_predict_pairs(h, begin, end, nullptr))) return rc;
+                if (learned_order) cfg.pred_cost = P<float>(h->pred) + 1;
+            }
+            if (learned && n) {
+                if ((rc = ensure(h, h->skipbuf, n))) return rc;
+                const float accept = opts->approx_threshold;
+                const float reject = opts->approx_reject < accept ? opts->approx_reject : accept;
+                hipLaunchKernelGGL(k_learned_decide, dim3(blocks_for(n)), dim3(256), 0, h->stream,
+                                   (const int8_t *)P<int8_t>(h->decided), (const float *)P<float>(h->pred), n, accept,
+                                   reject, W.result, P<uint8_t>(h->skipbuf));
+                HIPCHK(h, hipGetLastError());
+                cfg.skip = P<uint8_t>(h->skipbuf);
+            }
+            cfg.ws = W.desc;
+            cfg.cap_pairs = W.cap_pairs;
+            cfg.cap_pp = W.cap_pp;
+            cfg.cap_tiles = W.cap_tiles;
+            cfg.set_gen = h->set_gen;
+            cfg.pa = P<uint32_t>(h->pa);
+            cfg.pb = P<uint32_t>(h->pb);
+            cfg.grid = h->num_cus * 16;
+            rc = h->use_graphs ? isx_front_graph(h, S, W, cfg) : isx_front(h, S, W, cfg);
+            if (rc) return rc;
+            HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+            HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+            if (!approx && (rc = isx_refine(h, S, W, cfg))) return rc;
+        }
+        HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
+        hipLaunchKernelGGL(k_finalize, dim3(h->num_cus * 4), dim3(256), 0, h->stream, (const uint32_t *)W.result,
+                           (const IsxDesc *)W.desc, W.ctr);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipMemcpyAsync(c, W.ctr, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (mode == FARM_MODE_EXACT_PLAIN || approx) break;
+        // the pixel-pair list and the tile map are bounded buffers: a window
+        // that overflows them ran no refinement and is repeated with grown
+        // buffers, or fails when it exceeds 32-bit offsets or the limit
+        const unsigned long long npp = c[kCtrPixPairs], ntiles = c[kCtrTiles];
+        if (npp > h->max_pixel_pairs || npp > 0xffffffffull)
+            return fail(h, FARM_ECAPACITY,
+                        "%llu BORDER x BORDER pixel pairs in pairs [%llu,%llu) exceed the per-call limit of %llu; "
+                        "use smaller batches",
+                        npp, (unsigned long long)begin, (unsigned long long)end,
+                        (unsigned long long)std::min<uint64_t>(h->max_pixel_pairs, 0xffffffffull));
+        if (npp > W.cap_pp) {
+            h->isx_want_pp = npp + npp / 4 + 4096;
+            h->isx_want_tiles = std::max(h->isx_want_tiles, h->isx_want_pp);
+            continue;
+        }
+        if (ntiles > 0xffffffffull)
+            return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed 32-bit offsets; use smaller batches", ntiles);
+        if (ntiles > W.cap_tiles) {
+            h->isx_want_tiles = ntiles + ntiles / 4 + 4096;
+            continue;
+        }
+        break;
     }
-    unsigned long long c[16];
-    HIPCHK(h, hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (c[7] > 0xffffffffull)
-        return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed 32-bit offsets; use smaller batches", c[7]);
     if (st) {
         memset(st, 0, sizeof *st);
         st->pairs = n;
-        st->filter_bytes = c[0];
-        st->in_hits = c[1];
-        st->segment_tests = c[2];
-        st->refine_items = c[3];
-        st->refine_bytes = c[4];
-        st->found = c[5];
-        st->pruned_tests = c[8];
-        if (n) {
-            float t01 = 0, t23 = 0, t03 = 0;
-            HIPCHK(h, hipEventElapsedTime(&t01, h->ev[0], h->ev[1]));
-            HIPCHK(h, hipEventElapsedTime(&t23, h->ev[2], h->ev[3]));
-            HIPCHK(h, hipEventElapsedTime(&t03, h->ev[0], h->ev[3]));
-            filter_ms = t01;
-            refine_ms = t23;
-            st->filter_ms = filter_ms;
-            st->refine_ms = refine_ms;
-            st->total_ms = t03;
-        }
+        st->filter_bytes = c[kCtrFilterBytes];
+        st->in_hits = c[kCtrInHits];
+        st->bb_pixel_pairs = c[kCtrPixPairs];
+        st->segment_tests = c[kCtrTests];
+        st->refine_items = c[kCtrItems];
+        st->refine_bytes = c[kCtrRefineBytes];
+        st->found = c[kCtrFound];
+        st->pruned_tests = c[kCtrPruned];
+        float t01 = 0, t23 = 0, t03 = 0;
+        HIPCHK(h, hipEventElapsedTime(&t01, h->ev[0], h->ev[1]));
+        HIPCHK(h, hipEventElapsedTime(&t23, h->ev[2], h->ev[3]));
+        HIPCHK(h, hipEventElapsedTime(&t03, h->ev[0], h->ev[3]));
+        st->filter_ms = t01;
+        st->refine_ms = t23;
+        st->total_ms = t03;
     }
+    if (out_kind == OUT_HOST && n) HIPCHK(h, hipMemcpy(out, W.bits, n, hipMemcpyDeviceToHost));
     return FARM_OK;
 }
 
 int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
                        farm_stats *st) {
-    return run_intersect(h, begin, end, opts, out_dev, st);
+    return run_intersect(h, begin, end, opts, out_dev ? OUT_DEV : OUT_NONE, out_dev, st);
 }
 
 int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out,
                    farm_stats *st) {
-    if (!h) return FARM_EINVAL;
-    if (!out) return run_intersect(h, begin, end, opts, nullptr, st);
-    const uint64_t n = end > begin ? end - begin : 0;
-    DevBuf tmp;
-    int rc = ensure(h, tmp, n ? n : 1);
-    if (rc) return rc;
-    rc = run_intersect(h, begin, end, opts, (uint8_t *)tmp.p, st);
-    if (rc == FARM_OK && n) {
-        hipError_t e = hipMemcpy(out, tmp.p, n, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(h, FARM_EDEVICE, "copy of results failed: %s", hipGetErrorString(e));
-    }
-    (void)hipFree(tmp.p);
-    return rc;
+    return run_intersect(h, begin, end, opts, out ? OUT_HOST : OUT_NONE, out, st);
 }
 
 int farm_intersect_costs(farm_handle h, uint64_t begin, uint64_t end, uint8_t *bits, uint32_t *tests,
@@ -550,19 +668,24 @@ int farm_intersect_costs(farm_handle h, uint64_t begin, uint64_t end, uint8_t *b
     const uint64_t n = end > begin ? end - begin : 0;
     int rc = ensure(h, h->pair_tests, 4 * (n ? n : 1));
     if (rc) return rc;
-    DevBuf tmp;
-    if ((rc = ensure(h, tmp, n ? n : 1))) return rc;
     farm_opts o;
     memset(&o, 0, sizeof o);
     o.mode = FARM_MODE_EXACT_BATCHED;
-    rc = run_intersect(h, begin, end, &o, (uint8_t *)tmp.p, st, P<uint32_t>(h->pair_tests));
-    if (rc == FARM_OK && n) {
-        hipError_t e = hipMemcpy(bits, tmp.p, n, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(tests, h->pair_tests.p, 4 * n, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(h, FARM_EDEVICE, "copy of results failed: %s", hipGetErrorString(e));
-    }
-    (void)hipFree(tmp.p);
+    rc = run_intersect(h, begin, end, &o, bits ? OUT_HOST : OUT_NONE, bits, st, P<uint32_t>(h->pair_tests));
+    if (rc == FARM_OK && n && tests) HIPCHK(h, hipMemcpy(tests, h->pair_tests.p, 4 * n, hipMemcpyDeviceToHost));
     return rc;
+}
+
+int farm_set_pixel_pair_limit(farm_handle h, uint64_t max_pixel_pairs) {
+    if (!h) return FARM_EINVAL;
+    h->max_pixel_pairs = max_pixel_pairs ? std::min<uint64_t>(max_pixel_pairs, 0xffffffffull) : 0xffffffffull;
+    return FARM_OK;
+}
+
+int farm_set_graphs(farm_handle h, int on) {
+    if (!h) return FARM_EINVAL;
+    h->use_graphs = on != 0;  // captured graphs stay cached for when they are switched on again
+    return FARM_OK;
 }
 
 }  // extern "C"

@@ -65,13 +65,15 @@ struct Window {
 };
 
 // MyRaster::retrieve_pixels (MyRaster.cpp:332-346): pixel range of a box,
-// shrunk by 1e-6 on every side.
-__device__ __forceinline__ Window window_of(const farm_raster_hdr &r, double lx, double ly, double hx, double hy) {
+// shrunk by 1e-6 on every side.  shrink = 0 gives the unshrunk range the
+// reference GPU filter takes over the target MBB (query_intersect.cu:51-54).
+__device__ __forceinline__ Window window_of(const farm_raster_hdr &r, double lx, double ly, double hx, double hy,
+                                            double shrink = 1e-6) {
     Window w;
-    w.x0 = off_axis(lx + 1e-6, r.mbr[0], r.step[0], r.dimx);
-    w.y0 = off_axis(ly + 1e-6, r.mbr[1], r.step[1], r.dimy);
-    w.x1 = off_axis(hx - 1e-6, r.mbr[0], r.step[0], r.dimx);
-    w.y1 = off_axis(hy - 1e-6, r.mbr[1], r.step[1], r.dimy);
+    w.x0 = off_axis(lx + shrink, r.mbr[0], r.step[0], r.dimx);
+    w.y0 = off_axis(ly + shrink, r.mbr[1], r.step[1], r.dimy);
+    w.x1 = off_axis(hx - shrink, r.mbr[0], r.step[0], r.dimx);
+    w.y1 = off_axis(hy - shrink, r.mbr[1], r.step[1], r.dimy);
     return w;
 }
 

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "device_layout.h"
 
@@ -44,8 +45,20 @@ struct farm_handle_s {
     // pairs
     uint64_t npairs = 0;
     farm::DevBuf pa, pb;
-    // per-call scratch of the intersect driver
-    farm::DevBuf result, npp, pp_off, pps, bucket, order, tiles, tile_off, tile_map, ctr, hist, cursor, scan_tmp;
+    farm::DevBuf scan_tmp;  // hipcub temporaries (uploads, candidate pairs)
+    // per-call scratch of the intersect driver: one workspace (isect_work.h),
+    // grown when a call's pixel pairs or tiles exceed it
+    farm::DevBuf isx;
+    uint64_t isx_cap_pairs = 0, isx_cap_pp = 0, isx_cap_tiles = 0;
+    uint64_t isx_want_pp = 0, isx_want_tiles = 0;  // capacities a repeated call asked for
+    uint64_t max_pixel_pairs = 0xffffffffull;     // per-call limit (farm_set_pixel_pair_limit)
+    bool use_graphs = true;                        // capture the batch launches in a hipGraph
+    uint64_t set_gen = 0;                          // bumped by every upload that moves device arrays
+    struct IsxGraph {
+        unsigned char key[128];
+        hipGraphExec_t exec;
+    };
+    std::vector<IsxGraph> graphs;
     farm::DevBuf pair_tests;  // per-pair executed segment tests (cost labels)
     int tuned_tile = 4;       // source edges per lane of the refinement (farm_autotune)
     // learned part

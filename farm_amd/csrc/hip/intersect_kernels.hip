@@ -11,30 +11,46 @@
 #include <type_traits>
 
 #include "device_layout.h"
+#include "isect_work.h"
+#include "wave.h"
 
 namespace farm {
 
 // ------------------------------------------------------------------------
-// Raster filter: one lane per candidate pair.
+// Raster filter: one lane per candidate pair (grid-stride over the window).
 // pass 0: count BORDER x BORDER pixel pairs, decide IN hits (and the
-//         reference -a heuristic when approx != 0)
-// pass 1: emit the pixel pairs at the offsets of an exclusive scan.
+//         reference -a heuristic when approx != 0); the 64-bit total of the
+//         pixel pairs goes to ctr[kCtrPixPairs]
+// pass 1: emit the pixel pairs at the offsets of an exclusive scan; it does
+//         nothing when that total exceeds cap_pp (the buffer's capacity: the
+//         host grows it or fails the call, and no offset has wrapped).
 // The window is enumerated in full (no early exit), so the status bytes read
 // equal the algorithmic filter bytes.
-// ------------------------------------------------------------------------
 // skip (optional): pairs already decided by the caller (their result is left
 // untouched and they emit no pixel pairs).
-__global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__restrict__ pa_ids,
-                                                 const uint32_t *__restrict__ pb_ids, uint32_t npairs, int pass,
-                                                 int approx, const uint8_t *__restrict__ skip,
-                                                 uint32_t *__restrict__ npp, const uint32_t *__restrict__ pp_off,
-                                                 PixPair *__restrict__ pps, uint32_t *__restrict__ result,
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__restrict__ pa_all,
+                                                 const uint32_t *__restrict__ pb_all,
+                                                 const IsxDesc *__restrict__ desc, int pass, int approx,
+                                                 const uint8_t *__restrict__ skip, uint32_t *__restrict__ npp,
+                                                 const uint32_t *__restrict__ pp_off, PixPair *__restrict__ pps,
+                                                 uint64_t cap_pp, uint32_t *__restrict__ result,
                                                  unsigned long long *__restrict__ ctr) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long bytes = 0, inhit = 0;
-    if (k < npairs && skip && skip[k]) {
-        if (pass == 0) npp[k] = 0;
-    } else if (k < npairs) {
+    if (pass == 1 && ctr[kCtrPixPairs] > cap_pp) return;
+    const uint64_t n = desc->n;
+    const uint32_t *pa_ids = pa_all + desc->begin;
+    const uint32_t *pb_ids = pb_all + desc->begin;
+    uint32_t *tests = desc->pair_tests;
+    unsigned long long bytes = 0, inhit = 0, bb = 0;
+    for (uint64_t kk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < n;
+         kk += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = (uint32_t)kk;
+        if (pass == 0 && tests) tests[k] = 0;
+        if (skip && skip[k]) {
+            if (pass == 0) npp[k] = 0;
+            continue;
+        }
+        if (pass == 1 && npp[k] == 0) continue;  // decided by the count pass or no BORDER pair
         const uint32_t ia = pa_ids[k], ib = pb_ids[k];
         const farm_raster_hdr A = S.hdr[ia];
         const farm_raster_hdr B = S.hdr[ib];
@@ -42,17 +58,17 @@ __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__rest
         const uint8_t *sb = S.status + B.pix_start;
         const uint32_t *ea = S.pixedges + A.pix_start;
         const uint32_t *pofa = S.pixoff + A.pixoff_start;
-        const Window wa = window_of(A, B.mbr[0], B.mbr[1], B.mbr[2], B.mbr[3]);
+        // the reference's -a (its GPU filter) takes the source window over
+        // the target MBB without the 1e-6 shrink (query_intersect.cu:51-54);
+        // Ideal::intersect shrinks it (MyRaster.cpp:332-337)
+        const Window wa = approx ? window_of(A, B.mbr[0], B.mbr[1], B.mbr[2], B.mbr[3], 0.0)
+                                 : window_of(A, B.mbr[0], B.mbr[1], B.mbr[2], B.mbr[3]);
         const double area_a = A.step[0] * A.step[1], area_b = B.step[0] * B.step[1];
         const double amax = fmax(area_a, area_b);
         bool in_hit = false, heur = false;
         uint32_t cnt = 0;
-        uint32_t base = 0;
-        if (pass == 1) {
-            if (npp[k] == 0) return;  // decided by the count pass or no BORDER pair
-            base = pp_off[k];
-        }
-        bytes = 8 + 2 * 56;
+        const uint32_t base = pass == 1 ? pp_off[k] : 0u;
+        bytes += 8 + 2 * 56;
         for (int x = wa.x0; x <= wa.x1; x++) {
             for (int y = wa.y0; y <= wa.y1; y++) {
                 const int pa = y * A.dimx + x;
@@ -93,7 +109,8 @@ __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__rest
             const bool decided = in_hit || heur;
             npp[k] = decided ? 0u : cnt;
             result[k] = decided ? 1u : 0u;
-            inhit = in_hit ? 1 : 0;
+            inhit += in_hit ? 1 : 0;
+            bb += decided ? 0u : cnt;
         }
     }
     if (pass == 0) {
@@ -101,11 +118,114 @@ __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__rest
         for (int o = 32; o > 0; o >>= 1) {
             bytes += __shfl_xor(bytes, o);
             inhit += __shfl_xor(inhit, o);
+            bb += __shfl_xor(bb, o);
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&ctr[0], bytes);
-            atomicAdd(&ctr[1], inhit);
+            if (bytes) atomicAdd(&ctr[kCtrFilterBytes], bytes);
+            if (inhit) atomicAdd(&ctr[kCtrInHits], inhit);
+            if (bb) atomicAdd(&ctr[kCtrPixPairs], bb);
         }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Exclusive scan of n u32 values, n = bounded_count(nsrc, cap): out[0..n)
+// gets the prefix sums and out[n] the (32-bit) total.  Three kernels:
+// kScanParts blocks sum one contiguous segment each, one block scans the
+// segment sums, and every block rescans its segment from its offset.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void scan_segment(uint64_t n, uint32_t b, uint64_t &s0, uint64_t &s1) {
+    const uint64_t seg = (n + kScanParts - 1) / kScanParts;
+    s0 = min(n, (uint64_t)b * seg);
+    s1 = min(n, s0 + seg);
+}
+
+// block-wide sum (256 lanes) delivered to every lane
+__device__ __forceinline__ uint32_t block_sum256(uint32_t v, uint32_t *lds4) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return lds4[0] + lds4[1] + lds4[2] + lds4[3];
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t *__restrict__ in,
+                                                    const unsigned long long *__restrict__ nsrc, uint64_t cap,
+                                                    uint32_t *__restrict__ part) {
+    __shared__ uint32_t lds[4];
+    uint64_t s0, s1;
+    scan_segment(bounded_count(nsrc, cap), blockIdx.x, s0, s1);
+    uint32_t s = 0;
+    for (uint64_t i = s0 + threadIdx.x; i < s1; i += 256) s += in[i];
+    s = block_sum256(s, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// one block of kScanParts lanes
+__global__ __launch_bounds__(kScanParts) void k_scan_parts(uint32_t *__restrict__ part,
+                                                            const unsigned long long *__restrict__ nsrc,
+                                                            uint64_t cap, uint32_t *__restrict__ out) {
+    __shared__ uint32_t wsum[kScanParts / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t v = part[t];
+    uint32_t tot;
+    const uint32_t ex = wave_excl_scan(v, lane, &tot);
+    if (lane == 0) wsum[w] = tot;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int i = 0; i < kScanParts / 64; i++) {
+        before += i < w ? wsum[i] : 0u;
+        all += wsum[i];
+    }
+    part[t] = before + ex;
+    if (t == 0) {
+        part[kScanParts] = all;
+        out[bounded_count(nsrc, cap)] = all;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t *__restrict__ in,
+                                                     const unsigned long long *__restrict__ nsrc, uint64_t cap,
+                                                     const uint32_t *__restrict__ part, uint32_t *__restrict__ out) {
+    __shared__ uint32_t wsum[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint64_t s0, s1;
+    scan_segment(bounded_count(nsrc, cap), blockIdx.x, s0, s1);
+    uint32_t carry = part[blockIdx.x];
+    for (uint64_t base = s0; base < s1; base += 1024) {
+        // four consecutive values per lane
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t i = base + 4 * (uint64_t)t + j;
+            v[j] = i < s1 ? in[i] : 0u;
+            sum += v[j];
+        }
+        uint32_t wtot;
+        const uint32_t ex = wave_excl_scan(sum, lane, &wtot);
+        __syncthreads();
+        if (lane == 0) wsum[w] = wtot;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int i = 0; i < w; i++) before += wsum[i];
+        uint32_t run = carry + before + ex;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t i = base + 4 * (uint64_t)t + j;
+            if (i < s1) out[i] = run;
+            run += v[j];
+        }
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    }
+}
+
+__global__ void k_set_desc(IsxDesc *d, unsigned long long begin, unsigned long long n, uint8_t *out,
+                           uint32_t *pair_tests) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        d->begin = begin;
+        d->n = n;
+        d->out = out;
+        d->pair_tests = pair_tests;
     }
 }
 
@@ -120,16 +240,18 @@ __device__ __forceinline__ int cost_bucket(uint64_t cost) {
     return min(b, kBuckets - 1);
 }
 
-__global__ __launch_bounds__(256) void k_bucket_hist(DevSet S, const uint32_t *__restrict__ pa_ids,
-                                                      const uint32_t *__restrict__ pb_ids,
-                                                      const PixPair *__restrict__ pps, uint32_t n,
+__global__ __launch_bounds__(256) void k_bucket_hist(DevSet S, const uint32_t *__restrict__ pb_all,
+                                                      const IsxDesc *__restrict__ desc,
+                                                      const PixPair *__restrict__ pps,
+                                                      const unsigned long long *__restrict__ ctr, uint64_t cap_pp,
                                                       const float *__restrict__ pred_cost,
                                                       uint8_t *__restrict__ bucket, uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[kBuckets];
     for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    const uint64_t n = bounded_count(&ctr[kCtrPixPairs], cap_pp);
+    const uint32_t *pb_ids = pb_all + desc->begin;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const PixPair p = pps[i];
         int b;
         if (pred_cost) {
@@ -161,52 +283,62 @@ __global__ void k_bucket_scan(uint32_t *__restrict__ hist, uint32_t *__restrict_
     }
 }
 
-// positions are reserved per block and bucket (one global atomic per
-// non-empty bucket per block), then handed out from LDS
-__global__ __launch_bounds__(256) void k_bucket_scatter(const uint8_t *__restrict__ bucket, uint32_t n,
+// positions are reserved per block round and bucket (one global atomic per
+// non-empty bucket per round), then handed out from LDS
+__global__ __launch_bounds__(256) void k_bucket_scatter(const uint8_t *__restrict__ bucket,
+                                                         const unsigned long long *__restrict__ ctr, uint64_t cap_pp,
                                                          uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
     __shared__ uint32_t cnt[kBuckets], base[kBuckets];
-    for (int j = threadIdx.x; j < kBuckets; j += blockDim.x) cnt[j] = 0;
-    __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    int b = 0;
-    uint32_t local = 0;
-    if (i < n) {
-        b = bucket[i];
-        local = atomicAdd(&cnt[b], 1u);
+    const uint64_t n = bounded_count(&ctr[kCtrPixPairs], cap_pp);
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < n; b0 += (uint64_t)gridDim.x * blockDim.x) {
+        for (int j = threadIdx.x; j < kBuckets; j += blockDim.x) cnt[j] = 0;
+        __syncthreads();
+        const uint64_t i = b0 + threadIdx.x;
+        int b = 0;
+        uint32_t local = 0;
+        if (i < n) {
+            b = bucket[i];
+            local = atomicAdd(&cnt[b], 1u);
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < kBuckets; j += blockDim.x)
+            if (cnt[j]) base[j] = atomicAdd(&cursor[j], cnt[j]);
+        __syncthreads();
+        if (i < n) order[base[b] + local] = (uint32_t)i;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int j = threadIdx.x; j < kBuckets; j += blockDim.x)
-        if (cnt[j]) base[j] = atomicAdd(&cursor[j], cnt[j]);
-    __syncthreads();
-    if (i < n) order[base[b] + local] = i;
 }
 
-// tiles per ordered pixel pair (written at position i, scanned afterwards)
-// (the u64 total in ctr[7] lets the host detect a 32-bit offset overflow)
+// tiles per ordered pixel pair (written at position i, scanned afterwards);
+// the 64-bit total goes to ctr[kCtrTiles]
 template <int E>
 __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
-                                                uint32_t n, uint32_t *__restrict__ tiles,
-                                                unsigned long long *__restrict__ ctr) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                unsigned long long *__restrict__ ctr, uint64_t cap_pp,
+                                                uint32_t *__restrict__ tiles) {
+    const uint64_t n = bounded_count(&ctr[kCtrPixPairs], cap_pp);
     unsigned long long t = 0;
-    if (i < n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t slots = pps[order[i]].slots;
         const uint32_t per = 64u * E;
-        t = slots > 1 ? (slots - 1 + per - 1) / per : 0;
-        tiles[i] = (uint32_t)t;
+        const uint32_t ti = slots > 1 ? (slots - 1 + per - 1) / per : 0u;
+        tiles[i] = ti;
+        t += ti;
     }
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if ((threadIdx.x & 63) == 0 && t) atomicAdd(&ctr[7], t);
+    if ((threadIdx.x & 63) == 0 && t) atomicAdd(&ctr[kCtrTiles], t);
 }
 
-// (ordered pixel pair, tile within it) of every tile, from the scanned counts
-__global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ tile_off, uint32_t npp, uint32_t total,
-                                                  uint2 *__restrict__ map) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= npp) return;
-    const uint32_t t0 = tile_off[q], t1 = min(tile_off[q + 1], total);
-    for (uint32_t t = t0; t < t1; t++) map[t] = make_uint2(q, t - t0);
+// (ordered pixel pair, tile within it) of every tile, from the scanned
+// counts; nothing is written when the tiles exceed the map's capacity
+__global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ tile_off,
+                                                  const unsigned long long *__restrict__ ctr, uint64_t cap_pp,
+                                                  uint64_t cap_tiles, uint2 *__restrict__ map) {
+    const uint64_t npp = bounded_count(&ctr[kCtrPixPairs], cap_pp);
+    if (ctr[kCtrTiles] > cap_tiles) return;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npp; q += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t t0 = tile_off[q], t1 = tile_off[q + 1];
+        for (uint32_t t = t0; t < t1; t++) map[t] = make_uint2((uint32_t)q, t - t0);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -280,13 +412,20 @@ __global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, con
 constexpr uint32_t kRefQueue = 512;  // queued items per wave: a power of 2 above 63 + 64 kRefScreen
 
 template <int W>
-__global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_ids,
-                                                 const uint32_t *__restrict__ pb_ids,
+__global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_all,
+                                                 const uint32_t *__restrict__ pb_all,
+                                                 const IsxDesc *__restrict__ desc,
                                                  const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
-                                                 const uint2 *__restrict__ tile_map, uint32_t total,
-                                                 uint32_t *__restrict__ cursor, uint32_t *__restrict__ result,
-                                                 unsigned long long *__restrict__ ctr,
-                                                 uint32_t *__restrict__ pair_tests) {
+                                                 const uint2 *__restrict__ tile_map, uint64_t cap_pp,
+                                                 uint64_t cap_tiles, uint32_t *__restrict__ cursor,
+                                                 uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr) {
+    // the tile total, unless a buffer of an earlier phase overflowed (the
+    // host then grows the workspace and repeats the call, or fails it)
+    if (ctr[kCtrPixPairs] > cap_pp || ctr[kCtrTiles] > cap_tiles) return;
+    const uint32_t total = (uint32_t)ctr[kCtrTiles];
+    const uint32_t *pa_ids = pa_all + desc->begin;
+    const uint32_t *pb_ids = pb_all + desc->begin;
+    uint32_t *pair_tests = desc->pair_tests;
     // per wave: the tile's source slots (x then y, W + 1 each) and edge bits
     // of every lane, and the ring of queued items (first vertex of the target
     // chunk, source lane | steps << 8)
@@ -584,10 +723,10 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         pruned += __shfl_xor(pruned, o);
     }
     if (lane == 0) {
-        atomicAdd(&ctr[2], tests);
-        atomicAdd(&ctr[3], items);
-        atomicAdd(&ctr[4], rbytes);
-        atomicAdd(&ctr[8], pruned);
+        atomicAdd(&ctr[kCtrTests], tests);
+        atomicAdd(&ctr[kCtrItems], items);
+        atomicAdd(&ctr[kCtrRefineBytes], rbytes);
+        atomicAdd(&ctr[kCtrPruned], pruned);
     }
 }
 
@@ -641,7 +780,7 @@ __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restr
     if (s_flag) {
         if (tid == 0) {
             result[k] = 1;
-            atomicAdd(&ctr[1], 1ull);
+            atomicAdd(&ctr[kCtrInHits], 1ull);
         }
         return;
     }
@@ -704,7 +843,7 @@ __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restr
             }
     }
     for (int o = 32; o > 0; o >>= 1) tests += __shfl_xor(tests, o);
-    if ((tid & 63) == 0) atomicAdd(&ctr[2], tests);
+    if ((tid & 63) == 0) atomicAdd(&ctr[kCtrTests], tests);
     if (tid == 0 && hit) result[k] = 1;
 }
 
@@ -735,31 +874,31 @@ __global__ __launch_bounds__(256) void k_pixedges(DevSet S, int64_t npolys, uint
     }
 }
 
-__global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ result, uint32_t n,
-                                                   uint8_t *__restrict__ out, unsigned long long *__restrict__ ctr) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ result,
+                                                   const IsxDesc *__restrict__ desc,
+                                                   unsigned long long *__restrict__ ctr) {
+    const uint64_t n = desc->n;
+    uint8_t *out = desc->out;
     unsigned long long f = 0;
-    if (i < n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t r = result[i] ? 1u : 0u;
         if (out) out[i] = (uint8_t)r;
-        f = r;
+        f += r;
     }
     for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
-    if ((threadIdx.x & 63) == 0 && f) atomicAdd(&ctr[5], f);
+    if ((threadIdx.x & 63) == 0 && f) atomicAdd(&ctr[kCtrFound], f);
 }
 
-template __global__ void k_tiles<1>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
-template __global__ void k_tiles<2>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
-template __global__ void k_tiles<4>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 #define FARM_REFINE_INST(EE)                                                                                   \
-    template __global__ void k_refine<EE>(DevSet, const uint32_t *, const uint32_t *, const PixPair *,          \
-                                          const uint32_t *, const uint2 *, uint32_t, uint32_t *, uint32_t *,    \
-                                          unsigned long long *, uint32_t *);
+    template __global__ void k_tiles<EE>(const PixPair *, const uint32_t *, unsigned long long *, uint64_t,      \
+                                         uint32_t *);                                                          \
+    template __global__ void k_refine<EE>(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *,          \
+                                          const PixPair *, const uint32_t *, const uint2 *, uint64_t, uint64_t, \
+                                          uint32_t *, uint32_t *, unsigned long long *);
 FARM_REFINE_INST(1)
 FARM_REFINE_INST(2)
 FARM_REFINE_INST(4)
 FARM_REFINE_INST(8)
 #undef FARM_REFINE_INST
-template __global__ void k_tiles<8>(const PixPair *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 
 }  // namespace farm

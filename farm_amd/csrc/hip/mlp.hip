@@ -34,8 +34,8 @@ __device__ __forceinline__ short f2bf(float f) {
 __global__ __launch_bounds__(128) void k_features(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                    const uint32_t *__restrict__ pb_ids, uint32_t n,
                                                    float *__restrict__ feat, int8_t *__restrict__ decided) {
-    __shared__ uint16_t hist_all[128][64];
-    uint16_t *hist = hist_all[threadIdx.x];
+    __shared__ uint32_t hist_all[128][64];  // 32 KB: per-lane 32-bit bins (no wrap on huge pairs)
+    uint32_t *hist = hist_all[threadIdx.x];
     for (int i = 0; i < 64; i++) hist[i] = 0;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;

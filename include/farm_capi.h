@@ -198,6 +198,17 @@ int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts 
 int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
                        farm_stats *stats);
 
+/* Limit on the BORDER x BORDER pixel pairs one farm_intersect call may
+ * enumerate (default and maximum 2^32 - 1; 0 restores the default).  A
+ * window beyond it returns FARM_ECAPACITY before any refinement runs; split
+ * it into smaller batches.  The per-call buffers otherwise grow to fit. */
+int farm_set_pixel_pair_limit(farm_handle h, uint64_t max_pixel_pairs);
+
+/* on != 0 (default): the batched intersect replays one captured hipGraph
+ * per configuration (mode, tile width, queue order, buffers) for its
+ * filter, order and tile launches; 0: plain stream launches. */
+int farm_set_graphs(farm_handle h, int on);
+
 /* Exact intersect of pairs [begin, end) that also returns, per pair, the
  * segment tests the batched refinement executed for it (the cost labels of
  * the learned part).  No reference counterpart. */

@@ -357,6 +357,18 @@ struct Grid {
         for (int i = sx; i <= ex; i++)
             for (int j = sy; j <= ey; j++) ret.push_back(id(i, j));
     }
+    /* the reference GPU filter's source window over the target MBB: the same
+     * offsets without the 1e-6 shrink (query_intersect.cu:51-54,
+     * gpu_get_offset_x/_y geometry.cuh:72-82) */
+    void retrieve_unshrunk(const Box &t, std::vector<int> &ret) const {
+        ret.clear();
+        int sx = off_x(t.low[0]);
+        int sy = off_y(t.low[1]);
+        int ex = off_x(t.high[0]);
+        int ey = off_y(t.high[1]);
+        for (int i = sx; i <= ex; i++)
+            for (int j = sy; j <= ey; j++) ret.push_back(id(i, j));
+    }
     double pixel_area() const { return step_x * step_y; }
 };
 
@@ -1048,12 +1060,19 @@ struct QStats {
     int64_t bb_pairs = 0, seg_tests = 0, in_hits = 0;
 };
 
-/* Ideal::intersect (Ideal.cpp:1329-1422); mode 1 = the reference GPU -a
- * heuristic (query_intersect.cu:86-88,124-129) on the CPU pixel windows */
+/* Ideal::intersect (Ideal.cpp:1329-1422); mode 1 = the reference's -a, which
+ * only its GPU path has: kernel_filter_intersect's windows (the source window
+ * over the target MBB unshrunk, query_intersect.cu:51-54; the target window
+ * over a source pixel shrunk by 1e-6, :66-74), then true on an IN pixel on
+ * either side or on a BORDER x BORDER pair with probability > 1
+ * (query_intersect.cu:86-88,124-129) */
 static bool intersect(const Poly &A, const Poly &B, int mode, QStats &qs) {
     std::vector<std::tuple<double, int, int>> cand;
     std::vector<int> pxs, tpxs;
-    A.g.retrieve(B.mbr, pxs);
+    if (mode == 1)
+        A.g.retrieve_unshrunk(B.mbr, pxs);
+    else
+        A.g.retrieve(B.mbr, pxs);
     for (int pa : pxs) {
         if (A.g.show(pa) == ST_OUT) continue;
         Box bx = A.g.pixel_box(A.g.gx(pa), A.g.gy(pa));

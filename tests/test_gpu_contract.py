@@ -103,3 +103,81 @@ def test_device_pointer_variants(device):
             hip.hipFree(b)
     assert np.array_equal(got, want)
     assert np.array_equal(gc, wc) and np.array_equal(ga.view(np.uint64), wa.view(np.uint64))
+
+
+def _star_workload(seed=11, n=300):
+    (sa, sxy), (ta, txy) = datagen.star_pair_set(seed, n, n, 0.0, 30.0)
+    A, B = capi.HostSet.from_rings(sa, sxy), capi.HostSet.from_rings(ta, txy)
+    pa, pb = capi.candidates(A, B)
+    both = capi.HostSet.concat(A, B)
+    both.rasterize(10)
+    OA, OB = oracle.OracleSet(sa, sxy), oracle.OracleSet(ta, txy)
+    OA.rasterize(10)
+    OB.rasterize(10)
+    want, _ = oracle.intersect(OA, OB, pa, pb)
+    return both, len(A), pa, pb, want
+
+
+def test_pixel_pair_limit_fails_before_refinement():
+    both, na, pa, pb, want = _star_workload()
+    dev = capi.Device(0)
+    try:
+        dev.upload_polygons(both)
+        dev.upload_pairs(pa, pb + na)
+        st = dev.intersect().stats
+        total = st["bb_pixel_pairs"]
+        assert total > 10
+        dev.set_pixel_pair_limit(total - 1)
+        with pytest.raises(capi.FarmError) as e:
+            dev.intersect()
+        assert e.value.code == capi.FARM_ECAPACITY
+        dev.set_pixel_pair_limit(total)  # exactly at the limit: fine
+        assert np.array_equal(dev.intersect().bits, want)
+        dev.set_pixel_pair_limit(0)  # default again
+        assert np.array_equal(dev.intersect().bits, want)
+    finally:
+        dev.close()
+
+
+def test_workspace_grows_on_a_fresh_handle():
+    # the first call sizes the per-call buffers for a 1-pair window; the
+    # next windows overflow them, run no refinement, and are repeated with
+    # grown buffers
+    both, na, pa, pb, want = _star_workload(12, 400)
+    dev = capi.Device(0)
+    try:
+        dev.upload_polygons(both)
+        dev.upload_pairs(pa, pb + na)
+        assert np.array_equal(dev.intersect(0, 1).bits, want[:1])
+        n = len(pa)
+        assert np.array_equal(dev.intersect(0, n).bits, want)
+        assert np.array_equal(dev.intersect(n // 2, n, tile_a=2).bits, want[n // 2:])
+    finally:
+        dev.close()
+
+
+def test_graph_replays_match_eager_calls():
+    # every replay of a captured launch sequence against the eager result of
+    # the same window, with other work and synchronisations in between
+    both, na, pa, pb, want = _star_workload(13, 400)
+    dev = capi.Device(0)
+    try:
+        dev.upload_polygons(both)
+        dev.upload_pairs(pa, pb + na)
+        n = len(pa)
+        wins = [(0, n // 2), (n // 2, n), (n // 5, n - 3)]
+        dev.set_graphs(False)
+        eager = {w: dev.intersect(*w, tile_a=4).bits for w in wins}
+        for w in wins:
+            assert np.array_equal(eager[w], want[w[0]:w[1]])
+        dev.set_graphs(True)
+        for rep in range(4):
+            for w in wins:
+                got = dev.intersect(*w, tile_a=4).bits
+                assert np.array_equal(got, eager[w]), f"replay {rep} of window {w}"
+                dev.set_graphs(False)  # eager work between replays
+                assert np.array_equal(dev.intersect(0, 7, tile_a=2).bits, want[:7])
+                dev.set_graphs(True)
+                dev.intersect(mode=capi.MODE_APPROX_HEURISTIC)
+    finally:
+        dev.close()
