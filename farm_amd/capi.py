@@ -176,6 +176,10 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_intersect_noalloc": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Opts), C.c_void_p, C.POINTER(Stats)],
+    ),
     "farm_set_pixel_pair_limit": (C.c_int, [C.c_void_p, C.c_uint64]),
     "farm_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "farm_intersect_costs": (
@@ -481,6 +485,17 @@ class Device:
         st = Stats()
         rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr or 0), C.byref(st))
         self._check(rc, "intersect_dev")
+        return st.as_dict()
+
+    def intersect_noalloc(self, out_dev_ptr: int | None, begin: int = 0, end: int | None = None,
+                          mode: int = MODE_EXACT_BATCHED, tile_a: int = 0) -> dict:
+        """intersect_dev() through farm_intersect_noalloc: no device
+        allocation, the library's fixed per-GPU scratch."""
+        end = self.npairs if end is None else end
+        o = Opts(mode=mode, tile_a=tile_a)
+        st = Stats()
+        rc = lib.farm_intersect_noalloc(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr or 0), C.byref(st))
+        self._check(rc, "intersect_noalloc")
         return st.as_dict()
 
     def set_pixel_pair_limit(self, limit: int) -> None:

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -262,14 +263,25 @@ int isx_reserve(farm_handle h, uint64_t pairs, IsxWork *W) {
     return FARM_OK;
 }
 
+// Fixed scratch of farm_intersect_noalloc: at most kNaPairs pairs,
+// kNaPixPairs BORDER x BORDER pixel pairs and kNaTiles refinement tiles per
+// call (about 42 MiB, held by every GPU the library runs on until the
+// process ends).  Host code never passes the array's name to a kernel: it
+// uses the address hipGetSymbolAddress returns for the current device.
+constexpr uint64_t kNaPairs = 1ull << 18, kNaPixPairs = 1ull << 20, kNaTiles = 1ull << 20;
+constexpr uint64_t kNaBytes = isx_bytes(kNaPairs, kNaPixPairs, kNaTiles);
+
 }  // namespace
 
+namespace farm {
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+}  // namespace farm
 
 extern "C" {
 
 enum { OUT_NONE = 0, OUT_DEV = 1, OUT_HOST = 2 };
 static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, int out_kind,
-                         uint8_t *out, farm_stats *st, uint32_t *pair_tests = nullptr);
+                         uint8_t *out, farm_stats *st, uint32_t *pair_tests = nullptr, bool fixed = false);
 
 const char *farm_version(void) { return "farm-mi355x 0.1 (gfx950)"; }
 
@@ -353,6 +365,23 @@ int farm_open(int device, farm_handle *out) {
         return FARM_EDEVICE;
     }
     h->num_cus = prop.multiProcessorCount;
+    // this device's copy of the fixed scratch array (farm_intersect_noalloc):
+    // its device address, looked up once per device with the device current
+    {
+        static std::mutex mu;
+        static std::vector<void *> addr;
+        std::lock_guard<std::mutex> lock(mu);
+        if ((size_t)device >= addr.size()) addr.resize(device + 1, nullptr);
+        if (!addr[device]) {
+            void *p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(farm::scratch)) != hipSuccess || !p) {
+                delete h;
+                return FARM_EDEVICE;
+            }
+            addr[device] = p;
+        }
+        h->scratch = addr[device];
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return FARM_EDEVICE;
@@ -523,7 +552,7 @@ int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint6
 }
 
 static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, int out_kind,
-                         uint8_t *out, farm_stats *st, uint32_t *pair_tests) {
+                         uint8_t *out, farm_stats *st, uint32_t *pair_tests, bool fixed) {
     if (!h) return FARM_EINVAL;
     if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
     if (begin > end || end > h->npairs) return fail(h, FARM_EINVAL, "pair window [%llu,%llu) outside [0,%llu)",
@@ -541,13 +570,21 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
     const bool approx = mode == FARM_MODE_APPROX_HEURISTIC;
     const bool learned = mode == FARM_MODE_APPROX_LEARNED;
     const bool learned_order = opts && opts->order == 1 && mode == FARM_MODE_EXACT_BATCHED;
+    if (fixed) {
+        if (learned || learned_order)
+            return fail(h, FARM_EINVAL, "the learned modes need the MLP's own buffers: use farm_intersect_dev");
+        if (n64 > kNaPairs)
+            return fail(h, FARM_ECAPACITY, "%llu pairs exceed the fixed scratch's %llu per call",
+                        (unsigned long long)n64, (unsigned long long)kNaPairs);
+    }
     HIPCHK(h, hipSetDevice(h->device));
     const DevSet S = devset(h);
     unsigned long long c[kCtrSlots];
     IsxWork W;
     for (int attempt = 0;; attempt++) {
         int rc;
-        if ((rc = isx_reserve(h, n, &W))) return rc;
+        if (fixed) W = isx_carve(h->scratch, kNaPairs, kNaPixPairs, kNaTiles);
+        else if ((rc = isx_reserve(h, n, &W))) return rc;
         uint8_t *outp = out_kind == OUT_DEV ? out : (out_kind == OUT_HOST ? W.bits : nullptr);
         hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(64), 0, h->stream, W.desc, (unsigned long long)begin,
                            (unsigned long long)n, outp, pair_tests);
@@ -593,7 +630,7 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
             cfg.pa = P<uint32_t>(h->pa);
             cfg.pb = P<uint32_t>(h->pb);
             cfg.grid = h->num_cus * 16;
-            rc = h->use_graphs ? isx_front_graph(h, S, W, cfg) : isx_front(h, S, W, cfg);
+            rc = (h->use_graphs && !fixed) ? isx_front_graph(h, S, W, cfg) : isx_front(h, S, W, cfg);
             if (rc) return rc;
             HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
             HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
@@ -617,6 +654,9 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
                         npp, (unsigned long long)begin, (unsigned long long)end,
                         (unsigned long long)std::min<uint64_t>(h->max_pixel_pairs, 0xffffffffull));
         if (npp > W.cap_pp) {
+            if (fixed)
+                return fail(h, FARM_ECAPACITY, "%llu pixel pairs exceed the fixed scratch's %llu per call", npp,
+                            (unsigned long long)kNaPixPairs);
             h->isx_want_pp = npp + npp / 4 + 4096;
             h->isx_want_tiles = std::max(h->isx_want_tiles, h->isx_want_pp);
             continue;
@@ -624,6 +664,9 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
         if (ntiles > 0xffffffffull)
             return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed 32-bit offsets; use smaller batches", ntiles);
         if (ntiles > W.cap_tiles) {
+            if (fixed)
+                return fail(h, FARM_ECAPACITY, "%llu refinement tiles exceed the fixed scratch's %llu per call",
+                            ntiles, (unsigned long long)kNaTiles);
             h->isx_want_tiles = ntiles + ntiles / 4 + 4096;
             continue;
         }
@@ -655,6 +698,11 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
 int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
                        farm_stats *st) {
     return run_intersect(h, begin, end, opts, out_dev ? OUT_DEV : OUT_NONE, out_dev, st);
+}
+
+int farm_intersect_noalloc(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
+                           farm_stats *st) {
+    return run_intersect(h, begin, end, opts, out_dev ? OUT_DEV : OUT_NONE, out_dev, st, nullptr, true);
 }
 
 int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out,

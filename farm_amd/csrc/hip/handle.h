@@ -59,6 +59,7 @@ struct farm_handle_s {
         hipGraphExec_t exec;
     };
     std::vector<IsxGraph> graphs;
+    void *scratch = nullptr;  // this device's fixed scratch array (farm_intersect_noalloc)
     farm::DevBuf pair_tests;  // per-pair executed segment tests (cost labels)
     int tuned_tile = 4;       // source edges per lane of the refinement (farm_autotune)
     // learned part

@@ -1,7 +1,8 @@
 // Device scratch of one batched intersect call and the counters its kernels
-// share.  Every region is carved from one block of device memory, a
-// handle-owned allocation grown on demand.  The kernels read the pair window
-// and the element counts of later phases from device memory (IsxDesc, ctr), so one
+// share.  Every region is carved from one block of device memory: a
+// handle-owned allocation grown on demand, or the library's fixed scratch
+// array for farm_intersect_noalloc.  The kernels read the pair window and the
+// element counts of later phases from device memory (IsxDesc, ctr), so one
 // captured launch sequence serves every window and the host never waits in
 // the middle of a call.
 #pragma once

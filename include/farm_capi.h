@@ -198,6 +198,20 @@ int farm_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm_opts 
 int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
                        farm_stats *stats);
 
+/* farm_intersect_dev without any device allocation, for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No reference counterpart.  The
+ * per-call scratch is a fixed device array of the library (about 42 MiB on
+ * each GPU the library runs on, held until the process ends): at most
+ * 2^18 pairs, 2^20 BORDER x BORDER pixel pairs and 2^20 refinement tiles
+ * per call, else FARM_ECAPACITY (out_dev then holds no valid results).
+ * Modes: exact batched, exact plain and the reference -a heuristic (the
+ * learned modes need the MLP's buffers: FARM_EINVAL).  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (from several handles
+ * or threads) race; each call starts by clearing what it counts. */
+int farm_intersect_noalloc(farm_handle h, uint64_t begin, uint64_t end, const farm_opts *opts, uint8_t *out_dev,
+                           farm_stats *stats);
+
 /* Limit on the BORDER x BORDER pixel pairs one farm_intersect call may
  * enumerate (default and maximum 2^32 - 1; 0 restores the default).  A
  * window beyond it returns FARM_ECAPACITY before any refinement runs; split
