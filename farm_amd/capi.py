@@ -240,6 +240,7 @@ EXPORTS = {
         [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
     ),
     "farm_free": (None, [C.c_void_p]),
+    "farm_km_table": (C.c_int, [C.c_void_p]),
 }
 
 

@@ -733,6 +733,12 @@ const Dpk kDpk;
 
 }  // namespace
 
+extern "C" int farm_km_table(double *out) {
+    if (!out) return 1;
+    for (int k = 0; k < 900; k++) out[k] = kDpk.v[k];
+    return 0;
+}
+
 namespace farm {
 const double *km_lon_table() { return kDpk.v; }
 

@@ -76,6 +76,12 @@ int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b, double exp
                         int nthreads, uint32_t **pa, uint32_t **pb);
 void farm_free(void *p);
 
+/* The 900-entry degrees-per-kilometre longitude table the library uses for
+ * within distances and the within join's box expansion: out[k] covers
+ * latitudes [k/10, (k+1)/10) (degree_per_kilometer_longitude_arr,
+ * src/include/util.h:73-74). */
+int farm_km_table(double *out);
+
 #ifdef __cplusplus
 }
 #endif

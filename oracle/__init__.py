@@ -50,6 +50,8 @@ def _load() -> C.CDLL:
         "orc_layer": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_segment_intersect": (C.c_int, [C.c_void_p]),
         "orc_double_to_int": (C.c_int, [C.c_double]),
+        "orc_retrieve": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
+        "orc_pixel_box": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p]),
         "orc_point_to_segment_distance": (C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     }
     for k, (r, a) in sig.items():
@@ -122,6 +124,23 @@ class OracleSet:
         lib.orc_raster_copy(self._h, i, _p(r["status"]), _p(r["pixoff"]), _p(r["seqs"]), _p(r["voff"]),
                             _p(r["vnodes"]), _p(r["hoff"]), _p(r["hnodes"]), _p(r["areas"]))
         return r
+
+    def retrieve(self, i: int, box, unshrunk: bool = False) -> np.ndarray:
+        """Pixel ids of polygon i's raster over box (low x, low y, high x,
+        high y), in the reference's enumeration order."""
+        b = np.ascontiguousarray(box, np.float64)
+        n = int(lib.orc_retrieve(self._h, i, _p(b), int(unshrunk), None, 0))
+        if n < 0:
+            raise IndexError(i)
+        ids = np.zeros(max(n, 1), np.int32)
+        lib.orc_retrieve(self._h, i, _p(b), int(unshrunk), _p(ids), n)
+        return ids[:n]
+
+    def pixel_box(self, i: int, x: int, y: int) -> np.ndarray:
+        out = np.zeros(4)
+        if lib.orc_pixel_box(self._h, i, x, y, _p(out)):
+            raise IndexError(i)
+        return out
 
     def nlayers(self, i: int) -> int:
         low = np.zeros(2)

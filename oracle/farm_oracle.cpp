@@ -12,25 +12,34 @@
  * reference's Makefile builds its CPU path (src/Makefile:11): IEEE binary32
  * for Point arithmetic, binary64 for boxes, steps and distances, no FMA.
  *
- * How each function is pinned (the reference has no tests or golden data and
- * cannot be built here: it needs boost, which is absent):
- *   double_to_int, get_offset_*, retrieve_pixels  -> closed forms / exhaustive
- *       edge values (tests/test_oracle_primitives.py)
+ * How each function is pinned (the reference has no tests and cannot be built
+ * here: it needs boost, which is absent; its one data table, the degree-per-km
+ * longitude table, is a golden fixture):
+ *   double_to_int -> closed forms (tests/test_oracle_primitives.py)
+ *   retrieve_pixels (shrunk), the reference GPU filter's unshrunk window,
+ *       get_pixel_box -> closed forms on grid lines, 1e-6 and 1e-8 offsets,
+ *       clamping; exact rational pixel boxes (tests/test_oracle_windows.py)
  *   segment_intersect -> exact rational orientation on integer grids where
  *       FP32 is exact, plus hand-built collinear/touching cases
- *   .dt decode / MBB -> byte-level format tests (tests/test_dt_format.py)
+ *   .dt decode / MBB -> byte-level format tests (tests/test_host_io.py)
  *   init_raster / rasterization -> closed forms for axis-aligned rectangles
  *       and right triangles (exact fill classes), PIP brute force for IN/OUT
  *       pixel corners, sum(areas) == shoelace area, edge coverage of pixel
  *       edge sequences (tests/test_oracle_raster.py)
- *   intersect (exact) -> brute force "any FP32 edge crossing OR IN-pixel
- *       overlap" on small inputs, symmetry, containment cases
+ *   grid_align / layering / merge_status -> closed forms on a square snapped
+ *       to the universal grid: steps, boxes, dims, layer count and every
+ *       merged fill class (tests/test_host_hierarchy.py)
+ *   intersect (exact) -> both directions against brute force: (an FP32
+ *       crossing of any edge pair) or (an IN pixel overlap in the windows)
  *       (tests/test_oracle_intersect.py)
+ *   intersect mode 1 (the reference -a) -> closed forms on rectangles with
+ *       known fill classes, and the unshrunk source window
+ *       (tests/test_oracle_windows.py)
  *   candidate pairs -> brute-force MBB overlap (independent numpy check)
- *   within / distances / hierarchy -> closed-form point-segment distances,
- *       brute-force all-pairs segment distance (reference distance_test.cpp
- *       definition) for the decision when it is not near the threshold
- *       (tests/test_oracle_within.py)
+ *   within / distances -> closed-form point-segment distances, the 900 table
+ *       entries (golden fixture), brute-force all-pairs segment distance for
+ *       the decision when it is not near the threshold
+ *       (tests/test_oracle_within.py, tests/test_oracle_primitives.py)
  */
 #include "farm_oracle.h"
 
@@ -1585,6 +1594,37 @@ int orc_raster_info(const orc_set *s, int64_t i, double *mbr4, int *dims2, doubl
     counts[2] = (int64_t)P.vnodes.size();
     counts[3] = (int64_t)P.hnodes.size();
     counts[4] = (int64_t)P.p.size();
+    return ORC_OK;
+}
+
+/* MyRaster::retrieve_pixels (unshrunk = 0) or the reference GPU filter's
+ * unshrunk window (unshrunk = 1) of box4 in polygon i's raster: pixel ids in
+ * enumeration order; returns their count (at most cap are written), or -1 */
+int64_t orc_retrieve(const orc_set *s, int64_t i, const double *box4, int unshrunk, int32_t *ids, int64_t cap) {
+    if (i < 0 || i >= (int64_t)s->polys.size()) return -1;
+    const Poly &P = s->polys[i];
+    Box b;
+    b.low[0] = box4[0];
+    b.low[1] = box4[1];
+    b.high[0] = box4[2];
+    b.high[1] = box4[3];
+    std::vector<int> ids_v;
+    if (unshrunk)
+        P.g.retrieve_unshrunk(b, ids_v);
+    else
+        P.g.retrieve(b, ids_v);
+    for (int64_t k = 0; k < (int64_t)ids_v.size() && k < cap; k++) ids[k] = ids_v[k];
+    return (int64_t)ids_v.size();
+}
+
+/* MyRaster::get_pixel_box of pixel (x, y) of polygon i */
+int orc_pixel_box(const orc_set *s, int64_t i, int x, int y, double *out4) {
+    if (i < 0 || i >= (int64_t)s->polys.size()) return ORC_EINVAL;
+    const Box b = s->polys[i].g.pixel_box(x, y);
+    out4[0] = b.low[0];
+    out4[1] = b.low[1];
+    out4[2] = b.high[0];
+    out4[3] = b.high[1];
     return ORC_OK;
 }
 

@@ -98,6 +98,8 @@ int orc_failed(const orc_set *s, int64_t i);
 /* Primitive restatements exposed for unit tests. */
 int orc_segment_intersect(const float *abcd /* ax ay bx by cx cy dx dy */);
 int orc_double_to_int(double v);
+int64_t orc_retrieve(const orc_set *s, int64_t i, const double *box4, int unshrunk, int32_t *ids, int64_t cap);
+int orc_pixel_box(const orc_set *s, int64_t i, int x, int y, double *out4);
 double orc_point_to_segment_distance(const float *p, const float *p1, const float *p2, int geography);
 
 #ifdef __cplusplus

@@ -83,3 +83,71 @@ def test_containment_and_disjoint_cases():
     assert got[0] == 1 and st["in_hits"] >= 1     # containment decided by IN pixels
     assert got[1] == 0                             # MBB-disjoint pair: empty windows
     assert got[2] == 1                             # inner sits in ring_around's IN pixels
+
+
+def _windows_in_overlap(ra: dict, rb: dict, mbb_b: np.ndarray) -> tuple[bool, bool]:
+    """(IN overlap, ambiguous): some non-OUT pixel pa of A over B's MBB and
+    some non-OUT pixel pb of B over pa's box, one of them IN (class 15);
+    pixel ranges by floor-and-clamp of the 1e-6-shrunk box in numpy.
+    ambiguous: a range end within 1e-7 of an integer, where the reference's
+    double_to_int rounding could pick the neighbouring pixel."""
+    amb = False
+
+    def rng(lo, hi, r, axis):
+        nonlocal amb
+        o, s, d = r["mbr"][axis], r["step"][axis], (r["dimx"], r["dimy"])[axis]
+        f0, f1 = (lo + 1e-6 - o) / s, (hi - 1e-6 - o) / s
+        for f in (f0, f1):
+            if abs(f - np.rint(f)) < 1e-7:
+                amb = True
+        return int(min(max(np.floor(f0), 0), d - 1)), int(min(max(np.floor(f1), 0), d - 1))
+
+    sa = ra["status"].reshape(ra["dimy"], ra["dimx"])
+    sb = rb["status"].reshape(rb["dimy"], rb["dimx"])
+    x0, x1 = rng(mbb_b[0], mbb_b[2], ra, 0)
+    y0, y1 = rng(mbb_b[1], mbb_b[3], ra, 1)
+    for x in range(x0, x1 + 1):
+        for y in range(y0, y1 + 1):
+            fa = sa[y, x]
+            if fa == 0:
+                continue
+            lx = ra["mbr"][0] + x * ra["step"][0]
+            ly = ra["mbr"][1] + y * ra["step"][1]
+            hx = ra["mbr"][0] + (x + 1) * ra["step"][0]
+            hy = ra["mbr"][1] + (y + 1) * ra["step"][1]
+            u0, u1 = rng(lx, hx, rb, 0)
+            v0, v1 = rng(ly, hy, rb, 1)
+            win = sb[v0:v1 + 1, u0:u1 + 1]
+            if (win != 0).any() and (fa == 15 or (win == 15).any()):
+                return True, amb
+    return False, amb
+
+
+@pytest.mark.parametrize("seed,n", [(23, 400), (24, 400)])
+def test_intersect_is_crossing_or_in_overlap(seed, n):
+    # both directions: the oracle's answer equals (an FP32 crossing between
+    # any edge of A and any edge of B) or (an IN pixel overlap in the
+    # windows), each computed independently here
+    (sa, sxy), (ta, txy) = datagen.star_pair_set(seed, n, n, 0.0, 60.0)
+    A, B = oracle.OracleSet(sa, sxy), oracle.OracleSet(ta, txy)
+    A.rasterize(10)
+    B.rasterize(10)
+    pa, pb = oracle.candidates(A, B)
+    got, _ = oracle.intersect(A, B, pa, pb)
+    rasters_a = {int(i): A.raster(int(i)) for i in np.unique(pa)}
+    rasters_b = {int(j): B.raster(int(j)) for j in np.unique(pb)}
+    both = only_in = ambiguous = 0
+    for k in range(len(pa)):
+        i, j = int(pa[k]), int(pb[k])
+        ra = sxy[int(sa[i]):int(sa[i + 1])]
+        rb = txy[int(ta[j]):int(ta[j + 1])]
+        inov, amb = _windows_in_overlap(rasters_a[i], rasters_b[j], B.mbb(j))
+        want = inov or _any_crossing(ra, rb)
+        if amb:
+            ambiguous += 1
+            continue
+        assert bool(got[k]) == want, (k, i, j, inov)
+        both += want
+        only_in += inov and not _any_crossing(ra, rb)
+    assert ambiguous <= len(pa) // 100
+    assert both > 50 and only_in > 0  # both reasons occur

@@ -101,3 +101,31 @@ def test_longitude_km_table_entries():
         # zero-length segment: param = -1, foot = p1; 1 degree east
         d = oracle.point_to_segment_distance((1.0, lat), (0.0, lat), (0.0, lat))
         assert d == 1.0 / v, (k, d, 1.0 / v)
+
+
+def test_longitude_km_table_all_entries_golden():
+    # every entry of the reference's table (tests/golden/km_table.json, made
+    # from the reference's own header by tests/golden/make_km_table.py):
+    # the oracle's and the library's rebuilt tables equal it exactly
+    import json
+    import os
+
+    from farm_amd import capi
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "km_table.json")
+    ref = np.array(json.load(open(path))["values"], np.float64)
+    assert ref.shape == (900,)
+    lib = np.zeros(900, np.float64)
+    assert capi.lib.farm_km_table(lib.ctypes.data) == 0
+    assert np.array_equal(lib, ref)
+    got = np.zeros(900)
+    for k in range(900):
+        # a query point inside latitude band k, one degree east of a point segment
+        lat = np.float32(k / 10 + 0.05)
+        got[k] = 1.0 / oracle.point_to_segment_distance((1.0, lat), (0.0, lat), (0.0, lat))
+    # the oracle's distance divides by the entry: 1 / (1 / v) is v or its neighbour
+    assert np.all(np.abs(got - ref) <= 2 * np.spacing(ref))
+    # and exactly: 1 / v as the reference divides
+    for k in range(900):
+        lat = np.float32(k / 10 + 0.05)
+        assert oracle.point_to_segment_distance((1.0, lat), (0.0, lat), (0.0, lat)) == 1.0 / ref[k], k

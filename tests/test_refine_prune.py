@@ -180,3 +180,56 @@ def test_non_finite_and_huge_boxes_are_never_pruned():
     assert np.isinf(chunk_box(chunk[:, 0], chunk[:, 1])[2])
     box = chunk_box(chunk[:, 0], chunk[:, 1])
     assert not pruned(np.array([0, 5], f32), np.array([1, 5], f32), box)
+
+
+def pruned_at(a, b, box, k: int) -> bool:
+    """The kernel's prune with its margin scaled to 2^-k Q (|b - a|_1 + 2Q)
+    (the kernel uses k = 19)."""
+    cx, cy, hx, hy = box
+    ax, ay = f32(a[0]), f32(a[1])
+    bax, bay = f32(f32(b[0]) - ax), f32(f32(b[1]) - ay)
+    with np.errstate(all="ignore"):
+        dxc, dyc = f32(cx - ax), f32(cy - ay)
+        oc = fmaf(bax, dyc, -f32(bay * dxc))
+        rr = fmaf(abs(bay), hx, f32(abs(bax) * hy))
+        q = f32(f32(abs(dxc) + abs(dyc)) + f32(hx + hy))
+        sab = f32(abs(bax) + abs(bay))
+        m = fmaf(q, fmaf(2.0, q, sab), 2.0 ** -61)
+        return bool(f32(f32(abs(oc) - rr) * f32(2.0 ** k)) > m)
+
+
+def test_constructed_rounding_crossings_need_the_margin():
+    # Constructed, not sampled: a target segment (c, d) that lies strictly on
+    # one side of the source line ab in exact arithmetic, c within a few
+    # rounding units of the line beside the start of ab, d straight out on
+    # the same side.  Where the reference's binary32 a.cross(b, c) rounds to
+    # 0 or to the wrong sign it reports a crossing (a and b are on opposite
+    # sides of line cd), so the prune must keep the chunk.  The kernel's
+    # margin (2^-19) keeps every such chunk; a margin of 2^-26 already drops
+    # some, so this test fails for a margin shrunk that far.
+    rng = np.random.default_rng(21)
+    ref_true, dropped_small = 0, 0
+    for _ in range(60000):
+        a = rng.uniform(-1, 1, 2).astype(f32)
+        u = rng.normal(0, 1, 2)
+        u /= np.linalg.norm(u)
+        b = (a + rng.uniform(0.5, 1.5) * u).astype(f32)
+        dab = b.astype(np.float64) - a
+        nrm = np.array([-dab[1], dab[0]]) / np.linalg.norm(dab)
+        t = 10 ** rng.uniform(-4, -1)
+        c = (a + t * dab + rng.uniform(-3, 3) * 4e-8 * t * nrm).astype(f32)
+        d = (c.astype(np.float64) + t * rng.uniform(0.01, 0.3) * nrm).astype(f32)
+
+        def side(p):  # exact in f64 for these f32 inputs up to a few ulps; sign only
+            return dab[0] * (np.float64(p[1]) - np.float64(a[1])) - dab[1] * (np.float64(p[0]) - np.float64(a[0]))
+
+        if not (side(c) > 0 and side(d) > 0):
+            continue
+        if not oracle.segment_intersect(a, b, c, d):
+            continue
+        ref_true += 1
+        box = chunk_box(np.array([c[0], d[0]]), np.array([c[1], d[1]]))
+        assert not pruned_at(a, b, box, 19), (a, b, c, d)
+        dropped_small += pruned_at(a, b, box, 26)
+    assert ref_true > 200  # the construction reaches the rounding regime
+    assert dropped_small > 0  # and a too-small margin would drop reported crossings
