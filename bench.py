@@ -12,7 +12,16 @@ batch of candidate pairs already resident in HBM; the batches partition the
 candidate list.  value = pairs / s over the K timed steps.
 
 Also reported:
-  plain     the one-pair-per-workgroup kernel on the first timed batch
+  plain     the one-pair-per-workgroup kernel on the first timed batch, and
+            the batched kernel without its exact chunk prune (what batching
+            alone gains, and what the prune adds)
+  learned   the learned part on the plain intersect workload's distribution
+            (star polygons, 2,000 per side per [0,100]^2): trained on 2M pairs
+            labeled by the exact path, -a at 99% precision (Clopper-Pearson
+            calibration) evaluated on 10M fresh pairs against the exact bits,
+            beside the reference heuristic -a and the filter alone
+  learned_order  the cost head trained on this workload's warmup pairs
+            orders the timed steps' refinement queue
   roofline  the refinement kernel: 42 FP32 ops per executed segment test
             (6 orientations x 7 ops) / its event-timed duration, against the
             MI355X FP32 vector peak.  The kernel's exact prune skips most
@@ -511,6 +520,10 @@ def main() -> int:
     ap.add_argument("--train-pairs", type=int, default=1_000_000)
     ap.add_argument("--label-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--c5-train", type=int, default=2_000_000,
+                    help="learned part on the plain intersect distribution: training pairs")
+    ap.add_argument("--c5-eval", type=int, default=10_000_000,
+                    help="learned part on the plain intersect distribution: fresh evaluation pairs (0: skip)")
     ap.add_argument("--workload", choices=["intersect", "within", "intersection", "candidates"], default="intersect")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only time a plain device copy of this many bytes (HBM counter calibration)")
@@ -692,10 +705,21 @@ def main() -> int:
         stp = dev.intersect(b0, b1, capi.MODE_EXACT_PLAIN, want_bits=False).stats
         plain_s = time.time() - tp
         plain_rate = (b1 - b0) / plain_s
+        # the batched kernel without its exact chunk prune: what batching alone
+        # gains over the plain kernel, and what the prune adds on top
+        tq = time.time()
+        stn = dev.intersect(b0, b1, capi.MODE_EXACT_BATCHED, args.tile, want_bits=False, order=order,
+                            flags=capi.FLAG_NO_PRUNE).stats
+        noprune_s = time.time() - tq
         line["plain"] = {"pairs_per_s": plain_rate, "batch_pairs": b1 - b0, "found": int(stp["found"]),
                          "found_batched": int(per_step[0]["found"]),
-                         "speedup_batched_vs_plain": plain_s / batched_first if batched_first > 0 else None}
-        log(f"plain kernel on step-0 batch: {plain_s * 1000:.1f} ms ({plain_rate:.3e} pairs/s)")
+                         "speedup_batched_vs_plain": plain_s / batched_first if batched_first > 0 else None,
+                         "batched_no_prune": {"pairs_per_s": (b1 - b0) / noprune_s, "found": int(stn["found"]),
+                                              "segment_tests": int(stn["segment_tests"]),
+                                              "speedup_vs_plain": plain_s / noprune_s,
+                                              "prune_speedup": noprune_s / batched_first}}
+        log(f"plain kernel on step-0 batch: {plain_s * 1000:.1f} ms ({plain_rate:.3e} pairs/s); batched without "
+            f"the prune {noprune_s * 1000:.1f} ms")
 
     # exact bits of every pair (untimed, cost-bound queue order): the
     # approximate modes' truth, and a check that the timed steps' bits (the
@@ -730,9 +754,9 @@ def main() -> int:
                             "speedup_vs_exact": t_x / t_h},
             "filter_only": {"precision": p_f, "recall": r_f},
         })
-        line["learned"] = learned_info
-        log(f"-a learned P={p_l:.4f} R={r_l:.4f} ({t_l * 1000:.1f} ms) | heuristic P={p_h:.4f} R={r_h:.4f} "
-            f"({t_h * 1000:.1f} ms) | filter-only R={r_f:.4f}")
+        line["learned_order"] = learned_info
+        log(f"this workload's -a: learned P={p_l:.4f} R={r_l:.4f} ({t_l * 1000:.1f} ms) | heuristic P={p_h:.4f} "
+            f"R={r_h:.4f} ({t_h * 1000:.1f} ms) | filter-only R={r_f:.4f}")
 
     line["candidates"] = cand_info
 
@@ -758,6 +782,22 @@ def main() -> int:
                                  "timed_vs_cost_bound_order_mismatches": order_mismatch}
         log(f"cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; sample mismatches vs GPU: {mism}")
     dev.close()
+    if not args.no_learned and args.c5_eval > 0:
+        # the learned part on the plain intersect workload's distribution at
+        # its stated sizes: trained on c5_train pairs labeled by the exact
+        # path, -a evaluated on c5_eval fresh pairs against the exact bits
+        from farm_amd import learned
+
+        t = time.time()
+        d5 = capi.Device(0)
+        res5, _, _ = learned.config5(d5, args.c5_train, args.c5_eval, threads=args.threads, log=log)
+        d5.close()
+        res5["seconds"] = time.time() - t
+        line["learned"] = res5
+        log(f"config5 -a on {res5['eval_pairs']} pairs: learned P={res5['learned_a']['precision']:.4f} "
+            f"R={res5['learned_a']['recall']:.4f} | heuristic P={res5['heuristic_a']['precision']:.4f} "
+            f"R={res5['heuristic_a']['recall']:.4f} | filter-only R={res5['filter_only']['recall']:.4f} "
+            f"({res5['seconds']:.1f}s)")
     if not args.no_extra:
         # the other two queries at their own full sizes, each with its own
         # throughput, roofline, CPU baseline and parity sample

@@ -29,6 +29,8 @@ MODE_EXACT_BATCHED = 1
 MODE_APPROX_HEURISTIC = 2
 MODE_APPROX_LEARNED = 3
 
+FLAG_NO_PRUNE = 1
+
 
 class FarmError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -117,6 +119,9 @@ class PolySet(C.Structure):
         ("lstatus", C.POINTER(C.c_uint8)),
         ("nlstatus", C.c_uint64),
         ("areas", C.POINTER(C.c_double)),
+        ("fill", C.POINTER(C.c_uint8)),
+        ("category_count", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -129,9 +134,26 @@ class Opts(C.Structure):
         ("approx_threshold", C.c_float),
         ("approx_reject", C.c_float),
         ("merge_threshold", C.c_float),
-        ("reserved", C.c_int32),
+        ("flags", C.c_int32),
     ]
 
+
+class Piece(C.Structure):
+    _fields_ = [
+        ("pair", C.c_uint32),
+        ("ring", C.c_int32),
+        ("first", C.c_int32),
+        ("nvert", C.c_int32),
+        ("ax", C.c_float),
+        ("ay", C.c_float),
+        ("bx", C.c_float),
+        ("by", C.c_float),
+    ]
+
+
+assert C.sizeof(Piece) == 32
+PIECE_DTYPE = np.dtype([("pair", "<u4"), ("ring", "<i4"), ("first", "<i4"), ("nvert", "<i4"), ("ax", "<f4"),
+                        ("ay", "<f4"), ("bx", "<f4"), ("by", "<f4")])
 
 NFEAT = 80
 MLP_NPARAMS = 27138
@@ -202,6 +224,15 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(Stats)],
     ),
+    "farm_intersection_pieces": (
+        C.c_int,
+        [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+         C.POINTER(C.c_uint64), C.POINTER(Stats)],
+    ),
+    "farm_pieces_wkt": (
+        C.c_void_p,
+        [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)],
+    ),
     "farm_pair_features": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
     "farm_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "farm_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -234,6 +265,7 @@ EXPORTS = {
     "farm_hostset_view": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
     "farm_hostset_areas": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
     "farm_hostset_raster_failures": (C.c_int64, [C.c_void_p]),
+    "farm_hostset_set_categories": (C.c_int, [C.c_void_p, C.c_int]),
     "farm_hostset_raster_ok": (C.c_int, [C.c_void_p, C.c_void_p]),
     "farm_candidates": (
         C.c_int64,
@@ -241,6 +273,11 @@ EXPORTS = {
     ),
     "farm_free": (None, [C.c_void_p]),
     "farm_km_table": (C.c_int, [C.c_void_p]),
+    "farm_hostset_star_polygons": (
+        C.c_void_p,
+        [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+         C.c_int],
+    ),
 }
 
 
@@ -281,6 +318,14 @@ class HostSet:
         ring_off = np.ascontiguousarray(ring_off, dtype=np.uint64)
         xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
         return cls(lib.farm_hostset_from_rings(len(ring_off) - 1, _ptr(ring_off), _ptr(xy)))
+
+    @classmethod
+    def star_polygons(cls, seed: int, n: int, lo: float, hi: float, nv_lo: int, nv_hi: int, r_mu: float,
+                      r_sigma: float, amp: float, nthreads: int | None = None) -> "HostSet":
+        """Seeded star polygons generated in parallel by the library
+        (farm_hostset_star_polygons)."""
+        return cls(lib.farm_hostset_star_polygons(seed, n, lo, hi, nv_lo, nv_hi, r_mu, r_sigma, amp,
+                                                  nthreads or default_threads()))
 
     @classmethod
     def load_dt(cls, path: str) -> "HostSet":
@@ -338,6 +383,13 @@ class HostSet:
         st = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n,)).copy()
         return {"low": np.array(L.low[:]), "step": L.step, "dimx": L.dimx, "dimy": L.dimy, "status": st}
 
+    def set_categories(self, bits: int) -> None:
+        """-w bits: fill classes at 2^bits categories for the approximate
+        mode and the learned features (after rasterizing)."""
+        rc = lib.farm_hostset_set_categories(self._h, bits)
+        if rc:
+            raise FarmError(rc, f"set_categories({bits}) failed")
+
     def raster_failures(self) -> int:
         """Index of the first polygon the reference raster builder rejects, or -1."""
         return int(lib.farm_hostset_raster_failures(self._h))
@@ -375,6 +427,8 @@ class HostSet:
             "y": arr(ps.y, ps.nverts, np.float32),
             "voff": arr(ps.voff, ps.nvoff, np.uint32),
             "vnodes": arr(ps.vnodes, ps.nvnodes, np.float64),
+            "fill": arr(ps.fill, ps.nstatus, np.uint8) if ps.fill else None,
+            "category_count": int(ps.category_count),
         }
 
     def areas(self, i: int, npix: int) -> np.ndarray:
@@ -389,6 +443,41 @@ class HostSet:
         if h:
             lib.farm_hostset_free(h)
             self._h = None
+
+
+def pieces_wkt(s: HostSet, pa: np.ndarray, pb: np.ndarray, pieces: np.ndarray) -> str:
+    """The rings of A n B of every pair with pieces, assembled by the library
+    (farm_pieces_wkt): lines "<k>\tMULTIPOLYGON (...)" for window index k;
+    pa / pb are the window's pair ids into the uploaded set s."""
+    pa = np.ascontiguousarray(pa, np.uint32)
+    pb = np.ascontiguousarray(pb, np.uint32)
+    pieces = np.ascontiguousarray(pieces, PIECE_DTYPE)
+    n = C.c_uint64(0)
+    p = lib.farm_pieces_wkt(s._h, _ptr(pa), _ptr(pb), len(pa), _ptr(pieces) if len(pieces) else None, len(pieces),
+                            C.byref(n))
+    if not p:
+        raise FarmError(FARM_EINVAL, "farm_pieces_wkt rejected its input")
+    try:
+        return C.string_at(p, n.value).decode()
+    finally:
+        lib.farm_free(p)
+
+
+def wkt_rings(text: str) -> list[np.ndarray]:
+    """The rings of a MULTIPOLYGON (((x y, ...)), ...) line, as (n, 2) f64 arrays."""
+    body = text[text.index("(") + 1:text.rindex(")")]
+    return [np.array([[float(v) for v in q.split()] for q in ring.split("))")[0].split(",")], np.float64)
+            for ring in body.split("((")[1:]]
+
+
+def wkt_multipolygon_area(text: str) -> float:
+    """|sum of the rings' signed shoelace areas| (f64): outer rings and holes
+    turn opposite ways."""
+    total = 0.0
+    for pts in wkt_rings(text):
+        x, y = pts[:, 0] - pts[0, 0], pts[:, 1] - pts[0, 1]
+        total += 0.5 * float(np.sum(x[:-1] * y[1:] - x[1:] * y[:-1]))
+    return abs(total)
 
 
 def candidates(a: HostSet, b: HostSet, expand_km: float = 0.0, skip_self: bool = False,
@@ -462,13 +551,13 @@ class Device:
 
     def intersect(self, begin: int = 0, end: int | None = None, mode: int = MODE_EXACT_BATCHED,
                   tile_a: int = 0, want_bits: bool = True, order: int = 0, accept: float = 0.0,
-                  reject: float | None = None) -> IntersectResult:
+                  reject: float | None = None, flags: int = 0) -> IntersectResult:
         """Exact or approximate intersect of pairs [begin, end).  order=1 orders
         the refinement queue by the MLP cost head; mode MODE_APPROX_LEARNED
         accepts logits > accept, rejects logits < reject and refines the rest."""
         end = self.npairs if end is None else end
         o = Opts(mode=mode, tile_a=tile_a, order=order, approx_threshold=accept,
-                 approx_reject=accept if reject is None else reject)
+                 approx_reject=accept if reject is None else reject, flags=flags)
         st = Stats()
         out = np.zeros(max(end - begin, 1), dtype=np.uint8) if want_bits else None
         rc = lib.farm_intersect(self._h, begin, end, C.byref(o), _ptr(out) if want_bits else None, C.byref(st))
@@ -477,12 +566,12 @@ class Device:
 
     def intersect_dev(self, out_dev_ptr: int | None, begin: int = 0, end: int | None = None,
                       mode: int = MODE_EXACT_BATCHED, tile_a: int = 0, order: int = 0, accept: float = 0.0,
-                      reject: float | None = None) -> dict:
+                      reject: float | None = None, flags: int = 0) -> dict:
         """intersect() with the 0/1 results left in device memory at
         out_dev_ptr (end - begin bytes; None: counts only)."""
         end = self.npairs if end is None else end
         o = Opts(mode=mode, tile_a=tile_a, order=order, approx_threshold=accept,
-                 approx_reject=accept if reject is None else reject)
+                 approx_reject=accept if reject is None else reject, flags=flags)
         st = Stats()
         rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr or 0), C.byref(st))
         self._check(rc, "intersect_dev")
@@ -549,6 +638,26 @@ class Device:
         self._check(lib.farm_intersection_area(self._h, begin, end, _ptr(counts), _ptr(area), C.byref(st)),
                     "intersection")
         return counts[:n], area[:n], st.as_dict()
+
+    def intersection_pieces(self, begin: int = 0, end: int | None = None,
+                            cap: int = 1 << 20) -> tuple[np.ndarray, np.ndarray, np.ndarray, dict]:
+        """intersection_area() plus the IN pieces of the boundary of A n B of
+        every pair (a PIECE_DTYPE array, pair = index in the window)."""
+        end = self.npairs if end is None else end
+        n = end - begin
+        counts = np.zeros((max(n, 1), 5), np.int32)
+        area = np.zeros(max(n, 1), np.float64)
+        while True:
+            pcs = np.zeros(max(cap, 1), PIECE_DTYPE)
+            npc = C.c_uint64(0)
+            st = Stats()
+            rc = lib.farm_intersection_pieces(self._h, begin, end, _ptr(counts), _ptr(area), _ptr(pcs), cap,
+                                              C.byref(npc), C.byref(st))
+            if rc == FARM_ECAPACITY and npc.value > cap:
+                cap = int(npc.value)
+                continue
+            self._check(rc, "intersection_pieces")
+            return counts[:n], area[:n], pcs[:npc.value], st.as_dict()
 
     def pair_features(self, begin: int = 0, end: int | None = None) -> tuple[np.ndarray, np.ndarray]:
         end = self.npairs if end is None else end

@@ -30,11 +30,11 @@ __global__ void k_tiles(const PixPair *, const uint32_t *, unsigned long long *,
 template <int E>
 __global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, const PixPair *,
                          const uint32_t *, const uint2 *, uint64_t, uint64_t, uint32_t *, uint32_t *,
-                         unsigned long long *);
+                         unsigned long long *, int);
 __global__ void k_tile_map(const uint32_t *, const unsigned long long *, uint64_t, uint64_t, uint2 *);
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
-__global__ void k_vdiff(const float *, const float *, uint64_t, float *, float *);
+__global__ void k_vdiff(const float *, const float *, uint64_t, float4 *);
 __global__ void k_chunk_count(DevSet, int64_t, uint32_t *);
 __global__ void k_chunk_box(DevSet, int64_t, const uint32_t *, float4 *);
 __global__ void k_finalize(const uint32_t *, const IsxDesc *, unsigned long long *);
@@ -120,7 +120,7 @@ struct IsxCfg {
     uint64_t set_gen;         // device arrays of the uploaded set and pairs
     const uint32_t *pa, *pb;  // the pair list
     int32_t grid;             // blocks of the grid-stride kernels
-    int32_t pad;
+    int32_t prune;            // the refinement's exact chunk prune on / off
 };
 static_assert(sizeof(IsxCfg) <= sizeof(((farm_handle_s::IsxGraph *)nullptr)->key), "graph key too large");
 
@@ -185,19 +185,19 @@ int isx_refine(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c
     switch (c.tile) {
         case 1:
             hipLaunchKernelGGL(k_refine<1>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
             break;
         case 2:
             hipLaunchKernelGGL(k_refine<2>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
             break;
         case 4:
             hipLaunchKernelGGL(k_refine<4>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
             break;
         default:
             hipLaunchKernelGGL(k_refine<8>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr);
+                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
             break;
     }
     HIPCHK(h, hipGetLastError());
@@ -400,13 +400,13 @@ int farm_close(farm_handle h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
-    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->dx,       &h->dy,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
+    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->xyd,      &h->fill,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
                       &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->isx,
                       &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
                       &h->skipbuf, &h->layers, &h->lstatus, &h->areas, &h->wctr, &h->wovf, &h->wslab,
-                      &h->wslab_big, &h->wout, &h->wmd, &h->ring2a, &h->icounts, &h->iarea, &h->cg_boxa,
+                      &h->wslab_big, &h->wout, &h->wmd, &h->ring2a, &h->icounts, &h->iarea, &h->ipieces, &h->cg_boxa,
                       &h->cg_boxq, &h->cg_cellcnt, &h->cg_celloff, &h->cg_items, &h->cg_cnt, &h->cg_off,
                       &h->cg_keys, &h->cg_keys2, &h->cg_uoff};
     for (DevBuf *b : bufs)
@@ -471,8 +471,13 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
             }
         }
     }
+    const int cc = ps->category_count ? ps->category_count : FARM_CATEGORY_COUNT;
+    if (cc < 4 || cc > 256 || (cc & (cc - 1)) || (cc != FARM_CATEGORY_COUNT && !ps->fill))
+        return fail(h, FARM_EINVAL, "category count %d: a power of 2 in 4..256, with fill classes unless 16", cc);
     int rc;
     if ((rc = upload(h, h->hdr, ps->hdr, sizeof(farm_raster_hdr) * ps->npolys))) return rc;
+    if (cc != FARM_CATEGORY_COUNT && (rc = upload(h, h->fill, ps->fill, ps->nstatus))) return rc;
+    h->cc = cc;
     if ((rc = upload(h, h->status, ps->status, ps->nstatus))) return rc;
     if ((rc = upload(h, h->pixoff, ps->pixoff, 4 * ps->npixoff))) return rc;
     if ((rc = upload(h, h->seqs, ps->seqs, 8 * ps->nseqs))) return rc;
@@ -486,10 +491,9 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     if ((rc = upload(h, h->y, ps->y, 4 * ps->nverts))) return rc;
     // successive vertex differences over the padded arrays (the last one
     // reads no vertex past the padding and is 0)
-    for (DevBuf *b : {&h->dx, &h->dy})
-        if ((rc = ensure(h, *b, 4 * (ps->nverts + kVertexPad)))) return rc;
+    if ((rc = ensure(h, h->xyd, 16 * (ps->nverts + kVertexPad)))) return rc;
     hipLaunchKernelGGL(k_vdiff, dim3(blocks_for(ps->nverts + kVertexPad)), dim3(256), 0, h->stream,
-                       P<float>(h->x), P<float>(h->y), ps->nverts + kVertexPad, P<float>(h->dx), P<float>(h->dy));
+                       P<float>(h->x), P<float>(h->y), ps->nverts + kVertexPad, P<float4>(h->xyd));
     HIPCHK(h, hipGetLastError());
     if ((rc = upload(h, h->voff, ps->voff, 4 * ps->nvoff))) return rc;
     if ((rc = upload(h, h->vnodes, ps->vnodes, 8 * ps->nvnodes))) return rc;
@@ -630,6 +634,7 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
             cfg.pa = P<uint32_t>(h->pa);
             cfg.pb = P<uint32_t>(h->pb);
             cfg.grid = h->num_cus * 16;
+            cfg.prune = opts && (opts->flags & FARM_FLAG_NO_PRUNE) ? 0 : 1;
             rc = (h->use_graphs && !fixed) ? isx_front_graph(h, S, W, cfg) : isx_front(h, S, W, cfg);
             if (rc) return rc;
             HIPCHK(h, hipEventRecord(h->ev[1], h->stream));

@@ -17,9 +17,10 @@ struct DevSet {
     const uint2 *seqs;
     const float *x;
     const float *y;
-    // dx[v] = x[v + 1] - x[v] and dy likewise, rounded to f32 (refinement only)
-    const float *dx;
-    const float *dy;
+    // per vertex v: (x[v], y[v], x[v] - x[v - 1], y[v] - y[v - 1]), the
+    // differences rounded to f32, (0, 0) at v = 0 (refinement only: one
+    // 16-byte load per target step)
+    const float4 *xyd;
     // per edge sequence: index of its first 16-step chunk box; the boxes
     // (centre x, y, half extents x, y), refinement only
     const uint32_t *choff;
@@ -27,6 +28,10 @@ struct DevSet {
     const uint32_t *pixedges;  // per pixel: sum of sequence lengths (BORDER), else 0
     const uint32_t *voff;
     const double *vnodes;
+    // fill classes for the approximate mode and the features (the status
+    // bytes unless -w chose another category count) and that count
+    const uint8_t *fill;
+    int32_t cc;
     // hierarchy (-h) and per-pixel areas (within only; null otherwise)
     const farm_layer *layers;
     const uint8_t *lstatus;
@@ -99,6 +104,21 @@ __device__ __forceinline__ double apx_area(uint8_t f, double pix_area) {
     } else {
         lo = (1.0 * f - 1) / (kCC - 2) * pix_area;
         hi = (1.0 * f - 0) / (kCC - 2) * pix_area;
+    }
+    return (lo + hi) / 2;
+}
+
+// decodePixelArea at cc categories (gpu_decode_fullness, geometry.cuh:374-388,
+// with the filter's category_count), midpoint of the fill interval
+__device__ __forceinline__ double apx_area_cc(uint8_t f, double pix_area, int cc) {
+    double lo, hi;
+    if (f == 0) {
+        lo = hi = 0.0;
+    } else if (f == cc - 1) {
+        lo = hi = pix_area;
+    } else {
+        lo = (1.0 * f - 1) / (cc - 2) * pix_area;
+        hi = (1.0 * f - 0) / (cc - 2) * pix_area;
     }
     return (lo + hi) / 2;
 }

@@ -40,7 +40,9 @@ struct farm_handle_s {
     std::string err;
     // polygon set
     int64_t npolys = 0;
-    farm::DevBuf hdr, status, pixoff, seqs, x, y, dx, dy, choff, chcnt, chbox, voff, vnodes, pixedges;
+    farm::DevBuf fill;    // -w fill classes (when the set has them)
+    int cc = FARM_CATEGORY_COUNT;
+    farm::DevBuf hdr, status, pixoff, seqs, x, y, xyd, choff, chcnt, chbox, voff, vnodes, pixedges;
     uint64_t nstatus = 0;
     // pairs
     uint64_t npairs = 0;
@@ -79,7 +81,7 @@ struct farm_handle_s {
     farm::DevBuf wctr, wovf, wslab, wslab_big, wout, wmd;
     // intersection
     bool ring_area_ready = false;
-    farm::DevBuf ring2a, icounts, iarea;
+    farm::DevBuf ring2a, icounts, iarea, ipieces;
     // candidate generation
     farm::DevBuf cg_boxa, cg_boxq, cg_cellcnt, cg_celloff, cg_items, cg_uoff, cg_cnt, cg_off, cg_keys, cg_keys2;
 };
@@ -139,8 +141,9 @@ inline DevSet devset(farm_handle h) {
     S.seqs = P<uint2>(h->seqs);
     S.x = P<float>(h->x);
     S.y = P<float>(h->y);
-    S.dx = P<float>(h->dx);
-    S.dy = P<float>(h->dy);
+    S.xyd = P<float4>(h->xyd);
+    S.fill = h->cc != FARM_CATEGORY_COUNT ? P<uint8_t>(h->fill) : P<uint8_t>(h->status);
+    S.cc = h->cc;
     S.choff = P<uint32_t>(h->choff);
     S.chbox = P<float4>(h->chbox);
     S.pixedges = P<uint32_t>(h->pixedges);

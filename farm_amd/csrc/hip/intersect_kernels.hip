@@ -91,7 +91,10 @@ __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__rest
                             continue;
                         }
                         if (approx) {
-                            const double prob = (apx_area(fa, area_a) + apx_area(fb, area_b)) / amax;
+                            // fill classes at the set's category count (-w)
+                            const double prob = (apx_area_cc(S.fill[A.pix_start + pa], area_a, S.cc) +
+                                                 apx_area_cc(S.fill[B.pix_start + pb], area_b, S.cc)) /
+                                                amax;
                             if (prob > 1.0) heur = true;
                             continue;
                         }
@@ -409,16 +412,21 @@ __global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, con
     }
 }
 
+// Chunk boxes read through the constant address space: they do not change
+// during a launch, and uniform addresses then become scalar loads.
+typedef __attribute__((address_space(4))) const float cfloat;
+
 constexpr uint32_t kRefQueue = 512;  // queued items per wave: a power of 2 above 63 + 64 kRefScreen
 
 template <int W>
-__global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__restrict__ pa_all,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_refine(DevSet S, const uint32_t *__restrict__ pa_all,
                                                  const uint32_t *__restrict__ pb_all,
                                                  const IsxDesc *__restrict__ desc,
                                                  const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
                                                  const uint2 *__restrict__ tile_map, uint64_t cap_pp,
                                                  uint64_t cap_tiles, uint32_t *__restrict__ cursor,
-                                                 uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr) {
+                                                 uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr,
+                                                 int prune) {
     // the tile total, unless a buffer of an earlier phase overflowed (the
     // host then grows the workspace and repeats the call, or fails it)
     if (ctr[kCtrPixPairs] > cap_pp || ctr[kCtrTiles] > cap_tiles) return;
@@ -460,10 +468,7 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
         const uint2 *sqb = S.seqs + B.seq_start;
         const float *ax_ = S.x + A.vert_start;
         const float *ay_ = S.y + A.vert_start;
-        const float *bx_ = S.x + B.vert_start;
-        const float *by_ = S.y + B.vert_start;
-        const float *bdx_ = S.dx + B.vert_start;
-        const float *bdy_ = S.dy + B.vert_start;
+        const float4 *bq_ = S.xyd + B.vert_start;
         const uint32_t *bchoff = S.choff + B.seq_start;
         const uint32_t a_s0 = pofa[P.pa], a_s1 = pofa[P.pa + 1];
         const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
@@ -514,6 +519,7 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
             bool hit = false;
             bool stop = false;
             uint32_t qhead = 0, qn = 0, screened = 0;  // wave-uniform
+            uint32_t scr_steps = 0, queued_steps = 0;  // target steps screened (uniform) / queued by this lane
 
             // Evaluate the n <= 64 oldest queued items, one per lane: the source
             // lane's E edges against the chunk's nstep target segments.  Per
@@ -547,7 +553,8 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                     qinv[e] = ve ? -INFINITY : INFINITY;  // lifts a slot without an edge out of the screen
                     qnv += ve ? 1 : 0;
                 }
-                const float c0x = bx_[vs], c0y = by_[vs];
+                const float4 c04 = bq_[vs];
+                const float c0x = c04.x, c0y = c04.y;
                 float tx[E + 1], ty[E + 1], Op[E];
 #pragma unroll
                 for (int v = 0; v <= E; v++) {
@@ -565,9 +572,10 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
 #pragma unroll 4
                 for (int m = 0; m < (int)kRefChunk; m++) {
                     if ((uint32_t)m < ns) {
-                        const float qx = bx_[vs + 1 + m], qy = by_[vs + 1 + m];
-                        // the step's direction (q - c), precomputed with the same rounding
-                        const float ddx = bdx_[vs + m], ddy = bdy_[vs + m];
+                        // the step's end q and direction (q - c), precomputed with
+                        // the same rounding, in one load
+                        const float4 q4 = bq_[vs + 1 + m];
+                        const float qx = q4.x, qy = q4.y, ddx = q4.z, ddy = q4.w;
                         float o[E + 1];
 #pragma unroll
                         for (int v = 0; v <= E; v++) {
@@ -599,7 +607,8 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                     if (maybe) {
                         float ccx = c0x, ccy = c0y;
                         for (uint32_t m = 0; m < ns; m++) {
-                            const float dx = bx_[vs + 1 + m], dy = by_[vs + 1 + m];
+                            const float4 d4 = bq_[vs + 1 + m];
+                            const float dx = d4.x, dy = d4.y;
 #pragma unroll
                             for (int e = 0; e < E; e++)
                                 if (((eb >> e) & 1u) &&
@@ -621,52 +630,82 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 const uint2 sq = sqb[j];
                 const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
                 const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
-                const float4 *cbox = S.chbox + __builtin_amdgcn_readfirstlane(bchoff[j]);
+                const cfloat *cbox = (const cfloat *)(S.chbox + __builtin_amdgcn_readfirstlane(bchoff[j]));
                 const uint32_t nch = (L + kRefChunk - 1) / kRefChunk;
                 for (uint32_t g = 0; g < nch && !stop; g += kRefScreen) {
                     // Exact prune of this lane's edges against kRefScreen chunks
-                    // (their boxes are one scalar load; boxes past the sequence's
+                    // (their boxes are scalar loads; boxes past the sequence's
                     // last chunk are read, the array keeps spare entries, but not
-                    // used).  With M = 2^-19 Q (|b - a|_1 + 2Q), Q the L1 reach
-                    // from a to the chunk box, at least 2^-80: if every box point
-                    // q has |a.cross(b, q)| > M in exact arithmetic, the
-                    // reference's f32 a.cross(b, q) has one nonzero sign on the
-                    // whole chunk (rounding <= 3 2^-24 Q |b - a|_1), and its
-                    // collinear branch (c.cross(a, d) == c.cross(b, d) == 0)
-                    // needs a and b within rounding of line cd, which puts c
-                    // within 2^-20.7 Q (|b - a|_1 + 2Q) of line ab:
-                    // segment_intersect is false for every segment of the chunk.
-                    // The f32 evaluation below errs by a few 2^-24 Q |b - a|_1,
-                    // covered by M.  (A chunk with a segment of length 0 has an
-                    // infinite box and is never pruned.)
+                    // used).  q >= Q, the L1 reach from any vertex of the lane's
+                    // edges to the chunk box, is taken from the box of those
+                    // vertices.  If every box point p has
+                    // |a.cross(b, p)| > M_lin = 2^-19 q |b - a|_1 in exact
+                    // arithmetic, the reference's f32 a.cross(b, p) has one
+                    // nonzero sign on the whole chunk (rounding <= 3 2^-24 Q
+                    // |b - a|_1), so its sign test is false for every segment of
+                    // the chunk.  Its collinear branch (c.cross(a, d) ==
+                    // c.cross(b, d) == 0) answers with inter1's exact bounding-box
+                    // overlap, impossible when the chunk box and the box of the
+                    // lane's vertices are apart on an axis; otherwise it needs a
+                    // and b within rounding of line cd, which puts c within
+                    // 2^-20.7 Q (|b - a|_1 + 2Q) of line ab, and the margin is
+                    // M = 2^-19 q (|b - a|_1 + 2q).  Both margins get 2^-80 added.
+                    // The f32 evaluation below (the centre's cross, its spread
+                    // over the box plus M in two fmas) errs by a few
+                    // 2^-24 q |b - a|_1, covered by M.  (A chunk with a segment of
+                    // length 0 has an infinite box and is never pruned.)
                     // this lane's slots again, from LDS (not kept in registers
                     // across the evaluation of queued items)
-                    float spx[E + 1], spy[E + 1], sbx[E], sby[E];
+                    float spx[E], spy[E], sbx[E], sby[E], sab[E];
+                    float lxl, lxh, lyl, lyh;  // box of the lane's slot vertices
+                    {
+                        float nx = src[lane][0], ny = src[lane][E + 1];
+                        lxl = lxh = nx;
+                        lyl = lyh = ny;
 #pragma unroll
-                    for (int v = 0; v <= E; v++) {
-                        spx[v] = src[lane][v];
-                        spy[v] = src[lane][E + 1 + v];
-                    }
-#pragma unroll
-                    for (int e = 0; e < E; e++) {
-                        sbx[e] = spx[e + 1] - spx[e];
-                        sby[e] = spy[e + 1] - spy[e];
+                        for (int e = 0; e < E; e++) {
+                            spx[e] = nx;
+                            spy[e] = ny;
+                            nx = src[lane][e + 1];
+                            ny = src[lane][E + 2 + e];
+                            lxl = fminf(lxl, nx);
+                            lxh = fmaxf(lxh, nx);
+                            lyl = fminf(lyl, ny);
+                            lyh = fmaxf(lyh, ny);
+                            sbx[e] = nx - spx[e];
+                            sby[e] = ny - spy[e];
+                            // a slot without an edge gets sab = -inf: its margin and
+                            // threshold are -inf and the box is "pruned" for it (an
+                            // infinite box makes the threshold NaN instead, which
+                            // queues the item; the evaluation skips the slot)
+                            sab[e] = ((evb >> e) & 1u) ? fabsf(sbx[e]) + fabsf(sby[e]) : -INFINITY;
+                        }
                     }
                     bool need[kRefScreen];
 #pragma unroll
                     for (int i = 0; i < (int)kRefScreen; i++) {
-                        const float4 bb = cbox[g + i];
+                        const float4 bb = make_float4(cbox[4 * (g + i)], cbox[4 * (g + i) + 1],
+                                                      cbox[4 * (g + i) + 2], cbox[4 * (g + i) + 3]);
+                        // the reach q (rounded up), and whether the boxes are apart
+                        // on an axis (by more than the f32 rounding of the tests)
+                        const float ex0 = bb.x - lxl, ex1 = bb.x - lxh;
+                        const float ey0 = bb.y - lyl, ey1 = bb.y - lyh;
+                        const float q = ((fmaxf(fabsf(ex0), fabsf(ex1)) + fmaxf(fabsf(ey0), fabsf(ey1))) +
+                                         (bb.z + bb.w)) * 1.000001f;
+                        const float hx1 = bb.z * 1.000001f, hy1 = bb.w * 1.000001f;
+                        const bool apart = (ex1 > hx1) | (-ex0 > hx1) | (ey1 > hy1) | (-ey0 > hy1);
+                        const float k1 = q * 0x1p-19f;
+                        // (without the prune: the largest finite margin keeps every
+                        // chunk for every slot with an edge)
+                        const float k2 = !prune ? 0x1.fffffep127f : (apart ? 0x1p-80f : fmaf(q, q * 0x1p-18f, 0x1p-80f));
                         need[i] = false;
 #pragma unroll
                         for (int e = 0; e < E; e++) {
                             const float dxc = bb.x - spx[e], dyc = bb.y - spy[e];
-                            const float oc = fmaf(sbx[e], dyc, -(sby[e] * dxc));               // at the box centre
-                            const float rr = fmaf(fabsf(sby[e]), bb.z, fabsf(sbx[e]) * bb.w);  // its spread
-                            const float q = (fabsf(dxc) + fabsf(dyc)) + (bb.z + bb.w);
-                            const float sab = fabsf(sbx[e]) + fabsf(sby[e]);
-                            const float m19 = fmaf(q, fmaf(2.f, q, sab), 0x1p-61f);  // 2^19 M, plus 2^-61
-                            const bool prune = (fabsf(oc) - rr) * 0x1p19f > m19;
-                            need[i] = need[i] || (((evb >> e) & 1u) && !prune);
+                            const float oc = fmaf(sbx[e], dyc, -(sby[e] * dxc));  // at the box centre
+                            const float m = fmaf(sab[e], k1, k2);                 // M
+                            const float thr = fmaf(fabsf(sby[e]), bb.z, fmaf(fabsf(sbx[e]), bb.w, m));
+                            need[i] = need[i] | !(fabsf(oc) > thr);  // bitwise: no branch per edge
                         }
                     }
 #pragma unroll
@@ -674,17 +713,19 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                         const uint32_t c = g + i;
                         if (c >= nch) break;
                         const uint32_t nstep = min(kRefChunk, L - c * kRefChunk);
+                        scr_steps += nstep;
                         const unsigned long long bm = __ballot(need[i]);
                         if (bm) {
                             // append this chunk for every lane that needs it
                             const uint32_t below = __builtin_amdgcn_mbcnt_hi(
                                 (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-                            if (need[i])
+                            if (need[i]) {
                                 ring[(qhead + qn + below) & (kRefQueue - 1)] =
                                     make_uint2(v0 + c * kRefChunk, (uint32_t)lane | (nstep << 8));
+                                queued_steps += nstep;
+                            }
                             qn += (uint32_t)__popcll(bm);
                         }
-                        if (!need[i]) pruned += (unsigned long long)nvalid * nstep;
                     }
                     while (qn >= 64 && !stop) {
                         __builtin_amdgcn_wave_barrier();
@@ -714,6 +755,9 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
                 rbytes += 16 + 8ull * nb + 8ull * (S.pixedges[B.pix_start + P.pb] + nb);
             }
             rbytes += (unsigned long long)nvalid * 8;
+            // tests of this lane's edges the prune skipped: every screened step
+            // that was not queued
+            pruned += (unsigned long long)nvalid * (scr_steps - queued_steps);
         };
         run(std::integral_constant<int, W>());
     }
@@ -732,17 +776,61 @@ __global__ __launch_bounds__(256) void k_refine(DevSet S, const uint32_t *__rest
 
 // ------------------------------------------------------------------------
 // Plain kernel: one workgroup (256 lanes) per candidate pair, the direct
-// formulation.  The window is scanned for IN overlaps by all lanes; then each
-// BORDER x BORDER pixel pair is refined by striding the lanes over the
-// flattened (source edge, target edge) grid.
+// formulation with no pruning and no queue.  The window is scanned for IN
+// overlaps by all lanes; then each BORDER x BORDER pixel pair is refined in
+// full: both pixels' edges are flattened into LDS (a block scan over the
+// sequence lengths, then every lane fills edges, finding its sequence by a
+// binary search over the scanned starts), as coordinates, up to kPlainCap
+// edges per side at a time, and the lanes sweep the source x target edge
+// grid in 16 x 16 blocks (lane t tests source edge t / 16 and target edge
+// t % 16 of each block), with a block-wide early exit every 8 blocks.
 // ------------------------------------------------------------------------
-constexpr int kPlainCap = 2048;  // edges per side held in LDS per chunk
+constexpr int kPlainCap = 1024;   // edges per side held in LDS per chunk
+constexpr int kPlainSeqs = 256;   // sequences per pixel scanned per pass
+
+// Fills dst[0 .. len) with the coordinates (a.x, a.y, b.x, b.y) of edges
+// first .. first + len of the concatenation of a pixel's sequences
+// sq[0 .. nseq), seq_off[k] the scanned start of sequence k.
+__device__ void plain_fill(const uint2 *__restrict__ sq, const uint32_t *seq_off, uint32_t nseq, uint32_t first,
+                           uint32_t len, const float *__restrict__ X, const float *__restrict__ Y, float4 *dst) {
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const uint32_t g = first + i;
+        uint32_t lo = 0, hi = nseq;  // last k with seq_off[k] <= g
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (seq_off[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t v = sq[lo].x + (g - seq_off[lo]);
+        dst[i] = make_float4(X[v], Y[v], X[v + 1], Y[v + 1]);
+    }
+}
+
+// Exclusive block scan of the lengths of sequences sq[0 .. n) (n <=
+// kPlainSeqs): seq_off[k] for k < n, returns the total.
+__device__ uint32_t plain_scan(const uint2 *__restrict__ sq, uint32_t n, uint32_t *seq_off, uint32_t *wsum) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t v = (uint32_t)t < n ? sq[t].y : 0u;
+    uint32_t tot;
+    const uint32_t ex = wave_excl_scan(v, lane, &tot);
+    if (lane == 0) wsum[w] = tot;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int i = 0; i < 4; i++) {
+        before += i < w ? wsum[i] : 0u;
+        all += wsum[i];
+    }
+    if ((uint32_t)t < n) seq_off[t] = before + ex;
+    __syncthreads();
+    return all;
+}
 
 __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restrict__ pa_ids,
                                                 const uint32_t *__restrict__ pb_ids, uint32_t npairs,
                                                 uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr) {
-    __shared__ uint32_t ea[kPlainCap];
-    __shared__ uint32_t eb[kPlainCap];
+    __shared__ float4 ea[kPlainCap];
+    __shared__ float4 eb[kPlainCap];
+    __shared__ uint32_t offa[kPlainSeqs], offb[kPlainSeqs], wsum[4];
     __shared__ int s_flag;
     const uint32_t k = blockIdx.x;
     if (k >= npairs) return;
@@ -786,6 +874,7 @@ __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restr
     }
     unsigned long long tests = 0;
     bool hit = false;
+    const int ti = tid >> 4, tj = tid & 15;  // this lane's place in a 16 x 16 block
     // refinement: uniform loop over the BORDER x BORDER pixel pairs
     for (int w = 0; w < nwa && !hit; w++) {
         const int x = wa.x0 + w / nwy, y = wa.y0 + w % nwy;
@@ -798,46 +887,44 @@ __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restr
             for (int v = wb.y0; v <= wb.y1 && !hit; v++) {
                 const int pb = v * B.dimx + u;
                 if (show(sb[pb]) != 1) continue;
-                // flatten the edge lists (vertex index of each edge start)
-                int na = 0, nb = 0;
-                for (uint32_t s = pofa[pa]; s < pofa[pa + 1]; s++) na += sqa[s].y;
-                for (uint32_t s = pofb[pb]; s < pofb[pb + 1]; s++) nb += sqb[s].y;
-                for (int ca = 0; ca < na && !hit; ca += kPlainCap) {
-                    for (int cb = 0; cb < nb && !hit; cb += kPlainCap) {
-                        const int la = min(kPlainCap, na - ca), lb = min(kPlainCap, nb - cb);
-                        __syncthreads();
-                        if (tid == 0) {
-                            int c = 0, o = 0;
-                            for (uint32_t s = pofa[pa]; s < pofa[pa + 1]; s++)
-                                for (uint32_t e = 0; e < sqa[s].y; e++, c++)
-                                    if (c >= ca && c < ca + la) ea[o++] = sqa[s].x + e;
-                            c = 0;
-                            o = 0;
-                            for (uint32_t s = pofb[pb]; s < pofb[pb + 1]; s++)
-                                for (uint32_t e = 0; e < sqb[s].y; e++, c++)
-                                    if (c >= cb && c < cb + lb) eb[o++] = sqb[s].x + e;
-                        }
-                        __syncthreads();
-                        const int n = la * lb;
-                        const int iters = (n + (int)blockDim.x - 1) / (int)blockDim.x;  // uniform
-                        for (int it = 0; it < iters; it++) {
-                            const int idx = it * (int)blockDim.x + tid;
-                            if (idx < n) {
-                                const int i = idx / lb, j = idx % lb;
-                                const uint32_t va = ea[i], vb = eb[j];
-                                tests++;
-                                if (seg_intersect(AX[va], AY[va], AX[va + 1], AY[va + 1], BX[vb], BY[vb], BX[vb + 1],
-                                                  BY[vb + 1]))
-                                    hit = true;
-                            }
-                            if ((it & 15) == 15) {
-                                if (__syncthreads_or(hit)) {
-                                    hit = true;
-                                    break;
+                // the pixels' sequences, kPlainSeqs at a time on each side
+                for (uint32_t s0a = pofa[pa]; s0a < pofa[pa + 1] && !hit; s0a += kPlainSeqs) {
+                    const uint32_t nsa = min((uint32_t)kPlainSeqs, pofa[pa + 1] - s0a);
+                    const uint32_t na = plain_scan(sqa + s0a, nsa, offa, wsum);
+                    for (uint32_t s0b = pofb[pb]; s0b < pofb[pb + 1] && !hit; s0b += kPlainSeqs) {
+                        const uint32_t nsb = min((uint32_t)kPlainSeqs, pofb[pb + 1] - s0b);
+                        const uint32_t nb = plain_scan(sqb + s0b, nsb, offb, wsum);
+                        for (uint32_t ca = 0; ca < na && !hit; ca += kPlainCap) {
+                            const uint32_t la = min((uint32_t)kPlainCap, na - ca);
+                            __syncthreads();
+                            plain_fill(sqa + s0a, offa, nsa, ca, la, AX, AY, ea);
+                            for (uint32_t cb = 0; cb < nb && !hit; cb += kPlainCap) {
+                                const uint32_t lb = min((uint32_t)kPlainCap, nb - cb);
+                                __syncthreads();
+                                plain_fill(sqb + s0b, offb, nsb, cb, lb, BX, BY, eb);
+                                __syncthreads();
+                                const uint32_t bi = (la + 15) / 16, bj = (lb + 15) / 16;
+                                uint32_t blk = 0;
+                                bool done = false;  // block-uniform: set only from __syncthreads_or
+                                for (uint32_t i0 = 0; i0 < bi && !done; i0++) {
+                                    const uint32_t i = 16 * i0 + ti;
+                                    const float4 a = ea[min(i, la - 1)];
+                                    for (uint32_t j0 = 0; j0 < bj; j0++, blk++) {
+                                        const uint32_t j = 16 * j0 + tj;
+                                        if (i < la && j < lb) {
+                                            const float4 b = eb[j];
+                                            tests++;
+                                            if (seg_intersect(a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w)) hit = true;
+                                        }
+                                        if ((blk & 7) == 7 && __syncthreads_or(hit)) {
+                                            done = true;
+                                            break;
+                                        }
+                                    }
                                 }
+                                hit = __syncthreads_or(hit);
                             }
                         }
-                        hit = __syncthreads_or(hit);
                     }
                 }
             }
@@ -847,15 +934,15 @@ __global__ __launch_bounds__(256) void k_plain(DevSet S, const uint32_t *__restr
     if (tid == 0 && hit) result[k] = 1;
 }
 
-// dx[v] = x[v + 1] - x[v], dy[v] = y[v + 1] - y[v] over n padded vertices;
-// the same rounded f32 differences the refinement would form per step
+// xyd[v] = (x[v], y[v], x[v] - x[v - 1], y[v] - y[v - 1]) over n padded
+// vertices: the vertex and the same rounded f32 difference the refinement
+// would form for the step arriving at it
 __global__ __launch_bounds__(256) void k_vdiff(const float *__restrict__ x, const float *__restrict__ y, uint64_t n,
-                                               float *__restrict__ dx, float *__restrict__ dy) {
+                                               float4 *__restrict__ xyd) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const bool last = v + 1 == n;
-    dx[v] = last ? 0.f : x[v + 1] - x[v];
-    dy[v] = last ? 0.f : y[v + 1] - y[v];
+    const float vx = x[v], vy = y[v];
+    xyd[v] = v == 0 ? make_float4(vx, vy, 0.f, 0.f) : make_float4(vx, vy, vx - x[v - 1], vy - y[v - 1]);
 }
 
 // per-pixel edge count of every BORDER pixel (sum of its sequence lengths)
@@ -894,7 +981,7 @@ __global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ r
                                          uint32_t *);                                                          \
     template __global__ void k_refine<EE>(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *,          \
                                           const PixPair *, const uint32_t *, const uint2 *, uint64_t, uint64_t, \
-                                          uint32_t *, uint32_t *, unsigned long long *);
+                                          uint32_t *, uint32_t *, unsigned long long *, int);
 FARM_REFINE_INST(1)
 FARM_REFINE_INST(2)
 FARM_REFINE_INST(4)

@@ -212,6 +212,27 @@ __device__ int seg_contain_wave(const DevSet &S, const farm_raster_hdr &h, float
     return (cnt & 1) ? 2 : 0;
 }
 
+// The IN pieces of a pair go to buf (when requested), one atomic per wave
+// and chunk; beyond cap they are only counted.
+struct PieceOut {
+    farm_piece *buf;
+    uint32_t *cnt;
+    uint32_t cap;
+    uint32_t pair;
+};
+
+__device__ __forceinline__ void put_pieces(const PieceOut &po, bool mine, const farm_piece &p, int lane) {
+    const uint64_t m = __ballot(mine);
+    if (!po.buf || !m) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(po.cnt, (uint32_t)__popcll(m));
+    base = uni(__shfl(base, 0, 64));
+    if (mine) {
+        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (k < po.cap) po.buf[k] = p;
+    }
+}
+
 struct PairRes {
     int32_t c[5];  // [crossing points, pieces, IN, OUT, BORDER]
     double area;
@@ -242,7 +263,8 @@ __device__ __forceinline__ void path_pt(const float *X, const float *Y, int m, i
 // of the cross-product sum of the area pieces IN the other polygon.
 template <bool SRC>
 __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const farm_raster_hdr &hq, const Rec *D,
-                              uint32_t n, double ox, double oy, int lane, PairRes &r, uint32_t &pieces) {
+                              uint32_t n, double ox, double oy, int lane, PairRes &r, uint32_t &pieces,
+                              const PieceOut &po) {
     const float *X = S.x + hp.vert_start, *Y = S.y + hp.vert_start;
     const int m = (int)hp.nverts - 1;
     const double eps = (double)1e-6f;  // util.h:44
@@ -317,6 +339,7 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
         r.c[3] += __popcll(__ballot(piece && st == 0));
         r.c[4] += __popcll(__ballot(piece && st == 1));
         pieces += __popcll(pm);
+        put_pieces(po, in_area, farm_piece{po.pair, SRC ? 0 : 1, first, nvert, ax, ay, bx, by}, lane);
         // wave-cooperative path sums of this chunk's IN pieces
         uint64_t inm = __ballot(in_area);
         while (inm) {
@@ -576,7 +599,7 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
 // record slabs of cap records each (cap a power of two).
 __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const double *ring2a, uint32_t ia, uint32_t ib,
                                                      Rec *R, Rec *D, uint32_t cap, uint32_t *cnt, EdgeQueue &Q,
-                                                     int lane) {
+                                                     int lane, const PieceOut &po) {
     PairRes r;
     for (int k = 0; k < 5; k++) r.c[k] = 0;
     r.area = 0.0;
@@ -606,6 +629,7 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
     if (itn == npx) {
         r.c[0] = -2;
         r.area = fabs(ring2a[ib]) / 2.0;
+        put_pieces(po, lane == 0, farm_piece{po.pair, 1, 0, -1, 0.f, 0.f, 0.f, 0.f}, lane);  // all of B
         return r;
     }
     // 2. crossing points of the edges of BORDER x BORDER pixel pairs
@@ -689,15 +713,20 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
     double acc_a = 0.0, acc_b = 0.0;
     if (nk > 0) {
         bitonic<1>(D, nk, lane);
-        acc_a = ring_pieces<true>(S, hA, hB, D, nk, ox, oy, lane, r, pieces);
+        acc_a = ring_pieces<true>(S, hA, hB, D, nk, ox, oy, lane, r, pieces, po);
         wave_sync();
         bitonic<2>(D, nk, lane);
-        acc_b = ring_pieces<false>(S, hB, hA, D, nk, ox, oy, lane, r, pieces);
+        acc_b = ring_pieces<false>(S, hB, hA, D, nk, ox, oy, lane, r, pieces, po);
     }
     if (pieces == 0) {
         // containment or disjoint
-        if (seg_contain_wave(S, hB, AX[0], AY[0], lane) == 2) r.area = fabs(ring2a[ia]) / 2.0;
-        else if (seg_contain_wave(S, hA, BX[0], BY[0], lane) == 2) r.area = fabs(ring2a[ib]) / 2.0;
+        if (seg_contain_wave(S, hB, AX[0], AY[0], lane) == 2) {
+            r.area = fabs(ring2a[ia]) / 2.0;
+            put_pieces(po, lane == 0, farm_piece{po.pair, 0, 0, -1, 0.f, 0.f, 0.f, 0.f}, lane);  // all of A
+        } else if (seg_contain_wave(S, hA, BX[0], BY[0], lane) == 2) {
+            r.area = fabs(ring2a[ib]) / 2.0;
+            put_pieces(po, lane == 0, farm_piece{po.pair, 1, 0, -1, 0.f, 0.f, 0.f, 0.f}, lane);  // all of B
+        }
         return r;
     }
     const double sa = ring2a[ia] < 0 ? -1.0 : 1.0, sb = ring2a[ib] < 0 ? -1.0 : 1.0;
@@ -732,7 +761,8 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
                                                       int32_t *__restrict__ counts, double *__restrict__ area,
                                                       Rec *__restrict__ slab, uint32_t cap,
                                                       uint32_t *__restrict__ ctr, uint32_t *__restrict__ ovf,
-                                                      unsigned long long *__restrict__ stats) {
+                                                      unsigned long long *__restrict__ stats,
+                                                      farm_piece *__restrict__ pcs, uint32_t pcap) {
     const int lane = threadIdx.x & 63;
     const uint64_t gw = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     Rec *R = slab + gw * 2ull * cap;
@@ -756,7 +786,8 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
                 if (lane == 0) area[idx] = 0.0;
                 continue;
             }
-            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, cnt, Q, lane);
+            const PieceOut po{pcs, ctr + 2, pcap, idx};
+            const PairRes r = intersection_pair(S, ring2a, ia, ib, R, D, cap, cnt, Q, lane, po);
             if (r.over) {
                 if (lane == 0) {
                     if (BIG) {  // still too many: the host reruns it with larger slabs
@@ -787,10 +818,10 @@ __global__ __launch_bounds__(256) void k_intersection(DevSet S, const double *__
 
 template __global__ void k_intersection<false>(DevSet, const double *, const uint32_t *, const uint32_t *, uint32_t,
                                                const uint32_t *, int32_t *, double *, Rec *, uint32_t, uint32_t *,
-                                               uint32_t *, unsigned long long *);
+                                               uint32_t *, unsigned long long *, farm_piece *, uint32_t);
 template __global__ void k_intersection<true>(DevSet, const double *, const uint32_t *, const uint32_t *, uint32_t,
                                               const uint32_t *, int32_t *, double *, Rec *, uint32_t, uint32_t *,
-                                              uint32_t *, unsigned long long *);
+                                              uint32_t *, unsigned long long *, farm_piece *, uint32_t);
 
 }  // namespace farm
 
@@ -799,7 +830,7 @@ using namespace farm;
 namespace {
 
 int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,
-                     farm_stats *st) {
+                     farm_stats *st, farm_piece *pcs_dev = nullptr, uint32_t pcap = 0, uint64_t *npcs = nullptr) {
     if (!h) return FARM_EINVAL;
     if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
     if (h->has_layers) return fail(h, FARM_ESTATE, "intersection needs a set rasterized without -h");
@@ -835,7 +866,7 @@ int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *count
     HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
     hipLaunchKernelGGL(k_intersection<false>, dim3(nblocks), dim3(64 * kWaves), 0, h->stream, S,
                        (const double *)P<double>(h->ring2a), pa, pb, n, (const uint32_t *)nullptr, counts_dev, area_dev,
-                       P<Rec>(h->wslab), kRecCap, ctr, P<uint32_t>(h->wovf), ist);
+                       P<Rec>(h->wslab), kRecCap, ctr, P<uint32_t>(h->wovf), ist, pcs_dev, pcap);
     HIPCHK(h, hipGetLastError());
     uint32_t novf = 0;
     HIPCHK(h, hipMemcpyAsync(&novf, ctr + 1, 4, hipMemcpyDeviceToHost, h->stream));
@@ -847,7 +878,8 @@ int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *count
         if ((rc = fill_u32(h, ctr, 1, 0))) return rc;
         hipLaunchKernelGGL(k_intersection<true>, dim3(bblocks), dim3(64 * kWaves), 0, h->stream, S,
                            (const double *)P<double>(h->ring2a), pa, pb, novf, (const uint32_t *)P<uint32_t>(h->wovf),
-                           counts_dev, area_dev, P<Rec>(h->wslab_big), kRecCapBig, ctr, P<uint32_t>(h->wovf), ist);
+                           counts_dev, area_dev, P<Rec>(h->wslab_big), kRecCapBig, ctr, P<uint32_t>(h->wovf), ist,
+                           pcs_dev, pcap);
         HIPCHK(h, hipGetLastError());
     }
     HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
@@ -883,7 +915,7 @@ int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *count
             hipLaunchKernelGGL(k_intersection<true>, dim3(blocks), dim3(64 * kWaves), 0, h->stream, S,
                                (const double *)P<double>(h->ring2a), pa, pb, (uint32_t)big.size(),
                                (const uint32_t *)P<uint32_t>(h->wovf), counts_dev, area_dev, P<Rec>(h->wslab_big),
-                               (uint32_t)cap, ctr, P<uint32_t>(h->wovf), ist);
+                               (uint32_t)cap, ctr, P<uint32_t>(h->wovf), ist, pcs_dev, pcap);
             HIPCHK(h, hipGetLastError());
             HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
             HIPCHK(h, hipMemcpyAsync(hst, ist, sizeof hst, hipMemcpyDeviceToHost, h->stream));
@@ -896,6 +928,11 @@ int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *count
                                 (unsigned long long)(begin + idx), (unsigned long long)cap);
             }
         }
+    }
+    if (npcs) {
+        uint32_t np = 0;
+        HIPCHK(h, hipMemcpy(&np, ctr + 2, 4, hipMemcpyDeviceToHost));
+        *npcs = np;
     }
     if (st) {
         float ms = 0;
@@ -918,6 +955,28 @@ int farm_intersection_area_dev(farm_handle h, uint64_t begin, uint64_t end, int3
                                farm_stats *st) {
     if (!counts_dev || !area_dev) return fail(h, FARM_EINVAL, "null output buffer");
     return run_intersection(h, begin, end, counts_dev, area_dev, st);
+}
+
+int farm_intersection_pieces(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts, double *area,
+                             farm_piece *pieces, uint64_t cap, uint64_t *npieces, farm_stats *st) {
+    if (!h) return FARM_EINVAL;
+    if (begin > end || !npieces || (cap && !pieces)) return fail(h, FARM_EINVAL, "bad arguments");
+    if (cap > 0xffffffffull) cap = 0xffffffffull;
+    const uint64_t n = end - begin;
+    int rc;
+    if ((rc = ensure(h, h->icounts, 20 * n + 16))) return rc;
+    if ((rc = ensure(h, h->iarea, 8 * n + 16))) return rc;
+    if ((rc = ensure(h, h->ipieces, sizeof(farm_piece) * (cap ? cap : 1)))) return rc;
+    if ((rc = run_intersection(h, begin, end, P<int32_t>(h->icounts), P<double>(h->iarea), st,
+                               P<farm_piece>(h->ipieces), (uint32_t)cap, npieces)))
+        return rc;
+    if (n && counts) HIPCHK(h, hipMemcpy(counts, h->icounts.p, 20 * n, hipMemcpyDeviceToHost));
+    if (n && area) HIPCHK(h, hipMemcpy(area, h->iarea.p, 8 * n, hipMemcpyDeviceToHost));
+    if (*npieces > cap)
+        return fail(h, FARM_ECAPACITY, "%llu pieces, room for %llu", (unsigned long long)*npieces,
+                    (unsigned long long)cap);
+    if (*npieces) HIPCHK(h, hipMemcpy(pieces, h->ipieces.p, sizeof(farm_piece) * *npieces, hipMemcpyDeviceToHost));
+    return FARM_OK;
 }
 
 int farm_intersection_area(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts, double *area,

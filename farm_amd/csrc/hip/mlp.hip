@@ -73,7 +73,8 @@ __global__ __launch_bounds__(128) void k_features(DevSet S, const uint32_t *__re
                         continue;
                     }
                     n_bb++;
-                    const double xa = apx_area(fa, area_a), xb = apx_area(fb, area_b);
+                    const double xa = apx_area_cc(S.fill[A.pix_start + pa], area_a, S.cc);
+                    const double xb = apx_area_cc(S.fill[B.pix_start + pb], area_b, S.cc);
                     const double prob = (xa + xb) / amax;
                     max_prob = fmax(max_prob, prob);
                     sum_prob += prob;

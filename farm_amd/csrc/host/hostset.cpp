@@ -10,6 +10,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <thread>
 #include <vector>
 
@@ -37,6 +38,9 @@ struct farm_hostset {
     std::vector<uint32_t> pixoff, seqs, voff;
     std::vector<double> vnodes;
     std::vector<float> vx, vy;
+    // -w b: fill classes at cc = 2^b categories (empty: the status bytes, 16)
+    std::vector<uint8_t> fill;
+    int cc = 16;
     // hierarchy (-h)
     bool hier = false;
     std::vector<farm_layer> layers;
@@ -562,6 +566,44 @@ int farm_hostset_view(const farm_hostset *s, farm_polyset *o) {
     o->lstatus = s->hier ? s->lstatus.data() : nullptr;
     o->nlstatus = s->hier ? s->lstatus.size() : 0;
     o->areas = s->hier ? s->areas.data() : nullptr;
+    o->fill = s->fill.empty() ? nullptr : s->fill.data();
+    o->category_count = s->cc;
+    o->reserved = 0;
+    return FARM_OK;
+}
+
+int farm_hostset_set_categories(farm_hostset *s, int category_bits) {
+    if (!s->rastered) return FARM_ESTATE;
+    if (category_bits < 2 || category_bits > 8) return FARM_EINVAL;
+    const int cc = 1 << category_bits;
+    s->cc = cc;
+    s->fill.clear();
+    if (cc == FARM_CATEGORY_COUNT) return FARM_OK;
+    s->fill.assign(s->status.size(), 0);
+    for (const farm_raster_hdr &h : s->hdr) {
+        const double pix_area = h.step[0] * h.step[1];
+        const uint64_t np = (uint64_t)h.dimx * (uint64_t)h.dimy;
+        for (uint64_t p = 0; p < np; p++) {
+            const uint8_t st = s->status[h.pix_start + p];
+            uint8_t f = 0;
+            if (st == FARM_CATEGORY_COUNT - 1) {
+                f = (uint8_t)(cc - 1);
+            } else if (st != 0) {
+                // encodePixelArea (Ideal.cpp:64-88) with cc categories, then the
+                // BORDER clamp of calculate_fullness (Ideal.cpp:905-912)
+                const double ratio = s->areas[h.pix_start + p] / pix_area;
+                int c;
+                if (std::abs(ratio - 1.0) < 1e-9) c = cc - 1;
+                else if (std::abs(ratio) < 1e-9) c = 0;
+                else {
+                    c = (int)std::ceil(ratio * (cc - 2));
+                    if (c >= cc) c = cc - 1;
+                }
+                f = (uint8_t)std::max(1, std::min(c, cc - 2));
+            }
+            s->fill[h.pix_start + p] = f;
+        }
+    }
     return FARM_OK;
 }
 
@@ -732,6 +774,118 @@ struct Dpk {
 const Dpk kDpk;
 
 }  // namespace
+
+// Rings of A n B from the IN pieces of each pair (the chaining of the
+// reference's rebuild_polygons, query_intersection.cu:393-502): start at an
+// unused piece, follow a -> ring vertices -> b, continue with the unused
+// piece that starts at b, until the chain returns to its start.
+extern "C" char *farm_pieces_wkt(const farm_hostset *s, const uint32_t *pa, const uint32_t *pb, uint64_t npairs,
+                                 const farm_piece *pieces, uint64_t npieces, uint64_t *len) {
+    if (!s || !s->rastered || (npieces && !pieces) || (npairs && (!pa || !pb))) return nullptr;
+    std::vector<uint64_t> order(npieces);
+    for (uint64_t i = 0; i < npieces; i++) {
+        if (pieces[i].pair >= npairs || pieces[i].ring < 0 || pieces[i].ring > 1) return nullptr;
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint64_t x, uint64_t y) { return pieces[x].pair < pieces[y].pair; });
+    auto key = [](float x, float y) {
+        uint32_t ux, uy;
+        memcpy(&ux, &x, 4);
+        memcpy(&uy, &y, 4);
+        return ((uint64_t)ux << 32) | uy;
+    };
+    std::string out;
+    char num[64];
+    auto put = [&](float x, float y, bool first) {
+        snprintf(num, sizeof num, "%s%.9g %.9g", first ? "" : ", ", (double)x, (double)y);
+        out += num;
+    };
+    for (uint64_t lo = 0; lo < npieces;) {
+        uint64_t hi = lo;
+        const uint32_t k = pieces[order[lo]].pair;
+        while (hi < npieces && pieces[order[hi]].pair == k) hi++;
+        const uint32_t ids[2] = {pa[k], pb[k]};
+        if ((int64_t)ids[0] >= npoly(s) || (int64_t)ids[1] >= npoly(s)) return nullptr;
+        snprintf(num, sizeof num, "%u\tMULTIPOLYGON (", k);
+        out += num;
+        bool first_poly = true;
+        // pieces follow their ring's direction; when the two rings turn
+        // opposite ways the target's pieces are walked backwards, so that
+        // every piece ends where the next one starts
+        auto orient = [&](uint32_t id) {
+            const farm_raster_hdr &h = s->hdr[id];
+            double a = 0;
+            const double ox = s->vx[h.vert_start], oy = s->vy[h.vert_start];
+            for (uint32_t v = 0; v + 1 < h.nverts; v++)
+                a += ((double)s->vx[h.vert_start + v] - ox) * ((double)s->vy[h.vert_start + v + 1] - oy) -
+                     ((double)s->vx[h.vert_start + v + 1] - ox) * ((double)s->vy[h.vert_start + v] - oy);
+            return a < 0 ? -1 : 1;
+        };
+        const bool flip_b = orient(ids[0]) != orient(ids[1]);
+        auto rev = [&](const farm_piece &p) { return flip_b && p.ring == 1; };
+        std::unordered_map<uint64_t, std::vector<uint64_t>> starts;
+        std::vector<char> used(hi - lo, 0);
+        for (uint64_t i = lo; i < hi; i++) {
+            const farm_piece &p = pieces[order[i]];
+            if (p.nvert >= 0) starts[rev(p) ? key(p.bx, p.by) : key(p.ax, p.ay)].push_back(i - lo);
+        }
+        for (uint64_t i = lo; i < hi; i++) {
+            if (used[i - lo]) continue;
+            const farm_piece &p0 = pieces[order[i]];
+            const farm_raster_hdr &h0 = s->hdr[ids[p0.ring]];
+            out += first_poly ? "((" : ", ((";
+            first_poly = false;
+            if (p0.nvert < 0) {
+                // a whole ring (containment)
+                for (uint32_t v = 0; v < h0.nverts; v++) put(s->vx[h0.vert_start + v], s->vy[h0.vert_start + v], v == 0);
+                used[i - lo] = 1;
+                out += "))";
+                continue;
+            }
+            bool first_pt = true;
+            uint64_t cur = i - lo;
+            const float sx0 = rev(p0) ? p0.bx : p0.ax, sy0 = rev(p0) ? p0.by : p0.ay;
+            for (;;) {
+                used[cur] = 1;
+                const farm_piece &p = pieces[order[lo + cur]];
+                const farm_raster_hdr &h = s->hdr[ids[p.ring]];
+                const int m = (int)h.nverts - 1;
+                const bool r = rev(p);
+                put(r ? p.bx : p.ax, r ? p.by : p.ay, first_pt);
+                first_pt = false;
+                for (int t = 0; t < p.nvert; t++) {
+                    const int v = (p.first + (r ? p.nvert - 1 - t : t)) % m;
+                    put(s->vx[h.vert_start + v], s->vy[h.vert_start + v], false);
+                }
+                // the next piece starts where this one ends
+                const float ex = r ? p.ax : p.bx, ey = r ? p.ay : p.by;
+                uint64_t next = UINT64_MAX;
+                auto it = starts.find(key(ex, ey));
+                if (it != starts.end())
+                    for (uint64_t c : it->second)
+                        if (!used[c]) {
+                            next = c;
+                            break;
+                        }
+                if (next == UINT64_MAX) {
+                    put(ex, ey, false);
+                    break;
+                }
+                cur = next;
+            }
+            put(sx0, sy0, false);  // close the ring at its start
+            out += "))";
+        }
+        out += ")\n";
+        lo = hi;
+    }
+    char *r = (char *)malloc(out.size() + 1);
+    if (!r) return nullptr;
+    memcpy(r, out.c_str(), out.size() + 1);
+    if (len) *len = out.size();
+    return r;
+}
 
 extern "C" int farm_km_table(double *out) {
     if (!out) return 1;

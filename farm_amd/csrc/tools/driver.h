@@ -38,6 +38,8 @@ struct DriverArgs {
     int tile = 0;
     bool plain = false;
     bool gpu_candidates = false;  // build the candidate pairs on the device instead of the host R-tree
+    std::string model;            // trained MLP for the learned -a mode (--model)
+    std::string wkt;              // intersection: write the polygons of A n B here (--wkt)
 };
 
 inline double now_ms() {
@@ -60,16 +62,22 @@ inline void usage(const char *prog) {
            "  -t, --target PATH       target polygons (.dt)\n"
            "  -n, --threads N         host threads\n"
            "  -v, --vpr N             vertices per raster pixel (default 10)\n"
-           "  -b, --batch_size N      candidate pairs per GPU batch (0 = all)\n"
+           "  -b, --batch_size N      candidate pairs per GPU batch (0 = all; a batch whose pixel\n"
+           "                          pairs exceed the per-call limit is split in halves)\n"
            "  -m, --merge_threshold X accepted for compatibility (exact results do not depend on it)\n"
-           "  -w, --category_bits B   2^B fill categories for approximate modes (default 4)\n"
+           "  -w, --category_bits B   intersect: 2^B fill categories (B in 2..8, default 4) for -a and the\n"
+           "                          learned features; exact results do not depend on it\n"
            "  -d, --within_distance D within distance in km (default 10)\n"
            "  --NLow N                coarsest hierarchy layer size (default 1)\n"
            "  -u, --unorll_size N     accepted for compatibility\n"
            "  -r, -g, -a, -h          rasterize / gpu / approximation / hierarchy\n"
            "  --plain                 one-pair-per-workgroup kernel instead of the batched one\n"
-           "  --tile E                source edges per lane in the batched kernel (1, 2, 4)\n"
+           "  --tile E                source edges per lane in the batched kernel (1, 2, 4, 8)\n"
+           "  --model FILE            with -a: the learned mode with this trained MLP (farm_amd/learned.py\n"
+           "                          writes it) instead of the reference heuristic\n"
            "  --gpu-candidates        build the candidate pairs on the device (same pairs as the R-tree)\n"
+           "  --wkt FILE              intersection: write A n B of every intersecting pair as WKT lines\n"
+           "                          <source id> <target id> MULTIPOLYGON (...), tab-separated\n"
            "  --device N              HIP device\n",
            prog);
 }
@@ -112,6 +120,8 @@ inline bool parse_args(int argc, char **argv, DriverArgs &a) {
         else if (k == "--device") a.device = atoi(val());
         else if (k == "--tile") a.tile = atoi(val());
         else if (k == "--gpu-candidates") a.gpu_candidates = true;
+        else if (k == "--model") a.model = val();
+        else if (k == "--wkt") a.wkt = val();
         else {
             // boost::program_options rejection of the reference (query_context.cpp:250-255)
             printf("Undefined Option!\n");
@@ -119,6 +129,10 @@ inline bool parse_args(int argc, char **argv, DriverArgs &a) {
         }
     }
     if (a.threads < 1) a.threads = 1;
+    if (a.category_bits < 2 || a.category_bits > 8) {
+        fprintf(stderr, "-w must be in 2..8 (2^w fill categories in a byte)\n");
+        return false;
+    }
     if (a.vpr < 1) {
         fprintf(stderr, "-v must be positive\n");
         return false;
@@ -162,3 +176,48 @@ inline uint64_t device_candidates(farm_handle h, const farm_hostset *src, const 
     logf("device candidate pairs: %llu takes %.3f ms", (unsigned long long)n, now_ms() - t);
     return n;
 }
+
+// A trained MLP for the learned -a mode, as farm_amd/learned.py save_model
+// writes it: the 8 bytes "FARMMLP1", u32 parameter count, u32 feature count,
+// f32 accept and reject logits, the parameters, then the feature
+// normalisation (means, then reciprocal deviations), all little-endian.
+struct MlpModel {
+    std::vector<float> params, norm;
+    float accept = 0, reject = 0;
+};
+
+inline bool load_model(const std::string &path, MlpModel &m) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char magic[8];
+    uint32_t np = 0, nf = 0;
+    bool ok = fread(magic, 1, 8, f) == 8 && !memcmp(magic, "FARMMLP1", 8) && fread(&np, 4, 1, f) == 1 &&
+              fread(&nf, 4, 1, f) == 1 && np == FARM_MLP_NPARAMS && nf == FARM_NFEAT &&
+              fread(&m.accept, 4, 1, f) == 1 && fread(&m.reject, 4, 1, f) == 1;
+    if (ok) {
+        m.params.resize(np);
+        m.norm.resize(2 * nf);
+        ok = fread(m.params.data(), 4, np, f) == np && fread(m.norm.data(), 4, 2 * nf, f) == 2 * nf;
+    }
+    fclose(f);
+    return ok;
+}
+
+// farm_intersect over pairs [b, e), split in halves while a window's pixel
+// pairs exceed the per-call limit (FARM_ECAPACITY) or the device memory
+// (FARM_ENOMEM); adds the counters.
+inline void intersect_split(farm_handle h, uint64_t b, uint64_t e, const farm_opts *o, uint64_t &found,
+                            uint64_t &tests) {
+    farm_stats st;
+    const int rc = farm_intersect(h, b, e, o, nullptr, &st);
+    if ((rc == FARM_ECAPACITY || rc == FARM_ENOMEM) && e - b > 1) {
+        const uint64_t m = b + (e - b) / 2;
+        intersect_split(h, b, m, o, found, tests);
+        intersect_split(h, m, e, o, found, tests);
+        return;
+    }
+    DIE_IF(h, rc, "intersect");
+    found += st.found;
+    tests += st.segment_tests;
+}
+

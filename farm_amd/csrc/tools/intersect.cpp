@@ -45,6 +45,11 @@ int main(int argc, char **argv) {
              (long long)(bad < ns ? bad : bad - ns));
         return 1;
     }
+    // -w b: fill classes at 2^b categories for -a (exact results do not depend on it)
+    if (farm_hostset_set_categories(all, a.category_bits) != FARM_OK) {
+        logf("-w %d: bad category bits", a.category_bits);
+        return 1;
+    }
     logf("preprocess finished takes %.3f ms", now_ms() - t);
 
     farm_handle h = nullptr;
@@ -65,16 +70,29 @@ int main(int argc, char **argv) {
     o.mode = a.approx ? FARM_MODE_APPROX_HEURISTIC : (a.plain ? FARM_MODE_EXACT_PLAIN : FARM_MODE_EXACT_BATCHED);
     o.tile_a = a.tile;
     o.merge_threshold = a.merge_threshold;
-    uint64_t batch = a.batch ? a.batch : (uint64_t)npairs;
+    if (!a.model.empty()) {
+        if (!a.approx) {
+            logf("--model needs -a");
+            return 1;
+        }
+        MlpModel m;
+        if (!load_model(a.model, m)) {
+            logf("%s is not a trained MLP file", a.model.c_str());
+            return 1;
+        }
+        DIE_IF(h, farm_mlp_set_params(h, m.params.data(), m.params.size(), m.norm.data()), "load MLP");
+        o.mode = FARM_MODE_APPROX_LEARNED;
+        o.approx_threshold = m.accept;
+        o.approx_reject = m.reject;
+        logf("learned -a: accept logit > %g, reject < %g, exact refinement between", m.accept, m.reject);
+    }
+    // -b 0 is one batch of all pairs, as the reference; any batch whose
+    // BORDER x BORDER pixel pairs exceed the per-call limit is split in halves
+    const uint64_t batch = a.batch ? a.batch : std::max<uint64_t>(1, (uint64_t)npairs);
     uint64_t found = 0, tests = 0;
     t = now_ms();
-    for (uint64_t b = 0; b < (uint64_t)npairs; b += batch) {
-        const uint64_t e = std::min<uint64_t>(b + batch, (uint64_t)npairs);
-        farm_stats st;
-        DIE_IF(h, farm_intersect(h, b, e, &o, nullptr, &st), "intersect");
-        found += st.found;
-        tests += st.segment_tests;
-    }
+    for (uint64_t b = 0; b < (uint64_t)npairs; b += batch)
+        intersect_split(h, b, std::min<uint64_t>(b + batch, (uint64_t)npairs), &o, found, tests);
     const double qms = now_ms() - t;
     logf("count-found:\t%llu", (unsigned long long)found);
     logf("segment-tests:\t%llu", (unsigned long long)tests);

@@ -66,13 +66,57 @@ int main(int argc, char **argv) {
     double total_area = 0.0;
     std::vector<int32_t> counts;
     std::vector<double> area;
+    FILE *wkt = nullptr;
+    if (!a.wkt.empty() && !(wkt = fopen(a.wkt.c_str(), "w"))) {
+        logf("cannot write %s", a.wkt.c_str());
+        return 1;
+    }
+    std::vector<farm_piece> pcs(1 << 16);
+    std::vector<uint32_t> wa, wb;
     t = now_ms();
     for (uint64_t b = 0; b < (uint64_t)npairs; b += batch) {
         const uint64_t e = std::min<uint64_t>(b + batch, (uint64_t)npairs);
         counts.resize(5 * (e - b));
         area.resize(e - b);
         farm_stats st;
-        DIE_IF(h, farm_intersection_area(h, b, e, counts.data(), area.data(), &st), "intersection");
+        if (!wkt) {
+            DIE_IF(h, farm_intersection_area(h, b, e, counts.data(), area.data(), &st), "intersection");
+        } else {
+            // the boundary pieces of A n B, then their rings as WKT
+            uint64_t np = 0;
+            int rc = farm_intersection_pieces(h, b, e, counts.data(), area.data(), pcs.data(), pcs.size(), &np, &st);
+            if (rc == FARM_ECAPACITY && np > pcs.size()) {
+                pcs.resize(np);
+                rc = farm_intersection_pieces(h, b, e, counts.data(), area.data(), pcs.data(), pcs.size(), &np, &st);
+            }
+            DIE_IF(h, rc, "intersection pieces");
+            wa.resize(e - b);
+            wb.resize(e - b);
+            if (a.gpu_candidates) DIE_IF(h, farm_download_pairs(h, b, e, wa.data(), wb.data()), "download pairs");
+            else {
+                std::copy(pa + b, pa + e, wa.begin());
+                std::copy(pb + b, pb + e, wb.begin());
+            }
+            uint64_t len = 0;
+            char *text = farm_pieces_wkt(all, wa.data(), wb.data(), e - b, pcs.data(), np, &len);
+            if (!text) {
+                logf("could not assemble the polygons of A n B");
+                return 1;
+            }
+            // "<k>\t<wkt>" per window index k -> "<source id>\t<target id>\t<wkt>"
+            for (char *line = text; *line;) {
+                char *nl = strchr(line, '\n');
+                if (nl) *nl = 0;
+                char *tab = strchr(line, '\t');
+                if (tab) {
+                    const uint64_t k = strtoull(line, nullptr, 10);
+                    fprintf(wkt, "%u\t%u\t%s\n", wa[k], wb[k] - (uint32_t)ns, tab + 1);
+                }
+                if (!nl) break;
+                line = nl + 1;
+            }
+            farm_free(text);
+        }
         found += st.found;
         for (uint64_t k = 0; k < e - b; k++) {
             if (counts[5 * k] > 0) points += (uint64_t)counts[5 * k];
@@ -86,6 +130,7 @@ int main(int argc, char **argv) {
     logf("boundary-pieces:\t%llu", (unsigned long long)pieces);
     logf("total-area:\t%.12g", total_area);
     logf("query finished takes %.3f ms (%.1f pairs/s)", qms, npairs / (qms / 1000.0));
+    if (wkt) fclose(wkt);
     farm_close(h);
     farm_free(pa);
     farm_free(pb);

@@ -13,8 +13,9 @@ trained on the GPU by libfarm's own forward/backward/Adam kernels
                   pair decides false); pairs whose logit falls in a band
                   around the threshold can fall back to exact refinement.
 
-The workload is the star-polygon pair set (many seeded instances of 2,000 x
-2,000 polygons in [0,100]^2), so pairs never mix instances.
+The workload is the plain intersect workload's distribution: star polygons
+at its density, as many seeded instances of 2,000 x 2,000 polygons in
+[0,100]^2 (build_pairsets) or as one larger box (star_domain).
 """
 from __future__ import annotations
 
@@ -148,52 +149,6 @@ def precision_recall(pred: np.ndarray, truth: np.ndarray) -> tuple[float, float]
     return (tp / max(tp + fp, 1), tp / max(tp + fn, 1))
 
 
-def evaluate(train_pairs: int = 500_000, eval_pairs: int = 2_000_000, target: float = 0.99, epochs: int = 12,
-             seed: int = 1) -> dict:
-    """Learned -a on the star-pair distribution: train on 80% of
-    `train_pairs` labeled pairs of seeded instances, pick the threshold for
-    `target` precision (with a 2-sigma margin) on the held-out 20%, evaluate
-    on `eval_pairs` pairs of fresh seeds (exact labels, the reference
-    heuristic and the filter alone on the same pairs)."""
-    import time
-
-    per_instance = 4200  # candidate pairs of one 2,000 x 2,000 instance, about
-    n_tr = max(1, train_pairs // per_instance)
-    n_ev = max(1, eval_pairs // per_instance)
-    dev = capi.Device(0)
-    t = time.time()
-    tr_ps = build_pairsets([10_000 + seed * 100_000 + i for i in range(n_tr)])
-    tr, cal = split_calibration(label(dev, tr_ps), 0.2, seed)
-    norm, losses = train(dev, tr, seed=seed, epochs=epochs)
-    z_cal = dev.mlp_forward(apply_norm(cal.feat, norm))[:, 0]
-    thr = threshold_for_precision(cal.decided, z_cal, cal.bits.astype(bool), target)
-    t_train = time.time() - t
-    ev_ps = build_pairsets([50_000 + seed * 100_000 + i for i in range(n_ev)])
-    ev = label(dev, ev_ps)
-    truth = ev.bits.astype(bool)
-    res = {"train_pairs": int(len(tr.bits)), "eval_pairs": int(len(ev.bits)), "threshold": thr,
-           "train_seconds": t_train, "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
-    timings = {}
-    for name, kw in (("exact", dict(mode=capi.MODE_EXACT_BATCHED)),
-                     ("exact_learned_order", dict(mode=capi.MODE_EXACT_BATCHED, order=1)),
-                     ("heuristic_a", dict(mode=capi.MODE_APPROX_HEURISTIC)),
-                     ("learned_a", dict(mode=capi.MODE_APPROX_LEARNED, accept=thr))):
-        dev.intersect(**kw)  # warm
-        tq = time.time()
-        bits = dev.intersect(**kw).bits.astype(bool)
-        timings[name] = time.time() - tq
-        p, r = precision_recall(bits, truth)
-        res[name] = {"precision": p, "recall": r, "pairs_per_s": len(truth) / timings[name]}
-        if name.startswith("exact"):
-            assert np.array_equal(bits, truth)
-    for name in ("heuristic_a", "learned_a", "exact_learned_order"):
-        res[name]["speedup_vs_exact"] = timings["exact"] / timings[name]
-    p, r = precision_recall(ev.decided == 1, truth)
-    res["filter_only"] = {"precision": p, "recall": r}
-    dev.close()
-    return res
-
-
 def split_calibration(lab: Labeled, frac: float = 0.2, seed: int = 0) -> tuple[Labeled, Labeled]:
     """Random split of labeled pairs into a training part and a held-out
     calibration part (for the -a threshold)."""
@@ -203,10 +158,13 @@ def split_calibration(lab: Labeled, frac: float = 0.2, seed: int = 0) -> tuple[L
     return take(~cal), take(cal)
 
 
-def threshold_for_precision(decided, logit, truth, target: float = 0.99, zconf: float = 2.0) -> float:
-    """Smallest logit threshold whose overall precision, less zconf standard
-    errors of its binomial estimate, reaches target (pick it on pairs the
-    network was not trained on)."""
+def threshold_for_precision(decided, logit, truth, target: float = 0.99, confidence: float = 0.999) -> float:
+    """Smallest logit threshold whose overall precision is at least target
+    with the given confidence: the one-sided Clopper-Pearson lower bound of
+    the accepted pairs' precision (IN-decided pairs count as accepted) reaches
+    target.  Pick it on pairs the network was not trained on."""
+    from scipy.stats import beta
+
     und = decided == -1
     base_tp = int(((decided == 1) & truth).sum())
     base_fp = int(((decided == 1) & ~truth).sum())
@@ -215,9 +173,8 @@ def threshold_for_precision(decided, logit, truth, target: float = 0.99, zconf: 
     order = np.argsort(-z)
     tp = base_tp + np.cumsum(t[order])
     fp = base_fp + np.cumsum(~t[order])
-    n = np.maximum(tp + fp, 1)
-    prec = tp / n
-    lower = prec - zconf * np.sqrt(prec * (1 - prec) / n)
+    with np.errstate(all="ignore"):
+        lower = np.where(tp > 0, beta.ppf(1.0 - confidence, tp, fp + 1), 0.0)
     ok = np.nonzero(lower >= target)[0]
     if len(ok) == 0:
         return float(np.inf)
@@ -225,14 +182,149 @@ def threshold_for_precision(decided, logit, truth, target: float = 0.99, zconf: 
     return float(z[order][k]) - 1e-6 if k + 1 < len(z) else float(z.min()) - 1.0
 
 
+MODEL_MAGIC = b"FARMMLP1"
+
+
+def save_model(path: str, params: np.ndarray, norm: np.ndarray, accept: float, reject: float | None = None) -> None:
+    """A trained MLP for the executables' learned -a mode (intersect -a
+    --model FILE): magic, u32 parameter and feature counts, f32 accept and
+    reject logits, the parameters and the feature normalisation."""
+    params = np.ascontiguousarray(params, np.float32)
+    norm = np.ascontiguousarray(norm, np.float32)
+    assert params.size == capi.MLP_NPARAMS and norm.size == 2 * NFEAT
+    with open(path, "wb") as f:
+        f.write(MODEL_MAGIC)
+        f.write(np.array([params.size, NFEAT], "<u4").tobytes())
+        f.write(np.array([accept, accept if reject is None else reject], "<f4").tobytes())
+        f.write(params.astype("<f4").tobytes())
+        f.write(norm.astype("<f4").tobytes())
+
+
+def load_model(path: str) -> tuple[np.ndarray, np.ndarray, float, float]:
+    with open(path, "rb") as f:
+        if f.read(8) != MODEL_MAGIC:
+            raise ValueError(f"{path} is not a trained MLP file")
+        n, nf = np.frombuffer(f.read(8), "<u4")
+        acc, rej = np.frombuffer(f.read(8), "<f4")
+        params = np.frombuffer(f.read(4 * int(n)), "<f4").copy()
+        norm = np.frombuffer(f.read(8 * int(nf)), "<f4").copy()
+    return params, norm, float(acc), float(rej)
+
+
+# The plain intersect workload's polygon distribution: 50..500 vertices,
+# R ~ LogNormal(ln 0.5, 0.5), r = R (1 + 0.4 u), 2,000 polygons per side in
+# [0,100]^2.  Larger sets keep its density in a larger box.
+STAR = dict(nv_lo=50, nv_hi=500, r_mu=float(np.log(0.5)), r_sigma=0.5, amp=0.4)
+PAIRS_PER_INSTANCE = 4240  # candidate pairs per 2,000 x 2,000 polygons at that density
+
+
+def star_domain(seed: int, instances: int, threads: int | None = None, vpr: int = 10) -> PairSet:
+    """instances x 2,000 source and target star polygons in a box of side
+    100 sqrt(instances) (the plain intersect workload's density), generated
+    by the library in parallel, with their candidate pairs, rasterized."""
+    side = 100.0 * float(np.sqrt(instances))
+    n = 2000 * instances
+    A = capi.HostSet.star_polygons(2 * seed, n, 0.0, side, nthreads=threads, **STAR)
+    B = capi.HostSet.star_polygons(2 * seed + 1, n, 0.0, side, nthreads=threads, **STAR)
+    pa, pb = capi.candidates(A, B, nthreads=threads)
+    both = capi.HostSet.concat(A, B)
+    both.rasterize(vpr, threads)
+    return PairSet(both, pa, (pb.astype(np.int64) + len(A)).astype(np.uint32), instances)
+
+
+def label_set(dev: capi.Device, ps: PairSet) -> Labeled:
+    """Exact labels (bits, executed tests) and features of an uploaded set."""
+    dev.upload_polygons(ps.both)
+    dev.upload_pairs(ps.pa, ps.pb)
+    bits, tests, _ = dev.intersect_costs()
+    feat, dec = dev.pair_features()
+    return Labeled(feat, dec, bits, tests)
+
+
+def config5(dev: capi.Device, train_pairs: int = 2_000_000, eval_pairs: int = 10_000_000, cal_pairs: int = 500_000,
+            batch_pairs: int = 1_000_000, epochs: int = 8, seed: int = 5, threads: int | None = None,
+            log=None) -> tuple[dict, np.ndarray, np.ndarray]:
+    """The learned part on the plain intersect workload's distribution:
+    train on train_pairs pairs labeled by the exact path, choose the -a
+    threshold for 99% precision (Clopper-Pearson, 99.9% confidence) on
+    cal_pairs held-out pairs, then evaluate on eval_pairs fresh pairs in
+    batches: precision and recall of learned -a, of the reference heuristic
+    -a and of the filter alone against the exact bits, and each mode's
+    pairs/s (every mode over the same uploaded batch).  Returns the results
+    and the trained model (parameters, feature normalisation)."""
+    import time
+
+    say = log or (lambda m: None)
+    t0 = time.time()
+    tr = label_set(dev, star_domain(1000 + seed, max(1, round(train_pairs / PAIRS_PER_INSTANCE)), threads))
+    norm, losses = train(dev, tr, seed=seed, epochs=epochs)
+    params = dev.mlp_get_params()
+    cal = label_set(dev, star_domain(2000 + seed, max(1, round(cal_pairs / PAIRS_PER_INSTANCE)), threads))
+    dev.mlp_set_params(params, norm)
+    z_cal = dev.mlp_forward(apply_norm(cal.feat, norm))[:, 0]
+    thr = threshold_for_precision(cal.decided, z_cal, cal.bits.astype(bool), 0.99, 0.999)
+    t_train = time.time() - t0
+    say(f"config5: trained on {int((tr.decided == -1).sum())} undecided of {len(tr.bits)} pairs, "
+        f"calibrated on {len(cal.bits)}, threshold {thr:.3f}, {t_train:.1f}s")
+    counts = {k: np.zeros(3, np.int64) for k in ("learned_a", "heuristic_a", "filter_only")}
+    secs = {k: 0.0 for k in ("exact", "learned_a", "heuristic_a")}
+    n_eval, b = 0, 0
+    while n_eval < eval_pairs:
+        inst = max(1, round(min(batch_pairs, eval_pairs - n_eval) / PAIRS_PER_INSTANCE))
+        ps = star_domain(100_000 + 1000 * seed + b, inst, threads)
+        dev.upload_polygons(ps.both)
+        dev.upload_pairs(ps.pa, ps.pb)
+        n = len(ps.pa)
+        res = {}
+        for name, kw in (("exact", dict(mode=capi.MODE_EXACT_BATCHED)),
+                         ("heuristic_a", dict(mode=capi.MODE_APPROX_HEURISTIC)),
+                         ("learned_a", dict(mode=capi.MODE_APPROX_LEARNED, accept=thr))):
+            dev.intersect(**kw, want_bits=False)  # warm (buffers sized for this batch)
+            st = dev.intersect(**kw).stats
+            secs[name] += st["total_ms"] / 1000.0
+        truth = dev.intersect(mode=capi.MODE_EXACT_BATCHED).bits.astype(bool)
+        res["heuristic_a"] = dev.intersect(mode=capi.MODE_APPROX_HEURISTIC).bits.astype(bool)
+        res["learned_a"] = dev.intersect(mode=capi.MODE_APPROX_LEARNED, accept=thr).bits.astype(bool)
+        _, dec = dev.pair_features()
+        res["filter_only"] = dec == 1
+        for k, pred in res.items():
+            counts[k] += [int((pred & truth).sum()), int((pred & ~truth).sum()), int((~pred & truth).sum())]
+        n_eval += n
+        b += 1
+        say(f"config5: evaluated {n_eval} pairs ({b} batches)")
+    out = {"distribution": "plain intersect workload (star polygons, 50..500 vertices, 2,000 per side per "
+                           "[0,100]^2), -v 10",
+           "train_pairs": int(len(tr.bits)), "train_undecided": int((tr.decided == -1).sum()),
+           "calibration_pairs": int(len(cal.bits)), "eval_pairs": int(n_eval), "eval_batches": b,
+           "threshold": thr, "calibration": "Clopper-Pearson lower bound >= 0.99 at 99.9% confidence",
+           "epochs": epochs, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "train_seconds": t_train}
+    for k, (tp, fp, fn) in counts.items():
+        out[k] = {"precision": tp / max(tp + fp, 1), "recall": tp / max(tp + fn, 1)}
+    for k in ("learned_a", "heuristic_a"):
+        out[k]["pairs_per_s"] = n_eval / secs[k]
+        out[k]["speedup_vs_exact"] = secs["exact"] / secs[k]
+    out["exact_pairs_per_s"] = n_eval / secs["exact"]
+    out["learned_vs_heuristic_speed"] = secs["heuristic_a"] / secs["learned_a"]
+    return out, params, norm
+
+
 if __name__ == "__main__":
     import argparse
     import json
 
-    ap = argparse.ArgumentParser(description="train and evaluate the learned -a mode on star-polygon pairs")
-    ap.add_argument("--train-pairs", type=int, default=500_000)
-    ap.add_argument("--eval-pairs", type=int, default=2_000_000)
-    ap.add_argument("--epochs", type=int, default=12)
-    ap.add_argument("--seed", type=int, default=1)
+    ap = argparse.ArgumentParser(description="train the learned -a mode on the plain intersect workload's "
+                                             "distribution, evaluate it, and optionally save the model")
+    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--eval-pairs", type=int, default=10_000_000)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--save-model", default="", help="write the trained MLP here (for intersect -a --model)")
     a = ap.parse_args()
-    print(json.dumps(evaluate(a.train_pairs, a.eval_pairs, epochs=a.epochs, seed=a.seed)), flush=True)
+    dev = capi.Device(0)
+    res, params, norm = config5(dev, a.train_pairs, a.eval_pairs, epochs=a.epochs, seed=a.seed,
+                                log=lambda m: print(m, flush=True))
+    if a.save_model:
+        save_model(a.save_model, params, norm, res["threshold"])
+    print(json.dumps(res), flush=True)
+    dev.close()

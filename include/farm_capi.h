@@ -115,6 +115,13 @@ typedef struct farm_polyset {
     const uint8_t *lstatus;
     uint64_t nlstatus;
     const double *areas;    /* clipped polygon area per finest pixel, like status */
+    /* fill classes at category_count categories (-w b: 2^b, 4..256) for the
+     * approximate mode and the learned features, indexed like status; NULL
+     * (or category_count 0 / 16): the status bytes, 16 categories.  Exact
+     * results never depend on it: OUT, BORDER and IN do not change. */
+    const uint8_t *fill;
+    int32_t category_count;
+    int32_t reserved;
 } farm_polyset;
 
 /* Query modes of farm_intersect. */
@@ -132,8 +139,13 @@ typedef struct farm_opts {
     float approx_reject;   /* learned -a: logit below which a pair is rejected; pairs in
                               [reject, threshold] fall back to exact refinement */
     float merge_threshold; /* -m, accepted for compatibility; exact results do not depend on it */
-    int32_t reserved;
+    int32_t flags;         /* FARM_FLAG_*; 0 for the defaults */
 } farm_opts;
+
+/* farm_opts.flags: refine every (source edge, target chunk) of the batched
+ * kernel in full, without the exact chunk-box prune (the same results; for
+ * measuring what the prune saves). */
+#define FARM_FLAG_NO_PRUNE 1
 
 /* Counters of the work actually executed (device counters). */
 typedef struct farm_stats {
@@ -265,6 +277,30 @@ int farm_intersection_area(farm_handle h, uint64_t begin, uint64_t end, int32_t 
                            farm_stats *stats);
 int farm_intersection_area_dev(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,
                                farm_stats *stats);
+
+/* One piece of the boundary of A n B: a run of one polygon's boundary that
+ * lies IN the other polygon, from crossing point a through the ring
+ * vertices first .. first + nvert - 1 (indexes into the polygon's decoded
+ * ring, taken modulo its m = nverts - 1 distinct vertices) to crossing point
+ * b.  nvert == -1: the whole ring (one polygon contains the other).  The
+ * pieces of a pair chain into the rings of A n B (each piece's b is the a of
+ * the next), the assembly the reference's rebuild_polygons performs
+ * (src/cuda/query_intersection.cu:393-502, never launched there). */
+typedef struct farm_piece {
+    uint32_t pair;  /* index in the call's pair window */
+    int32_t ring;   /* 0: the source polygon, 1: the target polygon */
+    int32_t first;
+    int32_t nvert;
+    float ax, ay, bx, by;
+} farm_piece;
+
+/* farm_intersection_area that also returns the IN pieces of every pair with
+ * an intersection (in no particular order; farm_pieces_wkt assembles them).
+ * *npieces gets their count; with more than cap of them nothing is copied
+ * and the call returns FARM_ECAPACITY (repeat with cap >= *npieces).  counts
+ * and area may be NULL. */
+int farm_intersection_pieces(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts, double *area,
+                             farm_piece *pieces, uint64_t cap, uint64_t *npieces, farm_stats *stats);
 
 /* ---- learned part (no reference counterpart) ----
  * Pair features: FARM_NFEAT floats per candidate pair from the filter's

@@ -57,6 +57,12 @@ int farm_hostset_rasterize_h(farm_hostset *s, int vpr, int nlow, int nthreads);
  * set rasterized with farm_hostset_rasterize_h, for tests. */
 int farm_hostset_layer(const farm_hostset *s, int64_t i, int k, farm_layer *out, const uint8_t **status);
 int farm_hostset_view(const farm_hostset *s, farm_polyset *out);
+/* -w b: fill classes of every pixel at 2^b categories (b in 2..8), from the
+ * same clipped areas (encodePixelArea, Ideal.cpp:64-88, with 2^b in place of
+ * 16), for the approximate mode and the learned features; b = 4 keeps the
+ * status bytes alone.  Call after rasterizing; farm_hostset_view then
+ * exposes them. */
+int farm_hostset_set_categories(farm_hostset *s, int category_bits);
 /* Per-pixel clipped areas of polygon i (dimx*dimy doubles), for tests. */
 int farm_hostset_areas(const farm_hostset *s, int64_t i, const double **areas);
 /* Index of the first polygon the reference raster builder would reject
@@ -75,6 +81,27 @@ int farm_hostset_raster_ok(const farm_hostset *s, uint8_t *out);
 int64_t farm_candidates(const farm_hostset *a, const farm_hostset *b, double expand_km, int skip_self,
                         int nthreads, uint32_t **pa, uint32_t **pb);
 void farm_free(void *p);
+
+/* Seeded synthetic star-shaped polygons, generated on nthreads host threads
+ * (the same set for any thread count): n closed rings, centres uniform in
+ * [lo, hi]^2, vertex counts uniform in [nv_lo, nv_hi], sorted uniform
+ * angles, radius R (1 + amp u) per vertex with u ~ U(-1, 1) and
+ * R ~ LogNormal(r_mu, r_sigma), coordinates rounded to float32.  With
+ * nv 50..500, R ~ LogNormal(ln 0.5, 0.5), amp 0.4 and a [0, 100]^2 box per
+ * 2,000 polygons this is the plain intersect workload's distribution.
+ * Returns NULL on bad arguments. */
+farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, double lo, double hi, int nv_lo, int nv_hi,
+                                         double r_mu, double r_sigma, double amp, int nthreads);
+
+/* The polygons of A n B assembled from farm_intersection_pieces output, as
+ * WKT: one line "<k>\tMULTIPOLYGON (((x y, ...)), ...)" per pair k of the
+ * window that has pieces, k ascending.  s is the rasterized set that was
+ * uploaded, pa / pb the window's pair ids into it.  Pieces chain by their
+ * crossing points (each piece's b is the next one's a, compared as floats);
+ * a chain that does not close is closed at its start.  Returns text to free
+ * with farm_free (*len its length), NULL on bad input. */
+char *farm_pieces_wkt(const farm_hostset *s, const uint32_t *pa, const uint32_t *pb, uint64_t npairs,
+                      const struct farm_piece *pieces, uint64_t npieces, uint64_t *len);
 
 /* The 900-entry degrees-per-kilometre longitude table the library uses for
  * within distances and the within join's box expansion: out[k] covers

@@ -167,14 +167,16 @@ class OracleSet:
 
 
 def intersect(a: OracleSet, b: OracleSet, pa: np.ndarray, pb: np.ndarray, mode: int = 0,
-              nthreads: int = 8) -> tuple[np.ndarray, dict]:
-    """Ideal::intersect per pair (mode 1: the reference -a heuristic).
-    out == 2 marks pairs on which the reference would abort."""
+              nthreads: int = 8, category_bits: int = 4) -> tuple[np.ndarray, dict]:
+    """Ideal::intersect per pair (mode 1: the reference -a heuristic, its fill
+    classes at 2^category_bits categories).  out == 2 marks pairs on which
+    the reference would abort."""
     pa = np.ascontiguousarray(pa, dtype=np.uint32)
     pb = np.ascontiguousarray(pb, dtype=np.uint32)
     out = np.zeros(len(pa), dtype=np.uint8)
     stats = np.zeros(3, dtype=np.int64)
-    rc = lib.orc_intersect(a._h, b._h, len(pa), _p(pa), _p(pb), _p(out), mode, nthreads, _p(stats))
+    m = mode | ((category_bits if category_bits != 4 else 0) << 8)
+    rc = lib.orc_intersect(a._h, b._h, len(pa), _p(pa), _p(pb), _p(out), m, nthreads, _p(stats))
     if rc == ORC_EINVAL:
         raise ValueError("pair index out of range")
     return out, {"bb_pixel_pairs": int(stats[0]), "segment_tests": int(stats[1]), "in_hits": int(stats[2])}

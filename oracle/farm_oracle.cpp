@@ -426,6 +426,29 @@ struct Poly {
         if (f == kCC - 1) return pa;
         return (1.0 * f - isLow) / (kCC - 2) * pa;
     }
+    /* decodePixelArea (Ideal.cpp:47-62) of the fill class the same clipped
+     * area gets with cc categories (encodePixelArea, Ideal.cpp:64-88, with cc
+     * in place of 16; BORDER classes clamped to [1, cc - 2] as
+     * calculate_fullness does, Ideal.cpp:905-912): the reference GPU filter
+     * decodes with its category_count (gpu_decode_fullness,
+     * geometry.cuh:374-388) */
+    double decode_area_cc(int id, bool isLow, int cc) const {
+        if (cc == kCC) return decode_area(id, isLow);
+        const uint8_t s = status[id];
+        const double pa = g.pixel_area();
+        if (s == 0) return 0.0f;
+        if (s == kCC - 1) return pa;
+        const double ratio = areas[id] / pa;
+        int f;
+        if (std::fabs(ratio - 1.0) < 1e-9) f = cc - 1;
+        else if (std::fabs(ratio) < 1e-9) f = 0;
+        else {
+            f = static_cast<int>(std::ceil(ratio * (cc - 2)));
+            if (f >= cc) f = cc - 1;
+        }
+        f = std::max(1, std::min(f, cc - 2));
+        return (1.0 * f - isLow) / (cc - 2) * pa;
+    }
 };
 
 /* VertexSequence::getMBR (vertex_sequence.cpp:152-167): float bias 1e-9 */
@@ -1075,7 +1098,7 @@ struct QStats {
  * over a source pixel shrunk by 1e-6, :66-74), then true on an IN pixel on
  * either side or on a BORDER x BORDER pair with probability > 1
  * (query_intersect.cu:86-88,124-129) */
-static bool intersect(const Poly &A, const Poly &B, int mode, QStats &qs) {
+static bool intersect(const Poly &A, const Poly &B, int mode, QStats &qs, int cc = kCC) {
     std::vector<std::tuple<double, int, int>> cand;
     std::vector<int> pxs, tpxs;
     if (mode == 1)
@@ -1092,8 +1115,8 @@ static bool intersect(const Poly &A, const Poly &B, int mode, QStats &qs) {
                 qs.in_hits++;
                 return true;
             }
-            double s_apx = (A.decode_area(pa, true) + A.decode_area(pa, false)) / 2;
-            double t_apx = (B.decode_area(pb, true) + B.decode_area(pb, false)) / 2;
+            double s_apx = (A.decode_area_cc(pa, true, cc) + A.decode_area_cc(pa, false, cc)) / 2;
+            double t_apx = (B.decode_area_cc(pb, true, cc) + B.decode_area_cc(pb, false, cc)) / 2;
             double prob = (s_apx + t_apx) / std::max(A.g.pixel_area(), B.g.pixel_area());
             cand.emplace_back(prob, pa, pb);
         }
@@ -1669,6 +1692,10 @@ int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint
                   uint8_t *out, int mode, int nthreads, int64_t *stats) {
     for (int64_t k = 0; k < npairs; k++)
         if (pa[k] >= a->polys.size() || pb[k] >= b->polys.size()) return ORC_EINVAL;
+    /* mode bits 8..15: b of the -w b fill categories (cc = 2^b, 0 = 16) */
+    const int wbits = (mode >> 8) & 0xff;
+    if (wbits != 0 && (wbits < 2 || wbits > 8)) return ORC_EINVAL;
+    const int cc = wbits ? 1 << wbits : kCC;
     std::atomic<int64_t> next(0);
     std::atomic<int> err(ORC_OK);
     std::mutex mu;
@@ -1686,7 +1713,7 @@ int orc_intersect(const orc_set *a, const orc_set *b, int64_t npairs, const uint
                         err = ORC_EREF_ABORT;
                         continue;
                     }
-                    out[k] = intersect(a->polys[pa[k]], b->polys[pb[k]], mode, qs) ? 1 : 0;
+                    out[k] = intersect(a->polys[pa[k]], b->polys[pb[k]], mode & 0xff, qs, cc) ? 1 : 0;
                 } catch (const RefAbort &) {
                     out[k] = 2;
                     err = ORC_EREF_ABORT;

@@ -120,3 +120,11 @@ def test_reference_heuristic(device, headline):
     r = device.intersect(mode=capi.MODE_APPROX_HEURISTIC)
     diff = np.nonzero(r.bits != headline["heur"])[0]
     assert diff.size == 0, f"-a: {diff.size} pairs differ, first {diff[:10]}"
+
+
+def test_batched_without_the_prune(device, headline):
+    # every (source edge, target chunk) refined in full: the same bits
+    _upload(device, headline)
+    r = device.intersect(mode=capi.MODE_EXACT_BATCHED, tile_a=8, flags=capi.FLAG_NO_PRUNE)
+    _agree(r.bits, headline, "no prune")
+    assert r.stats["pruned_tests"] == 0

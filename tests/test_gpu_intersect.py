@@ -131,3 +131,24 @@ def test_zero_length_target_segments(device):
     w = Workload(rings_from_list(a), rings_from_list(b), pairs=([0, 0], [0, 1]))
     want, _ = _check(device, w)
     assert list(want) == [1, 1]
+
+
+@pytest.mark.parametrize("bits", [3, 8])
+def test_category_bits(device, bits):
+    # -w b: fill classes at 2^b categories.  Exact bits do not change; the
+    # reference -a decodes the finer classes (oracle mode 1 at 2^b)
+    w = star_pairs(6, n=600)
+    want, _ = w.oracle(0)
+    import oracle
+
+    heur, _ = oracle.intersect(w.OA, w.OB, w.pa, w.pb, 1, category_bits=bits)
+    heur16, _ = oracle.intersect(w.OA, w.OB, w.pa, w.pb, 1)
+    w.both.set_categories(bits)
+    try:
+        w.upload(device)
+        assert np.array_equal(device.intersect(mode=capi.MODE_EXACT_BATCHED).bits, want)
+        got = device.intersect(mode=capi.MODE_APPROX_HEURISTIC).bits
+        assert np.array_equal(got, heur), int((got != heur).sum())
+        assert not np.array_equal(heur, heur16) or bits == 4  # the category count matters to -a
+    finally:
+        w.both.set_categories(4)

@@ -137,3 +137,54 @@ def test_pairs_beyond_the_record_slabs(device):
     w = Workload(rings_from_list(a), rings_from_list(b), vpr=10, pairs=(pa, pb))
     c, _, st = _check(device, w)
     assert c[0, 0] == 40000 and c[1, 0] == 90000
+
+
+def _wkt_lines(text):
+    out = {}
+    for line in text.splitlines():
+        k, body = line.split("\t", 1)
+        out[int(k)] = body
+    return out
+
+
+def test_intersection_polygons_of_rectangles(device):
+    # A n B of axis-aligned rectangles is the rectangle of their overlap
+    a = [np.array([[0, 0], [2, 0], [2, 2], [0, 2]], np.float32),
+         np.array([[10, 10], [14, 10], [14, 14], [10, 14]], np.float32),
+         np.array([[20, 20], [21, 20], [21, 21], [20, 21]], np.float32)]
+    b = [np.array([[1, 1], [3, 1], [3, 3], [1, 3]], np.float32),
+         np.array([[11, 11], [12, 11], [12, 12], [11, 12]], np.float32),   # inside a[1]
+         np.array([[30, 30], [31, 30], [31, 31], [30, 31]], np.float32)]   # apart from a[2]
+    w = Workload(rings_from_list(a), rings_from_list(b), vpr=1, pairs=([0, 1, 2], [0, 1, 2]))
+    w.upload(device)
+    counts, area, pcs, _ = device.intersection_pieces()
+    text = capi.pieces_wkt(w.both, w.pa, w.pb + len(w.A), pcs)
+    lines = _wkt_lines(text)
+    assert sorted(lines) == [0, 1]
+    ring0 = capi.wkt_rings(lines[0])
+    assert len(ring0) == 1
+    corners = {tuple(p) for p in ring0[0].tolist()}
+    assert corners == {(1.0, 1.0), (2.0, 1.0), (2.0, 2.0), (1.0, 2.0)}
+    assert abs(capi.wkt_multipolygon_area(lines[0]) - 1.0) < 1e-12 and abs(area[0] - 1.0) < 1e-12
+    assert abs(capi.wkt_multipolygon_area(lines[1]) - 1.0) < 1e-12  # the contained square, whole
+    assert area[2] == 0.0
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_intersection_polygons_areas_match(device, seed):
+    # the assembled rings' shoelace area equals the device's Green's-theorem
+    # area pair by pair (the same boundary, summed independently)
+    src, tgt = datagen.star_pair_set(seed, 300, 300, 0.0, 20.0)
+    w = Workload(src, tgt, vpr=10)
+    w.upload(device)
+    counts, area, pcs, _ = device.intersection_pieces()
+    c2, a2, _ = device.intersection_area()
+    assert np.array_equal(counts, c2) and np.array_equal(area.view(np.uint64), a2.view(np.uint64))
+    lines = _wkt_lines(capi.pieces_wkt(w.both, w.pa, w.pb + len(w.A), pcs))
+    nz = np.nonzero(area > 0)[0]
+    assert len(nz) > 50
+    for k in nz:
+        assert k in lines, k
+        got = capi.wkt_multipolygon_area(lines[k])
+        assert abs(got - area[k]) <= 1e-9 * area[k] + 1e-15, (k, got, area[k])
+    assert set(lines) <= set(nz.tolist()) | set(np.nonzero(counts[:, 2] > 0)[0].tolist())

@@ -65,12 +65,16 @@ def test_training_reduces_loss(device):
 
 @pytest.fixture(scope="module")
 def star_model(device):
-    train_ps = learned.build_pairsets(list(range(500, 516)))
-    lab, cal = learned.split_calibration(learned.label(device, train_ps), 0.2, 0)
+    # the plain intersect workload's distribution: train on one seeded box,
+    # choose the -a threshold on a second, test on a third (fresh seeds)
+    train_ps = learned.star_domain(500, 16)
+    lab = learned.label_set(device, train_ps)
     norm, losses = learned.train(device, lab, seed=0, epochs=15)
     params = device.mlp_get_params()
-    test_ps = learned.build_pairsets(list(range(900, 908)))
-    test_lab = learned.label(device, test_ps)
+    cal_ps = learned.star_domain(600, 40)
+    cal = learned.label_set(device, cal_ps)
+    test_ps = learned.star_domain(900, 16)
+    test_lab = learned.label_set(device, test_ps)
     return {"norm": norm, "params": params, "train": lab, "cal": cal, "test": test_lab, "test_ps": test_ps,
             "losses": losses}
 
@@ -86,7 +90,7 @@ def test_learned_approx_beats_heuristic_at_99_precision(device, star_model):
     z_te = device.mlp_predict_pairs()
     truth = te.bits.astype(bool)
     z_cal = device.mlp_forward(learned.apply_norm(cal.feat, M["norm"]))
-    thr = learned.threshold_for_precision(cal.decided, z_cal[:, 0], cal.bits.astype(bool), 0.99)
+    thr = learned.threshold_for_precision(cal.decided, z_cal[:, 0], cal.bits.astype(bool), 0.99, 0.999)
     pred = learned.decisions(te.decided, z_te[:, 0], thr)
     p_l, r_l = learned.precision_recall(pred, truth)
     # the reference heuristic on the same pairs
@@ -95,7 +99,7 @@ def test_learned_approx_beats_heuristic_at_99_precision(device, star_model):
     # the filter alone (IN overlaps only): the method without refinement or MLP
     p_f, r_f = learned.precision_recall(te.decided == 1, truth)
     print(f"learned P={p_l:.4f} R={r_l:.4f} | heuristic P={p_h:.4f} R={r_h:.4f} | filter-only R={r_f:.4f}")
-    assert p_l >= 0.985
+    assert p_l >= 0.99
     assert r_l > r_h + 0.05 and r_l > r_f + 0.05
     # the device learned mode takes the same decisions as the host rule
     r = device.intersect(mode=capi.MODE_APPROX_LEARNED, accept=thr)

@@ -54,6 +54,19 @@ def test_intersect_and_intersection_executables(tmp_path):
     for extra in ([], ["-b", "1000"], ["--gpu-candidates"]):
         assert _found([os.path.join(BIN, "intersection"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (area > 0).sum())
+    # --wkt: the polygons of A n B, one line per intersecting pair, their
+    # shoelace areas equal to the oracle's areas
+    want = {(int(i), int(j)): float(x) for i, j, x in zip(pa, pb, area) if x > 0}
+    for extra in ([], ["-b", "777", "--gpu-candidates"]):
+        out = tmp_path / "ab.wkt"
+        _found([os.path.join(BIN, "intersection"), "-s", s, "-t", t, "-r", "-g", "--wkt", str(out)] + extra)
+        got = {}
+        for line in out.read_text().splitlines():
+            i, j, body = line.split("\t", 2)
+            got[(int(i), int(j))] = capi.wkt_multipolygon_area(body)
+        assert set(want) <= set(got)
+        for key, x in want.items():
+            assert abs(got[key] - x) <= 1e-9 * x + 1e-15, (key, got[key], x)
 
 
 def test_within_polygon_executable(tmp_path):
@@ -71,3 +84,43 @@ def test_within_polygon_executable(tmp_path):
     # without -h the reference dereferences an empty layer list; here it is an error
     p = subprocess.run([exe, "-s", s, "-r", "-g"], capture_output=True, text=True, timeout=60)
     assert p.returncode != 0 and "-h" in p.stderr
+
+
+def test_intersect_learned_a_from_a_model_file(tmp_path):
+    # intersect -a --model FILE runs the learned -a mode with a saved MLP:
+    # its count equals the learned mode's on the same pairs through the C ABI
+    from farm_amd import learned
+
+    (ra, xa), (rb, xb) = datagen.star_pair_set(24, 300, 300, 0.0, 30.0)
+    s, t = _dt(tmp_path, "src", ra, xa), _dt(tmp_path, "tgt", rb, xb)
+    A, B = capi.HostSet.load_dt(s), capi.HostSet.load_dt(t)
+    pa, pb = capi.candidates(A, B)
+    both = capi.HostSet.concat(A, B)
+    both.rasterize(10)
+    dev = capi.Device(0)
+    try:
+        dev.upload_polygons(both)
+        dev.upload_pairs(pa, pb + len(A))
+        feat, dec = dev.pair_features()
+        rng = np.random.default_rng(5)
+        params = (rng.standard_normal(capi.MLP_NPARAMS) * 0.05).astype(np.float32)
+        norm = learned.fit_norm(feat)
+        dev.mlp_set_params(params, norm)
+        z = dev.mlp_predict_pairs()[:, 0]
+        thr = float(np.median(z[dec == -1])) if (dec == -1).any() else 0.0
+        want = int(dev.intersect(mode=capi.MODE_APPROX_LEARNED, accept=thr).bits.sum())
+    finally:
+        dev.close()
+    path = str(tmp_path / "mlp.bin")
+    learned.save_model(path, params, norm, thr)
+    p2, n2, acc, _ = learned.load_model(path)
+    assert np.array_equal(p2, params) and np.array_equal(n2, norm) and acc == np.float32(thr)
+    exe = os.path.join(BIN, "intersect")
+    assert _found([exe, "-s", s, "-t", t, "-r", "-g", "-a", "--model", path]) == want
+    # a model without -a, or a file that is not a model, is an error
+    p = subprocess.run([exe, "-s", s, "-t", t, "--model", path], capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"not a model")
+    p = subprocess.run([exe, "-s", s, "-t", t, "-a", "--model", str(bad)], capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0

@@ -241,3 +241,40 @@ def test_product_raster_matches_oracle_fine_pixels_far_from_origin():
     xy = (xy.astype(np.float64) * 0.01 + [178.0, -60.0]).astype(np.float32)
     _compare_rasters(ro, xy, 1)
     _compare_rasters(ro, xy, 4)
+
+
+@pytest.mark.parametrize("bits", [2, 3, 8])
+def test_fill_classes_at_other_category_counts(bits):
+    # -w b: every pixel's class at cc = 2^b from the same clipped area; a
+    # subdivided rectangle's boundary ring is nearly full (class cc - 2), its
+    # interior IN (cc - 1), as at 16 (14 and 15); OUT / BORDER / IN unchanged
+    ring = _subdivided_rect(10.3, 20.7, 4.0, 3.0, 40)
+    H = capi.HostSet.from_rings(*rings_from_list([ring]))
+    H.rasterize(1)
+    L = H.layout()
+    assert L["fill"] is None and L["category_count"] == 16
+    st = L["status"].copy()
+    H.set_categories(bits)
+    L = H.layout()
+    cc = 1 << bits
+    assert L["category_count"] == cc
+    if cc == 16:
+        assert L["fill"] is None
+        return
+    fill = L["fill"]
+    assert np.array_equal(fill == 0, st == 0) and np.array_equal(fill == cc - 1, st == 15)
+    assert np.array_equal(fill[(st > 0) & (st < 15)], np.full(((st > 0) & (st < 15)).sum(), cc - 2))
+    # a right triangle: BORDER classes follow ceil((cc - 2) * ratio), clamped to [1, cc - 2]
+    tri = np.array([[3.3125, 7.40625], [9.4125, 7.40625], [3.3125, 11.70625]], np.float32)
+    T = capi.HostSet.from_rings(*rings_from_list([tri]))
+    T.rasterize(1)
+    T.set_categories(bits)
+    Lt = T.layout()
+    h = Lt["hdr"][0]
+    npix = int(h["dimx"] * h["dimy"])
+    areas = T.areas(0, npix)
+    ratio = areas / (h["step"][0] * h["step"][1])
+    stt = Lt["status"][:npix]
+    border = (stt > 0) & (stt < 15)
+    want = np.clip(np.ceil(ratio[border] * (cc - 2)), 1, cc - 2).astype(np.uint8)
+    assert np.array_equal(Lt["fill"][:npix][border], want)

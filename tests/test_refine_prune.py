@@ -2,8 +2,9 @@
 
 k_refine (farm_amd/csrc/hip/intersect_kernels.hip) skips a source edge (a, b)
 against a chunk of up to 16 target segments when the chunk's box lies on one
-side of line ab by more than M = 2^-19 Q (|b - a|_1 + 2Q), Q the L1 reach
-from a to the box.  The skip is only
+side of line ab by more than M = 2^-19 q |b - a|_1 (plus 2^-18 q^2 when the
+chunk box overlaps the box of the lane's edge vertices), q >= the L1 reach
+from those vertices to the chunk box.  The skip is only
 allowed if the reference predicate (segment_intersect,
 src/include/geometry_computation.h:215-221) is false for every segment of
 the chunk.  These tests restate the box (k_chunk_box) and the test in f32
@@ -40,19 +41,39 @@ def fmaf(a, b, c) -> np.float32:
     return f32(np.float64(a) * np.float64(b) + np.float64(c))
 
 
-def pruned(a, b, box) -> bool:
-    """The kernel's per-edge prune decision, in f32."""
+def pruned_at(a, b, box, k: int = 19, tile=None) -> bool:
+    """The kernel's per-edge prune decision, in f32, with its margin scaled to
+    2^-k (the kernel uses k = 19).  The reach q is taken from the box of the
+    vertices of the lane's edges (default: the edge's own two vertices, the
+    smallest such box and so the most aggressive prune); the margin is
+    2^-k q |b - a|_1 when that box and the chunk box are apart on an axis
+    (the reference's collinear branch needs their exact overlap), else
+    2^-k q (|b - a|_1 + 2q)."""
     cx, cy, hx, hy = box
     ax, ay = f32(a[0]), f32(a[1])
-    bax, bay = f32(f32(b[0]) - ax), f32(f32(b[1]) - ay)
+    bx_, by_ = f32(b[0]), f32(b[1])
+    pts = np.array([[ax, ay], [bx_, by_]] if tile is None else tile, f32)
+    lxl, lxh = pts[:, 0].min(), pts[:, 0].max()
+    lyl, lyh = pts[:, 1].min(), pts[:, 1].max()
+    bax, bay = f32(bx_ - ax), f32(by_ - ay)
     with np.errstate(all="ignore"):
+        ex0, ex1 = f32(cx - lxl), f32(cx - lxh)
+        ey0, ey1 = f32(cy - lyl), f32(cy - lyh)
+        q = f32(f32(f32(max(abs(ex0), abs(ex1)) + max(abs(ey0), abs(ey1))) + f32(hx + hy)) * f32(1.000001))
+        hx1, hy1 = f32(hx * f32(1.000001)), f32(hy * f32(1.000001))
+        apart = (ex1 > hx1) or (-ex0 > hx1) or (ey1 > hy1) or (-ey0 > hy1)
+        k1 = f32(q * f32(2.0 ** -k))
+        k2 = f32(2.0 ** -80) if apart else fmaf(q, f32(q * f32(2.0 ** (1 - k))), 2.0 ** -80)
         dxc, dyc = f32(cx - ax), f32(cy - ay)
         oc = fmaf(bax, dyc, -f32(bay * dxc))
-        rr = fmaf(abs(bay), hx, f32(abs(bax) * hy))
-        q = f32(f32(abs(dxc) + abs(dyc)) + f32(hx + hy))
         sab = f32(abs(bax) + abs(bay))
-        m19 = fmaf(q, fmaf(2.0, q, sab), 2.0 ** -61)  # 2^19 M, plus 2^-61
-        return bool(f32(f32(abs(oc) - rr) * f32(2.0 ** 19)) > m19)
+        m = fmaf(sab, k1, k2)
+        thr = fmaf(abs(bay), hx, fmaf(abs(bax), hy, m))
+        return bool(abs(oc) > thr)
+
+
+def pruned(a, b, box) -> bool:
+    return pruned_at(a, b, box)
 
 
 def check(a, b, chunk: np.ndarray) -> bool:
@@ -182,21 +203,6 @@ def test_non_finite_and_huge_boxes_are_never_pruned():
     assert not pruned(np.array([0, 5], f32), np.array([1, 5], f32), box)
 
 
-def pruned_at(a, b, box, k: int) -> bool:
-    """The kernel's prune with its margin scaled to 2^-k Q (|b - a|_1 + 2Q)
-    (the kernel uses k = 19)."""
-    cx, cy, hx, hy = box
-    ax, ay = f32(a[0]), f32(a[1])
-    bax, bay = f32(f32(b[0]) - ax), f32(f32(b[1]) - ay)
-    with np.errstate(all="ignore"):
-        dxc, dyc = f32(cx - ax), f32(cy - ay)
-        oc = fmaf(bax, dyc, -f32(bay * dxc))
-        rr = fmaf(abs(bay), hx, f32(abs(bax) * hy))
-        q = f32(f32(abs(dxc) + abs(dyc)) + f32(hx + hy))
-        sab = f32(abs(bax) + abs(bay))
-        m = fmaf(q, fmaf(2.0, q, sab), 2.0 ** -61)
-        return bool(f32(f32(abs(oc) - rr) * f32(2.0 ** k)) > m)
-
 
 def test_constructed_rounding_crossings_need_the_margin():
     # Constructed, not sampled: a target segment (c, d) that lies strictly on
@@ -205,8 +211,9 @@ def test_constructed_rounding_crossings_need_the_margin():
     # the same side.  Where the reference's binary32 a.cross(b, c) rounds to
     # 0 or to the wrong sign it reports a crossing (a and b are on opposite
     # sides of line cd), so the prune must keep the chunk.  The kernel's
-    # margin (2^-19) keeps every such chunk; a margin of 2^-26 already drops
-    # some, so this test fails for a margin shrunk that far.
+    # margin (2^-19) keeps every such chunk, even with q taken from a alone;
+    # a margin of 2^-27 already drops some, so this test fails for a margin
+    # shrunk that far.
     rng = np.random.default_rng(21)
     ref_true, dropped_small = 0, 0
     for _ in range(60000):
@@ -229,7 +236,11 @@ def test_constructed_rounding_crossings_need_the_margin():
             continue
         ref_true += 1
         box = chunk_box(np.array([c[0], d[0]]), np.array([c[1], d[1]]))
+        # q from a alone: the smallest reach any tile holding this edge can
+        # give, so the most aggressive prune the kernel could make
+        point = np.array([a, a], f32)
         assert not pruned_at(a, b, box, 19), (a, b, c, d)
-        dropped_small += pruned_at(a, b, box, 26)
+        assert not pruned_at(a, b, box, 19, tile=point), (a, b, c, d)
+        dropped_small += pruned_at(a, b, box, 27, tile=point)
     assert ref_true > 200  # the construction reaches the rounding regime
     assert dropped_small > 0  # and a too-small margin would drop reported crossings
