@@ -32,6 +32,9 @@ Also reported:
   cpu_baseline  the CPU oracle (restatement of the reference intersect_cpu
             query phase) on a bounded random sample of the same pairs,
             all host threads; the sample is also checked bit for bit
+  rasterize the set is rasterized on the device (farm_rasterize), checked
+            byte for byte against the host builder's arrays and used as the
+            device set; both times are reported
 """
 from __future__ import annotations
 
@@ -145,9 +148,42 @@ def build_workload(scale: float, seed: int, vpr: int, threads: int):
     t = time.time()
     both = capi.HostSet.concat(A, B)
     both.rasterize(vpr, threads)
+    RASTER_TIMES["host_s"] = time.time() - t
     bad = both.raster_failures()
-    log(f"rasterized {len(both)} polygons in {time.time() - t:.1f}s (first rejected polygon: {bad})")
+    log(f"rasterized {len(both)} polygons on {threads} host threads in {RASTER_TIMES['host_s']:.1f}s "
+        f"(first rejected polygon: {bad})")
     return (cro, cxy, lro, lxy), A, B, both, pa, pb
+
+
+RASTER_TIMES: dict = {}
+
+
+def device_rasterize(dev, H, vpr: int, nlow: int, threads: int) -> dict:
+    """The set rasterized on the device (farm_rasterize) becomes the device
+    set; its arrays must equal the host builder's byte for byte."""
+    t = time.time()
+    off, x, y = H.closed_rings(threads)
+    t_rings = time.time() - t
+    t = time.time()
+    info = dev.rasterize(off, x, y, vpr, nlow)
+    wall = time.time() - t
+    D = dev.download_polyset()
+    L = H.layout()
+    same = D["hdr"].tobytes() == L["hdr"].tobytes()
+    for k in ("status", "pixoff", "seqs", "voff", "vnodes"):
+        same = same and np.array_equal(D[k], L[k])
+    if nlow:
+        same = same and D["layers"].tobytes() == L["layers"].tobytes() and np.array_equal(D["lstatus"], L["lstatus"])
+        same = same and np.array_equal(D["areas"].view(np.uint64), L["areas"].view(np.uint64))
+    if not same:
+        raise RuntimeError("the device-built raster set differs from the host builder's")
+    log(f"device rasterization: {len(off) - 1} polygons, {int(off[-1])} vertices in {wall:.2f}s "
+        f"(build {info['build_ms']:.0f} ms after the vertex upload), identical to the host's "
+        f"{RASTER_TIMES.get('host_s', float('nan')):.1f}s on {threads} threads")
+    return {"polygons": int(len(off) - 1), "vertices": int(off[-1]), "vpr": vpr, "nlow": nlow,
+            "host_s": RASTER_TIMES.get("host_s"), "host_threads": threads, "device_s": wall,
+            "device_build_ms": info["build_ms"], "closed_rings_s": t_rings, "rejected": info["nrejected"],
+            "identical": True}
 
 
 ONE_THREAD_SECONDS = 4.0
@@ -235,7 +271,8 @@ def within_workload(scale: float, seed: int, threads: int):
     t = time.time()
     H = capi.HostSet.from_rings(ro, xy)
     H.rasterize_h(10, 1, threads)
-    log(f"within: -h rasterized (-v 10, --NLow 1) in {time.time() - t:.1f}s")
+    RASTER_TIMES["host_s"] = time.time() - t
+    log(f"within: -h rasterized (-v 10, --NLow 1) on {threads} host threads in {RASTER_TIMES['host_s']:.1f}s")
     t = time.time()
     pa, pb = capi.candidates(H, H, expand_km=10.0, skip_self=True, nthreads=threads)
     CAND_TIMES["host_rtree_s"] = time.time() - t
@@ -281,7 +318,7 @@ def bench_within(args) -> dict:
 
     (ro, xy), H, pa, pb = within_workload(args.scale, args.seed, args.threads)
     dev = capi.Device(0)
-    dev.upload_polygons(H)
+    raster_info = device_rasterize(dev, H, 10, 1, args.threads)
     m = H.mbbs()
     cand_info = device_candidates(dev, m, m, pa, pb, 0, expand_km=10.0, skip_self=True)
     nb = args.steps + args.warmup
@@ -326,6 +363,7 @@ def bench_within(args) -> dict:
                      "traffic_source": WITHIN_TRAFFIC if traffic else None,
                      "ops_per_point_segment": OPS_PER_PT_SEG, "ops_per_box_pair": OPS_PER_BOX_PAIR},
         "candidates": cand_info,
+        "rasterize": raster_info,
     }
     if not args.no_cpu:
         bits, md, _ = dev.within(10.0)
@@ -554,9 +592,7 @@ def main() -> int:
 
     rings, A, B, both, pa, pb = build_workload(args.scale, args.seed, args.vpr, args.threads)
     dev = capi.Device(0)
-    t = time.time()
-    dev.upload_polygons(both)
-    log(f"upload {time.time() - t:.1f}s")
+    raster_info = device_rasterize(dev, both, args.vpr, 0, args.threads)
     # the pair list is built on the device (and checked against the host's)
     cand_info = device_candidates(dev, A.mbbs(), B.mbbs(), pa, pb, len(A))
 
@@ -759,6 +795,7 @@ def main() -> int:
             f"R={r_h:.4f} ({t_h * 1000:.1f} ms) | filter-only R={r_f:.4f}")
 
     line["candidates"] = cand_info
+    line["rasterize"] = raster_info
 
     if not args.no_cpu:
         rng = np.random.default_rng(args.seed + 7)

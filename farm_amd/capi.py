@@ -74,6 +74,29 @@ class Layer(C.Structure):
 
 
 assert C.sizeof(Layer) == 40
+LAYER_DTYPE = np.dtype([("low", "<f8", (2,)), ("step", "<f8"), ("dimx", "<i4"), ("dimy", "<i4"),
+                        ("status_start", "<u8")])
+assert LAYER_DTYPE.itemsize == 40
+
+
+class RingSet(C.Structure):
+    _fields_ = [("npolys", C.c_int64), ("vert_off", C.c_void_p), ("x", C.c_void_p), ("y", C.c_void_p)]
+
+
+class RasterInfo(C.Structure):
+    _fields_ = [("nrejected", C.c_int64), ("first_rejected", C.c_int64), ("nverts", C.c_uint64),
+                ("nstatus", C.c_uint64), ("npixoff", C.c_uint64), ("nseqs", C.c_uint64), ("nvoff", C.c_uint64),
+                ("nvnodes", C.c_uint64), ("nlayers", C.c_uint64), ("nlstatus", C.c_uint64),
+                ("build_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PolySetBuf(C.Structure):
+    _fields_ = [("hdr", C.c_void_p), ("status", C.c_void_p), ("pixoff", C.c_void_p), ("seqs", C.c_void_p),
+                ("voff", C.c_void_p), ("vnodes", C.c_void_p), ("layers", C.c_void_p), ("lstatus", C.c_void_p),
+                ("areas", C.c_void_p)]
 
 HDR_DTYPE = np.dtype(
     [
@@ -187,6 +210,8 @@ EXPORTS = {
     "farm_last_error": (C.c_char_p, [C.c_void_p]),
     "farm_upload_polygons": (C.c_int, [C.c_void_p, C.POINTER(PolySet)]),
     "farm_upload_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "farm_rasterize": (C.c_int, [C.c_void_p, C.POINTER(RingSet), C.c_int, C.c_int, C.POINTER(RasterInfo)]),
+    "farm_download_polyset": (C.c_int, [C.c_void_p, C.POINTER(PolySetBuf)]),
     "farm_generate_pairs": (
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_double, C.c_int, C.c_uint32, C.c_uint32,
@@ -259,6 +284,7 @@ EXPORTS = {
     "farm_hostset_size": (C.c_int64, [C.c_void_p]),
     "farm_hostset_ring": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "farm_hostset_mbbs": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "farm_hostset_closed_rings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "farm_hostset_rasterize": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "farm_hostset_rasterize_h": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "farm_hostset_layer": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(Layer), C.POINTER(C.c_void_p)]),
@@ -429,7 +455,24 @@ class HostSet:
             "vnodes": arr(ps.vnodes, ps.nvnodes, np.float64),
             "fill": arr(ps.fill, ps.nstatus, np.uint8) if ps.fill else None,
             "category_count": int(ps.category_count),
+            "layers": (np.ctypeslib.as_array(C.cast(ps.layers, C.POINTER(C.c_uint8)), shape=(ps.nlayers * 40,))
+                       .view(LAYER_DTYPE) if ps.nlayers else np.zeros(0, LAYER_DTYPE)),
+            "lstatus": arr(ps.lstatus, ps.nlstatus, np.uint8),
+            "areas": arr(ps.areas, ps.nstatus, np.float64) if ps.areas else None,
         }
+
+    def closed_rings(self, nthreads: int | None = None) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(vert_off, x, y): every ring with its first vertex repeated, the
+        input of Device.rasterize."""
+        n = len(self)
+        off = np.zeros(n + 1, np.uint64)
+        if lib.farm_hostset_closed_rings(self._h, _ptr(off), None, None, 1):
+            raise FarmError(FARM_EINVAL, "closed_rings failed")
+        x = np.empty(max(int(off[-1]), 1), np.float32)
+        y = np.empty(max(int(off[-1]), 1), np.float32)
+        if lib.farm_hostset_closed_rings(self._h, _ptr(off), _ptr(x), _ptr(y), nthreads or default_threads()):
+            raise FarmError(FARM_EINVAL, "closed_rings failed")
+        return off, x[:int(off[-1])], y[:int(off[-1])]
 
     def areas(self, i: int, npix: int) -> np.ndarray:
         p = C.c_void_p()
@@ -464,9 +507,12 @@ def pieces_wkt(s: HostSet, pa: np.ndarray, pb: np.ndarray, pieces: np.ndarray) -
 
 
 def wkt_rings(text: str) -> list[np.ndarray]:
-    """The rings of a MULTIPOLYGON (((x y, ...)), ...) line, as (n, 2) f64 arrays."""
+    """The rings of a MULTIPOLYGON (((x y, ...)), ...) line, as (n, 2) f64
+    arrays of float32 coordinates: like the .dt loader (std::stof), every
+    number is rounded to float first, which gives back the printed floats."""
     body = text[text.index("(") + 1:text.rindex(")")]
-    return [np.array([[float(v) for v in q.split()] for q in ring.split("))")[0].split(",")], np.float64)
+    return [np.array([[float(v) for v in q.split()] for q in ring.split("))")[0].split(",")],
+                     np.float64).astype(np.float32).astype(np.float64)
             for ring in body.split("((")[1:]]
 
 
@@ -522,6 +568,38 @@ class Device:
     def upload_polygons(self, s: HostSet) -> None:
         ps = s.view()
         self._check(lib.farm_upload_polygons(self._h, C.byref(ps)), "upload_polygons")
+
+    def rasterize(self, vert_off: np.ndarray, x: np.ndarray, y: np.ndarray, vpr: int, nlow: int = 0) -> dict:
+        """Rasterize the rings on the device (farm_rasterize) and keep the
+        result as the polygon set; returns the array lengths and rejections."""
+        off = np.ascontiguousarray(vert_off, np.uint64)
+        xs = np.ascontiguousarray(x, np.float32)
+        ys = np.ascontiguousarray(y, np.float32)
+        rs = RingSet(len(off) - 1, _ptr(off), _ptr(xs), _ptr(ys))
+        info = RasterInfo()
+        self._check(lib.farm_rasterize(self._h, C.byref(rs), int(vpr), int(nlow), C.byref(info)), "rasterize")
+        self.raster_info = info.as_dict()
+        self.raster_info["npolys"] = len(off) - 1
+        return self.raster_info
+
+    def download_polyset(self) -> dict:
+        """The device set's arrays (after rasterize), as HostSet.layout names them."""
+        i = self.raster_info
+        out = {
+            "hdr": np.zeros(i["npolys"], HDR_DTYPE),
+            "status": np.zeros(i["nstatus"], np.uint8),
+            "pixoff": np.zeros(i["npixoff"], np.uint32),
+            "seqs": np.zeros((i["nseqs"], 2), np.uint32),
+            "voff": np.zeros(i["nvoff"], np.uint32),
+            "vnodes": np.zeros(i["nvnodes"], np.float64),
+            "layers": np.zeros(i["nlayers"], LAYER_DTYPE),
+            "lstatus": np.zeros(i["nlstatus"], np.uint8),
+            "areas": np.zeros(i["nstatus"], np.float64),
+        }
+        buf = PolySetBuf(*[(_ptr(out[k]) if out[k].size else None) for k in
+                           ("hdr", "status", "pixoff", "seqs", "voff", "vnodes", "layers", "lstatus", "areas")])
+        self._check(lib.farm_download_polyset(self._h, C.byref(buf)), "download_polyset")
+        return out
 
     def upload_pairs(self, a: np.ndarray, b: np.ndarray) -> None:
         a = np.ascontiguousarray(a, dtype=np.uint32)

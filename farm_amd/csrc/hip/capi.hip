@@ -277,6 +277,57 @@ namespace farm {
 __device__ __align__(256) unsigned char scratch[kNaBytes];
 }  // namespace farm
 
+namespace farm {
+
+// Everything the queries derive from an uploaded or device-built set: the
+// vertex padding, per-vertex step differences, per-pixel edge counts and the
+// refinement's 16-step chunk boxes.  Synchronises the stream.
+int finish_set(farm_handle h) {
+    int rc;
+    const uint64_t nverts = h->nverts, nseqs = h->nseqs;
+    for (DevBuf *b : {&h->x, &h->y}) {
+        if ((rc = ensure(h, *b, 4 * (nverts + kVertexPad)))) return rc;
+        HIPCHK(h, hipMemsetAsync((char *)b->p + 4 * nverts, 0, 4 * kVertexPad, h->stream));
+    }
+    // successive vertex differences over the padded arrays (the last one
+    // reads no vertex past the padding and is 0)
+    if ((rc = ensure(h, h->xyd, 16 * (nverts + kVertexPad)))) return rc;
+    hipLaunchKernelGGL(k_vdiff, dim3(blocks_for(nverts + kVertexPad)), dim3(256), 0, h->stream, P<float>(h->x),
+                       P<float>(h->y), nverts + kVertexPad, P<float4>(h->xyd));
+    HIPCHK(h, hipGetLastError());
+    if ((rc = ensure(h, h->pixedges, 4 * h->nstatus))) return rc;
+    h->ring_area_ready = false;
+    h->set_gen++;
+    if (h->npolys > 0) {
+        hipLaunchKernelGGL(k_pixedges, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+                           P<uint32_t>(h->pixedges));
+        HIPCHK(h, hipGetLastError());
+    }
+    // boxes of the refinement's 16-step target chunks, per edge sequence
+    if ((rc = ensure(h, h->chcnt, 4 * (nseqs + 1)))) return rc;
+    if ((rc = ensure(h, h->choff, 4 * (nseqs + 1)))) return rc;
+    if ((rc = fill_u32(h, h->chcnt.p, nseqs + 1, 0))) return rc;
+    if (h->npolys > 0) {
+        hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+                           P<uint32_t>(h->chcnt));
+        HIPCHK(h, hipGetLastError());
+    }
+    if ((rc = exclusive_scan(h, P<uint32_t>(h->chcnt), P<uint32_t>(h->choff), nseqs + 1))) return rc;
+    uint32_t nchunks = 0;
+    HIPCHK(h, hipMemcpyAsync(&nchunks, P<uint32_t>(h->choff) + nseqs, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if ((rc = ensure(h, h->chbox, 16ull * (nchunks + kRefScreen)))) return rc;
+    if (h->npolys > 0) {
+        hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+                           (const uint32_t *)P<uint32_t>(h->choff), P<float4>(h->chbox));
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FARM_OK;
+}
+
+}  // namespace farm
+
 extern "C" {
 
 enum { OUT_NONE = 0, OUT_DEV = 1, OUT_HOST = 2 };
@@ -483,23 +534,14 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     if ((rc = upload(h, h->seqs, ps->seqs, 8 * ps->nseqs))) return rc;
     // vertex arrays carry kVertexPad zeroed floats past the end: the
     // refinement kernel reads target vertices in fixed-size chunks
-    for (DevBuf *b : {&h->x, &h->y}) {
+    for (DevBuf *b : {&h->x, &h->y})
         if ((rc = ensure(h, *b, 4 * (ps->nverts + kVertexPad)))) return rc;
-        HIPCHK(h, hipMemsetAsync((char *)b->p + 4 * ps->nverts, 0, 4 * kVertexPad, h->stream));
-    }
     if ((rc = upload(h, h->x, ps->x, 4 * ps->nverts))) return rc;
     if ((rc = upload(h, h->y, ps->y, 4 * ps->nverts))) return rc;
-    // successive vertex differences over the padded arrays (the last one
-    // reads no vertex past the padding and is 0)
-    if ((rc = ensure(h, h->xyd, 16 * (ps->nverts + kVertexPad)))) return rc;
-    hipLaunchKernelGGL(k_vdiff, dim3(blocks_for(ps->nverts + kVertexPad)), dim3(256), 0, h->stream,
-                       P<float>(h->x), P<float>(h->y), ps->nverts + kVertexPad, P<float4>(h->xyd));
-    HIPCHK(h, hipGetLastError());
     if ((rc = upload(h, h->voff, ps->voff, 4 * ps->nvoff))) return rc;
     if ((rc = upload(h, h->vnodes, ps->vnodes, 8 * ps->nvnodes))) return rc;
-    if ((rc = ensure(h, h->pixedges, 4 * ps->nstatus))) return rc;
     h->has_layers = false;
-    h->ring_area_ready = false;
+    h->areas_ready = layered;
     if (layered) {
         if ((rc = upload(h, h->layers, ps->layers, sizeof(farm_layer) * ps->nlayers))) return rc;
         if ((rc = upload(h, h->lstatus, ps->lstatus, ps->nlstatus))) return rc;
@@ -510,33 +552,14 @@ int farm_upload_polygons(farm_handle h, const farm_polyset *ps) {
     }
     h->npolys = ps->npolys;
     h->nstatus = ps->nstatus;
-    h->set_gen++;
-    if (ps->npolys > 0) {
-        hipLaunchKernelGGL(k_pixedges, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h), ps->npolys,
-                           P<uint32_t>(h->pixedges));
-        HIPCHK(h, hipGetLastError());
-    }
-    // boxes of the refinement's 16-step target chunks, per edge sequence
-    if ((rc = ensure(h, h->chcnt, 4 * (ps->nseqs + 1)))) return rc;
-    if ((rc = ensure(h, h->choff, 4 * (ps->nseqs + 1)))) return rc;
-    if ((rc = fill_u32(h, h->chcnt.p, ps->nseqs + 1, 0))) return rc;
-    if (ps->npolys > 0) {
-        hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h),
-                           ps->npolys, P<uint32_t>(h->chcnt));
-        HIPCHK(h, hipGetLastError());
-    }
-    if ((rc = exclusive_scan(h, P<uint32_t>(h->chcnt), P<uint32_t>(h->choff), ps->nseqs + 1))) return rc;
-    uint32_t nchunks = 0;
-    HIPCHK(h, hipMemcpyAsync(&nchunks, P<uint32_t>(h->choff) + ps->nseqs, 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    if ((rc = ensure(h, h->chbox, 16ull * (nchunks + kRefScreen)))) return rc;
-    if (ps->npolys > 0) {
-        hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)ps->npolys), dim3(256), 0, h->stream, devset(h), ps->npolys,
-                           (const uint32_t *)P<uint32_t>(h->choff), P<float4>(h->chbox));
-        HIPCHK(h, hipGetLastError());
-    }
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    return FARM_OK;
+    h->nverts = ps->nverts;
+    h->nseqs = ps->nseqs;
+    h->npixoff = ps->npixoff;
+    h->nvoff = ps->nvoff;
+    h->nvnodes = ps->nvnodes;
+    h->nlayers = layered ? ps->nlayers : 0;
+    h->nlstatus = layered ? ps->nlstatus : 0;
+    return finish_set(h);
 }
 
 int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint64_t n) {

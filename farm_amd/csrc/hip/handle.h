@@ -44,6 +44,8 @@ struct farm_handle_s {
     int cc = FARM_CATEGORY_COUNT;
     farm::DevBuf hdr, status, pixoff, seqs, x, y, xyd, choff, chcnt, chbox, voff, vnodes, pixedges;
     uint64_t nstatus = 0;
+    // array lengths of the set (farm_polyset's counts)
+    uint64_t nverts = 0, nseqs = 0, npixoff = 0, nvoff = 0, nvnodes = 0, nlayers = 0, nlstatus = 0;
     // pairs
     uint64_t npairs = 0;
     farm::DevBuf pa, pb;
@@ -78,12 +80,15 @@ struct farm_handle_s {
     uint64_t max_layer_pix = 0;   // largest layer (pixels) of the set
     uint64_t max_border_pix = 0;  // most BORDER pixels of a finest layer
     farm::DevBuf layers, lstatus, areas;
+    bool areas_ready = false;  // areas hold the set's per-pixel clipped areas
     farm::DevBuf wctr, wovf, wslab, wslab_big, wout, wmd;
     // intersection
     bool ring_area_ready = false;
     farm::DevBuf ring2a, icounts, iarea, ipieces;
     // candidate generation
     farm::DevBuf cg_boxa, cg_boxq, cg_cellcnt, cg_celloff, cg_items, cg_uoff, cg_cnt, cg_off, cg_keys, cg_keys2;
+    // device raster builder temporaries (freed after each build)
+    std::vector<farm::DevBuf> rtmp;
 };
 
 namespace farm {
@@ -163,6 +168,10 @@ inline int fill_u32(farm_handle h, void *p, uint64_t n, uint32_t v) {
     HIPCHK(h, hipGetLastError());
     return FARM_OK;
 }
+
+// The vertex padding, step differences, per-pixel edge counts and chunk
+// boxes of the handle's set (after its arrays are in place).
+int finish_set(farm_handle h);
 
 inline void free_buf(DevBuf &b) {
     if (b.p) (void)hipFree(b.p);

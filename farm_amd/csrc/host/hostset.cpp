@@ -394,6 +394,35 @@ int farm_hostset_mbbs(const farm_hostset *s, double *out) {
     return FARM_OK;
 }
 
+int farm_hostset_closed_rings(const farm_hostset *s, uint64_t *vert_off, float *x, float *y, int nthreads) {
+    if (!s || !vert_off) return FARM_EINVAL;
+    const int64_t n = npoly(s);
+    uint64_t t = 0;
+    for (int64_t i = 0; i < n; i++) {
+        vert_off[i] = t;
+        t += s->ring_off[i + 1] - s->ring_off[i] + 1;
+    }
+    vert_off[n] = t;
+    if (!x || !y) return FARM_OK;
+    const int nth = std::max(1, nthreads);
+    auto work = [&](int k) {
+        for (int64_t i = k; i < n; i += nth) {
+            const uint64_t r0 = s->ring_off[i], r1 = s->ring_off[i + 1], o = vert_off[i];
+            for (uint64_t v = r0; v < r1; v++) {
+                x[o + v - r0] = s->xy[2 * v];
+                y[o + v - r0] = s->xy[2 * v + 1];
+            }
+            x[o + r1 - r0] = s->xy[2 * r0];  // the decoder's repeated first vertex
+            y[o + r1 - r0] = s->xy[2 * r0 + 1];
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < nth; k++) th.emplace_back(work, k);
+    work(0);
+    for (auto &q : th) q.join();
+    return FARM_OK;
+}
+
 namespace {
 
 int rasterize_all(farm_hostset *s, int vpr, bool hier, int nlow, int nthreads) {
@@ -868,9 +897,30 @@ extern "C" char *farm_pieces_wkt(const farm_hostset *s, const uint32_t *pa, cons
                             next = c;
                             break;
                         }
-                if (next == UINT64_MAX) {
-                    put(ex, ey, false);
+                if (next == UINT64_MAX && ex == sx0 && ey == sy0) {
+                    put(ex, ey, false);  // back at the ring's start
                     break;
+                }
+                if (next == UINT64_MAX) {
+                    // Two crossing points a few ulps apart (one crossing
+                    // computed from two edge pairs) with a BORDER piece
+                    // between them: bridge to the nearest unused start
+                    // within 64 ulps, else close the ring here.
+                    const double tol = std::ldexp(std::max({1.0, (double)std::fabs(ex), (double)std::fabs(ey)}), -17);
+                    double best = tol;
+                    for (uint64_t c = 0; c < hi - lo; c++) {
+                        if (used[c]) continue;
+                        const farm_piece &q = pieces[order[lo + c]];
+                        if (q.nvert < 0) continue;
+                        const float qx = rev(q) ? q.bx : q.ax, qy = rev(q) ? q.by : q.ay;
+                        const double dd = std::max(std::fabs((double)qx - ex), std::fabs((double)qy - ey));
+                        if (dd <= best) {
+                            best = dd;
+                            next = c;
+                        }
+                    }
+                    put(ex, ey, false);
+                    if (next == UINT64_MAX) break;
                 }
                 cur = next;
             }

@@ -680,19 +680,11 @@ inline int grid_off(double v, double lo, double step, int dim) {
 
 }  // namespace
 
-void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, int nlow, RasterOut &out,
-                       std::vector<LayerOut> &layers, RasterScratch *S) {
-    layers.clear();
-    out.g = size_grid(mbb, nv / vpr);
-    if (!align_grid(out.g)) {
-        out.ok = false;
-        make_empty(out);
-        layers.push_back({out.g, {}});
-        return;
-    }
-    build_raster_impl(ring, nv, out, S);
-    if (!out.ok) make_empty(out);
-    const RasterGrid &f = out.g;
+bool hier_grids(const double mbb[4], int nv, int vpr, int nlow, RasterGrid &fine, std::vector<RasterGrid> &coarse) {
+    coarse.clear();
+    fine = size_grid(mbb, nv / vpr);
+    if (!align_grid(fine)) return false;
+    const RasterGrid &f = fine;
     // Ideal::layering (Ideal.cpp:1061-1099): coarser layers by doubling the
     // step until the layer has at most nlow pixels
     const int high_layers = static_cast<int>(
@@ -705,12 +697,12 @@ void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, 
         dy = static_cast<int>(std::ceil(dy / 2.0));
         low_layers--;
     }
-    const int nl = high_layers - low_layers + 1;
-    layers.resize(nl);
-    layers[nl - 1].g = f;
-    for (int i = nl - 2; i >= 0; i--) {
-        RasterGrid &g = layers[i].g;
-        const double s0 = layers[i + 1].g.step[0] * 2, s1 = layers[i + 1].g.step[1] * 2;
+    const int nc = high_layers - low_layers;
+    coarse.resize(nc);
+    for (int i = nc - 1; i >= 0; i--) {
+        RasterGrid &g = coarse[i];
+        const RasterGrid &finer = i + 1 < nc ? coarse[i + 1] : f;
+        const double s0 = finer.step[0] * 2, s1 = finer.step[1] * 2;
         g.step[0] = s0;
         g.step[1] = s1;
         g.low[0] = std::floor(f.low[0] / s0) * s0;
@@ -719,6 +711,30 @@ void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, 
         g.high[1] = std::ceil(f.high[1] / s1) * s1;
         g.dimx = static_cast<int>(std::round((g.high[0] - g.low[0]) / s0));
         g.dimy = static_cast<int>(std::round((g.high[1] - g.low[1]) / s1));
+    }
+    return true;
+}
+
+void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, int nlow, RasterOut &out,
+                       std::vector<LayerOut> &layers, RasterScratch *S) {
+    layers.clear();
+    std::vector<RasterGrid> coarse;
+    if (!hier_grids(mbb, nv, vpr, nlow, out.g, coarse)) {
+        out.ok = false;
+        make_empty(out);
+        layers.push_back({out.g, {}});
+        return;
+    }
+    build_raster_impl(ring, nv, out, S);
+    if (!out.ok) make_empty(out);
+    const RasterGrid &f = out.g;
+    const int nl = (int)coarse.size() + 1;
+    layers.resize(nl);
+    layers[nl - 1].g = f;
+    for (int i = nl - 2; i >= 0; i--) {
+        RasterGrid &g = layers[i].g;
+        g = coarse[i];
+        const double s0 = g.step[0], s1 = g.step[1];
         // Ideal::merge_status (Ideal.cpp:1040-1059): each coarse pixel sums the
         // finest layer's clipped areas over the fine pixels it covers; it is
         // BORDER if any of them is, else it takes the first one's class

@@ -50,4 +50,9 @@ struct LayerOut {
 void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, int nlow, RasterOut &out,
                        std::vector<LayerOut> &layers, RasterScratch *s);
 
+// The grids build_raster_hier uses: the snapped finest grid and the coarse
+// layers (coarse to fine).  false when grid_align fails: the reference
+// raster builder asserts on such a polygon.
+bool hier_grids(const double mbb[4], int nv, int vpr, int nlow, RasterGrid &fine, std::vector<RasterGrid> &coarse);
+
 }  // namespace farm

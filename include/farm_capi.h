@@ -175,6 +175,56 @@ const char *farm_last_error(farm_handle h);
  * host-to-device copies, src/cuda/preprocessing.cu:128-343). */
 int farm_upload_polygons(farm_handle h, const farm_polyset *ps);
 
+/* Rings of a polygon set before rasterization: ring i is x/y[vert_off[i] ..
+ * vert_off[i+1]), its first vertex repeated at the end (the decoded ring of
+ * VertexSequence, vertex_sequence.cpp:206-218), 2 <= length < 2^31. */
+typedef struct farm_ringset {
+    int64_t npolys;
+    const uint64_t *vert_off; /* npolys + 1 */
+    const float *x;
+    const float *y;
+} farm_ringset;
+
+/* Array lengths of the set farm_rasterize built (farm_polyset's counts) and
+ * the polygons the reference raster builder rejects. */
+typedef struct farm_raster_info {
+    int64_t nrejected;
+    int64_t first_rejected; /* -1 if none */
+    uint64_t nverts, nstatus, npixoff, nseqs, nvoff, nvnodes, nlayers, nlstatus;
+    double build_ms;        /* device time from the uploaded vertices to the finished set */
+} farm_raster_info;
+
+/* Rasterize every ring on the device, vpr vertices per pixel (-v), and make
+ * the result the handle's polygon set: the bytes farm_hostset_rasterize
+ * (nlow = 0) or farm_hostset_rasterize_h (nlow > 0: -h with --NLow nlow)
+ * gives, without the host raster build or upload.  Replaces preprocess /
+ * process_rasterization (src/geometry/preprocessing.cpp:4-87,
+ * Ideal::rasterization src/geometry/Ideal.cpp:920-955, with
+ * evaluate_edges, process_crosses, process_intersection,
+ * scanline_reandering and calculate_fullness, Ideal.cpp:35-918; -h:
+ * MyRaster::grid_align MyRaster.cpp:444-491, Ideal::layering and
+ * merge_status Ideal.cpp:1008-1099) and preprocess_for_gpu
+ * (src/cuda/preprocessing.cu:128-343).  A rejected polygon (where the
+ * reference builder asserts or never ends) becomes an empty raster flagged
+ * FARM_HDR_REJECTED.  category_count is 16. */
+int farm_rasterize(farm_handle h, const farm_ringset *rs, int vpr, int nlow, farm_raster_info *info);
+
+/* Copies of the handle's device set arrays (lengths from farm_raster_info);
+ * NULL pointers are skipped; layers and lstatus only for -h sets, areas for
+ * -h sets and sets farm_rasterize built. */
+typedef struct farm_polyset_buf {
+    farm_raster_hdr *hdr;
+    uint8_t *status;
+    uint32_t *pixoff;
+    uint32_t *seqs; /* 2 * nseqs */
+    uint32_t *voff;
+    double *vnodes;
+    farm_layer *layers;
+    uint8_t *lstatus;
+    double *areas;
+} farm_polyset_buf;
+int farm_download_polyset(farm_handle h, farm_polyset_buf *out);
+
 /* Upload candidate pairs (ids index the uploaded set); replaces the pair
  * upload at src/cuda/preprocessing.cu:352-358. */
 int farm_upload_pairs(farm_handle h, const uint32_t *a, const uint32_t *b, uint64_t n);

@@ -44,6 +44,10 @@ int farm_hostset_ring(const farm_hostset *s, int64_t i, const float **xy, int64_
 /* MBB (low x, low y, high x, high y) with the float 1e-9 bias of
  * VertexSequence::getMBR (vertex_sequence.cpp:152-167). */
 int farm_hostset_mbbs(const farm_hostset *s, double *out4n);
+/* The rings as the raster builder reads them (each stored ring plus its
+ * first vertex once more), for farm_rasterize: vert_off gets npolys + 1
+ * offsets; x / y (may be NULL: offsets only) vert_off[npolys] floats each. */
+int farm_hostset_closed_rings(const farm_hostset *s, uint64_t *vert_off, float *x, float *y, int nthreads);
 
 /* Rasterize every polygon with vpr vertices per pixel (-v), nthreads host
  * threads (-n).  Fills the device layout returned by farm_hostset_view. */

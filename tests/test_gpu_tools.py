@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from farm_amd import capi, datagen
+from tests.helpers import check_wkt_areas
 
 pytestmark = pytest.mark.gpu
 
@@ -55,7 +56,7 @@ def test_intersect_and_intersection_executables(tmp_path):
         assert _found([os.path.join(BIN, "intersection"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (area > 0).sum())
     # --wkt: the polygons of A n B, one line per intersecting pair, their
-    # shoelace areas equal to the oracle's areas
+    # shoelace areas equal to the oracle's areas (helpers.check_wkt_areas)
     want = {(int(i), int(j)): float(x) for i, j, x in zip(pa, pb, area) if x > 0}
     for extra in ([], ["-b", "777", "--gpu-candidates"]):
         out = tmp_path / "ab.wkt"
@@ -64,9 +65,7 @@ def test_intersect_and_intersection_executables(tmp_path):
         for line in out.read_text().splitlines():
             i, j, body = line.split("\t", 2)
             got[(int(i), int(j))] = capi.wkt_multipolygon_area(body)
-        assert set(want) <= set(got)
-        for key, x in want.items():
-            assert abs(got[key] - x) <= 1e-9 * x + 1e-15, (key, got[key], x)
+        check_wkt_areas(got, want)
 
 
 def test_within_polygon_executable(tmp_path):
