@@ -130,17 +130,15 @@ def device_candidates(dev, mbb_a, mbb_b, pa, pb, b_base: int, **kw) -> dict:
 
 
 def build_workload(scale: float, seed: int, vpr: int, threads: int):
-    from farm_amd import capi, datagen
+    from farm_amd import capi
 
     side = 40.0 * np.sqrt(scale)
     n_c = max(1, int(round(20000 * scale)))
     n_l = max(1, int(round(500000 * scale)))
     t = time.time()
-    cro, cxy = datagen.complex_set(seed, n_c, 0.0, side)
-    lro, lxy = datagen.lake_set(seed + 1, n_l, 0.0, side)
-    log(f"generated {n_c} complex ({len(cxy)} vertices) + {n_l} lakes ({len(lxy)} vertices) in {time.time() - t:.1f}s")
-    A = capi.HostSet.from_rings(cro, cxy)
-    B = capi.HostSet.from_rings(lro, lxy)
+    A = capi.HostSet.complex_polygons(seed, n_c, 0.0, side, nthreads=threads)
+    B = capi.HostSet.lake_polygons(seed + 1, n_l, 0.0, side, nthreads=threads)
+    log(f"generated {n_c} complex + {n_l} lakes on {threads} host threads in {time.time() - t:.1f}s")
     t = time.time()
     pa, pb = capi.candidates(A, B, nthreads=threads)
     CAND_TIMES["host_rtree_s"] = time.time() - t
@@ -152,7 +150,7 @@ def build_workload(scale: float, seed: int, vpr: int, threads: int):
     bad = both.raster_failures()
     log(f"rasterized {len(both)} polygons on {threads} host threads in {RASTER_TIMES['host_s']:.1f}s "
         f"(first rejected polygon: {bad})")
-    return (cro, cxy, lro, lxy), A, B, both, pa, pb
+    return (A, B), A, B, both, pa, pb
 
 
 RASTER_TIMES: dict = {}
@@ -216,9 +214,8 @@ def one_thread(run, n_sample: int, dt: float, threads: int) -> dict:
 def cpu_baseline(rings, pa, pb, vpr, threads, target_s, rng, plain_bits_fn):
     """Oracle query-phase throughput on a random sample of the pairs."""
     import oracle
-    from farm_amd import datagen
 
-    cro, cxy, lro, lxy = rings
+    HA, HB = rings
     n = len(pa)
     sample = np.sort(rng.choice(n, size=min(n, 400), replace=False))
     results = {}
@@ -226,8 +223,8 @@ def cpu_baseline(rings, pa, pb, vpr, threads, target_s, rng, plain_bits_fn):
     while True:
         ua, ia = np.unique(pa[sample], return_inverse=True)
         ub, ib = np.unique(pb[sample], return_inverse=True)
-        OA = oracle.OracleSet(*datagen.subset(cro, cxy, ua))
-        OB = oracle.OracleSet(*datagen.subset(lro, lxy, ub))
+        OA = oracle.OracleSet(*HA.subset_rings(ua))
+        OB = oracle.OracleSet(*HB.subset_rings(ub))
         OA.rasterize(vpr, nthreads=threads)
         OB.rasterize(vpr, nthreads=threads)
         t = time.time()
@@ -261,15 +258,14 @@ OPS_PER_BOX_PAIR = 15
 
 
 def within_workload(scale: float, seed: int, threads: int):
-    from farm_amd import capi, datagen
+    from farm_amd import capi
 
     side = 20.0 * np.sqrt(scale)
     n = max(2, int(round(300000 * scale)))
     t = time.time()
-    ro, xy = datagen.lake_set(seed + 3, n, 0.0, side)
-    log(f"within: generated {n} lakes ({len(xy)} vertices) in a [0,{side:.1f}]^2 box in {time.time() - t:.1f}s")
+    H = capi.HostSet.lake_polygons(seed + 3, n, 0.0, side, nthreads=threads)
+    log(f"within: generated {n} lakes in a [0,{side:.1f}]^2 box in {time.time() - t:.1f}s")
     t = time.time()
-    H = capi.HostSet.from_rings(ro, xy)
     H.rasterize_h(10, 1, threads)
     RASTER_TIMES["host_s"] = time.time() - t
     log(f"within: -h rasterized (-v 10, --NLow 1) on {threads} host threads in {RASTER_TIMES['host_s']:.1f}s")
@@ -278,19 +274,18 @@ def within_workload(scale: float, seed: int, threads: int):
     CAND_TIMES["host_rtree_s"] = time.time() - t
     log(f"within: {len(pa)} ordered candidate pairs (MBB expanded by 10 km, host R-tree) in "
         f"{CAND_TIMES['host_rtree_s']:.1f}s")
-    return (ro, xy), H, pa, pb
+    return H, pa, pb
 
 
-def within_cpu_baseline(ro, xy, pa, pb, threads, target_s, rng):
+def within_cpu_baseline(H, pa, pb, threads, target_s, rng):
     """Oracle within throughput on a random sample of the pairs (query phase)."""
     import oracle
-    from farm_amd import datagen
 
     n = len(pa)
     sample = np.sort(rng.choice(n, size=min(n, 2000), replace=False))
     while True:
         u, inv = np.unique(np.concatenate([pa[sample], pb[sample]]), return_inverse=True)
-        O = oracle.OracleSet(*datagen.subset(ro, xy, u))
+        O = oracle.OracleSet(*H.subset_rings(u))
         O.rasterize(10, hierarchy=True, nlow=1, nthreads=threads)
         ia = inv[: len(sample)].astype(np.uint32)
         ib = inv[len(sample):].astype(np.uint32)
@@ -316,7 +311,7 @@ def bench_within(args) -> dict:
     """Within-distance self-join: 300,000 lakes in [0,20]^2, 10 km, -h."""
     from farm_amd import capi
 
-    (ro, xy), H, pa, pb = within_workload(args.scale, args.seed, args.threads)
+    H, pa, pb = within_workload(args.scale, args.seed, args.threads)
     dev = capi.Device(0)
     raster_info = device_rasterize(dev, H, 10, 1, args.threads)
     m = H.mbbs()
@@ -353,7 +348,7 @@ def bench_within(args) -> dict:
         "vs_baseline": None,
         "dtype": "f64 (box and point-segment distances)",
         "data": "synthetic, seeded",
-        "config": {"workload": f"within self-join, {len(ro) - 1} lakes, 10 km, -h --NLow 1 -v 10",
+        "config": {"workload": f"within self-join, {len(H)} lakes, 10 km, -h --NLow 1 -v 10",
                    "candidate_pairs": int(len(pa)), "timed_pairs": timed,
                    "found": int(sum(p["found"] for p in per)), "scale": args.scale},
         "point_segment_distances_per_s": sum(p["segment_tests"] for p in per) / wall,
@@ -368,7 +363,7 @@ def bench_within(args) -> dict:
     if not args.no_cpu:
         bits, md, _ = dev.within(10.0)
         rng = np.random.default_rng(args.seed + 9)
-        res = within_cpu_baseline(ro, xy, pa, pb, args.threads, args.cpu_seconds, rng)
+        res = within_cpu_baseline(H, pa, pb, args.threads, args.cpu_seconds, rng)
         samp = res["sample"]
         mism = int((bits[samp] != res["bits"]).sum())
         mdm = int((md[samp].view(np.uint64) != res["md"].view(np.uint64)).sum())
@@ -387,12 +382,11 @@ def bench_within(args) -> dict:
 def bench_candidates(args) -> dict:
     """Candidate pairs of the within self-join (300,000 lakes in [0,20]^2,
     boxes expanded by 10 km) built on the device, against the host R-tree."""
-    from farm_amd import capi, datagen
+    from farm_amd import capi
 
     side = 20.0 * np.sqrt(args.scale)
     n = max(1, int(round(300000 * args.scale)))
-    ro, xy = datagen.lake_set(args.seed + 3, n, 0.0, side)
-    H = capi.HostSet.from_rings(ro, xy)
+    H = capi.HostSet.lake_polygons(args.seed + 3, n, 0.0, side, nthreads=args.threads)
     t = time.time()
     pa, pb = capi.candidates(H, H, expand_km=10.0, skip_self=True, nthreads=args.threads)
     host_s = time.time() - t
@@ -437,18 +431,15 @@ def bench_intersection(args) -> dict:
     """Intersection (crossing points, boundary pieces, area of A n B):
     5,000 complex x 100,000 lakes in [0,20]^2, -v 256."""
     import oracle
-    from farm_amd import capi, datagen
+    from farm_amd import capi
 
     side = 20.0 * np.sqrt(args.scale)
     n_c = max(1, int(round(5000 * args.scale)))
     n_l = max(1, int(round(100000 * args.scale)))
     t = time.time()
-    cro, cxy = datagen.complex_set(args.seed + 5, n_c, 0.0, side)
-    lro, lxy = datagen.lake_set(args.seed + 6, n_l, 0.0, side)
-    log(f"intersection: generated {n_c} complex ({len(cxy)} vertices) + {n_l} lakes ({len(lxy)} vertices) "
-        f"in {time.time() - t:.1f}s")
-    A = capi.HostSet.from_rings(cro, cxy)
-    B = capi.HostSet.from_rings(lro, lxy)
+    A = capi.HostSet.complex_polygons(args.seed + 5, n_c, 0.0, side, nthreads=args.threads)
+    B = capi.HostSet.lake_polygons(args.seed + 6, n_l, 0.0, side, nthreads=args.threads)
+    log(f"intersection: generated {n_c} complex + {n_l} lakes in {time.time() - t:.1f}s")
     pa, pb = capi.candidates(A, B, nthreads=args.threads)
     both = capi.HostSet.concat(A, B)
     t = time.time()
@@ -508,8 +499,8 @@ def bench_intersection(args) -> dict:
         while True:
             ua, ia = np.unique(pa[sample], return_inverse=True)
             ub, ib = np.unique(pb[sample], return_inverse=True)
-            OA = oracle.OracleSet(*datagen.subset(cro, cxy, ua))
-            OB = oracle.OracleSet(*datagen.subset(lro, lxy, ub))
+            OA = oracle.OracleSet(*A.subset_rings(ua))
+            OB = oracle.OracleSet(*B.subset_rings(ub))
             OA.rasterize(args.vpr, nthreads=args.threads)
             OB.rasterize(args.vpr, nthreads=args.threads)
             t = time.time()
@@ -548,6 +539,8 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--vpr", type=int, default=256)
+    ap.add_argument("--category-bits", type=int, default=8,
+                    help="-w: fill classes at 2^b categories for the learned features and -a (exact bits do not depend on it)")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -593,6 +586,7 @@ def main() -> int:
     rings, A, B, both, pa, pb = build_workload(args.scale, args.seed, args.vpr, args.threads)
     dev = capi.Device(0)
     raster_info = device_rasterize(dev, both, args.vpr, 0, args.threads)
+    dev.set_categories(args.category_bits)
     # the pair list is built on the device (and checked against the host's)
     cand_info = device_candidates(dev, A.mbbs(), B.mbbs(), pa, pb, len(A))
 
@@ -689,8 +683,9 @@ def main() -> int:
         "dtype": "f32 (segment predicates), f64 (raster windows)",
         "data": "synthetic, seeded",
         "config": {
-            "workload": f"exact intersect, {len(A)} complex x {len(B)} lakes, -v {args.vpr}, batched kernel, "
-                        f"{'learned' if order else 'cost-bound'} queue order",
+            "workload": f"exact intersect, {len(A)} complex x {len(B)} lakes, -v {args.vpr} -w {args.category_bits}, "
+                        f"batched kernel, {'learned' if order else 'cost-bound'} queue order",
+            "category_bits": args.category_bits,
             "candidate_pairs": int(len(pa)),
             "timed_pairs": timed_pairs,
             "found": int(found),

@@ -96,7 +96,7 @@ class RasterInfo(C.Structure):
 class PolySetBuf(C.Structure):
     _fields_ = [("hdr", C.c_void_p), ("status", C.c_void_p), ("pixoff", C.c_void_p), ("seqs", C.c_void_p),
                 ("voff", C.c_void_p), ("vnodes", C.c_void_p), ("layers", C.c_void_p), ("lstatus", C.c_void_p),
-                ("areas", C.c_void_p)]
+                ("areas", C.c_void_p), ("fill", C.c_void_p)]
 
 HDR_DTYPE = np.dtype(
     [
@@ -212,6 +212,7 @@ EXPORTS = {
     "farm_upload_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "farm_rasterize": (C.c_int, [C.c_void_p, C.POINTER(RingSet), C.c_int, C.c_int, C.POINTER(RasterInfo)]),
     "farm_download_polyset": (C.c_int, [C.c_void_p, C.POINTER(PolySetBuf)]),
+    "farm_set_categories": (C.c_int, [C.c_void_p, C.c_int]),
     "farm_generate_pairs": (
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_double, C.c_int, C.c_uint32, C.c_uint32,
@@ -304,6 +305,11 @@ EXPORTS = {
         [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
          C.c_int],
     ),
+    "farm_hostset_star_polygons_ex": (
+        C.c_void_p,
+        [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+         C.c_int, C.c_double, C.c_int],
+    ),
 }
 
 
@@ -352,6 +358,27 @@ class HostSet:
         (farm_hostset_star_polygons)."""
         return cls(lib.farm_hostset_star_polygons(seed, n, lo, hi, nv_lo, nv_hi, r_mu, r_sigma, amp,
                                                   nthreads or default_threads()))
+
+    @classmethod
+    def complex_polygons(cls, seed: int, n: int, lo: float, hi: float, nv_cap: int = 200_000,
+                         nthreads: int | None = None) -> "HostSet":
+        """The complex polygons of the headline workload (datagen.complex_set's
+        distribution), generated in parallel by the library."""
+        return cls(lib.farm_hostset_star_polygons_ex(seed, n, lo, hi, 1000, nv_cap, 1.2, float(np.log(0.8)), 0.6, 1,
+                                                     0.3, nthreads or default_threads()))
+
+    @classmethod
+    def lake_polygons(cls, seed: int, n: int, lo: float, hi: float, nthreads: int | None = None) -> "HostSet":
+        """Lake-like polygons (datagen.lake_set's distribution), generated in
+        parallel by the library."""
+        return cls(lib.farm_hostset_star_polygons_ex(seed, n, lo, hi, 20, 2000, 0.0, float(np.log(0.05)), 0.8, 0,
+                                                     0.4, nthreads or default_threads()))
+
+    def subset_rings(self, idx: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """(ring_off, xy) of polygons idx as stored, for the oracle."""
+        parts = [self.ring(int(i)) for i in idx]
+        off = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+        return off, (np.concatenate(parts) if parts else np.zeros((0, 2), np.float32))
 
     @classmethod
     def load_dt(cls, path: str) -> "HostSet":
@@ -578,9 +605,15 @@ class Device:
         rs = RingSet(len(off) - 1, _ptr(off), _ptr(xs), _ptr(ys))
         info = RasterInfo()
         self._check(lib.farm_rasterize(self._h, C.byref(rs), int(vpr), int(nlow), C.byref(info)), "rasterize")
+        self.category_bits = 4
         self.raster_info = info.as_dict()
         self.raster_info["npolys"] = len(off) - 1
         return self.raster_info
+
+    def set_categories(self, bits: int) -> None:
+        """-w bits on the device set (farm_set_categories)."""
+        self._check(lib.farm_set_categories(self._h, int(bits)), "set_categories")
+        self.category_bits = int(bits)
 
     def download_polyset(self) -> dict:
         """The device set's arrays (after rasterize), as HostSet.layout names them."""
@@ -595,9 +628,11 @@ class Device:
             "layers": np.zeros(i["nlayers"], LAYER_DTYPE),
             "lstatus": np.zeros(i["nlstatus"], np.uint8),
             "areas": np.zeros(i["nstatus"], np.float64),
+            "fill": np.zeros(i["nstatus"] if getattr(self, "category_bits", 4) != 4 else 0, np.uint8),
         }
         buf = PolySetBuf(*[(_ptr(out[k]) if out[k].size else None) for k in
-                           ("hdr", "status", "pixoff", "seqs", "voff", "vnodes", "layers", "lstatus", "areas")])
+                           ("hdr", "status", "pixoff", "seqs", "voff", "vnodes", "layers", "lstatus", "areas",
+                            "fill")])
         self._check(lib.farm_download_polyset(self._h, C.byref(buf)), "download_polyset")
         return out
 

@@ -11,7 +11,7 @@
 //   area = |sA * sum(A pieces IN B) + sB * sum(B pieces IN A)| / 2
 // (cross-product sums of the piece paths relative to the low corner of the
 // two MBBs' overlap; sX the orientation of ring X; a piece is IN when the
-// midpoint of its middle sub-segment is).  Without two distinct crossing
+// midpoint of its longest sub-segment is).  Without two distinct crossing
 // points the pair is contained (first vertex IN the other) or disjoint.
 //
 // One wavefront per pair:
@@ -308,10 +308,25 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
                 else nvert = (eb0 - ea0 + m) % m + ((eb0 == ea0 && tb0 <= ta0) ? m : 0);
                 if (nvert < 0) nvert = 0;
                 first = ea0 + 1;
-                const int hh = (nvert + 1) / 2;
-                float ux, uy, vx, vy;
-                path_pt(X, Y, m, first, nvert, ax, ay, bx, by, hh, ux, uy);
-                path_pt(X, Y, m, first, nvert, ax, ay, bx, by, hh + 1, vx, vy);
+                // the area's test point: the midpoint of the path's longest
+                // sub-segment (the first on ties), well inside the piece
+                float ux = ax, uy = ay, vx = ax, vy = ay, px = ax, py = ay;
+                double best = -1.0;
+                for (int k = 1; k <= nvert + 1; k++) {
+                    float qx, qy;
+                    path_pt(X, Y, m, first, nvert, ax, ay, bx, by, k, qx, qy);
+                    const double dx = (double)qx - px, dy = (double)qy - py;
+                    const double l = dx * dx + dy * dy;
+                    if (l > best) {
+                        best = l;
+                        ux = px;
+                        uy = py;
+                        vx = qx;
+                        vy = qy;
+                    }
+                    px = qx;
+                    py = qy;
+                }
                 mx = (ux + vx) * 0.5f;
                 my = (uy + vy) * 0.5f;
             }

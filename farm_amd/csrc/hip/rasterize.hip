@@ -1054,6 +1054,36 @@ __global__ __launch_bounds__(kRB) void k_rborder_max(const RPoly *__restrict__ P
     if (threadIdx.x == 0) atomicMax(out, (unsigned long long)tot);
 }
 
+// -w b: fill classes at cc categories from the clipped areas
+// (encodePixelArea, Ideal.cpp:64-88, with cc in place of 16, and the BORDER
+// clamp of calculate_fullness, Ideal.cpp:905-912); one workgroup per polygon.
+__global__ __launch_bounds__(kRB) void k_rfill(const farm_raster_hdr *__restrict__ hdr,
+                                               const uint8_t *__restrict__ status,
+                                               const double *__restrict__ areas, int cc,
+                                               uint8_t *__restrict__ fill) {
+    const farm_raster_hdr h = hdr[blockIdx.x];
+    const double pix_area = h.step[0] * h.step[1];
+    const uint64_t npx = (uint64_t)h.dimx * h.dimy;
+    for (uint64_t p = threadIdx.x; p < npx; p += kRB) {
+        const uint8_t st = status[h.pix_start + p];
+        uint8_t f = 0;
+        if (st == kCC - 1) {
+            f = (uint8_t)(cc - 1);
+        } else if (st != 0) {
+            const double ratio = areas[h.pix_start + p] / pix_area;
+            int c;
+            if (fabs(ratio - 1.0) < 1e-9) c = cc - 1;
+            else if (fabs(ratio) < 1e-9) c = 0;
+            else {
+                c = cvt_x86(ceil(ratio * (cc - 2)));
+                if (c >= cc) c = cc - 1;
+            }
+            f = (uint8_t)max(1, min(c, cc - 2));
+        }
+        fill[h.pix_start + p] = f;
+    }
+}
+
 // Device temporaries of one build, freed when it returns.
 struct Tmp {
     std::vector<void *> bufs;
@@ -1500,6 +1530,25 @@ extern "C" int farm_rasterize(farm_handle h, const farm_ringset *rs, int vpr, in
     return FARM_OK;
 }
 
+extern "C" int farm_set_categories(farm_handle h, int category_bits) {
+    if (!h || category_bits < 2 || category_bits > 8) return fail(h, FARM_EINVAL, "-w %d: bits in 2..8", category_bits);
+    if (h->npolys <= 0 || !h->areas_ready) return fail(h, FARM_ESTATE, "no device set with per-pixel areas");
+    HIPCHK(h, hipSetDevice(h->device));
+    const int cc = 1 << category_bits;
+    if (cc != FARM_CATEGORY_COUNT) {
+        int rc;
+        if ((rc = ensure(h, h->fill, h->nstatus))) return rc;
+        hipLaunchKernelGGL(k_rfill, dim3((unsigned)h->npolys), dim3(kRB), 0, h->stream,
+                           (const farm_raster_hdr *)P<farm_raster_hdr>(h->hdr), (const uint8_t *)P<uint8_t>(h->status),
+                           (const double *)P<double>(h->areas), cc, P<uint8_t>(h->fill));
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    h->cc = cc;
+    h->set_gen++;
+    return FARM_OK;
+}
+
 extern "C" int farm_download_polyset(farm_handle h, farm_polyset_buf *o) {
     if (!h || !o) return fail(h, FARM_EINVAL, "farm_download_polyset: bad arguments");
     if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygon set on the device");
@@ -1518,6 +1567,7 @@ extern "C" int farm_download_polyset(farm_handle h, farm_polyset_buf *o) {
          (rc = get(o->lstatus, h->lstatus, h->nlstatus))))
         return rc;
     if (h->areas_ready && (rc = get(o->areas, h->areas, 8 * h->nstatus))) return rc;
+    if (h->cc != FARM_CATEGORY_COUNT && (rc = get(o->fill, h->fill, h->nstatus))) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return FARM_OK;
 }

@@ -118,6 +118,31 @@ bool read_ring(const char *w, size_t &off, size_t len, std::vector<float> &out) 
 
 }  // namespace
 
+namespace farm {
+
+// A set from closed rings already in the stored layout (every ring has at
+// least 3 stored vertices), taking the arrays over; MBBs in parallel.
+farm_hostset *hostset_adopt(std::vector<uint64_t> &&ring_off, std::vector<float> &&xy, int nthreads) {
+    farm_hostset *s = new farm_hostset();
+    const int64_t n = (int64_t)ring_off.size() - 1;
+    s->ring_off = std::move(ring_off);
+    s->xy = std::move(xy);
+    s->hole_first.assign((size_t)n + 1, 0);
+    s->mbb.resize(4 * (size_t)n);
+    const int nth = std::max(1, nthreads);
+    auto work = [&](int t) {
+        for (int64_t i = t; i < n; i += nth)
+            ring_mbb(s->xy.data() + 2 * s->ring_off[i], s->ring_off[i + 1] - s->ring_off[i], &s->mbb[4 * i]);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &q : th) q.join();
+    return s;
+}
+
+}  // namespace farm
+
 extern "C" {
 
 farm_hostset *farm_hostset_from_rings(int64_t n, const uint64_t *ring_off, const float *xy) {

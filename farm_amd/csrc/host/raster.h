@@ -56,3 +56,10 @@ void build_raster_hier(const float *ring, int nv, const double mbb[4], int vpr, 
 bool hier_grids(const double mbb[4], int nv, int vpr, int nlow, RasterGrid &fine, std::vector<RasterGrid> &coarse);
 
 }  // namespace farm
+
+struct farm_hostset;
+namespace farm {
+// A host set over closed rings in the stored layout (each ring >= 3 stored
+// vertices), taking the arrays over; MBBs computed on nthreads threads.
+farm_hostset *hostset_adopt(std::vector<uint64_t> &&ring_off, std::vector<float> &&xy, int nthreads);
+}  // namespace farm

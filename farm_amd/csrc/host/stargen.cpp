@@ -1,6 +1,6 @@
 // Seeded synthetic star-shaped polygon sets, generated in parallel on the
-// host: the workloads' polygon generators at the sizes the learned part's
-// training and evaluation need (millions of polygons per set).
+// host: the workloads' polygon generators (star pairs, complex polygons,
+// lakes) at full size -- hundreds of millions of vertices in about a second.
 //
 // Polygon i draws from its own counter-based stream (splitmix64 of the seed
 // and i), so a set does not depend on the thread count.  Per polygon: the
@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../../include/farm_host.h"
+#include "raster.h"
 
 namespace {
 
@@ -43,16 +44,30 @@ struct Stream {
 
 }  // namespace
 
-extern "C" farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, double lo, double hi, int nv_lo,
-                                                    int nv_hi, double r_mu, double r_sigma, double amp,
-                                                    int nthreads) {
-    if (n < 0 || nv_lo < 3 || nv_hi < nv_lo || !(hi >= lo)) return nullptr;
+namespace {
+
+// Shared body: vertex counts uniform in [nv_lo, nv_hi], or with
+// pareto_alpha > 0 Pareto(alpha) with scale nv_lo, floored and clipped to
+// [nv_lo, nv_hi]; noise 0: r = R (1 + amp u), u ~ U(-1, 1) per vertex;
+// noise 1: three octaves, r = R (1 + amp s / 1.75),
+// s = sum_o 0.5^o sin(f 2^o theta + phi_o), f ~ U{2..6}, phi_o ~ U[0, 2 pi)
+// per polygon.
+farm_hostset *star_set(uint64_t seed, int64_t n, double lo, double hi, int nv_lo, int nv_hi, double pareto_alpha,
+                       double r_mu, double r_sigma, int noise, double amp, int nthreads) {
+    if (n < 0 || nv_lo < 3 || nv_hi < nv_lo || !(hi >= lo) || noise < 0 || noise > 1 || pareto_alpha < 0)
+        return nullptr;
     const int nth = std::max(1, nthreads);
     // vertex counts first (their own streams), then the ring offsets
     std::vector<uint64_t> off((size_t)n + 1, 0);
     for (int64_t i = 0; i < n; i++) {
         Stream r(seed, (uint64_t)i, 0);
-        const int nv = nv_lo + (int)std::min<double>(nv_hi - nv_lo, std::floor(r.uniform() * (nv_hi - nv_lo + 1)));
+        int nv;
+        if (pareto_alpha > 0) {
+            const double v = std::floor(nv_lo * std::pow(r.uniform(), -1.0 / pareto_alpha));
+            nv = (int)std::min<double>(nv_hi, std::max<double>(nv_lo, v));
+        } else {
+            nv = nv_lo + (int)std::min<double>(nv_hi - nv_lo, std::floor(r.uniform() * (nv_hi - nv_lo + 1)));
+        }
         off[i + 1] = off[i] + (uint64_t)nv + 1;
     }
     std::vector<float> xy(2 * off[n]);
@@ -64,6 +79,11 @@ extern "C" farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, do
             const double cx = lo + (hi - lo) * r.uniform();
             const double cy = lo + (hi - lo) * r.uniform();
             const double R = std::exp(r_mu + r_sigma * r.normal());
+            double f = 0, ph[3] = {0, 0, 0};
+            if (noise == 1) {
+                f = 2 + std::min(4.0, std::floor(r.uniform() * 5));
+                for (double &p : ph) p = 2.0 * M_PI * r.uniform();
+            }
             e.resize((size_t)nv + 1);
             double acc = 0;
             for (int k = 0; k <= nv; k++) {
@@ -74,7 +94,14 @@ extern "C" farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, do
             float *p = xy.data() + 2 * off[i];
             for (int k = 0; k < nv; k++) {
                 const double th = e[k] * inv;
-                const double rad = R * (1.0 + amp * (2.0 * r.uniform() - 1.0));
+                double rad;
+                if (noise == 1) {
+                    double sn = 0;
+                    for (int o = 0; o < 3; o++) sn += std::ldexp(1.0, -o) * std::sin(f * (1 << o) * th + ph[o]);
+                    rad = R * (1.0 + amp * sn / 1.75);
+                } else {
+                    rad = R * (1.0 + amp * (2.0 * r.uniform() - 1.0));
+                }
                 p[2 * k] = (float)(cx + rad * std::cos(th));
                 p[2 * k + 1] = (float)(cy + rad * std::sin(th));
             }
@@ -86,5 +113,19 @@ extern "C" farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, do
     for (int t = 1; t < nth; t++) th.emplace_back(work, t);
     work(0);
     for (auto &x : th) x.join();
-    return farm_hostset_from_rings(n, off.data(), xy.data());
+    return farm::hostset_adopt(std::move(off), std::move(xy), nth);
+}
+
+}  // namespace
+
+extern "C" farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, double lo, double hi, int nv_lo,
+                                                    int nv_hi, double r_mu, double r_sigma, double amp,
+                                                    int nthreads) {
+    return star_set(seed, n, lo, hi, nv_lo, nv_hi, 0.0, r_mu, r_sigma, 0, amp, nthreads);
+}
+
+extern "C" farm_hostset *farm_hostset_star_polygons_ex(uint64_t seed, int64_t n, double lo, double hi, int nv_lo,
+                                                       int nv_hi, double pareto_alpha, double r_mu, double r_sigma,
+                                                       int noise, double amp, int nthreads) {
+    return star_set(seed, n, lo, hi, nv_lo, nv_hi, pareto_alpha, r_mu, r_sigma, noise, amp, nthreads);
 }

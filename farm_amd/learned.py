@@ -51,6 +51,7 @@ class PairSet:
     pa: np.ndarray
     pb: np.ndarray  # already offset into the combined set
     n_instances: int
+    on_device: bool = False  # rasterized on the device: it is that device's current set
 
 
 def build_pairsets(seeds: list[int], n_per_side: int = 2000) -> PairSet:
@@ -218,24 +219,36 @@ STAR = dict(nv_lo=50, nv_hi=500, r_mu=float(np.log(0.5)), r_sigma=0.5, amp=0.4)
 PAIRS_PER_INSTANCE = 4240  # candidate pairs per 2,000 x 2,000 polygons at that density
 
 
-def star_domain(seed: int, instances: int, threads: int | None = None, vpr: int = 10) -> PairSet:
+def star_domain(seed: int, instances: int, threads: int | None = None, vpr: int = 10,
+                dev: capi.Device | None = None) -> PairSet:
     """instances x 2,000 source and target star polygons in a box of side
     100 sqrt(instances) (the plain intersect workload's density), generated
-    by the library in parallel, with their candidate pairs, rasterized."""
+    by the library in parallel, with their candidate pairs, rasterized on
+    the host, or with dev on that device (farm_rasterize, the same bytes),
+    where it becomes the current set."""
     side = 100.0 * float(np.sqrt(instances))
     n = 2000 * instances
     A = capi.HostSet.star_polygons(2 * seed, n, 0.0, side, nthreads=threads, **STAR)
     B = capi.HostSet.star_polygons(2 * seed + 1, n, 0.0, side, nthreads=threads, **STAR)
     pa, pb = capi.candidates(A, B, nthreads=threads)
     both = capi.HostSet.concat(A, B)
-    both.rasterize(vpr, threads)
-    return PairSet(both, pa, (pb.astype(np.int64) + len(A)).astype(np.uint32), instances)
+    if dev is None:
+        both.rasterize(vpr, threads)
+    else:
+        dev.rasterize(*both.closed_rings(threads), vpr)
+    return PairSet(both, pa, (pb.astype(np.int64) + len(A)).astype(np.uint32), instances, dev is not None)
+
+
+def load(dev: capi.Device, ps: PairSet) -> None:
+    """Make ps the device's polygon set and pair list."""
+    if not ps.on_device:
+        dev.upload_polygons(ps.both)
+    dev.upload_pairs(ps.pa, ps.pb)
 
 
 def label_set(dev: capi.Device, ps: PairSet) -> Labeled:
     """Exact labels (bits, executed tests) and features of an uploaded set."""
-    dev.upload_polygons(ps.both)
-    dev.upload_pairs(ps.pa, ps.pb)
+    load(dev, ps)
     bits, tests, _ = dev.intersect_costs()
     feat, dec = dev.pair_features()
     return Labeled(feat, dec, bits, tests)
@@ -256,10 +269,10 @@ def config5(dev: capi.Device, train_pairs: int = 2_000_000, eval_pairs: int = 10
 
     say = log or (lambda m: None)
     t0 = time.time()
-    tr = label_set(dev, star_domain(1000 + seed, max(1, round(train_pairs / PAIRS_PER_INSTANCE)), threads))
+    tr = label_set(dev, star_domain(1000 + seed, max(1, round(train_pairs / PAIRS_PER_INSTANCE)), threads, dev=dev))
     norm, losses = train(dev, tr, seed=seed, epochs=epochs)
     params = dev.mlp_get_params()
-    cal = label_set(dev, star_domain(2000 + seed, max(1, round(cal_pairs / PAIRS_PER_INSTANCE)), threads))
+    cal = label_set(dev, star_domain(2000 + seed, max(1, round(cal_pairs / PAIRS_PER_INSTANCE)), threads, dev=dev))
     dev.mlp_set_params(params, norm)
     z_cal = dev.mlp_forward(apply_norm(cal.feat, norm))[:, 0]
     thr = threshold_for_precision(cal.decided, z_cal, cal.bits.astype(bool), 0.99, 0.999)
@@ -271,9 +284,8 @@ def config5(dev: capi.Device, train_pairs: int = 2_000_000, eval_pairs: int = 10
     n_eval, b = 0, 0
     while n_eval < eval_pairs:
         inst = max(1, round(min(batch_pairs, eval_pairs - n_eval) / PAIRS_PER_INSTANCE))
-        ps = star_domain(100_000 + 1000 * seed + b, inst, threads)
-        dev.upload_polygons(ps.both)
-        dev.upload_pairs(ps.pa, ps.pb)
+        ps = star_domain(100_000 + 1000 * seed + b, inst, threads, dev=dev)
+        load(dev, ps)
         n = len(ps.pa)
         res = {}
         for name, kw in (("exact", dict(mode=capi.MODE_EXACT_BATCHED)),

@@ -222,8 +222,17 @@ typedef struct farm_polyset_buf {
     farm_layer *layers;
     uint8_t *lstatus;
     double *areas;
+    uint8_t *fill; /* after farm_set_categories with b != 4 */
 } farm_polyset_buf;
 int farm_download_polyset(farm_handle h, farm_polyset_buf *out);
+
+/* -w b on the device set (b in 2..8): fill classes at 2^b categories for the
+ * approximate mode and the learned features, from the per-pixel clipped
+ * areas of a set farm_rasterize built or an -h upload -- the bytes
+ * farm_hostset_set_categories gives (encodePixelArea, Ideal.cpp:64-88, with
+ * 2^b in place of query_context::category_count, query_context.h:108).
+ * b = 4: the status bytes.  Exact results do not change. */
+int farm_set_categories(farm_handle h, int category_bits);
 
 /* Upload candidate pairs (ids index the uploaded set); replaces the pair
  * upload at src/cuda/preprocessing.cu:352-358. */

@@ -96,6 +96,15 @@ void farm_free(void *p);
  * Returns NULL on bad arguments. */
 farm_hostset *farm_hostset_star_polygons(uint64_t seed, int64_t n, double lo, double hi, int nv_lo, int nv_hi,
                                          double r_mu, double r_sigma, double amp, int nthreads);
+/* The same generator with the other workloads' options: pareto_alpha > 0
+ * draws vertex counts from Pareto(alpha) with scale nv_lo, floored and
+ * clipped to [nv_lo, nv_hi] (complex polygons: 1.2, 1,000, 200,000);
+ * noise 1 replaces the per-vertex noise by three octaves of radial noise,
+ * r = R (1 + amp s / 1.75) with s = sum_o 0.5^o sin(f 2^o theta + phi_o),
+ * f ~ U{2..6} and phi_o ~ U[0, 2 pi) per polygon. */
+farm_hostset *farm_hostset_star_polygons_ex(uint64_t seed, int64_t n, double lo, double hi, int nv_lo, int nv_hi,
+                                            double pareto_alpha, double r_mu, double r_sigma, int noise, double amp,
+                                            int nthreads);
 
 /* The polygons of A n B assembled from farm_intersection_pieces output, as
  * WKT: one line "<k>\tMULTIPOLYGON (((x y, ...)), ...)" per pair k of the

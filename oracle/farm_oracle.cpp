@@ -1350,9 +1350,10 @@ static int segment_contain(const Poly &P, const Pt &p) {
 }
 
 /* Cross-product sum of the path a -> ring vertices -> b relative to origin o,
- * and its representative point: the midpoint (f32) of the middle sub-segment
- * of the path.  first/last: ring vertex range (cyclic over nv - 1 vertices),
- * nvert vertices in it. */
+ * and its representative point: the midpoint (f32) of the longest
+ * sub-segment of the path (squared length in f64 from the f32 points, the
+ * first one on ties), a point well inside the piece.  first/last: ring
+ * vertex range (cyclic over nv - 1 vertices), nvert vertices in it. */
 static double path_cross(const Poly &P, const Pt &a, const Pt &b, int first, int nvert, double ox, double oy,
                          Pt *rep) {
     const int m = P.nv() - 1;
@@ -1366,7 +1367,17 @@ static double path_cross(const Poly &P, const Pt &a, const Pt &b, int first, int
         const Pt u = at(k), v = at(k + 1);
         sum += ((double)u.x - ox) * ((double)v.y - oy) - ((double)v.x - ox) * ((double)u.y - oy);
     }
-    const int h = (nvert + 1) / 2;
+    int h = 0;
+    double best = -1.0;
+    for (int k = 0; k <= nvert; k++) {
+        const Pt u = at(k), v = at(k + 1);
+        const double dx = (double)v.x - u.x, dy = (double)v.y - u.y;
+        const double l = dx * dx + dy * dy;
+        if (l > best) {
+            best = l;
+            h = k;
+        }
+    }
     const Pt u = at(h), v = at(h + 1);
     rep->x = (u.x + v.x) * 0.5f;
     rep->y = (u.y + v.y) * 0.5f;
@@ -1397,7 +1408,7 @@ struct IxOut {
  * where a piece runs along one ring between consecutive intersection points
  * (ordered by edge, then parameter), its sum is the cross-product sum of its
  * path relative to the low corner of the two MBBs' overlap, it counts when
- * the midpoint of its middle sub-segment is IN the other polygon
+ * the midpoint of its longest sub-segment is IN the other polygon
  * (segment_contain; BORDER pieces are not credited), and sX = sign of ring
  * X's signed area.  With fewer than two distinct intersection points the
  * area is area(A) if A's first vertex is IN B, area(B) if B's is IN A, else

@@ -173,3 +173,26 @@ def test_queries_on_a_device_built_set(device):
     device.upload_pairs(qa, qb)
     want_w, wmin, _ = device.within(10.0)
     assert np.array_equal(got, want_w) and np.array_equal(gmin.view(np.uint64), wmin.view(np.uint64))
+
+
+@pytest.mark.parametrize("bits", [3, 8])
+def test_fill_categories_on_the_device(device, bits):
+    # -w b on a device-built set: the host's fill bytes at 2^b categories,
+    # and the reference -a at those categories (oracle mode 1)
+    (a, b) = datagen.star_pair_set(31, 500, 500)
+    A, B = capi.HostSet.from_rings(*a), capi.HostSet.from_rings(*b)
+    both = capi.HostSet.concat(A, B)
+    _build(device, both, 10)
+    device.set_categories(bits)
+    D = device.download_polyset()
+    both.rasterize(10)
+    both.set_categories(bits)
+    assert np.array_equal(D["fill"], both.layout()["fill"])
+    pa, pb = capi.candidates(A, B)
+    OA, OB = oracle.OracleSet(*a), oracle.OracleSet(*b)
+    OA.rasterize(10)
+    OB.rasterize(10)
+    want, _ = oracle.intersect(OA, OB, pa, pb, 1, category_bits=bits)
+    device.upload_pairs(pa, pb + len(A))
+    assert np.array_equal(device.intersect(mode=capi.MODE_APPROX_HEURISTIC).bits, want)
+    device.set_categories(4)
