@@ -105,8 +105,11 @@ def measured_bytes_per_pair(name: str) -> float | None:
     return float(f.group(1)) + float(w.group(1)) if f and w else None
 
 
+T_START = time.time()
+
+
 def log(msg: str) -> None:
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.time() - T_START:6.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 CAND_TIMES: dict = {}

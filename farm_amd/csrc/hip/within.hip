@@ -91,11 +91,26 @@ __device__ __forceinline__ bool border(uint8_t s) { return s != 0 && s != kCC - 
 
 // degree_per_kilometer_longitude(lat) (util.h:88-96); the index is clamped
 // so that |lat| > 90 cannot read outside the table
-__device__ __forceinline__ double km_lon(double lat) {
+__device__ __forceinline__ int km_idx(double lat) {
     double a = fabs(lat);
     if (a == 90) a = 89.9;
-    int i = (int)(a * 10);
-    return c_km_lon[min(max(i, 0), 899)];
+    const int i = (int)(a * 10);
+    return min(max(i, 0), 899);
+}
+
+// The table entries one pair can need (its latitude span is a few tenths of
+// a degree), copied per wave into LDS: a lookup is an LDS read instead of a
+// dependent global load; an index outside the copy reads the table.
+constexpr int kKmC = 16;
+struct KmC {
+    double v[kKmC];
+    int lo, n;
+};
+
+__device__ __forceinline__ double km_lon(double lat, const KmC &c) {
+    const int i = km_idx(lat);
+    const int k = i - c.lo;
+    return (unsigned)k < (unsigned)c.n ? c.v[k] : c_km_lon[i];
 }
 
 // MyRaster::get_pixel_box (MyRaster.cpp:312-322), square pixels
@@ -107,7 +122,7 @@ __device__ __forceinline__ void pbox(const LGrid &g, int x, int y, double b[4]) 
 }
 
 // box::distance(box, geography) (Box.cpp:89-120), b = this
-__device__ __forceinline__ double box_min_dist(const double b[4], const double t[4]) {
+__device__ __forceinline__ double box_min_dist(const double b[4], const double t[4], const KmC &kc) {
     if (!(t[0] > b[2] || t[2] < b[0] || t[1] > b[3] || t[3] < b[1])) return 0.0;
     double dx = 0, dy = 0;
     if (t[1] > b[3]) dy = t[1] - b[3];
@@ -115,26 +130,27 @@ __device__ __forceinline__ double box_min_dist(const double b[4], const double t
     if (t[0] > b[2]) dx = t[0] - b[2];
     else if (t[2] < b[0]) dx = b[0] - t[2];
     dy = dy / kKmLat;
-    dx = dx / km_lon(b[1]);
+    dx = dx / km_lon(b[1], kc);
     return sqrt(dx * dx + dy * dy);
 }
 
 // box::max_distance(box, geography) (Box.cpp:122-149): corners as f32
 // points, f32 differences promoted to double; the largest |difference| per
 // axis gives the maximum over the 16 corner pairs
-__device__ __forceinline__ double box_max_dist(const double b[4], const double t[4]) {
+__device__ __forceinline__ double box_max_dist(const double b[4], const double t[4], const KmC &kc) {
     const float b0 = (float)b[0], b1 = (float)b[1], b2 = (float)b[2], b3 = (float)b[3];
     const float t0 = (float)t[0], t1 = (float)t[1], t2 = (float)t[2], t3 = (float)t[3];
     const float fx = fmaxf(fmaxf(fabsf(b0 - t0), fabsf(b0 - t2)), fmaxf(fabsf(b2 - t0), fabsf(b2 - t2)));
     const float fy = fmaxf(fmaxf(fabsf(b1 - t1), fabsf(b1 - t3)), fmaxf(fabsf(b3 - t1), fabsf(b3 - t3)));
     const double dy = (double)fy / kKmLat;
-    const double dx = (double)fx / km_lon(b[1]);
+    const double dx = (double)fx / km_lon(b[1], kc);
     return sqrt(dx * dx + dy * dy);
 }
 
 // point_to_segment_distance (geometry_computation.h:49-89), geography on,
 // before the square root
-__device__ __forceinline__ double pt_seg_d2(float px, float py, float ax, float ay, float bx, float by) {
+__device__ __forceinline__ double pt_seg_d2(float px, float py, float ax, float ay, float bx, float by,
+                                            const KmC &kc) {
     const double A = (double)(px - ax);
     const double B = (double)(py - ay);
     const double C = (double)(bx - ax);
@@ -156,7 +172,7 @@ __device__ __forceinline__ double pt_seg_d2(float px, float py, float ax, float 
     }
     double dx = px - xx;
     double dy = py - yy;
-    dx = dx / km_lon(py);
+    dx = dx / km_lon(py, kc);
     dy = dy / kKmLat;
     return dx * dx + dy * dy;
 }
@@ -298,6 +314,7 @@ __device__ __forceinline__ void expand_side(const T *list, uint32_t n, bool next
 struct SeqTab {
     uint32_t fa[64], la[64], fb[64], lb[64], w1[64], pre[65];
     unsigned long long mins[64];  // squared distance bits (non-negative doubles order as integers)
+    KmC km;                       // the km table entries of the current pair
 };
 
 constexpr int kBatch = 4;  // candidates refined together (their sequence pairs share one table)
@@ -384,7 +401,7 @@ __device__ __forceinline__ int refine_batch(const DevSet &S, const farm_raster_h
             const uint32_t i = e2 / m, j = e2 - i * m;
             const float *ptx = tgt_pt ? txv + fb_ : sxv + fa_, *pty = tgt_pt ? tyv + fb_ : syv + fa_;
             const float *sgx = tgt_pt ? sxv + fa_ : txv + fb_, *sgy = tgt_pt ? syv + fa_ : tyv + fb_;
-            const double d2 = pt_seg_d2(ptx[i], pty[i], sgx[j], sgy[j], sgx[j + 1], sgy[j + 1]);
+            const double d2 = pt_seg_d2(ptx[i], pty[i], sgx[j], sgy[j], sgx[j + 1], sgy[j + 1], tab.km);
             acc = fmin(acc, d2);
         }
         if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
@@ -483,6 +500,19 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
     const farm_raster_hdr &hs = S.hdr[ia];
     const farm_raster_hdr &ht = S.hdr[ib];
     const int s_level = hs.nlayers - 1, t_level = ht.nlayers - 1;
+    {
+        // the km table entries of the pair's latitudes (its two grid boxes)
+        const double ylo = fmin(hs.mbr[1], ht.mbr[1]), yhi = fmax(hs.mbr[3], ht.mbr[3]);
+        const int i0 = (ylo <= 0 && yhi >= 0) ? 0 : km_idx(fmin(fabs(ylo), fabs(yhi)));
+        const int i1 = km_idx(fmax(fabs(ylo), fabs(yhi)));
+        const int n = min(i1 - i0 + 1, kKmC);
+        if (lane < n) tab.km.v[lane] = c_km_lon[i0 + lane];
+        if (lane == 0) {
+            tab.km.lo = i0;
+            tab.km.n = n;
+        }
+        wave_sync();
+    }
     // every layer must fit the pixel buffers (the finest is the largest,
     // but check all: a coarse layer may be one pixel wider)
     for (int k = 0; k <= s_level; k++) {
@@ -555,7 +585,7 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                 double b1[4], b2[4];
                 pbox(gs, unpack_x<SH>(va), unpack_y<SH>(va), b1);
                 pbox(gt, unpack_x<SH>(vb), unpack_y<SH>(vb), b2);
-                const double mx = box_max_dist(b1, b2);
+                const double mx = box_max_dist(b1, b2, tab.km);
                 hit |= (mx <= d);
                 maxd = fmin(maxd, mx);
                 a += q;
@@ -594,14 +624,14 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                     double b1[4], b2[4];
                     pbox(gs, xa, ya, b1);
                     pbox(gt, xb, yb, b2);
-                    const double mn = box_min_dist(b1, b2);
+                    const double mn = box_min_dist(b1, b2, tab.km);
                     if (mn <= d && mn < maxd) {
                         keep = true;
                         const uint32_t id1 = (uint32_t)(ya * gs.dimx + xa), id2 = (uint32_t)(yb * gt.dimx + xb);
                         set_bit(sb, id1);
                         set_bit(tb, id2);
                         if (final_level) {
-                            const double hi = box_max_dist(b1, b2);
+                            const double hi = box_max_dist(b1, b2, tab.km);
                             const int pf = classify(S.areas[hs.pix_start + id1] + S.areas[ht.pix_start + id2],
                                                     sa + ta, 20);
                             c.apx = mn + (double)c_mean[pf] * (hi - mn);
@@ -693,7 +723,7 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
 // pair indices come from `list` (pairs the LDS path could not hold) and the
 // lists / bits live in a per-wave global slab.
 template <bool BIG>
-__global__ __launch_bounds__(256) void k_within(DevSet S, const uint32_t *__restrict__ pa,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_within(DevSet S, const uint32_t *__restrict__ pa,
                                                 const uint32_t *__restrict__ pb, uint32_t n,
                                                 const uint32_t *__restrict__ list, double d, uint8_t *__restrict__ out,
                                                 double *__restrict__ md, char *__restrict__ slab, uint64_t slab_stride,
