@@ -87,7 +87,7 @@ class RasterInfo(C.Structure):
     _fields_ = [("nrejected", C.c_int64), ("first_rejected", C.c_int64), ("nverts", C.c_uint64),
                 ("nstatus", C.c_uint64), ("npixoff", C.c_uint64), ("nseqs", C.c_uint64), ("nvoff", C.c_uint64),
                 ("nvnodes", C.c_uint64), ("nlayers", C.c_uint64), ("nlstatus", C.c_uint64),
-                ("build_ms", C.c_double)]
+                ("build_ms", C.c_double), ("upload_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -286,6 +286,9 @@ EXPORTS = {
     "farm_hostset_ring": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "farm_hostset_mbbs": (C.c_int, [C.c_void_p, C.c_void_p]),
     "farm_hostset_closed_rings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "farm_hostset_rasterize_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                                C.POINTER(RasterInfo)]),
+    "farm_set_staging": (C.c_int, [C.c_void_p, C.c_uint64]),
     "farm_hostset_rasterize": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "farm_hostset_rasterize_h": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "farm_hostset_layer": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(Layer), C.POINTER(C.c_void_p)]),
@@ -614,6 +617,21 @@ class Device:
         """-w bits on the device set (farm_set_categories)."""
         self._check(lib.farm_set_categories(self._h, int(bits)), "set_categories")
         self.category_bits = int(bits)
+
+    def rasterize_hostset(self, s: "HostSet", vpr: int, nlow: int = 0, nthreads: int | None = None) -> dict:
+        """The host set rasterized on the device, its rings streamed through
+        pinned staging buffers (farm_hostset_rasterize_device)."""
+        info = RasterInfo()
+        self._check(lib.farm_hostset_rasterize_device(self._h, s._h, int(vpr), int(nlow),
+                                                      nthreads or default_threads(), C.byref(info)),
+                    "rasterize_hostset")
+        self.raster_info = info.as_dict()
+        self.raster_info["npolys"] = len(s)
+        self.category_bits = 4
+        return self.raster_info
+
+    def set_staging(self, vertices: int) -> None:
+        self._check(lib.farm_set_staging(self._h, int(vertices)), "set_staging")
 
     def download_polyset(self) -> dict:
         """The device set's arrays (after rasterize), as HostSet.layout names them."""

@@ -451,6 +451,10 @@ int farm_close(farm_handle h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
+    for (int k = 0; k < 2; k++) {
+        if (h->stage[k]) (void)hipHostFree(h->stage[k]);
+        if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
+    }
     DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->xyd,      &h->fill,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
                       &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->isx,
                       &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,

@@ -87,8 +87,10 @@ struct farm_handle_s {
     farm::DevBuf ring2a, icounts, iarea, ipieces;
     // candidate generation
     farm::DevBuf cg_boxa, cg_boxq, cg_cellcnt, cg_celloff, cg_items, cg_uoff, cg_cnt, cg_off, cg_keys, cg_keys2;
-    // device raster builder temporaries (freed after each build)
-    std::vector<farm::DevBuf> rtmp;
+    // pinned staging of the streamed vertex upload (two buffers, their DMA events)
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {};
+    uint64_t stage_verts = 1ull << 22;  // vertices per staging buffer (8 B each)
 };
 
 namespace farm {

@@ -10,6 +10,7 @@
 // (Ideal::intersect, Ideal.cpp:1329-1357).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -175,6 +176,104 @@ __global__ __launch_bounds__(256) void k_gemm(const float *__restrict__ A, int64
                 *dst = v;
             }
         }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Inference in one pass: normalisation and the three layers of 64 pairs at a
+// time, the weights staged in LDS once per workgroup, the activations never
+// leaving it.  The operands are rounded to bf16 exactly where k_gemm rounds
+// them and every output takes the same MFMA sequence over K (16 at a time,
+// ascending), so the logits equal the three-launch path's bit for bit.
+// Wave w owns rows 16w .. 16w + 15 of the tile.
+// ------------------------------------------------------------------------
+constexpr int kInferRows = 64;
+
+template <int K, bool RELU>
+__device__ __forceinline__ void infer_layer(const short (*A)[K], const short (*B)[kHidden], const float *bias,
+                                            short (*H)[kHidden], int w, int l) {
+    f32x4 acc[kHidden / 16];
+#pragma unroll
+    for (int tt = 0; tt < kHidden / 16; tt++) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int ar = 16 * w + (l & 15), ak = 4 * (l >> 4);
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a;
+#pragma unroll
+        for (int j = 0; j < 4; j++) a[j] = A[ar][k0 + ak + j];
+#pragma unroll
+        for (int tt = 0; tt < kHidden / 16; tt++) {
+            bf16x4 b;
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = B[k0 + ak + j][16 * tt + (l & 15)];
+            acc[tt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int tt = 0; tt < kHidden / 16; tt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int col = 16 * tt + (l & 15);
+            float v = acc[tt][j] + bias[col];
+            if (RELU) v = fmaxf(v, 0.f);
+            H[16 * w + 4 * (l >> 4) + j][col] = f2bf(v);
+        }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ feat, const float *__restrict__ norm,
+                                                    const float *__restrict__ W, uint64_t n,
+                                                    float *__restrict__ pred) {
+    __shared__ short sW1[kNFeat][kHidden];
+    __shared__ short sW2[kHidden][kHidden];
+    __shared__ short sW3[kHidden][16];  // columns >= kOut are zero
+    __shared__ float sb[2 * kHidden + kOut];
+    __shared__ short sX[kInferRows][kNFeat];
+    __shared__ short sH1[kInferRows][kHidden];
+    __shared__ short sH2[kInferRows][kHidden];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    for (int i = t; i < kNFeat * kHidden; i += 256) sW1[i / kHidden][i % kHidden] = f2bf(W[kOffW1 + i]);
+    for (int i = t; i < kHidden * kHidden; i += 256) sW2[i / kHidden][i % kHidden] = f2bf(W[kOffW2 + i]);
+    for (int i = t; i < kHidden * 16; i += 256) {
+        const int k = i / 16, c = i % 16;
+        sW3[k][c] = c < kOut ? f2bf(W[kOffW3 + k * kOut + c]) : (short)0;
+    }
+    for (int i = t; i < kHidden; i += 256) {
+        sb[i] = W[kOffB1 + i];
+        sb[kHidden + i] = W[kOffB2 + i];
+    }
+    if (t < kOut) sb[2 * kHidden + t] = W[kOffB3 + t];
+    __syncthreads();
+    for (uint64_t r0 = (uint64_t)blockIdx.x * kInferRows; r0 < n; r0 += (uint64_t)gridDim.x * kInferRows) {
+        for (int i = t; i < kInferRows * kNFeat; i += 256) {
+            const int r = i / kNFeat, c = i % kNFeat;
+            float v = 0.f;
+            if (r0 + r < n) v = (feat[(r0 + r) * kNFeat + c] - norm[c]) * norm[kNFeat + c];
+            sX[r][c] = f2bf(v);
+        }
+        __syncthreads();
+        infer_layer<kNFeat, true>(sX, sW1, sb, sH1, w, l);
+        __syncthreads();
+        infer_layer<kHidden, true>(sH1, sW2, sb + kHidden, sH2, w, l);
+        __syncthreads();
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int ar = 16 * w + (l & 15), ak = 4 * (l >> 4);
+#pragma unroll
+        for (int k0 = 0; k0 < kHidden; k0 += 16) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                a[j] = sH2[ar][k0 + ak + j];
+                b[j] = sW3[k0 + ak + j][l & 15];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        const int col = l & 15;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t row = r0 + 16 * w + 4 * (l >> 4) + j;
+            if (col < kOut && row < n) pred[2 * row + col] = acc[j] + sb[2 * kHidden + col];
+        }
+        __syncthreads();
     }
 }
 
@@ -510,17 +609,12 @@ int farm_mlp_predict_pairs(farm_handle h, uint64_t begin, uint64_t end, float *o
     const uint64_t n = end - begin;
     if ((rc = ensure(h, h->pred, 8ull * (n ? n : 1)))) return rc;
     if (!n) return FARM_OK;
-    hipLaunchKernelGGL(k_normalize, dim3(blocks_for(n * kNFeat)), dim3(256), 0, h->stream, P<float>(h->feat), n,
-                       (const float *)P<float>(h->mlp_norm));
+    // normalisation and the three layers in one pass (the logits of the
+    // three-launch path, bit for bit)
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + kInferRows - 1) / kInferRows, 2ull * h->num_cus);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(256), 0, h->stream, (const float *)P<float>(h->feat),
+                       (const float *)P<float>(h->mlp_norm), (const float *)P<float>(h->mlp_w), n, P<float>(h->pred));
     HIPCHK(h, hipGetLastError());
-    const int chunk = 1 << 18;
-    for (uint64_t s = 0; s < n; s += chunk) {
-        const int M = (int)std::min<uint64_t>(chunk, n - s);
-        if ((rc = mlp_buffers(h, M))) return rc;
-        if ((rc = mlp_forward_dev(h, P<float>(h->feat) + s * kNFeat, M))) return rc;
-        HIPCHK(h, hipMemcpyAsync(P<float>(h->pred) + 2 * s, h->mlp_z3.p, 8ull * M, hipMemcpyDeviceToDevice,
-                                 h->stream));
-    }
     if (out) HIPCHK(h, hipMemcpyAsync(out, h->pred.p, 8ull * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return FARM_OK;

@@ -35,9 +35,12 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <chrono>
 #include <hipcub/hipcub.hpp>
+#include <thread>
 #include <vector>
 
+#include "../../../include/farm_host.h"
 #include "../host/raster.h"
 #include "handle.h"
 #include "wave.h"
@@ -1148,28 +1151,16 @@ unsigned grid_for(uint64_t n) { return (unsigned)std::min<uint64_t>(std::max<uin
 
 using namespace farm;
 
-extern "C" int farm_rasterize(farm_handle h, const farm_ringset *rs, int vpr, int nlow, farm_raster_info *info) {
-    if (!h || !rs || rs->npolys < 0 || vpr <= 0 || nlow < 0 || (rs->npolys && (!rs->vert_off || !rs->x || !rs->y)))
-        return fail(h, FARM_EINVAL, "farm_rasterize: bad arguments");
-    const int64_t np = rs->npolys;
+namespace farm {
+namespace {
+
+// Everything after the vertex upload: x / y of the handle hold the closed
+// rings, ring i at [voff[i], voff[i + 1]).
+int rasterize_uploaded(farm_handle h, int64_t np, const std::vector<uint64_t> &voff, uint64_t nvt, int vpr,
+                       int nlow, farm_raster_info *info) {
     const bool hier = nlow > 0;
-    const uint64_t vbase = np ? rs->vert_off[0] : 0;
-    for (int64_t i = 0; i < np; i++) {
-        const uint64_t a = rs->vert_off[i], b = rs->vert_off[i + 1];
-        if (b < a || b - a < 2 || b - a >= (1ull << 31))
-            return fail(h, FARM_EINVAL, "polygon %lld: ring of %lld vertices", (long long)i, (long long)(b - a));
-    }
-    const uint64_t nvt = np ? rs->vert_off[np] - vbase : 0;
-    HIPCHK(h, hipSetDevice(h->device));
     int rc;
     Tmp T;
-    // vertices, then every ring's MBB on the device
-    for (DevBuf *b : {&h->x, &h->y})
-        if ((rc = ensure(h, *b, 4 * (nvt + kVertexPad)))) return rc;
-    if ((rc = upload(h, h->x, rs->x + vbase, 4 * nvt))) return rc;
-    if ((rc = upload(h, h->y, rs->y + vbase, 4 * nvt))) return rc;
-    std::vector<uint64_t> voff(np + 1);
-    for (int64_t i = 0; i <= np; i++) voff[i] = np ? rs->vert_off[i] - vbase : 0;
     uint64_t *d_voff;
     double *d_mbb;
     float *d_absx;
@@ -1527,6 +1518,129 @@ extern "C" int farm_rasterize(farm_handle h, const farm_ringset *rs, int vpr, in
         info->nlayers = h->nlayers;
         info->nlstatus = h->nlstatus;
     }
+    return FARM_OK;
+}
+
+}  // namespace
+}  // namespace farm
+
+extern "C" int farm_rasterize(farm_handle h, const farm_ringset *rs, int vpr, int nlow, farm_raster_info *info) {
+    if (!h || !rs || rs->npolys < 0 || vpr <= 0 || nlow < 0 || (rs->npolys && (!rs->vert_off || !rs->x || !rs->y)))
+        return fail(h, FARM_EINVAL, "farm_rasterize: bad arguments");
+    const int64_t np = rs->npolys;
+    const uint64_t vbase = np ? rs->vert_off[0] : 0;
+    for (int64_t i = 0; i < np; i++) {
+        const uint64_t a = rs->vert_off[i], b = rs->vert_off[i + 1];
+        if (b < a || b - a < 2 || b - a >= (1ull << 31))
+            return fail(h, FARM_EINVAL, "polygon %lld: ring of %lld vertices", (long long)i, (long long)(b - a));
+    }
+    const uint64_t nvt = np ? rs->vert_off[np] - vbase : 0;
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (DevBuf *b : {&h->x, &h->y})
+        if ((rc = ensure(h, *b, 4 * (nvt + kVertexPad)))) return rc;
+    if ((rc = upload(h, h->x, rs->x + vbase, 4 * nvt))) return rc;
+    if ((rc = upload(h, h->y, rs->y + vbase, 4 * nvt))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double up_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<uint64_t> voff(np + 1);
+    for (int64_t i = 0; i <= np; i++) voff[i] = np ? rs->vert_off[i] - vbase : 0;
+    rc = rasterize_uploaded(h, np, voff, nvt, vpr, nlow, info);
+    if (!rc && info) info->upload_ms = up_ms;
+    return rc;
+}
+
+// The host set's rings go to the device through two pinned staging buffers:
+// host threads write chunk k + 1 as closed SoA rings while chunk k's DMA
+// runs, then the set is rasterized there.
+extern "C" int farm_hostset_rasterize_device(farm_handle h, const farm_hostset *s, int vpr, int nlow, int nthreads,
+                                             farm_raster_info *info) {
+    if (!h || !s || vpr <= 0 || nlow < 0) return fail(h, FARM_EINVAL, "farm_hostset_rasterize_device: bad arguments");
+    const int64_t np = farm_hostset_size(s);
+    std::vector<uint64_t> voff(np + 1, 0);
+    std::vector<const float *> rings(np);
+    for (int64_t i = 0; i < np; i++) {
+        int64_t nv = 0;
+        if (farm_hostset_ring(s, i, &rings[i], &nv) || nv + 1 >= (1ll << 31))
+            return fail(h, FARM_EINVAL, "polygon %lld: bad ring", (long long)i);
+        voff[i + 1] = voff[i] + (uint64_t)nv + 1;
+    }
+    const uint64_t nvt = voff[np];
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (DevBuf *b : {&h->x, &h->y})
+        if ((rc = ensure(h, *b, 4 * (nvt + kVertexPad)))) return rc;
+    const uint64_t kStage = h->stage_verts;  // vertices per staging buffer (x then y)
+    for (int k = 0; k < 2; k++) {
+        if (!h->stage[k]) HIPCHK(h, hipHostMalloc(&h->stage[k], 8 * kStage, hipHostMallocDefault));
+        if (!h->stage_ev[k]) HIPCHK(h, hipEventCreateWithFlags(&h->stage_ev[k], hipEventDisableTiming));
+    }
+    const int nth = std::max(1, nthreads);
+    int64_t i0 = 0;
+    int k = 0;
+    bool pending[2] = {false, false};
+    while (i0 < np) {
+        // the next chunk: whole rings up to kStage vertices (a longer ring alone)
+        int64_t i1 = i0 + 1;
+        while (i1 < np && voff[i1 + 1] - voff[i0] <= kStage) i1++;
+        const uint64_t v0 = voff[i0], nvc = voff[i1] - v0;
+        float *sx = (float *)h->stage[k], *sy = sx + kStage;
+        DevBuf big;  // a ring longer than a staging buffer goes through a pinned buffer of its own
+        if (nvc > kStage) {
+            HIPCHK(h, hipHostMalloc(&big.p, 8 * nvc, hipHostMallocDefault));
+            sx = (float *)big.p;
+            sy = sx + nvc;
+        } else if (pending[k]) {
+            HIPCHK(h, hipEventSynchronize(h->stage_ev[k]));  // buffer k's last DMA is done
+        }
+        auto work = [&](int t) {
+            for (int64_t i = i0 + t; i < i1; i += nth) {
+                const uint64_t o = voff[i] - v0, n = voff[i + 1] - voff[i] - 1;
+                const float *r = rings[i];
+                for (uint64_t v = 0; v < n; v++) {
+                    sx[o + v] = r[2 * v];
+                    sy[o + v] = r[2 * v + 1];
+                }
+                sx[o + n] = r[0];  // the decoder's repeated first vertex
+                sy[o + n] = r[1];
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth && t < i1 - i0; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto &q : th) q.join();
+        HIPCHK(h, hipMemcpyAsync(P<float>(h->x) + v0, sx, 4 * nvc, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(P<float>(h->y) + v0, sy, 4 * nvc, hipMemcpyHostToDevice, h->stream));
+        if (big.p) {
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            HIPCHK(h, hipHostFree(big.p));
+        } else {
+            HIPCHK(h, hipEventRecord(h->stage_ev[k], h->stream));
+            pending[k] = true;
+            k ^= 1;
+        }
+        i0 = i1;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double up_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    rc = rasterize_uploaded(h, np, voff, nvt, vpr, nlow, info);
+    if (!rc && info) info->upload_ms = up_ms;
+    return rc;
+}
+
+extern "C" int farm_set_staging(farm_handle h, uint64_t vertices) {
+    if (!h || vertices < 1024 || vertices > (1ull << 28)) return fail(h, FARM_EINVAL, "staging of %llu vertices",
+                                                                      (unsigned long long)vertices);
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 2; k++)
+        if (h->stage[k]) {
+            HIPCHK(h, hipHostFree(h->stage[k]));
+            h->stage[k] = nullptr;
+        }
+    h->stage_verts = vertices;
     return FARM_OK;
 }
 

@@ -235,7 +235,7 @@ def star_domain(seed: int, instances: int, threads: int | None = None, vpr: int 
     if dev is None:
         both.rasterize(vpr, threads)
     else:
-        dev.rasterize(*both.closed_rings(threads), vpr)
+        dev.rasterize_hostset(both, vpr, 0, threads)
     return PairSet(both, pa, (pb.astype(np.int64) + len(A)).astype(np.uint32), instances, dev is not None)
 
 

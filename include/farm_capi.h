@@ -192,6 +192,7 @@ typedef struct farm_raster_info {
     int64_t first_rejected; /* -1 if none */
     uint64_t nverts, nstatus, npixoff, nseqs, nvoff, nvnodes, nlayers, nlstatus;
     double build_ms;        /* device time from the uploaded vertices to the finished set */
+    double upload_ms;       /* host wall time of the vertex upload */
 } farm_raster_info;
 
 /* Rasterize every ring on the device, vpr vertices per pixel (-v), and make

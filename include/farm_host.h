@@ -44,6 +44,19 @@ int farm_hostset_ring(const farm_hostset *s, int64_t i, const float **xy, int64_
 /* MBB (low x, low y, high x, high y) with the float 1e-9 bias of
  * VertexSequence::getMBR (vertex_sequence.cpp:152-167). */
 int farm_hostset_mbbs(const farm_hostset *s, double *out4n);
+/* farm_rasterize for a host set, with the rings streamed to the device
+ * through two pinned staging buffers (host threads convert chunk k + 1 into
+ * closed SoA rings while chunk k's DMA runs): the same set, without the
+ * full-size host copy of farm_hostset_closed_rings or a pageable upload.
+ * Replaces the upload half of preprocess_for_gpu (src/cuda/preprocessing.cu:
+ * 128-343) together with the rasterization (see farm_rasterize). */
+int farm_hostset_rasterize_device(farm_handle h, const farm_hostset *s, int vpr, int nlow, int nthreads,
+                                  farm_raster_info *info);
+/* Vertices per pinned staging buffer of farm_hostset_rasterize_device
+ * (default 2^22, 1,024 .. 2^28; two buffers of 8 B per vertex); a ring
+ * longer than that goes through a pinned buffer of its own.  No counterpart
+ * in the reference (its upload is one pageable copy). */
+int farm_set_staging(farm_handle h, uint64_t vertices);
 /* The rings as the raster builder reads them (each stored ring plus its
  * first vertex once more), for farm_rasterize: vert_off gets npolys + 1
  * offsets; x / y (may be NULL: offsets only) vert_off[npolys] floats each. */

@@ -130,3 +130,18 @@ def test_learned_queue_order_keeps_exact_results(device, star_model):
     a = device.intersect(order=0).bits
     b = device.intersect(order=1).bits
     assert np.array_equal(a, b) and np.array_equal(a, M["test"].bits)
+
+
+def test_fused_inference_equals_the_three_launch_forward(device, star_model):
+    # farm_mlp_predict_pairs runs normalisation and the three layers in one
+    # kernel; its logits and cost predictions equal the generic MFMA GEMM
+    # path's (farm_mlp_forward on the same normalised features) bit for bit
+    M = star_model
+    device.mlp_set_params(M["params"], M["norm"])
+    device.upload_polygons(M["test_ps"].both)
+    device.upload_pairs(M["test_ps"].pa, M["test_ps"].pb)
+    fused = device.mlp_predict_pairs()
+    feat, _ = device.pair_features()
+    three = device.mlp_forward(learned.apply_norm(feat, M["norm"]))
+    assert fused.shape == three.shape and len(fused) > 1000
+    assert np.array_equal(fused.view(np.uint32), three.view(np.uint32))

@@ -196,3 +196,22 @@ def test_fill_categories_on_the_device(device, bits):
     device.upload_pairs(pa, pb + len(A))
     assert np.array_equal(device.intersect(mode=capi.MODE_APPROX_HEURISTIC).bits, want)
     device.set_categories(4)
+
+
+def test_streamed_upload_through_pinned_staging(device):
+    # farm_hostset_rasterize_device: the same set as farm_rasterize, with the
+    # rings streamed through two pinned buffers -- many chunks, and one ring
+    # longer than a buffer (its own pinned buffer)
+    c = datagen.complex_set(32, 3, 0.0, 3.0, nv_cap=30000)
+    lk = datagen.lake_set(33, 3000, 0.0, 3.0)
+    H = capi.HostSet.concat(capi.HostSet.from_rings(*c), capi.HostSet.from_rings(*lk))
+    _, want = _build(device, H, 16)
+    try:
+        device.set_staging(8192)
+        info = device.rasterize_hostset(H, 16, nthreads=4)
+        got = device.download_polyset()
+        assert info["upload_ms"] > 0
+        for k in want:
+            assert np.asarray(got[k]).tobytes() == np.asarray(want[k]).tobytes(), k
+    finally:
+        device.set_staging(1 << 22)
