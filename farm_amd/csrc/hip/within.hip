@@ -20,10 +20,13 @@
 //     the largest |dx| and |dy| (the corner sets are a full product and every
 //     step after the f32 differences is monotone), so one division per axis
 //     and one square root give the identical double.
-//  3. candidates (apx, low, high, id1, id2) live in a per-wave global slab
-//     and are taken in tuple order by argmin selection (the order of
-//     std::sort over the tuples); the refinement stops at the reference's
-//     first return, so usually after one or two candidates.
+//  3. candidates (apx, low, high, id1, id2) of the last level are handed to
+//     the lanes by lane permutes as they are found, candidate k in register
+//     slot k / 64 of lane k % 64, up to 128 of them; a pair with more keeps
+//     them in a per-wave global slab.  They are taken in tuple order by
+//     argmin selection (the order of std::sort over the tuples); the
+//     refinement stops at the reference's first return, so usually after a
+//     few candidates.
 //  4. the truncated point-segment distances of one edge-sequence pair are
 //     spread over the lanes; the minimum is taken on the squared distances
 //     and one square root follows (sqrt is monotone and correctly rounded).
@@ -50,9 +53,10 @@ namespace {
 constexpr double kKmLat = 360.0 / 40076.0;  // degree_per_kilometer_latitude (util.h:72)
 constexpr double kFar = 100000.0;           // the reference's "no distance yet"
 constexpr int kWaves = 4;                   // wavefronts per workgroup
-constexpr int kPixMax = 1024;               // pixels of a layer handled in LDS
+constexpr int kPixMax = 512;                // pixels of a layer handled in LDS
 constexpr int kBitWords = kPixMax / 32;
 constexpr uint32_t kCandLds = 2048;         // candidate slab of a wave in the LDS path
+constexpr int kCandRegs = 1;                // last-level candidates per lane held in registers
 constexpr uint32_t kChunk = 8;              // pairs per queue grab
 
 struct Cand {
@@ -214,47 +218,28 @@ __device__ __forceinline__ int unpack_y(T v) {
     return (int)((uint32_t)v >> SH);
 }
 
-// Pixel set -> compact list (ascending ids).  bits has ceil(npix / 32)
-// words; lanes take one word each.  The exclusive prefix sum of the per-lane
-// counts (0..32) is assembled from six bit-plane ballots, with no
-// cross-lane data movement.
+// Pixel set -> compact list (ascending ids): rounds of 64 ids, one per lane,
+// each member placed by the ballot count of the members below it.  (x, y)
+// of a lane's id is advanced by 64 ids per round instead of divided out.
 template <int SH, class T>
 __device__ __forceinline__ uint32_t compact(const uint32_t *bits, int npix, int dimx, T *list, int lane) {
-    const int nw = (npix + 31) >> 5;
     const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t ux = (uint32_t)dimx;
+    uint32_t y = (uint32_t)lane / ux, x = (uint32_t)lane - y * ux;
+    const uint32_t sy = 64u / ux, sx = 64u - sy * ux;  // sx < dimx
     uint32_t total = 0;
-    for (int w0 = 0; w0 < nw; w0 += 64) {
-        const int w = w0 + lane;
-        uint32_t word = w < nw ? bits[w] : 0u;
-        const uint32_t c = (uint32_t)__popc(word);
-        uint32_t pos = 0, sum = 0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const uint64_t m = __ballot((c >> k) & 1u);
-            pos += (uint32_t)__popcll(m & below) << k;
-            sum += (uint32_t)__popcll(m) << k;
+    for (int id0 = 0; id0 < npix; id0 += 64) {
+        const int id = id0 + lane;
+        const bool in = id < npix && ((bits[id >> 5] >> (id & 31)) & 1u);
+        const uint64_t m = __ballot(in);
+        if (in) list[total + (uint32_t)__popcll(m & below)] = pack_xy<SH, T>(x, y);
+        total += (uint32_t)__popcll(m);
+        x += sx;
+        y += sy;
+        if (x >= ux) {
+            x -= ux;
+            y++;
         }
-        pos += total;
-        if (word) {
-            const uint32_t id0 = (uint32_t)w * 32u;
-            const uint32_t y0 = id0 / (uint32_t)dimx, x0 = id0 - y0 * (uint32_t)dimx;
-            while (word) {
-                const uint32_t b = (uint32_t)(__ffs(word) - 1);
-                word &= word - 1;
-                uint32_t x = x0 + b, y = y0;
-                if (dimx >= 32) {  // b < 32 <= dimx: at most one row further
-                    if (x >= (uint32_t)dimx) {
-                        x -= (uint32_t)dimx;
-                        y++;
-                    }
-                } else {
-                    y = (id0 + b) / (uint32_t)dimx;
-                    x = id0 + b - y * (uint32_t)dimx;
-                }
-                list[pos++] = pack_xy<SH, T>(x, y);
-            }
-        }
-        total += sum;
     }
     wave_sync();
     return uni(total);
@@ -436,60 +421,131 @@ __device__ __forceinline__ int refine_batch(const DevSet &S, const farm_raster_h
 
 // Argmin (tuple order) over the live candidates, and reference_dist: the
 // smallest low bound among the other live candidates (each lane keeps its
-// two smallest low bounds).  Returns the slot, or -1 when none is live.
-__device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lane, Cand &out, double &ref) {
+// two smallest low bounds).  Lanes feed their candidates (global index k) to
+// add(); finish() reduces over the wave and returns the index of the
+// smallest, or -1 when none is live.
+struct SelAcc {
     Cand best;
-    best.apx = INFINITY;
-    best.lo = INFINITY;
-    best.hi = INFINITY;
-    best.id1 = best.id2 = 0xffffffffu;
     int bi = -1;
     double lo1 = INFINITY, lo2 = INFINITY;
     int i1 = -1;
-    for (uint32_t k = lane; k < n; k += 64) {
-        const Cand c = cand[k];
-        if (c.apx == INFINITY) continue;  // taken
+    __device__ __forceinline__ void add(const Cand &c, int k) {
         if (bi < 0 || tuple_less(c, best)) {
             best = c;
-            bi = (int)k;
+            bi = k;
         }
         if (c.lo < lo1) {
             lo2 = lo1;
             lo1 = c.lo;
-            i1 = (int)k;
+            i1 = k;
         } else if (c.lo < lo2) {
             lo2 = c.lo;
         }
     }
+    __device__ __forceinline__ int finish(Cand &out, double &ref) {
+        if (bi < 0) {
+            best.apx = INFINITY;
+            best.lo = INFINITY;
+            best.hi = INFINITY;
+            best.id1 = best.id2 = 0xffffffffu;
+        }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        Cand c;
-        c.apx = __shfl_xor(best.apx, o, 64);
-        c.lo = __shfl_xor(best.lo, o, 64);
-        c.hi = __shfl_xor(best.hi, o, 64);
-        c.id1 = __shfl_xor(best.id1, o, 64);
-        c.id2 = __shfl_xor(best.id2, o, 64);
-        const int ci = __shfl_xor(bi, o, 64);
-        if (ci >= 0 && (bi < 0 || tuple_less(c, best))) {
-            best = c;
-            bi = ci;
+        for (int o = 32; o > 0; o >>= 1) {
+            Cand c;
+            c.apx = __shfl_xor(best.apx, o, 64);
+            c.lo = __shfl_xor(best.lo, o, 64);
+            c.hi = __shfl_xor(best.hi, o, 64);
+            c.id1 = __shfl_xor(best.id1, o, 64);
+            c.id2 = __shfl_xor(best.id2, o, 64);
+            const int ci = __shfl_xor(bi, o, 64);
+            if (ci >= 0 && (bi < 0 || tuple_less(c, best))) {
+                best = c;
+                bi = ci;
+            }
+        }
+        out.apx = uni(best.apx);
+        out.lo = uni(best.lo);
+        out.hi = uni(best.hi);
+        out.id1 = uni(best.id1);
+        out.id2 = uni(best.id2);
+        const int b = uni(bi);
+        ref = fmin(kFar, wave_min(i1 == b ? lo2 : lo1));
+        return b;
+    }
+};
+
+// Selection over a candidate slab; a taken candidate has apx = +inf.
+__device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lane, Cand &out, double &ref) {
+    SelAcc acc;
+    for (uint32_t k = lane; k < n; k += 64) {
+        const Cand c = cand[k];
+        if (c.apx == INFINITY) continue;  // taken
+        acc.add(c, (int)k);
+    }
+    return acc.finish(out, ref);
+}
+
+// Last-level candidates held in registers: slot q of lane l is candidate
+// q * 64 + l, live while bit q of `live` is set.
+struct CandRegs {
+    double apx[kCandRegs], lo[kCandRegs], hi[kCandRegs];
+    uint32_t id1[kCandRegs], id2[kCandRegs];
+    uint32_t live = 0;
+};
+
+__device__ __forceinline__ int select_regs(const CandRegs &cr, int lane, Cand &out, double &ref) {
+    SelAcc acc;
+#pragma unroll
+    for (int q = 0; q < kCandRegs; q++) {
+        if ((cr.live >> q) & 1u) {
+            Cand c;
+            c.apx = cr.apx[q];
+            c.lo = cr.lo[q];
+            c.hi = cr.hi[q];
+            c.id1 = cr.id1[q];
+            c.id2 = cr.id2[q];
+            acc.add(c, q * 64 + lane);
         }
     }
-    out.apx = uni(best.apx);
-    out.lo = uni(best.lo);
-    out.hi = uni(best.hi);
-    out.id1 = uni(best.id1);
-    out.id2 = uni(best.id2);
-    bi = uni(bi);
-    ref = fmin(kFar, wave_min(i1 == bi ? lo2 : lo1));
-    return bi;
+    return acc.finish(out, ref);
+}
+
+// Lane index of the r-th (from 0) set bit of m, for r < popcount(m).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, uint32_t r) {
+    uint32_t w = (uint32_t)m;
+    int base = 0;
+    const uint32_t c32 = (uint32_t)__popc(w);
+    if (r >= c32) {
+        r -= c32;
+        w = (uint32_t)(m >> 32);
+        base = 32;
+    }
+#pragma unroll
+    for (int sh = 16; sh > 0; sh >>= 1) {
+        const uint32_t c = (uint32_t)__popc(w & ((1u << sh) - 1u));
+        if (r >= c) {
+            r -= c;
+            w >>= sh;
+            base += sh;
+        }
+    }
+    return base;
+}
+
+__device__ __forceinline__ uint32_t bperm_u(int addr, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
+}
+__device__ __forceinline__ double bperm_d(int addr, double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = bperm_u(addr, (uint32_t)b), hi = bperm_u(addr, (uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Ideal::within for one pair; the wave's buffers: lists / bits (LDS or a
 // global slab) and the candidate slab.
 template <int SH, class T>
 __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uint32_t ib, double d, T *sl,
-                                               T *tl, uint32_t *sb, uint32_t *tb, uint8_t *stage, Cand *cand,
+                                               T *tl, uint32_t *sb, uint32_t *tb, Cand *cand,
                                                uint32_t pixcap, uint32_t candcap, SeqTab &tab, int lane) {
     PairOut r;
     r.bit = 0;
@@ -541,6 +597,10 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
     }
     int i = 0, j = 0;
     bool final_level = false;
+    // last-level candidates: in registers while they fit (not on the path
+    // with global buffers), else in the slab (`spilled`)
+    CandRegs cr;
+    bool spilled = SH != 8;
     for (;;) {
         const double s_step = lgrid(S, hs, i).step, t_step = lgrid(S, ht, j).step;
         bool s_next = false, t_next = false;
@@ -559,13 +619,6 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         clear_bits(sb, npix_s, lane);
         clear_bits(tb, npix_t, lane);
         const uint8_t *sts = gs.st, *stt = gt.st;
-        if (stage) {  // the two layers' fill classes, staged in LDS
-            for (int k = lane; k < npix_s; k += 64) stage[k] = gs.st[k];
-            for (int k = lane; k < npix_t; k += 64) stage[pixcap + k] = gt.st[k];
-            wave_sync();
-            sts = stage;
-            stt = stage + pixcap;
-        }
         expand_side<SH>(sl, ns, s_next, s_next ? lgrid(S, hs, i - 1) : gs, gs, sts, sb, lane);
         expand_side<SH>(tl, nt, t_next, t_next ? lgrid(S, ht, j - 1) : gt, gt, stt, tb, lane);
         ns = compact<SH>(sb, npix_s, gs.dimx, sl, lane);
@@ -617,6 +670,8 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
             for (uint64_t it = 0; it < rounds; it++) {
                 bool keep = false;
                 Cand c;
+                c.apx = c.lo = c.hi = 0.0;
+                c.id1 = c.id2 = 0;
                 if (a < ns) {
                     const T va = sl[a], vb = tl[b];
                     const int xa = unpack_x<SH>(va), ya = unpack_y<SH>(va);
@@ -628,9 +683,10 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                     if (mn <= d && mn < maxd) {
                         keep = true;
                         const uint32_t id1 = (uint32_t)(ya * gs.dimx + xa), id2 = (uint32_t)(yb * gt.dimx + xb);
-                        set_bit(sb, id1);
-                        set_bit(tb, id2);
-                        if (final_level) {
+                        if (!final_level) {
+                            set_bit(sb, id1);
+                            set_bit(tb, id2);
+                        } else {
                             const double hi = box_max_dist(b1, b2, tab.km);
                             const int pf = classify(S.areas[hs.pix_start + id1] + S.areas[ht.pix_start + id2],
                                                     sa + ta, 20);
@@ -644,12 +700,52 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                 }
                 if (final_level) {
                     const uint64_t m = __ballot(keep);
-                    const uint32_t pos = nc + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    if (keep) {
-                        if (pos < candcap) cand[pos] = c;
-                        else over = true;
+                    const uint32_t cnt = (uint32_t)__popcll(m);
+                    if (cnt && !spilled && nc + cnt <= 64u * kCandRegs) {
+                        // candidate nc + r (r-th kept lane of this round) goes
+                        // to lane (nc + r) % 64, slot (nc + r) / 64
+                        const uint32_t rk = (uint32_t)(lane - (int)nc) & 63u;
+                        const bool recv = rk < cnt;
+                        const int addr = (recv ? nth_set_bit(m, rk) : lane) << 2;
+                        const double apx = bperm_d(addr, c.apx), lo = bperm_d(addr, c.lo), hi = bperm_d(addr, c.hi);
+                        const uint32_t i1 = bperm_u(addr, c.id1), i2 = bperm_u(addr, c.id2);
+                        const uint32_t qs = (nc + rk) >> 6;
+#pragma unroll
+                        for (int q = 0; q < kCandRegs; q++) {
+                            if (recv && qs == (uint32_t)q) {
+                                cr.apx[q] = apx;
+                                cr.lo[q] = lo;
+                                cr.hi[q] = hi;
+                                cr.id1[q] = i1;
+                                cr.id2[q] = i2;
+                                cr.live |= 1u << q;
+                            }
+                        }
+                    } else if (cnt) {
+                        if (!spilled) {
+                            // the registers overflow: their candidates go to
+                            // the slab, in order, and the rest follow them
+#pragma unroll
+                            for (int q = 0; q < kCandRegs; q++) {
+                                if ((cr.live >> q) & 1u) {
+                                    Cand t;
+                                    t.apx = cr.apx[q];
+                                    t.lo = cr.lo[q];
+                                    t.hi = cr.hi[q];
+                                    t.id1 = cr.id1[q];
+                                    t.id2 = cr.id2[q];
+                                    cand[q * 64 + lane] = t;  // < nc <= 64 kCandRegs < candcap
+                                }
+                            }
+                            spilled = true;
+                        }
+                        const uint32_t pos = nc + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                        if (keep) {
+                            if (pos < candcap) cand[pos] = c;
+                            else over = true;
+                        }
                     }
-                    nc += (uint32_t)__popcll(m);
+                    nc += cnt;
                 }
                 a += q;
                 b += rr;
@@ -681,7 +777,8 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
     double min_dist = kFar;
     Cand c;
     double ref;
-    int slot = select_next(cand, nc, lane, c, ref);
+    auto next = [&]() { return spilled ? select_next(cand, nc, lane, c, ref) : select_regs(cr, lane, c, ref); };
+    int slot = next();
     while (slot >= 0) {
         CandBatch cb;
         bool open = true;
@@ -701,9 +798,13 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                     cb.row0[q] = cb.nrows;
                     cb.nrows += rows;
                     cb.n = q + 1;
-                    if (lane == 0) cand[slot].apx = INFINITY;
-                    wave_sync();
-                    slot = select_next(cand, nc, lane, c, ref);
+                    if (spilled) {
+                        if (lane == 0) cand[slot].apx = INFINITY;
+                        wave_sync();
+                    } else if ((slot & 63) == lane) {
+                        cr.live &= ~(1u << (slot >> 6));
+                    }
+                    slot = next();
                     if (cb.nrows >= 64) open = false;
                 }
             }
@@ -737,17 +838,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     constexpr int SH = BIG ? 16 : 8;
     LT *sl, *tl;
     uint32_t *sb, *tb;
-    uint8_t *stage = nullptr;
     Cand *cand;
     if constexpr (!BIG) {
         __shared__ LT lists[kWaves][2][kPixMax];
         __shared__ uint32_t bits[kWaves][2][kBitWords];
-        __shared__ uint8_t stages[kWaves][2 * kPixMax];
         sl = lists[wv][0];
         tl = lists[wv][1];
         sb = bits[wv][0];
         tb = bits[wv][1];
-        stage = stages[wv];
         cand = reinterpret_cast<Cand *>(slab + gw * slab_stride);
     } else {
         char *base = slab + gw * slab_stride;
@@ -777,7 +875,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 }
                 continue;
             }
-            const PairOut r = within_pair<SH>(S, ia, ib, d, sl, tl, sb, tb, stage, cand, pixcap, candcap, tabs[wv], lane);
+            const PairOut r = within_pair<SH>(S, ia, ib, d, sl, tl, sb, tb, cand, pixcap, candcap, tabs[wv], lane);
             if (r.bit == 3) {
                 if (lane == 0) {
                     if (BIG) {
@@ -856,7 +954,13 @@ int run_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t
     uint32_t *ctr = P<uint32_t>(h->wctr);
     unsigned long long *wst = reinterpret_cast<unsigned long long *>(ctr + 8);
     if ((rc = fill_u32(h, ctr, 16, 0))) return rc;
-    const unsigned nblocks = (unsigned)h->num_cus * 4;
+    // as many resident workgroups as the registers and LDS allow (the waves
+    // are persistent)
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_within<false>, 64 * kWaves, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 4;
+    const unsigned nblocks = (unsigned)h->num_cus * (unsigned)std::min(per_cu, 8);
     const uint64_t nwaves = (uint64_t)nblocks * kWaves;
     const uint64_t stride = (uint64_t)kCandLds * sizeof(Cand);
     if ((rc = ensure(h, h->wslab, nwaves * stride))) return rc;
