@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 OPS_PER_TEST = 42
 
 
-def measured_bytes_per_test(tile: int) -> float | None:
+def measured_bytes_per_test(tile: int, chunk: int) -> float | None:
     """HBM bytes per executed segment test of k_refine<tile> on this workload,
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already
     scaled by the copy-kernel calibration); None for a width it has no
@@ -66,8 +66,8 @@ def measured_bytes_per_test(tile: int) -> float | None:
         text = open(path).read()
     except OSError:
         return None
-    f = re.search(rf"fetch_bytes_per_segment_test W{tile}: ([0-9.eE+-]+)", text)
-    w = re.search(rf"write_bytes_per_segment_test W{tile}: ([0-9.eE+-]+)", text)
+    f = re.search(rf"fetch_bytes_per_segment_test W{tile} C{chunk}: ([0-9.eE+-]+)", text)
+    w = re.search(rf"write_bytes_per_segment_test W{tile} C{chunk}: ([0-9.eE+-]+)", text)
     return float(f.group(1)) + float(w.group(1)) if f and w else None
 
 
@@ -76,7 +76,7 @@ def measured_bytes_per_test(tile: int) -> float | None:
 VALU_ISSUE_PEAK_TOPS = 256 * 4 * 32 * 2.4e9 / 1e12
 
 
-def measured_valu_ops_per_test(tile: int) -> float | None:
+def measured_valu_ops_per_test(tile: int, chunk: int) -> float | None:
     """VALU lane-ops k_refine<tile> issues per executed segment test on this
     workload, from the committed rocprofv3 SQ_INSTS_VALU summary; None for a
     width it has no figure for."""
@@ -86,7 +86,7 @@ def measured_valu_ops_per_test(tile: int) -> float | None:
         text = open(os.path.join(ROOT, "profiles", "k_refine_valu_complex_x_lakes.md")).read()
     except OSError:
         return None
-    m = re.search(rf"valu_lane_ops_per_segment_test W{tile}: ([0-9.eE+-]+)", text)
+    m = re.search(rf"valu_lane_ops_per_segment_test W{tile} C{chunk}: ([0-9.eE+-]+)", text)
     return float(m.group(1)) if m else None
 
 
@@ -542,7 +542,8 @@ def main() -> int:
     ap.add_argument("--vpr", type=int, default=256)
     ap.add_argument("--category-bits", type=int, default=8,
                     help="-w: fill classes at 2^b categories for the learned features and -a (exact bits do not depend on it)")
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0, help="source edges per lane (0: autotuned with --chunk)")
+    ap.add_argument("--chunk", type=int, default=0, help="target segments per chunk box (with --tile; default 16)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
@@ -597,13 +598,17 @@ def main() -> int:
     # back afterwards for the parity sample.
     nb = args.steps + args.warmup
     cuts = np.linspace(0, len(pa), nb + 1).astype(np.int64)
-    tile = args.tile
-    if not tile:
-        # autotune the refinement tile width on part of the warmup window
+    if not args.tile:
+        # autotune the refinement (tile width, chunk size, workgroups per CU)
+        # on part of the warmup window
         t = time.time()
-        tile = dev.autotune(0, int(min(cuts[1], 3_000_000)))
-        log(f"autotuned tile width {tile} in {time.time() - t:.1f}s")
-    args.tile = tile
+        tun = dev.autotune(0, int(min(cuts[1], 3_000_000)))
+        log(f"autotuned tile {tun['tile_a']}, chunk {tun['chunk']}, {tun['blocks_per_cu']} workgroups per CU "
+            f"in {time.time() - t:.1f}s")
+    else:
+        dev.set_tuning(args.tile, args.chunk or 16, 8)
+    tun = dev.tuning()
+    args.tile, args.chunk, args.blocks_per_cu = tun["tile_a"], tun["chunk"], tun["blocks_per_cu"]
 
     # Warmup doubles as the learned part's training data: the exact path
     # labels the warmup window (bits + executed tests per pair), the MLP is
@@ -665,9 +670,9 @@ def main() -> int:
     achieved = tests * OPS_PER_TEST / (refine_ms / 1000.0) / 1e12 if refine_ms > 0 else 0.0
     found = sum(s["found"] for s in per_step)
     full_size = args.scale == 1.0 and args.vpr == 256
-    bpt = measured_bytes_per_test(args.tile) if full_size else None
+    bpt = measured_bytes_per_test(args.tile, args.chunk) if full_size else None
     traffic = tests / args.steps * bpt if bpt else None
-    opt = measured_valu_ops_per_test(args.tile) if full_size else None
+    opt = measured_valu_ops_per_test(args.tile, args.chunk) if full_size else None
     issued = tests * opt / (refine_ms / 1000.0) / 1e12 if opt and refine_ms > 0 else None
 
     line = {
@@ -692,6 +697,8 @@ def main() -> int:
             "found": int(found),
             "scale": args.scale,
             "tile_a": args.tile,
+            "chunk": args.chunk,
+            "blocks_per_cu": args.blocks_per_cu,
         },
         "segment_tests_per_s": tests / wall,
         "filter_ms": filter_ms,
@@ -811,7 +818,7 @@ def main() -> int:
         }
         line["parity_sample"] = {"pairs": int(timed.sum()), "mismatches": mism,
                                  "bits_from": f"the {args.steps} timed steps ({'learned' if order else 'cost-bound'} "
-                                              f"queue order, tile {args.tile})",
+                                              f"queue order, tile {args.tile}, chunk {args.chunk})",
                                  "timed_vs_cost_bound_order_mismatches": order_mismatch}
         log(f"cpu baseline: {len(samp)} pairs in {res['seconds']:.1f}s; sample mismatches vs GPU: {mism}")
     dev.close()

@@ -161,6 +161,19 @@ class Opts(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("tile_a", C.c_int32),
+        ("chunk", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("shape_bucket", C.c_int32),
+        ("ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Piece(C.Structure):
     _fields_ = [
         ("pair", C.c_uint32),
@@ -274,7 +287,12 @@ EXPORTS = {
     ),
     "farm_mlp_predict_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]),
     "farm_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_int)]),
+    "farm_autotune_config": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(Tuning)]),
     "farm_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "farm_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "farm_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "farm_tuning_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "farm_tuning_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "farm_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "farm_hostset_from_rings": (C.c_void_p, [C.c_int64, C.c_void_p, C.c_void_p]),
     "farm_hostset_load_dt": (C.c_void_p, [C.c_char_p]),
@@ -682,12 +700,12 @@ class Device:
 
     def intersect(self, begin: int = 0, end: int | None = None, mode: int = MODE_EXACT_BATCHED,
                   tile_a: int = 0, want_bits: bool = True, order: int = 0, accept: float = 0.0,
-                  reject: float | None = None, flags: int = 0) -> IntersectResult:
+                  reject: float | None = None, flags: int = 0, chunk: int = 0) -> IntersectResult:
         """Exact or approximate intersect of pairs [begin, end).  order=1 orders
         the refinement queue by the MLP cost head; mode MODE_APPROX_LEARNED
         accepts logits > accept, rejects logits < reject and refines the rest."""
         end = self.npairs if end is None else end
-        o = Opts(mode=mode, tile_a=tile_a, order=order, approx_threshold=accept,
+        o = Opts(mode=mode, tile_a=tile_a, tile_b=chunk, order=order, approx_threshold=accept,
                  approx_reject=accept if reject is None else reject, flags=flags)
         st = Stats()
         out = np.zeros(max(end - begin, 1), dtype=np.uint8) if want_bits else None
@@ -697,11 +715,11 @@ class Device:
 
     def intersect_dev(self, out_dev_ptr: int | None, begin: int = 0, end: int | None = None,
                       mode: int = MODE_EXACT_BATCHED, tile_a: int = 0, order: int = 0, accept: float = 0.0,
-                      reject: float | None = None, flags: int = 0) -> dict:
+                      reject: float | None = None, flags: int = 0, chunk: int = 0) -> dict:
         """intersect() with the 0/1 results left in device memory at
         out_dev_ptr (end - begin bytes; None: counts only)."""
         end = self.npairs if end is None else end
-        o = Opts(mode=mode, tile_a=tile_a, order=order, approx_threshold=accept,
+        o = Opts(mode=mode, tile_a=tile_a, tile_b=chunk, order=order, approx_threshold=accept,
                  approx_reject=accept if reject is None else reject, flags=flags)
         st = Stats()
         rc = lib.farm_intersect_dev(self._h, begin, end, C.byref(o), C.c_void_p(out_dev_ptr or 0), C.byref(st))
@@ -840,32 +858,40 @@ class Device:
         self._check(lib.farm_mlp_predict_pairs(self._h, begin, end, _ptr(out)), "mlp_predict_pairs")
         return out[: end - begin]
 
-    def autotune(self, begin: int = 0, end: int | None = None, cache: str | None = None, key: str = "") -> int:
-        """Pick the refinement tile width on pairs [begin, end).  With a cache
-        path, a choice stored under `key` (a workload-shape descriptor) is
-        reused instead of re-timing."""
-        import json
-
-        if cache and key and os.path.exists(cache):
-            try:
-                known = json.load(open(cache))
-            except (OSError, ValueError):
-                known = {}
-            if key in known:
-                self._check(lib.farm_set_tile(self._h, int(known[key])), "set_tile")
-                return int(known[key])
+    def autotune(self, begin: int = 0, end: int | None = None, cache: str | None = None) -> dict:
+        """Pick the refinement configuration (tile width, chunk size,
+        workgroups per CU) on pairs [begin, end); it becomes the default of
+        later calls and the entry of the uploaded set's shape bucket.  With a
+        cache path (farm_tuning_save's text format), a stored entry for the
+        set's bucket is applied instead of re-timing, and a new choice is
+        added to the file."""
+        if cache and os.path.exists(cache):
+            self._check(lib.farm_tuning_load(self._h, cache.encode()), "tuning_load")
+            t = self.tuning()
+            if t["ms"] > 0:  # the bucket had an entry
+                return t
         end = self.npairs if end is None else end
-        best = C.c_int(0)
-        self._check(lib.farm_autotune(self._h, begin, end, C.byref(best)), "autotune")
-        if cache and key:
-            try:
-                known = json.load(open(cache)) if os.path.exists(cache) else {}
-            except (OSError, ValueError):
-                known = {}
-            known[key] = best.value
-            with open(cache, "w") as f:
-                json.dump(known, f, indent=1, sort_keys=True)
-        return best.value
+        best = Tuning()
+        self._check(lib.farm_autotune_config(self._h, begin, end, C.byref(best)), "autotune_config")
+        if cache:
+            self._check(lib.farm_tuning_save(self._h, cache.encode()), "tuning_save")
+        return best.as_dict()
+
+    def tuning(self) -> dict:
+        """The configuration calls with tile_a / tile_b == 0 use."""
+        t = Tuning()
+        self._check(lib.farm_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, tile_a: int, chunk: int, blocks_per_cu: int = 8) -> None:
+        t = Tuning(tile_a=tile_a, chunk=chunk, blocks_per_cu=blocks_per_cu)
+        self._check(lib.farm_set_tuning(self._h, C.byref(t)), "set_tuning")
+
+    def tuning_save(self, path: str) -> None:
+        self._check(lib.farm_tuning_save(self._h, path.encode()), "tuning_save")
+
+    def tuning_load(self, path: str) -> None:
+        self._check(lib.farm_tuning_load(self._h, path.encode()), "tuning_load")
 
     def copy_probe(self, nbytes: int, iters: int = 10) -> float:
         """GB/s (read + write) of a plain streaming copy of nbytes."""

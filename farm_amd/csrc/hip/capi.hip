@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -27,7 +29,7 @@ __global__ void k_bucket_scan(uint32_t *, uint32_t *);
 __global__ void k_bucket_scatter(const uint8_t *, const unsigned long long *, uint64_t, uint32_t *, uint32_t *);
 template <int E>
 __global__ void k_tiles(const PixPair *, const uint32_t *, unsigned long long *, uint64_t, uint32_t *);
-template <int E>
+template <int E, int C>
 __global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, const PixPair *,
                          const uint32_t *, const uint2 *, uint64_t, uint64_t, uint32_t *, uint32_t *,
                          unsigned long long *, int);
@@ -35,8 +37,8 @@ __global__ void k_tile_map(const uint32_t *, const unsigned long long *, uint64_
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
 __global__ void k_vdiff(const float *, const float *, uint64_t, float4 *);
-__global__ void k_chunk_count(DevSet, int64_t, uint32_t *);
-__global__ void k_chunk_box(DevSet, int64_t, const uint32_t *, float4 *);
+__global__ void k_chunk_count(DevSet, int64_t, uint32_t, uint32_t *);
+__global__ void k_chunk_box(DevSet, int64_t, uint32_t, const uint32_t *, float4 *);
 __global__ void k_finalize(const uint32_t *, const IsxDesc *, unsigned long long *);
 }  // namespace farm
 
@@ -121,6 +123,8 @@ struct IsxCfg {
     const uint32_t *pa, *pb;  // the pair list
     int32_t grid;             // blocks of the grid-stride kernels
     int32_t prune;            // the refinement's exact chunk prune on / off
+    int32_t chunk;            // target segments per chunk box of the refinement
+    int32_t bpc;              // refinement workgroups per CU
 };
 static_assert(sizeof(IsxCfg) <= sizeof(((farm_handle_s::IsxGraph *)nullptr)->key), "graph key too large");
 
@@ -175,29 +179,43 @@ int isx_front(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c)
     return fill_u32(h, W.cursor + 63, 1, 0);  // the refinement's tile cursor
 }
 
-// The persistent refinement with the tile width of the call.
-int isx_refine(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
-    const dim3 grid((unsigned)(h->num_cus * 8)), blk(256);
-    const IsxDesc *d = W.desc;
-    const PixPair *pps = W.pps;
-    const uint32_t *ord = W.order;
-    const uint2 *map = W.tile_map;
-    switch (c.tile) {
-        case 1:
-            hipLaunchKernelGGL(k_refine<1>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
+// The persistent refinement with the tile width and chunk size of the call.
+template <int E, int C>
+void launch_refine(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
+    const dim3 grid((unsigned)(h->num_cus * c.bpc)), blk(256);
+    hipLaunchKernelGGL((k_refine<E, C>), grid, blk, 0, h->stream, S, c.pa, c.pb, (const IsxDesc *)W.desc,
+                       (const PixPair *)W.pps, (const uint32_t *)W.order, (const uint2 *)W.tile_map, W.cap_pp,
+                       W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
+}
+
+template <int E>
+void launch_refine_chunk(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
+    switch (c.chunk) {
+        case 8:
+            launch_refine<E, 8>(h, S, W, c);
             break;
-        case 2:
-            hipLaunchKernelGGL(k_refine<2>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_refine<4>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
+        case 32:
+            launch_refine<E, 32>(h, S, W, c);
             break;
         default:
-            hipLaunchKernelGGL(k_refine<8>, grid, blk, 0, h->stream, S, c.pa, c.pb, d, pps, ord, map, W.cap_pp,
-                               W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
+            launch_refine<E, 16>(h, S, W, c);
+            break;
+    }
+}
+
+int isx_refine(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
+    switch (c.tile) {
+        case 1:
+            launch_refine_chunk<1>(h, S, W, c);
+            break;
+        case 2:
+            launch_refine_chunk<2>(h, S, W, c);
+            break;
+        case 4:
+            launch_refine_chunk<4>(h, S, W, c);
+            break;
+        default:
+            launch_refine_chunk<8>(h, S, W, c);
             break;
     }
     HIPCHK(h, hipGetLastError());
@@ -279,12 +297,34 @@ __device__ __align__(256) unsigned char scratch[kNaBytes];
 
 namespace farm {
 
+int build_chunk_boxes(farm_handle h, uint32_t C);
+
+// Sum and count of the nonzero per-pixel edge counts (BORDER pixels).
+__global__ __launch_bounds__(256) void k_shape_stats(const uint32_t *__restrict__ pixedges, uint64_t n,
+                                                     uint64_t *__restrict__ out) {
+    unsigned long long sum = 0, cnt = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = pixedges[i];
+        sum += e;
+        cnt += e ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        cnt += __shfl_xor(cnt, o);
+    }
+    if ((threadIdx.x & 63) == 0 && cnt) {
+        atomicAdd((unsigned long long *)&out[0], sum);
+        atomicAdd((unsigned long long *)&out[1], cnt);
+    }
+}
+
 // Everything the queries derive from an uploaded or device-built set: the
-// vertex padding, per-vertex step differences, per-pixel edge counts and the
-// refinement's 16-step chunk boxes.  Synchronises the stream.
+// vertex padding, per-vertex step differences, per-pixel edge counts, the
+// set's shape bucket and the refinement's chunk boxes.  Synchronises the
+// stream.
 int finish_set(farm_handle h) {
     int rc;
-    const uint64_t nverts = h->nverts, nseqs = h->nseqs;
+    const uint64_t nverts = h->nverts;
     for (DevBuf *b : {&h->x, &h->y}) {
         if ((rc = ensure(h, *b, 4 * (nverts + kVertexPad)))) return rc;
         HIPCHK(h, hipMemsetAsync((char *)b->p + 4 * nverts, 0, 4 * kVertexPad, h->stream));
@@ -303,12 +343,42 @@ int finish_set(farm_handle h) {
                            P<uint32_t>(h->pixedges));
         HIPCHK(h, hipGetLastError());
     }
-    // boxes of the refinement's 16-step target chunks, per edge sequence
+    // the set's shape bucket: half-octaves of its mean edges per BORDER pixel
+    uint64_t *shp = nullptr;
+    if ((rc = ensure(h, h->shape_ctr, 16))) return rc;
+    shp = P<uint64_t>(h->shape_ctr);
+    HIPCHK(h, hipMemsetAsync(shp, 0, 16, h->stream));
+    if (h->nstatus > 0) {
+        hipLaunchKernelGGL(k_shape_stats, dim3((unsigned)std::min<uint64_t>(blocks_for(h->nstatus), 4096)), dim3(256),
+                           0, h->stream, (const uint32_t *)P<uint32_t>(h->pixedges), h->nstatus, shp);
+        HIPCHK(h, hipGetLastError());
+    }
+    uint64_t hs[2] = {0, 0};
+    HIPCHK(h, hipMemcpyAsync(hs, shp, 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->shape_bucket = hs[1] ? (int)std::floor(2.0 * std::log2(std::max(1.0, (double)hs[0] / (double)hs[1]))) : 0;
+    // the bucket's tuned configuration, if the table holds one
+    auto it = h->tunings.find(h->shape_bucket);
+    if (it != h->tunings.end()) {
+        h->tuned_tile = it->second.tile_a;
+        h->tuned_chunk = it->second.chunk;
+        h->tuned_bpc = it->second.blocks_per_cu;
+    }
+    return build_chunk_boxes(h, (uint32_t)h->tuned_chunk);
+}
+
+// Boxes of the refinement's C-step target chunks, per edge sequence.
+// Synchronises the stream; device arrays move, so captured graphs go.
+int build_chunk_boxes(farm_handle h, uint32_t C) {
+    int rc;
+    const uint64_t nseqs = h->nseqs;
+    drop_graphs(h);
+    h->set_gen++;
     if ((rc = ensure(h, h->chcnt, 4 * (nseqs + 1)))) return rc;
     if ((rc = ensure(h, h->choff, 4 * (nseqs + 1)))) return rc;
     if ((rc = fill_u32(h, h->chcnt.p, nseqs + 1, 0))) return rc;
     if (h->npolys > 0) {
-        hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+        hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys, C,
                            P<uint32_t>(h->chcnt));
         HIPCHK(h, hipGetLastError());
     }
@@ -318,11 +388,12 @@ int finish_set(farm_handle h) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if ((rc = ensure(h, h->chbox, 16ull * (nchunks + kRefScreen)))) return rc;
     if (h->npolys > 0) {
-        hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+        hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys, C,
                            (const uint32_t *)P<uint32_t>(h->choff), P<float4>(h->chbox));
         HIPCHK(h, hipGetLastError());
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->chunk = C;
     return FARM_OK;
 }
 
@@ -362,37 +433,161 @@ int farm_copy_probe(farm_handle h, uint64_t bytes, int iters, double *gbps) {
     return FARM_OK;
 }
 
-int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile) {
+static bool valid_tuning(const farm_tuning &t) {
+    return (t.tile_a == 1 || t.tile_a == 2 || t.tile_a == 4 || t.tile_a == 8) &&
+           (t.chunk == 8 || t.chunk == 16 || t.chunk == 32) && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 32;
+}
+
+// Make t the default configuration and the entry of the set's bucket.
+static int apply_tuning(farm_handle h, const farm_tuning &t) {
+    h->tuned_tile = t.tile_a;
+    h->tuned_chunk = t.chunk;
+    h->tuned_bpc = t.blocks_per_cu;
+    farm_tuning e = t;
+    e.shape_bucket = h->shape_bucket;
+    h->tunings[h->shape_bucket] = e;
+    if (h->npolys > 0 && (uint32_t)t.chunk != h->chunk) return build_chunk_boxes(h, (uint32_t)t.chunk);
+    return FARM_OK;
+}
+
+int farm_autotune_config(farm_handle h, uint64_t begin, uint64_t end, farm_tuning *best) {
     if (!h) return FARM_EINVAL;
-    // time the refinement of one representative pair window with every tile
-    // width, keep the fastest for later calls with opts->tile_a == 0
-    const int cand[] = {1, 2, 4, 8};
-    double best_ms = 1e300;
-    int best = h->tuned_tile;
-    for (int E : cand) {
+    if (h->npolys <= 0) return fail(h, FARM_ESTATE, "no polygons uploaded");
+    HIPCHK(h, hipSetDevice(h->device));
+    // time one representative pair window with every tile width and chunk
+    // size (the chunk-major order rebuilds the boxes three times), then the
+    // workgroups per CU at the fastest pair; each configuration runs twice
+    // and its second (warm) time counts
+    const int saved_bpc = h->tuned_bpc;
+    auto time_cfg = [&](int E, int C, int bpc, double *ms) {
         farm_opts o;
         memset(&o, 0, sizeof o);
         o.mode = FARM_MODE_EXACT_BATCHED;
         o.tile_a = E;
+        o.tile_b = C;
+        h->tuned_bpc = bpc;
         farm_stats st;
         int rc = run_intersect(h, begin, end, &o, OUT_NONE, nullptr, &st);
-        if (rc) return rc;
-        rc = run_intersect(h, begin, end, &o, OUT_NONE, nullptr, &st);  // second run: warm caches
-        if (rc) return rc;
-        if (st.total_ms < best_ms) {
-            best_ms = st.total_ms;
-            best = E;
+        if (!rc) rc = run_intersect(h, begin, end, &o, OUT_NONE, nullptr, &st);
+        *ms = st.total_ms;
+        return rc;
+    };
+    farm_tuning b;
+    memset(&b, 0, sizeof b);
+    b.ms = 1e300;
+    int rc = FARM_OK;
+    for (int C : {8, 16, 32}) {
+        for (int E : {1, 2, 4, 8}) {
+            double ms = 0;
+            if ((rc = time_cfg(E, C, 8, &ms))) break;
+            if (ms < b.ms) {
+                b.tile_a = E;
+                b.chunk = C;
+                b.blocks_per_cu = 8;
+                b.ms = ms;
+            }
         }
+        if (rc) break;
     }
-    h->tuned_tile = best;
-    if (best_tile) *best_tile = best;
+    if (!rc)
+        for (int bpc : {4, 16}) {
+            double ms = 0;
+            if ((rc = time_cfg(b.tile_a, b.chunk, bpc, &ms))) break;
+            if (ms < b.ms) {
+                b.blocks_per_cu = bpc;
+                b.ms = ms;
+            }
+        }
+    h->tuned_bpc = saved_bpc;
+    if (rc) return rc;
+    if ((rc = apply_tuning(h, b))) return rc;
+    b.shape_bucket = h->shape_bucket;
+    if (best) *best = b;
     return FARM_OK;
+}
+
+int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile) {
+    farm_tuning t;
+    const int rc = farm_autotune_config(h, begin, end, &t);
+    if (!rc && best_tile) *best_tile = t.tile_a;
+    return rc;
 }
 
 int farm_set_tile(farm_handle h, int tile) {
     if (!h) return FARM_EINVAL;
     if (tile != 1 && tile != 2 && tile != 4 && tile != 8) return fail(h, FARM_EINVAL, "tile must be 1, 2, 4 or 8");
     h->tuned_tile = tile;
+    return FARM_OK;
+}
+
+int farm_get_tuning(farm_handle h, farm_tuning *out) {
+    if (!h || !out) return FARM_EINVAL;
+    memset(out, 0, sizeof *out);
+    out->tile_a = h->tuned_tile;
+    out->chunk = h->tuned_chunk;
+    out->blocks_per_cu = h->tuned_bpc;
+    out->shape_bucket = h->shape_bucket;
+    auto it = h->tunings.find(h->shape_bucket);
+    if (it != h->tunings.end() && it->second.tile_a == out->tile_a && it->second.chunk == out->chunk &&
+        it->second.blocks_per_cu == out->blocks_per_cu)
+        out->ms = it->second.ms;
+    return FARM_OK;
+}
+
+int farm_set_tuning(farm_handle h, const farm_tuning *t) {
+    if (!h || !t) return FARM_EINVAL;
+    if (!valid_tuning(*t))
+        return fail(h, FARM_EINVAL, "tuning (tile %d, chunk %d, %d per CU): tile 1, 2, 4 or 8, chunk 8, 16 or 32, "
+                                    "1..32 workgroups per CU", t->tile_a, t->chunk, t->blocks_per_cu);
+    HIPCHK(h, hipSetDevice(h->device));
+    return apply_tuning(h, *t);
+}
+
+int farm_tuning_save(farm_handle h, const char *path) {
+    if (!h || !path) return FARM_EINVAL;
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(h, FARM_EINVAL, "cannot write %s", path);
+    fprintf(f, "farm-tuning 1\n");
+    for (const auto &kv : h->tunings)
+        fprintf(f, "%d %d %d %d %.6f\n", kv.first, kv.second.tile_a, kv.second.chunk, kv.second.blocks_per_cu,
+                kv.second.ms);
+    const bool ok = fclose(f) == 0;
+    return ok ? FARM_OK : fail(h, FARM_EINVAL, "cannot write %s", path);
+}
+
+int farm_tuning_load(farm_handle h, const char *path) {
+    if (!h || !path) return FARM_EINVAL;
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(h, FARM_EINVAL, "cannot read %s", path);
+    char line[256];
+    int version = 0;
+    if (!fgets(line, sizeof line, f) || sscanf(line, "farm-tuning %d", &version) != 1 || version != 1) {
+        fclose(f);
+        return fail(h, FARM_EINVAL, "%s: not a farm tuning file", path);
+    }
+    std::map<int, farm_tuning> got;
+    int lineno = 1;
+    while (fgets(line, sizeof line, f)) {
+        lineno++;
+        farm_tuning t;
+        memset(&t, 0, sizeof t);
+        char tail = 0;
+        const int n = sscanf(line, "%d %d %d %d %lf %c", &t.shape_bucket, &t.tile_a, &t.chunk, &t.blocks_per_cu,
+                             &t.ms, &tail);
+        if (n == EOF || (n == 0 && line[0] == '\n')) continue;
+        if (n != 5 || !valid_tuning(t)) {
+            fclose(f);
+            return fail(h, FARM_EINVAL, "%s:%d: malformed tuning entry", path, lineno);
+        }
+        got[t.shape_bucket] = t;
+    }
+    fclose(f);
+    for (const auto &kv : got) h->tunings[kv.first] = kv.second;
+    auto it = h->tunings.find(h->shape_bucket);
+    if (h->npolys > 0 && it != h->tunings.end()) {
+        HIPCHK(h, hipSetDevice(h->device));
+        return apply_tuning(h, it->second);
+    }
     return FARM_OK;
 }
 
@@ -456,7 +651,7 @@ int farm_close(farm_handle h) {
         if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
     }
     DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->xyd,      &h->fill,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
-                      &h->vnodes, &h->pixedges, &h->pa,    &h->pb,     &h->isx,
+                      &h->vnodes, &h->pixedges, &h->shape_ctr, &h->pa, &h->pb, &h->isx,
                       &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,
                       &h->mlp_dh1, &h->mlp_y, &h->mlp_c, &h->mlp_loss, &h->feat, &h->decided, &h->pred,
@@ -596,6 +791,10 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
     const int mode = opts ? opts->mode : FARM_MODE_EXACT_BATCHED;
     int E = opts && opts->tile_a ? opts->tile_a : h->tuned_tile;
     if (E != 1 && E != 2 && E != 4 && E != 8) return fail(h, FARM_EINVAL, "tile_a must be 1, 2, 4 or 8");
+    // chunk size: the call's, else the tuned one (the built one for the
+    // fixed-scratch entry, which never rebuilds the boxes on its own)
+    const int C = opts && opts->tile_b ? opts->tile_b : (fixed ? (int)h->chunk : h->tuned_chunk);
+    if (C != 8 && C != 16 && C != 32) return fail(h, FARM_EINVAL, "tile_b (chunk size) must be 8, 16 or 32");
     if (mode < FARM_MODE_EXACT_PLAIN || mode > FARM_MODE_APPROX_LEARNED)
         return fail(h, FARM_EINVAL, "mode %d not supported by farm_intersect", mode);
     const bool approx = mode == FARM_MODE_APPROX_HEURISTIC;
@@ -609,6 +808,14 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
                         (unsigned long long)n64, (unsigned long long)kNaPairs);
     }
     HIPCHK(h, hipSetDevice(h->device));
+    if ((uint32_t)C != h->chunk && mode != FARM_MODE_EXACT_PLAIN && mode != FARM_MODE_APPROX_HEURISTIC) {
+        // boxes for another chunk size: rebuilt here, which allocates
+        if (fixed)
+            return fail(h, FARM_EINVAL, "chunk size %d: the set's boxes are built for %u (farm_set_tuning first)", C,
+                        h->chunk);
+        int rc = build_chunk_boxes(h, (uint32_t)C);
+        if (rc) return rc;
+    }
     const DevSet S = devset(h);
     unsigned long long c[kCtrSlots];
     IsxWork W;
@@ -662,6 +869,8 @@ static int run_intersect(farm_handle h, uint64_t begin, uint64_t end, const farm
             cfg.pb = P<uint32_t>(h->pb);
             cfg.grid = h->num_cus * 16;
             cfg.prune = opts && (opts->flags & FARM_FLAG_NO_PRUNE) ? 0 : 1;
+            cfg.chunk = approx ? 0 : C;
+            cfg.bpc = approx ? 0 : h->tuned_bpc;
             rc = (h->use_graphs && !fixed) ? isx_front_graph(h, S, W, cfg) : isx_front(h, S, W, cfg);
             if (rc) return rc;
             HIPCHK(h, hipEventRecord(h->ev[1], h->stream));

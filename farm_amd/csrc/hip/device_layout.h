@@ -21,7 +21,7 @@ struct DevSet {
     // differences rounded to f32, (0, 0) at v = 0 (refinement only: one
     // 16-byte load per target step)
     const float4 *xyd;
-    // per edge sequence: index of its first 16-step chunk box; the boxes
+    // per edge sequence: index of its first C-step chunk box; the boxes
     // (centre x, y, half extents x, y), refinement only
     const uint32_t *choff;
     const float4 *chbox;
@@ -38,10 +38,11 @@ struct DevSet {
     const double *areas;
 };
 
-// The refinement walks a target edge sequence in chunks of kRefChunk steps and
-// screens kRefScreen chunk boxes per scalar load (the box array keeps kRefScreen
-// spare entries past its end).
-constexpr uint32_t kRefChunk = 16;
+// The refinement walks a target edge sequence in chunks of C steps (8, 16 or
+// 32; kRefChunkDefault until the autotuner or the caller picks another) and
+// screens kRefScreen chunk boxes per scalar load (the box array keeps
+// kRefScreen spare entries past its end).
+constexpr uint32_t kRefChunkDefault = 16;
 constexpr uint32_t kRefScreen = 4;
 
 // A BORDER x BORDER pixel pair of one candidate pair.

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "device_layout.h"
@@ -42,7 +43,7 @@ struct farm_handle_s {
     int64_t npolys = 0;
     farm::DevBuf fill;    // -w fill classes (when the set has them)
     int cc = FARM_CATEGORY_COUNT;
-    farm::DevBuf hdr, status, pixoff, seqs, x, y, xyd, choff, chcnt, chbox, voff, vnodes, pixedges;
+    farm::DevBuf hdr, status, pixoff, seqs, x, y, xyd, choff, chcnt, chbox, voff, vnodes, pixedges, shape_ctr;
     uint64_t nstatus = 0;
     // array lengths of the set (farm_polyset's counts)
     uint64_t nverts = 0, nseqs = 0, npixoff = 0, nvoff = 0, nvnodes = 0, nlayers = 0, nlstatus = 0;
@@ -65,7 +66,17 @@ struct farm_handle_s {
     std::vector<IsxGraph> graphs;
     void *scratch = nullptr;  // this device's fixed scratch array (farm_intersect_noalloc)
     farm::DevBuf pair_tests;  // per-pair executed segment tests (cost labels)
-    int tuned_tile = 4;       // source edges per lane of the refinement (farm_autotune)
+    // refinement launch configuration used when a call names none: source
+    // edges per lane, target segments per chunk box and workgroups per CU
+    // (farm_autotune_config / farm_set_tuning); `chunk` is the chunk size the
+    // set's boxes are built for, and `tunings` holds the autotuner's choice
+    // per shape bucket (farm_tuning_save / farm_tuning_load persist it)
+    int tuned_tile = 4;
+    int tuned_chunk = 16;
+    int tuned_bpc = 8;
+    uint32_t chunk = 16;
+    int shape_bucket = 0;
+    std::map<int, farm_tuning> tunings;
     // learned part
     bool mlp_ready = false;
     farm::DevBuf mlp_w, mlp_m, mlp_v, mlp_g;  // f32 master params, Adam moments, grads

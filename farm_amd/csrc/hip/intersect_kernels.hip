@@ -348,7 +348,8 @@ __global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ t
 // Persistent refinement.  Each wave takes the next tile from a global
 // cursor.  A tile is 64 lanes x W consecutive edge slots of one source
 // pixel's sequences (consecutive lanes share one vertex), refined against
-// the edge sequences of one target pixel, walked in chunks of 16 segments.
+// the edge sequences of one target pixel, walked in chunks of C segments
+// (C = 8, 16 or 32, chosen per call; the autotuner picks it).
 // Per chunk every lane first tests its W edges against the chunk's box (an
 // exact prune: no edge whose line keeps the whole box at a safe distance can
 // meet a segment of the chunk under the reference predicate); the (lane,
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ t
 // ------------------------------------------------------------------------
 
 // Boxes of the refinement's target chunks.  A sequence (first, len) is walked
-// in chunks c = 0, 1, ..: steps base = 1 + 16c .. min(len, base + 15), whose
+// in chunks c = 0, 1, ..: steps base = 1 + C c .. min(len, base + C - 1), whose
 // segments span vertices first + base - 1 .. first + base + n - 1.  The box is
 // stored as a centre and half extents rounded up, so [c - h, c + h] holds
 // every vertex of the chunk in exact arithmetic.  Infinite extents (the prune
@@ -368,16 +369,17 @@ __global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ t
 // c.cross(a, d) is 0 for every a, so segment_intersect takes its collinear
 // branch and answers with the bounding-box overlap alone, on either side of
 // line ab.
-__global__ __launch_bounds__(256) void k_chunk_count(DevSet S, int64_t npolys, uint32_t *__restrict__ cnt) {
+__global__ __launch_bounds__(256) void k_chunk_count(DevSet S, int64_t npolys, uint32_t C,
+                                                     uint32_t *__restrict__ cnt) {
     const int64_t p = blockIdx.x;
     if (p >= npolys) return;
     const farm_raster_hdr H = S.hdr[p];
     for (uint32_t s = threadIdx.x; s < H.nseqs; s += blockDim.x)
-        cnt[H.seq_start + s] = (S.seqs[H.seq_start + s].y + kRefChunk - 1) / kRefChunk;
+        cnt[H.seq_start + s] = (S.seqs[H.seq_start + s].y + C - 1) / C;
 }
 
-__global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, const uint32_t *__restrict__ choff,
-                                                   float4 *__restrict__ box) {
+__global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, uint32_t C,
+                                                   const uint32_t *__restrict__ choff, float4 *__restrict__ box) {
     const int64_t p = blockIdx.x;
     if (p >= npolys) return;
     const farm_raster_hdr H = S.hdr[p];
@@ -385,8 +387,8 @@ __global__ __launch_bounds__(256) void k_chunk_box(DevSet S, int64_t npolys, con
     for (uint32_t s = threadIdx.x; s < H.nseqs; s += blockDim.x) {
         const uint2 sq = S.seqs[H.seq_start + s];
         uint32_t o = choff[H.seq_start + s];
-        for (uint32_t base = 1; base <= sq.y; base += kRefChunk, o++) {
-            const uint32_t n = min(kRefChunk, sq.y - base + 1);
+        for (uint32_t base = 1; base <= sq.y; base += C, o++) {
+            const uint32_t n = min(C, sq.y - base + 1);
             float x0 = x[sq.x + base - 1], x1 = x0, y0 = y[sq.x + base - 1], y1 = y0;
             bool open = isfinite(x0) && isfinite(y0);  // false: never prune this chunk
             float lx = x0, ly = y0;
@@ -418,7 +420,7 @@ typedef __attribute__((address_space(4))) const float cfloat;
 
 constexpr uint32_t kRefQueue = 512;  // queued items per wave: a power of 2 above 63 + 64 kRefScreen
 
-template <int W>
+template <int W, int C>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_refine(DevSet S, const uint32_t *__restrict__ pa_all,
                                                  const uint32_t *__restrict__ pb_all,
                                                  const IsxDesc *__restrict__ desc,
@@ -445,9 +447,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     float(*src)[2 * (W + 1)] = s_src[wv];
     uint32_t *sev = s_ev[wv];
     uint2 *ring = s_ring[wv];
-    unsigned long long tests = 0, items = 0, rbytes = 0, pruned = 0;
+    // work counters: per tile in 32-bit lane registers, summed over the wave
+    // at the end of the tile into this wave's 64-bit totals in LDS (64-bit
+    // lane totals live across the whole loop and push the kernel past its
+    // register budget)
+    __shared__ unsigned long long s_acc[4][3];  // tests, refine bytes, pruned tests
+    if (lane < 3) s_acc[wv][lane] = 0;
+    unsigned long long items = 0;
     for (;;) {
-        const unsigned long long tests_before = tests;
+        uint32_t tests = 0, rbytes = 0, pruned = 0;
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(cursor, 1u);
         t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
@@ -570,7 +578,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 // running minimum of the screen, even and odd edges apart
                 float gmin[2] = {INFINITY, INFINITY};
 #pragma unroll 4
-                for (int m = 0; m < (int)kRefChunk; m++) {
+                for (int m = 0; m < C; m++) {
                     if ((uint32_t)m < ns) {
                         // the step's end q and direction (q - c), precomputed with
                         // the same rounding, in one load
@@ -619,7 +627,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         }
                     }
                 }
-                tests += (unsigned long long)qnv * ns;
+                tests += (uint32_t)qnv * ns;
                 qhead = (qhead + n) & (kRefQueue - 1);
                 qn -= n;
                 if (__any(hit) || __hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
@@ -631,7 +639,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const uint32_t v0 = __builtin_amdgcn_readfirstlane(sq.x);
                 const uint32_t L = __builtin_amdgcn_readfirstlane(sq.y);
                 const cfloat *cbox = (const cfloat *)(S.chbox + __builtin_amdgcn_readfirstlane(bchoff[j]));
-                const uint32_t nch = (L + kRefChunk - 1) / kRefChunk;
+                const uint32_t nch = (L + C - 1) / C;
                 for (uint32_t g = 0; g < nch && !stop; g += kRefScreen) {
                     // Exact prune of this lane's edges against kRefScreen chunks
                     // (their boxes are scalar loads; boxes past the sequence's
@@ -712,7 +720,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     for (int i = 0; i < (int)kRefScreen; i++) {
                         const uint32_t c = g + i;
                         if (c >= nch) break;
-                        const uint32_t nstep = min(kRefChunk, L - c * kRefChunk);
+                        const uint32_t nstep = min((uint32_t)C, L - c * C);
                         scr_steps += nstep;
                         const unsigned long long bm = __ballot(need[i]);
                         if (bm) {
@@ -721,7 +729,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                 (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
                             if (need[i]) {
                                 ring[(qhead + qn + below) & (kRefQueue - 1)] =
-                                    make_uint2(v0 + c * kRefChunk, (uint32_t)lane | (nstep << 8));
+                                    make_uint2(v0 + c * C, (uint32_t)lane | (nstep << 8));
                                 queued_steps += nstep;
                             }
                             qn += (uint32_t)__popcll(bm);
@@ -743,34 +751,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (__any(hit)) {
                 if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (pair_tests) {  // executed tests of this tile, for the cost labels
-                unsigned long long tt = tests - tests_before;
-                for (int o = 32; o > 0; o >>= 1) tt += __shfl_xor(tt, o);
-                if (lane == 0) atomicAdd(&pair_tests[pair], (uint32_t)min(tt, 0xffffffffull));
-            }
+
             if (lane == 0) {
                 // algorithmic refine bytes of this tile: its source vertices plus
                 // the target pixel's sequences once
                 const uint32_t nb = b_s1 - b_s0;
-                rbytes += 16 + 8ull * nb + 8ull * (S.pixedges[B.pix_start + P.pb] + nb);
+                rbytes += 16 + 8u * nb + 8u * (S.pixedges[B.pix_start + P.pb] + nb);
             }
-            rbytes += (unsigned long long)nvalid * 8;
+            rbytes += (uint32_t)nvalid * 8u;
             // tests of this lane's edges the prune skipped: every screened step
             // that was not queued
-            pruned += (unsigned long long)nvalid * (scr_steps - queued_steps);
+            pruned += (uint32_t)nvalid * (scr_steps - queued_steps);
         };
         run(std::integral_constant<int, W>());
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        tests += __shfl_xor(tests, o);
-        rbytes += __shfl_xor(rbytes, o);
-        pruned += __shfl_xor(pruned, o);
+        // the tile's totals (a tile holds at most 512 edges against one target
+        // pixel's: its sums stay far below 2^64 and each lane's below 2^32)
+        unsigned long long t64 = tests, b64 = rbytes, p64 = pruned;
+        for (int o = 32; o > 0; o >>= 1) {
+            t64 += __shfl_xor(t64, o);
+            b64 += __shfl_xor(b64, o);
+            p64 += __shfl_xor(p64, o);
+        }
+        if (lane == 0) {
+            s_acc[wv][0] += t64;
+            s_acc[wv][1] += b64;
+            s_acc[wv][2] += p64;
+            // executed tests of this tile, for the cost labels
+            if (pair_tests) atomicAdd(&pair_tests[pair], (uint32_t)min(t64, 0xffffffffull));
+        }
     }
     if (lane == 0) {
-        atomicAdd(&ctr[kCtrTests], tests);
+        atomicAdd(&ctr[kCtrTests], s_acc[wv][0]);
         atomicAdd(&ctr[kCtrItems], items);
-        atomicAdd(&ctr[kCtrRefineBytes], rbytes);
-        atomicAdd(&ctr[kCtrPruned], pruned);
+        atomicAdd(&ctr[kCtrRefineBytes], s_acc[wv][1]);
+        atomicAdd(&ctr[kCtrPruned], s_acc[wv][2]);
     }
 }
 
@@ -976,16 +990,30 @@ __global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ r
     if ((threadIdx.x & 63) == 0 && f) atomicAdd(&ctr[kCtrFound], f);
 }
 
-#define FARM_REFINE_INST(EE)                                                                                   \
+#define FARM_TILES_INST(EE)                                                                                    \
     template __global__ void k_tiles<EE>(const PixPair *, const uint32_t *, unsigned long long *, uint64_t,      \
-                                         uint32_t *);                                                          \
-    template __global__ void k_refine<EE>(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *,          \
-                                          const PixPair *, const uint32_t *, const uint2 *, uint64_t, uint64_t, \
-                                          uint32_t *, uint32_t *, unsigned long long *, int);
-FARM_REFINE_INST(1)
-FARM_REFINE_INST(2)
-FARM_REFINE_INST(4)
-FARM_REFINE_INST(8)
+                                         uint32_t *);
+#define FARM_REFINE_INST(EE, CC)                                                                               \
+    template __global__ void k_refine<EE, CC>(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *,      \
+                                              const PixPair *, const uint32_t *, const uint2 *, uint64_t,       \
+                                              uint64_t, uint32_t *, uint32_t *, unsigned long long *, int);
+FARM_TILES_INST(1)
+FARM_TILES_INST(2)
+FARM_TILES_INST(4)
+FARM_TILES_INST(8)
+FARM_REFINE_INST(1, 8)
+FARM_REFINE_INST(2, 8)
+FARM_REFINE_INST(4, 8)
+FARM_REFINE_INST(8, 8)
+FARM_REFINE_INST(1, 16)
+FARM_REFINE_INST(2, 16)
+FARM_REFINE_INST(4, 16)
+FARM_REFINE_INST(8, 16)
+FARM_REFINE_INST(1, 32)
+FARM_REFINE_INST(2, 32)
+FARM_REFINE_INST(4, 32)
+FARM_REFINE_INST(8, 32)
+#undef FARM_TILES_INST
 #undef FARM_REFINE_INST
 
 }  // namespace farm

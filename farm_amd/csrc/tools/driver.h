@@ -36,6 +36,8 @@ struct DriverArgs {
     bool rasterize = false, gpu = false, approx = false, hierarchy = false;
     int device = 0;
     int tile = 0;
+    int chunk = 0;                // target segments per chunk box (--chunk)
+    std::string tune;             // tuning table file (--tune)
     bool plain = false;
     bool gpu_candidates = false;  // build the candidate pairs on the device instead of the host R-tree
     std::string model;            // trained MLP for the learned -a mode (--model)
@@ -72,7 +74,11 @@ inline void usage(const char *prog) {
            "  -u, --unorll_size N     accepted for compatibility\n"
            "  -r, -g, -a, -h          rasterize / gpu / approximation / hierarchy\n"
            "  --plain                 one-pair-per-workgroup kernel instead of the batched one\n"
-           "  --tile E                source edges per lane in the batched kernel (1, 2, 4, 8)\n"
+           "  --tile E                source edges per lane in the batched kernel (1, 2, 4 or 8)\n"
+           "  --chunk C               target segments per chunk box in the batched kernel (8, 16 or 32)\n"
+           "  --tune FILE             intersect: the refinement configuration of the set's shape bucket\n"
+           "                          from FILE; when FILE has none, autotune on the first batch and\n"
+           "                          add it to FILE (--tile / --chunk override it)\n"
            "  --model FILE            with -a: the learned mode with this trained MLP (farm_amd/learned.py\n"
            "                          writes it) instead of the reference heuristic\n"
            "  --gpu-candidates        build the candidate pairs on the device (same pairs as the R-tree)\n"
@@ -119,6 +125,8 @@ inline bool parse_args(int argc, char **argv, DriverArgs &a) {
         else if (k == "-d" || k == "--within_distance") a.within_km = atoi(val());
         else if (k == "--device") a.device = atoi(val());
         else if (k == "--tile") a.tile = atoi(val());
+        else if (k == "--chunk") a.chunk = atoi(val());
+        else if (k == "--tune") a.tune = val();
         else if (k == "--gpu-candidates") a.gpu_candidates = true;
         else if (k == "--model") a.model = val();
         else if (k == "--wkt") a.wkt = val();

@@ -69,7 +69,28 @@ int main(int argc, char **argv) {
     memset(&o, 0, sizeof o);
     o.mode = a.approx ? FARM_MODE_APPROX_HEURISTIC : (a.plain ? FARM_MODE_EXACT_PLAIN : FARM_MODE_EXACT_BATCHED);
     o.tile_a = a.tile;
+    o.tile_b = a.chunk;
     o.merge_threshold = a.merge_threshold;
+    if (!a.tune.empty() && o.mode == FARM_MODE_EXACT_BATCHED && npairs > 0) {
+        // the set's shape bucket from the file, else tuned on the first batch
+        FILE *f = fopen(a.tune.c_str(), "r");
+        if (f) {
+            fclose(f);
+            DIE_IF(h, farm_tuning_load(h, a.tune.c_str()), "load tuning");
+        }
+        farm_tuning tn;
+        DIE_IF(h, farm_get_tuning(h, &tn), "tuning");
+        if (tn.ms <= 0) {
+            const uint64_t w = std::min<uint64_t>((uint64_t)npairs, a.batch ? a.batch : 3000000);
+            DIE_IF(h, farm_autotune_config(h, 0, w, &tn), "autotune");
+            DIE_IF(h, farm_tuning_save(h, a.tune.c_str()), "save tuning");
+            logf("autotuned on %llu pairs: tile %d, chunk %d, %d workgroups per CU (%.3f ms)",
+                 (unsigned long long)w, tn.tile_a, tn.chunk, tn.blocks_per_cu, tn.ms);
+        } else {
+            logf("tuning of shape bucket %d from %s: tile %d, chunk %d, %d workgroups per CU", tn.shape_bucket,
+                 a.tune.c_str(), tn.tile_a, tn.chunk, tn.blocks_per_cu);
+        }
+    }
     if (!a.model.empty()) {
         if (!a.approx) {
             logf("--model needs -a");

@@ -180,7 +180,10 @@ def threshold_for_precision(decided, logit, truth, target: float = 0.99, confide
     if len(ok) == 0:
         return float(np.inf)
     k = ok[-1]  # accept the top k+1 logits
-    return float(z[order][k]) - 1e-6 if k + 1 < len(z) else float(z.min()) - 1.0
+    # a binary32 value (the device compares binary32 logits against a
+    # binary32 threshold): the largest one below the (k+1)-th logit
+    zk = np.float32(z[order][k]) if k + 1 < len(z) else np.float32(z.min() - 1.0)
+    return float(np.nextafter(zk, np.float32(-np.inf)))
 
 
 MODEL_MAGIC = b"FARMMLP1"

@@ -133,7 +133,9 @@ typedef struct farm_polyset {
 typedef struct farm_opts {
     int32_t mode;
     int32_t tile_a;        /* source edges per lane of the batched refinement; 0 = autotuned */
-    int32_t tile_b;        /* reserved (target vertices stream in fixed 16-vertex chunks) */
+    int32_t tile_b;        /* target segments per chunk box of the batched refinement (8, 16
+                              or 32); 0 = autotuned.  A size other than the one the set's
+                              boxes are built for rebuilds them (allocates, synchronises) */
     int32_t order;         /* batched refine queue order: 0 exact cost bound, 1 learned cost head */
     float approx_threshold;/* learned -a: logit above which a pair is accepted */
     float approx_reject;   /* learned -a: logit below which a pair is rejected; pairs in
@@ -392,14 +394,40 @@ int farm_mlp_train(farm_handle h, const float *X, const float *y, const float *c
 /* features + normalisation + forward for pairs [begin, end); out: n x 2 */
 int farm_mlp_predict_pairs(farm_handle h, uint64_t begin, uint64_t end, float *out);
 
+/* Launch configuration of the batched refinement. */
+typedef struct farm_tuning {
+    int32_t tile_a;        /* source edges per lane: 1, 2, 4 or 8 */
+    int32_t chunk;         /* target segments per chunk box: 8, 16 or 32 */
+    int32_t blocks_per_cu; /* persistent refinement workgroups per CU: 1..32 */
+    int32_t shape_bucket;  /* output: the shape bucket of the uploaded set this entry is for
+                              (half-octaves of its mean edges per BORDER pixel) */
+    double ms;             /* output: device time of the tuning window with it (0: not timed) */
+} farm_tuning;
+
 /* Autotuner of the batched refinement: times pairs [begin, end) with every
- * supported tile width (source edges per lane: 1, 2, 4, 8) and keeps the
- * fastest for later calls whose opts->tile_a is 0.  No reference
+ * tile width (1, 2, 4, 8) and chunk size (8, 16, 32), then with 4, 8 and 16
+ * workgroups per CU at the fastest pair, and keeps the fastest as the
+ * configuration of later calls whose opts->tile_a / tile_b are 0 and as the
+ * entry of the set's shape bucket (see farm_tuning_save).  No reference
  * counterpart (the reference hard-codes 16x16 tiles,
  * src/cuda/query_intersect.cu:194). */
+int farm_autotune_config(farm_handle h, uint64_t begin, uint64_t end, farm_tuning *best);
+/* farm_autotune_config, reporting only the tile width. */
 int farm_autotune(farm_handle h, uint64_t begin, uint64_t end, int *best_tile);
 /* Set the tile width used when opts->tile_a is 0 (e.g. a persisted choice). */
 int farm_set_tile(farm_handle h, int tile);
+/* The configuration calls with opts->tile_a / tile_b == 0 use, and the set's
+ * shape bucket. */
+int farm_get_tuning(farm_handle h, farm_tuning *out);
+/* Make t that configuration and the entry of the set's shape bucket
+ * (rebuilds the chunk boxes when t->chunk differs from the built size). */
+int farm_set_tuning(farm_handle h, const farm_tuning *t);
+/* Persist the per-bucket entries as text ("farm-tuning 1", then one line
+ * "bucket tile_a chunk blocks_per_cu ms" per entry), and read them back:
+ * load merges the file's entries and applies the uploaded set's bucket when
+ * the file holds it.  FARM_EINVAL for an unreadable or malformed file. */
+int farm_tuning_save(farm_handle h, const char *path);
+int farm_tuning_load(farm_handle h, const char *path);
 
 /* HBM calibration: times `iters` launches of a plain streaming copy of
  * `bytes` bytes between two scratch buffers; *gbps = bytes read + written

@@ -18,6 +18,7 @@ MIRRORS = {
     "farm_polyset": capi.PolySet,
     "farm_opts": capi.Opts,
     "farm_stats": capi.Stats,
+    "farm_tuning": capi.Tuning,
 }
 
 

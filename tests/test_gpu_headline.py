@@ -6,7 +6,7 @@ without graph capture, host and device outputs -- bit-exact against the
 oracle.  The set is the full-size workload's densities in a smaller box
 (80 complex x 2,000 lakes, 75,502 candidate pairs), so that the oracle
 finishes in seconds; its long target sequences run the refinement's full
-16-step chunks and its exact chunk prune."""
+chunks (8, 16 and 32 steps) and its exact chunk prune."""
 import os
 
 import numpy as np
@@ -55,12 +55,13 @@ def _agree(got, w, what):
     assert diff.size == 0, f"{what}: {diff.size} of {len(got)} pairs differ from the oracle, first {diff[:10]}"
 
 
+@pytest.mark.parametrize("chunk", [8, 16, 32])
 @pytest.mark.parametrize("tile", [1, 2, 4, 8])
-def test_batched_every_tile_width(device, headline, tile):
-    print(f"batched tile={tile}", flush=True)
+def test_batched_every_tile_width(device, headline, tile, chunk):
+    print(f"batched tile={tile} chunk={chunk}", flush=True)
     _upload(device, headline)
-    r = device.intersect(mode=capi.MODE_EXACT_BATCHED, tile_a=tile)
-    _agree(r.bits, headline, f"batched W={tile}")
+    r = device.intersect(mode=capi.MODE_EXACT_BATCHED, tile_a=tile, chunk=chunk)
+    _agree(r.bits, headline, f"batched W={tile} C={chunk}")
     st = r.stats
     assert st["found"] == int(headline["want"].sum())
     assert st["bb_pixel_pairs"] > 0 and st["segment_tests"] > 0
@@ -128,3 +129,32 @@ def test_batched_without_the_prune(device, headline):
     r = device.intersect(mode=capi.MODE_EXACT_BATCHED, tile_a=8, flags=capi.FLAG_NO_PRUNE)
     _agree(r.bits, headline, "no prune")
     assert r.stats["pruned_tests"] == 0
+
+
+def test_autotuned_configuration(device, headline, tmp_path):
+    # the autotuner's choice (tile width, chunk size, workgroups per CU) is
+    # exact, persists per shape bucket and is applied again from the file
+    _upload(device, headline)
+    n = len(headline["pa"])
+    cache = str(tmp_path / "tuning.txt")
+    t = device.autotune(0, min(n, 20000), cache=cache)
+    print(f"tuned {t}", flush=True)
+    assert t["tile_a"] in (1, 2, 4, 8) and t["chunk"] in (8, 16, 32) and t["blocks_per_cu"] in (4, 8, 16)
+    assert t["ms"] > 0
+    r = device.intersect(mode=capi.MODE_EXACT_BATCHED)  # tile_a = tile_b = 0: the tuned configuration
+    _agree(r.bits, headline, "autotuned")
+    lines = open(cache).read().split("\n")
+    assert lines[0] == "farm-tuning 1"
+    assert lines[1].split()[:4] == [str(t["shape_bucket"]), str(t["tile_a"]), str(t["chunk"]),
+                                    str(t["blocks_per_cu"])]
+    # another configuration, then the file's entry applied back on re-upload
+    device.set_tuning(1, 8 if t["chunk"] != 8 else 32, 4)
+    assert device.tuning()["tile_a"] == 1
+    _upload(device, headline)  # the handle's table still holds the set's bucket (the last set_tuning)
+    device.tuning_load(cache)
+    back = device.tuning()
+    assert (back["tile_a"], back["chunk"], back["blocks_per_cu"]) == (t["tile_a"], t["chunk"], t["blocks_per_cu"])
+    r = device.intersect(mode=capi.MODE_EXACT_BATCHED)
+    _agree(r.bits, headline, "tuning loaded from the file")
+    # served from the file without re-timing
+    assert device.autotune(0, min(n, 20000), cache=cache)["tile_a"] == t["tile_a"]

@@ -65,12 +65,40 @@ def test_autotuned_tile_is_exact(device, tmp_path):
     w = star_pairs(8, n=800)
     want, _ = w.oracle(0)
     w.upload(device)
-    cache = str(tmp_path / "tune.json")
-    tile = device.autotune(cache=cache, key="star-pairs-800")
-    assert tile in (1, 2, 4, 8)
-    assert device.autotune(cache=cache, key="star-pairs-800") == tile  # served from the cache
-    r = device.intersect()  # tile_a = 0: the tuned width
+    cache = str(tmp_path / "tune.txt")
+    t = device.autotune(cache=cache)
+    assert t["tile_a"] in (1, 2, 4, 8) and t["chunk"] in (8, 16, 32)
+    again = device.autotune(cache=cache)  # served from the cache
+    assert [again[k] for k in ("tile_a", "chunk", "blocks_per_cu", "shape_bucket")] == \
+        [t[k] for k in ("tile_a", "chunk", "blocks_per_cu", "shape_bucket")]
+    r = device.intersect()  # tile_a = tile_b = 0: the tuned configuration
     assert np.array_equal(r.bits, want)
+
+
+@pytest.mark.parametrize("chunk", [8, 32])
+def test_chunk_sizes(device, chunk):
+    # the refinement with 8- and 32-step target chunks (16 is the default of
+    # the tests above) on 10 small seeded instances, every tile width
+    for seed in range(130, 140):
+        print(f"star pairs 400 x 400 seed={seed} chunk={chunk}", flush=True)
+        w = star_pairs(seed, n=400)
+        want, _ = w.oracle(0)
+        w.upload(device)
+        for tile in (1, 2, 4, 8):
+            r = device.intersect(mode=capi.MODE_EXACT_BATCHED, tile_a=tile, chunk=chunk)
+            diff = np.nonzero(r.bits != want)[0]
+            assert diff.size == 0, f"tile {tile} chunk {chunk}: {diff.size} pairs differ, first {diff[:10]}"
+
+
+def test_tuning_rejects_bad_values(device):
+    w = star_pairs(8, n=200)
+    w.upload(device)
+    with pytest.raises(capi.FarmError):
+        device.set_tuning(3, 16, 8)
+    with pytest.raises(capi.FarmError):
+        device.set_tuning(8, 12, 8)
+    with pytest.raises(capi.FarmError):
+        device.intersect(mode=capi.MODE_EXACT_BATCHED, tile_a=8, chunk=24)
 
 
 def test_batch_windows_concatenate(device):

@@ -46,9 +46,12 @@ def test_intersect_and_intersection_executables(tmp_path):
     assert OA.rasterize(10) == -1 and OB.rasterize(10) == -1
     bits, _ = oracle.intersect(OA, OB, pa, pb)
     _, area = oracle.intersection(OA, OB, pa, pb)
-    for extra in ([], ["-b", "1000"], ["--gpu-candidates"], ["--gpu-candidates", "-b", "777"]):
+    tune = str(tmp_path / "tuning.txt")
+    for extra in ([], ["-b", "1000"], ["--gpu-candidates"], ["--gpu-candidates", "-b", "777"],
+                  ["--tile", "2", "--chunk", "32"], ["--tune", tune], ["--tune", tune, "-b", "900"]):
         assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-n", "8"] + extra) == int(
             (bits == 1).sum())
+    assert open(tune).readline().strip() == "farm-tuning 1"  # written by the first --tune run
     # -a: the reference's heuristic (IN overlap or a BORDER pair with prob > 1)
     abits, _ = oracle.intersect(OA, OB, pa, pb, 1)
     assert _found([os.path.join(BIN, "intersect"), "-s", s, "-t", t, "-r", "-g", "-a"]) == int((abits == 1).sum())
