@@ -18,15 +18,21 @@
 //     the filtered cross product of the two lists.
 //  2. box distances: box::max_distance over the 16 corner pairs reduces to
 //     the largest |dx| and |dy| (the corner sets are a full product and every
-//     step after the f32 differences is monotone), so one division per axis
-//     and one square root give the identical double.
+//     step after the f32 differences is monotone).  Distances are compared
+//     as squares against the squared bounds' thresholds (sq_le / sq_lt: the
+//     correctly rounded root is monotone), so a square root is taken only
+//     for the smallest maximum of a level and for kept candidates.  The
+//     divisions by the km table entries, the latitude constant and the
+//     raster steps go through precomputed reciprocals with two exact fma
+//     corrections (div_rcp), which give the same rounded quotient.
 //  3. candidates (apx, low, high, id1, id2) of the last level are handed to
-//     the lanes by lane permutes as they are found, candidate k in register
-//     slot k / 64 of lane k % 64, up to 128 of them; a pair with more keeps
-//     them in a per-wave global slab.  They are taken in tuple order by
-//     argmin selection (the order of std::sort over the tuples); the
-//     refinement stops at the reference's first return, so usually after a
-//     few candidates.
+//     the lanes by lane permutes as they are found, candidate k on lane k, up
+//     to 64 of them, and sorted there once into tuple order (the order of
+//     std::sort over the tuples) with each one's reference_dist by a suffix
+//     minimum; a pair with more keeps them in a per-wave global slab,
+//     selects the first kSelectFirst by argmin (most such pairs are decided
+//     by their first few) and then sorts the rest of the slab once.  The
+//     refinement stops at the reference's first return.
 //  4. the truncated point-segment distances of one edge-sequence pair are
 //     spread over the lanes; the minimum is taken on the squared distances
 //     and one square root follows (sqrt is monotone and correctly rounded).
@@ -46,11 +52,13 @@ namespace farm {
 const double *km_lon_table();  // host table shared with candidate generation
 
 __constant__ double c_km_lon[900];  // degree_per_kilometer_longitude (util.h:73-96)
+__constant__ double c_km_lon_inv[900];  // 1 / each entry, rounded to nearest (host division)
 __constant__ float c_mean[20];      // mean[20] (util.h:68)
 
 namespace {
 
 constexpr double kKmLat = 360.0 / 40076.0;  // degree_per_kilometer_latitude (util.h:72)
+constexpr double kKmLatInv = 1.0 / kKmLat;  // rounded to nearest (constant folding is IEEE)
 constexpr double kFar = 100000.0;           // the reference's "no distance yet"
 constexpr int kWaves = 4;                   // wavefronts per workgroup
 constexpr int kPixMax = 512;                // pixels of a layer handled in LDS
@@ -107,15 +115,38 @@ __device__ __forceinline__ int km_idx(double lat) {
 // dependent global load; an index outside the copy reads the table.
 constexpr int kKmC = 16;
 struct KmC {
-    double v[kKmC];
+    double v[kKmC], inv[kKmC];
     int lo, n;
 };
 
-__device__ __forceinline__ double km_lon(double lat, const KmC &c) {
+// a / b rounded to nearest, for b > 0 (a table entry, the latitude
+// constant or a raster step, 2^-20 .. 2^10) and y = RN(1 / b): q = a y, then two corrections
+// q' = q + (a - b q) y by fma.  The residual a - b q of a q within one ulp of
+// a / b is exact, and with y within half an ulp of 1 / b one correction of
+// such a q is the rounded quotient (Markstein); the first brings a y (within
+// 1.5 ulp) within one ulp, the second rounds.  A dividend whose residual
+// could leave the normal range (|a| < 2^-900, not 0) or that is not finite
+// takes the division.
+__device__ __forceinline__ double div_rcp(double a, double b, double y) {
+    double q = a * y;
+    q = fma(fma(-b, q, a), y, q);
+    q = fma(fma(-b, q, a), y, q);
+    const double m = fabs(a);
+    if (__builtin_expect(!(m < 0x1p1000) || (m < 0x1p-900 && m != 0.0), 0)) q = a / b;
+    return q;
+}
+
+// x / degree_per_kilometer_longitude(lat)
+__device__ __forceinline__ double div_km_lon(double x, double lat, const KmC &c) {
     const int i = km_idx(lat);
     const int k = i - c.lo;
-    return (unsigned)k < (unsigned)c.n ? c.v[k] : c_km_lon[i];
+    const bool in = (unsigned)k < (unsigned)c.n;
+    const double b = in ? c.v[k] : c_km_lon[i];
+    const double y = in ? c.inv[k] : c_km_lon_inv[i];
+    return div_rcp(x, b, y);
 }
+
+__device__ __forceinline__ double div_km_lat(double x) { return div_rcp(x, kKmLat, kKmLatInv); }
 
 // MyRaster::get_pixel_box (MyRaster.cpp:312-322), square pixels
 __device__ __forceinline__ void pbox(const LGrid &g, int x, int y, double b[4]) {
@@ -125,30 +156,54 @@ __device__ __forceinline__ void pbox(const LGrid &g, int x, int y, double b[4]) 
     b[3] = g.ly + (y + 1) * g.step;
 }
 
-// box::distance(box, geography) (Box.cpp:89-120), b = this
-__device__ __forceinline__ double box_min_dist(const double b[4], const double t[4], const KmC &kc) {
+// box::distance(box, geography) (Box.cpp:89-120), b = this, before the
+// square root (the root is monotone and correctly rounded: comparisons of
+// the distance against a bound are comparisons of this value against the
+// bound's threshold, sq_le / sq_lt)
+__device__ __forceinline__ double box_min_d2(const double b[4], const double t[4], const KmC &kc) {
     if (!(t[0] > b[2] || t[2] < b[0] || t[1] > b[3] || t[3] < b[1])) return 0.0;
     double dx = 0, dy = 0;
     if (t[1] > b[3]) dy = t[1] - b[3];
     else if (t[3] < b[1]) dy = b[1] - t[3];
     if (t[0] > b[2]) dx = t[0] - b[2];
     else if (t[2] < b[0]) dx = b[0] - t[2];
-    dy = dy / kKmLat;
-    dx = dx / km_lon(b[1], kc);
-    return sqrt(dx * dx + dy * dy);
+    dy = div_km_lat(dy);
+    dx = div_km_lon(dx, b[1], kc);
+    return dx * dx + dy * dy;
 }
 
 // box::max_distance(box, geography) (Box.cpp:122-149): corners as f32
 // points, f32 differences promoted to double; the largest |difference| per
-// axis gives the maximum over the 16 corner pairs
-__device__ __forceinline__ double box_max_dist(const double b[4], const double t[4], const KmC &kc) {
+// axis gives the maximum over the 16 corner pairs.  Before the square root.
+__device__ __forceinline__ double box_max_d2(const double b[4], const double t[4], const KmC &kc) {
     const float b0 = (float)b[0], b1 = (float)b[1], b2 = (float)b[2], b3 = (float)b[3];
     const float t0 = (float)t[0], t1 = (float)t[1], t2 = (float)t[2], t3 = (float)t[3];
     const float fx = fmaxf(fmaxf(fabsf(b0 - t0), fabsf(b0 - t2)), fmaxf(fabsf(b2 - t0), fabsf(b2 - t2)));
     const float fy = fmaxf(fmaxf(fabsf(b1 - t1), fabsf(b1 - t3)), fmaxf(fabsf(b3 - t1), fabsf(b3 - t3)));
-    const double dy = (double)fy / kKmLat;
-    const double dx = (double)fx / km_lon(b[1], kc);
-    return sqrt(dx * dx + dy * dy);
+    const double dy = div_km_lat((double)fy);
+    const double dx = div_km_lon((double)fx, b[1], kc);
+    return dx * dx + dy * dy;
+}
+
+// The largest v >= 0 with sqrt(v) <= d (sq_le) or sqrt(v) < d (sq_lt), -1
+// when there is none: sqrt(v) <= d <=> v <= sq_le(d), as the correctly
+// rounded square root is monotone.
+__device__ __forceinline__ double sq_le(double d) {
+    if (!(d >= 0)) return -1.0;
+    if (isinf(d)) return INFINITY;
+    double v = d * d;
+    while (v > 0 && sqrt(v) > d) v = nextafter(v, 0.0);
+    while (sqrt(nextafter(v, INFINITY)) <= d) v = nextafter(v, INFINITY);
+    return v;
+}
+__device__ __forceinline__ double sq_lt(double d) {
+    if (!(d > 0)) return -1.0;
+    if (isinf(d)) return INFINITY;
+    double v = d * d;
+    while (v > 0 && !(sqrt(v) < d)) v = nextafter(v, 0.0);
+    if (!(sqrt(v) < d)) return -1.0;
+    while (sqrt(nextafter(v, INFINITY)) < d) v = nextafter(v, INFINITY);
+    return v;
 }
 
 // point_to_segment_distance (geometry_computation.h:49-89), geography on,
@@ -176,8 +231,8 @@ __device__ __forceinline__ double pt_seg_d2(float px, float py, float ax, float 
     }
     double dx = px - xx;
     double dy = py - yy;
-    dx = dx / km_lon(py, kc);
-    dy = dy / kKmLat;
+    dx = div_km_lon(dx, py, kc);
+    dy = div_km_lat(dy);
     return dx * dx + dy * dy;
 }
 
@@ -266,6 +321,12 @@ struct PairOut {
 template <int SH, class T>
 __device__ __forceinline__ void expand_side(const T *list, uint32_t n, bool next, const LGrid &pg,
                                             const LGrid &g, const uint8_t *st, uint32_t *bits, int lane) {
+    // get_offset_x/_y's (v - low) / step through the step's reciprocal
+    // (div_rcp: the same rounded quotient)
+    const double ystep = next ? 1.0 / g.step : 0.0;
+    auto off = [&](double v, double lo, int dim) {
+        return min(max(d2i(div_rcp(v - lo, g.step, ystep)), 0), dim - 1);
+    };
     for (uint32_t k = lane; k < n; k += 64) {
         const T v = list[k];
         const int px = unpack_x<SH>(v), py = unpack_y<SH>(v);
@@ -282,10 +343,10 @@ __device__ __forceinline__ void expand_side(const T *list, uint32_t n, bool next
         b[1] += 1e-6;
         b[2] -= 1e-6;
         b[3] -= 1e-6;
-        const int sx = off_axis(b[0] + 1e-6, g.lx, g.step, g.dimx);
-        const int sy = off_axis(b[1] + 1e-6, g.ly, g.step, g.dimy);
-        const int ex = off_axis(b[2] - 1e-6, g.lx, g.step, g.dimx);
-        const int ey = off_axis(b[3] - 1e-6, g.ly, g.step, g.dimy);
+        const int sx = off(b[0] + 1e-6, g.lx, g.dimx);
+        const int sy = off(b[1] + 1e-6, g.ly, g.dimy);
+        const int ex = off(b[2] - 1e-6, g.lx, g.dimx);
+        const int ey = off(b[3] - 1e-6, g.ly, g.dimy);
         for (int x = sx; x <= ex; x++)
             for (int y = sy; y <= ey; y++) {
                 const uint32_t id = (uint32_t)(y * g.dimx + x);
@@ -303,15 +364,27 @@ struct SeqTab {
 };
 
 constexpr int kBatch = 4;  // candidates refined together (their sequence pairs share one table)
+constexpr uint32_t kSelectFirst = 8;  // slab candidates selected one by one before the slab is sorted
 
 // A run of candidates taken in tuple order: pixel-pair sequence ranges and
 // each one's reference_dist.  All values are uniform across the wave.
 struct CandBatch {
     int n = 0;
     uint32_t nrows = 0;
-    uint32_t s0[kBatch], t0[kBatch], n2[kBatch], row0[kBatch];
+    uint32_t s0[kBatch], t0[kBatch], n1[kBatch], n2[kBatch], row0[kBatch];
     double ref[kBatch];
 };
+
+// floor(n / m) for n < 2^22 and 1 <= m: the binary32 quotient is within one
+// of it (relative error below 2^-22), and one step against the remainder
+// corrects it.
+__device__ __forceinline__ uint32_t udiv_small(uint32_t n, uint32_t m) {
+    uint32_t q = (uint32_t)((float)n * __builtin_amdgcn_rcpf((float)m));
+    const int r = (int)n - (int)(q * m);
+    if (r < 0) q--;
+    else if (r >= (int)m) q++;
+    return q;
+}
 
 // Refines a batch of candidate pixel pairs: the truncated distances of all
 // their edge-sequence pairs (segment_to_segment_within_batch with vs1 = the
@@ -366,30 +439,34 @@ __device__ __forceinline__ int refine_batch(const DevSet &S, const farm_raster_h
         if (lane == 63) tab.pre[64] = W;
         tab.mins[lane] = ~0ull;
         wave_sync();
-        int kk = 0, cur = -1;
-        double acc = INFINITY;
-        for (uint32_t u = lane; u < W; u += 64) {
-            while (tab.pre[kk + 1] <= u) kk++;
-            if (kk != cur) {
-                if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
-                cur = kk;
-                acc = INFINITY;
+        // e < w1: point i of the target sequence, segment j of the source
+        // one; else point i of the source sequence, segment j of the target
+        // one.  One distance per lane either way: the operands are selected,
+        // so lanes on both sides of w1 share one evaluation.
+        auto sweep = [&](auto small_tag) {
+            constexpr bool SM = decltype(small_tag)::value;
+            int kk = 0, cur = -1;
+            double acc = INFINITY;
+            for (uint32_t u = lane; u < W; u += 64) {
+                while (tab.pre[kk + 1] <= u) kk++;
+                if (kk != cur) {
+                    if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
+                    cur = kk;
+                    acc = INFINITY;
+                }
+                const uint32_t e = u - tab.pre[kk];
+                const uint32_t fa_ = tab.fa[kk], fb_ = tab.fb[kk], w1_ = tab.w1[kk];
+                const bool tgt_pt = e < w1_;
+                const uint32_t e2 = tgt_pt ? e : e - w1_, m = (tgt_pt ? tab.la[kk] : tab.lb[kk]) - 1;
+                const uint32_t i = SM ? udiv_small(e2, m) : e2 / m, j = e2 - i * m;
+                const float *ptx = tgt_pt ? txv + fb_ : sxv + fa_, *pty = tgt_pt ? tyv + fb_ : syv + fa_;
+                const float *sgx = tgt_pt ? sxv + fa_ : txv + fb_, *sgy = tgt_pt ? syv + fa_ : tyv + fb_;
+                acc = fmin(acc, pt_seg_d2(ptx[i], pty[i], sgx[j], sgy[j], sgx[j + 1], sgy[j + 1], tab.km));
             }
-            const uint32_t e = u - tab.pre[kk];
-            const uint32_t fa_ = tab.fa[kk], fb_ = tab.fb[kk], w1_ = tab.w1[kk];
-            // e < w1: point i of the target sequence, segment j of the source
-            // one; else point i of the source sequence, segment j of the
-            // target one.  One distance per lane either way: the operands are
-            // selected, so lanes on both sides of w1 share one evaluation.
-            const bool tgt_pt = e < w1_;
-            const uint32_t e2 = tgt_pt ? e : e - w1_, m = (tgt_pt ? tab.la[kk] : tab.lb[kk]) - 1;
-            const uint32_t i = e2 / m, j = e2 - i * m;
-            const float *ptx = tgt_pt ? txv + fb_ : sxv + fa_, *pty = tgt_pt ? tyv + fb_ : syv + fa_;
-            const float *sgx = tgt_pt ? sxv + fa_ : txv + fb_, *sgy = tgt_pt ? syv + fa_ : tyv + fb_;
-            const double d2 = pt_seg_d2(ptx[i], pty[i], sgx[j], sgy[j], sgx[j + 1], sgy[j + 1], tab.km);
-            acc = fmin(acc, d2);
-        }
-        if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
+            if (cur >= 0) atomicMin(&tab.mins[cur], (unsigned long long)__double_as_longlong(acc));
+        };
+        if (W < (1u << 22)) sweep(std::true_type());  // every e2 below 2^22
+        else sweep(std::false_type());
         wave_sync();
         // min of the square roots = square root of the min (sqrt is monotone)
         const double dist = ok ? sqrt(__longlong_as_double((long long)tab.mins[lane])) : INFINITY;
@@ -493,22 +570,6 @@ struct CandRegs {
     uint32_t live = 0;
 };
 
-__device__ __forceinline__ int select_regs(const CandRegs &cr, int lane, Cand &out, double &ref) {
-    SelAcc acc;
-#pragma unroll
-    for (int q = 0; q < kCandRegs; q++) {
-        if ((cr.live >> q) & 1u) {
-            Cand c;
-            c.apx = cr.apx[q];
-            c.lo = cr.lo[q];
-            c.hi = cr.hi[q];
-            c.id1 = cr.id1[q];
-            c.id2 = cr.id2[q];
-            acc.add(c, q * 64 + lane);
-        }
-    }
-    return acc.finish(out, ref);
-}
 
 // Lane index of the r-th (from 0) set bit of m, for r < popcount(m).
 __device__ __forceinline__ int nth_set_bit(uint64_t m, uint32_t r) {
@@ -541,6 +602,129 @@ __device__ __forceinline__ double bperm_d(int addr, double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// One compare-exchange step of a bitonic network over the wave: element i
+// (held by this lane) meets element i ^ j (lane ^ j); the lower one keeps
+// the smaller tuple when the run is ascending ((i & kk) == 0), the larger
+// otherwise.  Equal tuples (only the +inf padding) stay where they are.
+__device__ __forceinline__ void bitonic_step(Cand &x, uint32_t i, uint32_t kk, int j) {
+    Cand p;
+    p.apx = __shfl_xor(x.apx, j, 64);
+    p.lo = __shfl_xor(x.lo, j, 64);
+    p.hi = __shfl_xor(x.hi, j, 64);
+    p.id1 = __shfl_xor(x.id1, j, 64);
+    p.id2 = __shfl_xor(x.id2, j, 64);
+    const bool up = (i & kk) == 0, low = (i & (uint32_t)j) == 0;
+    if (low == up ? tuple_less(p, x) : tuple_less(x, p)) x = p;
+}
+
+// The in-register part of a bitonic network for the 64 elements b .. b + 63
+// (lane l holds b + l): stages kk = k0 .. 64 (k0 = 2: the whole 64-element
+// sort) with their strides below 64, or, for k0 > 64, the strides 32 .. 1 of
+// stage k0.  Loops, not unrolled: the kernel's code stays small enough for
+// the instruction cache.
+__device__ __forceinline__ void bitonic_block(Cand &x, uint32_t b, uint32_t k0, int lane) {
+    const uint32_t i = b + (uint32_t)lane;
+    if (k0 > 64) {
+#pragma unroll 1
+        for (int j = 32; j > 0; j >>= 1) bitonic_step(x, i, k0, j);
+        return;
+    }
+#pragma unroll 1
+    for (uint32_t kk = k0; kk <= 64; kk <<= 1)
+#pragma unroll 1
+        for (int j = (int)kk >> 1; j > 0; j >>= 1) bitonic_step(x, i, kk, j);
+}
+
+__device__ __forceinline__ Cand cand_inf() {
+    Cand c;
+    c.apx = c.lo = c.hi = INFINITY;
+    c.id1 = c.id2 = 0xffffffffu;
+    return c;
+}
+
+// Sorts the (at most 64) register candidates into tuple order, candidate k
+// on lane k: *ids gets its pixel ids (id1 | id2 << 16, both below 2^16 on the
+// LDS path), and the return value is its reference_dist, the smallest low
+// bound among the candidates after it, at most kFar (Ideal.cpp:2611-2615).
+__device__ __forceinline__ double sort_regs(const CandRegs &cr, uint32_t nc, int lane, uint32_t *ids) {
+    Cand x = cand_inf();
+    if (cr.live & 1u) {
+        x.apx = cr.apx[0];
+        x.lo = cr.lo[0];
+        x.hi = cr.hi[0];
+        x.id1 = cr.id1[0];
+        x.id2 = cr.id2[0];
+    }
+    bitonic_block(x, 0, 2, lane);
+    *ids = (x.id1 & 0xffffu) | (x.id2 << 16);
+    // suffix minimum of the low bounds, then shifted by one lane
+    double s = (uint32_t)lane < nc ? x.lo : INFINITY;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double t = __shfl_down(s, o, 64);
+        if (lane + o < 64) s = fmin(s, t);
+    }
+    const double after = __shfl_down(s, 1, 64);
+    return fmin(kFar, lane < 63 ? after : INFINITY);
+}
+
+// Sorts the slab's n candidates (64 < n <= 2048) into tuple order with a
+// bitonic network over the next power of two (+inf padding): strides below
+// 64 in registers, a 64-element block per lane set, wider strides as
+// compare-exchanges in the slab.  Taken candidates (apx = +inf) sort after
+// the live ones.  Then the lo field of each of the first nlive entries is
+// replaced by its reference_dist (smallest low bound of the later live
+// entries, <= kFar).
+__device__ __forceinline__ void sort_slab(Cand *cand, uint32_t n, uint32_t nlive, int lane) {
+    uint32_t N = 128;
+    while (N < n) N <<= 1;
+    for (uint32_t i = n + (uint32_t)lane; i < N; i += 64) cand[i] = cand_inf();
+    wave_sync();
+    for (uint32_t b = 0; b < N; b += 64) {
+        Cand x = cand[b + lane];
+        bitonic_block(x, b, 2, lane);
+        cand[b + lane] = x;
+    }
+    wave_sync();
+    for (uint32_t kk = 128; kk <= N; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j >= 64; j >>= 1) {
+            for (uint32_t p0 = 0; p0 < N / 2; p0 += 64) {
+                const uint32_t p = p0 + (uint32_t)lane;         // compare pair p of N / 2
+                const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));  // its lower element
+                const Cand a = cand[i], b = cand[i + j];
+                const bool up = (i & kk) == 0;
+                if (up ? tuple_less(b, a) : tuple_less(a, b)) {
+                    cand[i] = b;
+                    cand[i + j] = a;
+                }
+            }
+            wave_sync();
+        }
+        for (uint32_t b = 0; b < N; b += 64) {
+            Cand x = cand[b + lane];
+            bitonic_block(x, b, kk, lane);
+            cand[b + lane] = x;
+        }
+        wave_sync();
+    }
+    // reference_dist of every live entry, from the last block to the first
+    double carry = INFINITY;  // smallest low bound past the current block
+    for (int b = (int)((nlive + 63) / 64) - 1; b >= 0; b--) {
+        const uint32_t i = (uint32_t)b * 64 + (uint32_t)lane;
+        double s = i < nlive ? cand[i].lo : INFINITY;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double t = __shfl_down(s, o, 64);
+            if (lane + o < 64) s = fmin(s, t);
+        }
+        const double after = __shfl_down(s, 1, 64);
+        const double rd = fmin(kFar, fmin(lane < 63 ? after : INFINITY, carry));
+        carry = fmin(carry, uni(__shfl(s, 0, 64)));
+        if (i < nlive) cand[i].lo = rd;
+    }
+    wave_sync();
+}
+
 // Ideal::within for one pair; the wave's buffers: lists / bits (LDS or a
 // global slab) and the candidate slab.
 template <int SH, class T>
@@ -548,6 +732,7 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                                                T *tl, uint32_t *sb, uint32_t *tb, Cand *cand,
                                                uint32_t pixcap, uint32_t candcap, SeqTab &tab, int lane) {
     PairOut r;
+    const double d_le2 = sq_le(d);  // min distance <= d <=> its square <= d_le2
     r.bit = 0;
     r.min_dist = kFar;
     r.ncand = 0;
@@ -562,7 +747,10 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         const int i0 = (ylo <= 0 && yhi >= 0) ? 0 : km_idx(fmin(fabs(ylo), fabs(yhi)));
         const int i1 = km_idx(fmax(fabs(ylo), fabs(yhi)));
         const int n = min(i1 - i0 + 1, kKmC);
-        if (lane < n) tab.km.v[lane] = c_km_lon[i0 + lane];
+        if (lane < n) {
+            tab.km.v[lane] = c_km_lon[i0 + lane];
+            tab.km.inv[lane] = c_km_lon_inv[i0 + lane];
+        }
         if (lane == 0) {
             tab.km.lo = i0;
             tab.km.n = n;
@@ -626,8 +814,9 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         if (ns == 0 || nt == 0) break;  // no BORDER pixel on a side: nothing to refine
         // pass 1: any BORDER box pair within d by its farthest corners
         // decides; else the smallest such maximum bounds the useful pairs
-        double maxd = kFar;
-        bool hit = false;
+        // (on squared distances: the smallest maximum is the root of the
+        // smallest square, and it is <= d exactly when some maximum is)
+        double mx2 = INFINITY;
         const uint64_t total = (uint64_t)ns * nt;
         r.boxes += total;
         {
@@ -638,9 +827,7 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                 double b1[4], b2[4];
                 pbox(gs, unpack_x<SH>(va), unpack_y<SH>(va), b1);
                 pbox(gt, unpack_x<SH>(vb), unpack_y<SH>(vb), b2);
-                const double mx = box_max_dist(b1, b2, tab.km);
-                hit |= (mx <= d);
-                maxd = fmin(maxd, mx);
+                mx2 = fmin(mx2, box_max_d2(b1, b2, tab.km));
                 a += q;
                 b += rr;
                 if (b >= nt) {
@@ -649,11 +836,14 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                 }
             }
         }
-        if (wave_any(hit)) {
+        const double mxmin = sqrt(wave_min(mx2));
+        if (mxmin <= d) {
             r.bit = 1;
             return r;
         }
-        maxd = wave_min(maxd);
+        const double maxd = fmin(kFar, mxmin);
+        // min distance <= d and < maxd, as bounds on the squared distance
+        const double keep2 = fmin(d_le2, sq_lt(maxd));
         r.boxes += total;
         // pass 2: keep pairs with min distance <= d and below maxd; their
         // projections are the next level's sets, or (last level) they are
@@ -679,15 +869,16 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                     double b1[4], b2[4];
                     pbox(gs, xa, ya, b1);
                     pbox(gt, xb, yb, b2);
-                    const double mn = box_min_dist(b1, b2, tab.km);
-                    if (mn <= d && mn < maxd) {
+                    const double mn2 = box_min_d2(b1, b2, tab.km);
+                    if (mn2 <= keep2) {
                         keep = true;
                         const uint32_t id1 = (uint32_t)(ya * gs.dimx + xa), id2 = (uint32_t)(yb * gt.dimx + xb);
                         if (!final_level) {
                             set_bit(sb, id1);
                             set_bit(tb, id2);
                         } else {
-                            const double hi = box_max_dist(b1, b2, tab.km);
+                            const double mn = sqrt(mn2);
+                            const double hi = sqrt(box_max_d2(b1, b2, tab.km));
                             const int pf = classify(S.areas[hs.pix_start + id1] + S.areas[ht.pix_start + id2],
                                                     sa + ta, 20);
                             c.apx = mn + (double)c_mean[pf] * (hi - mn);
@@ -770,15 +961,56 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
     if (!final_level || r.ncand == 0) return r;
 
     // refinement in tuple order
-    // candidates are taken one by one in tuple order and refined in runs of
-    // up to kBatch whose sequence pairs fit one 64-row table (a candidate
-    // with more rows runs alone, 64 rows at a time)
+    // candidates are taken in tuple order and refined in runs of up to
+    // kBatch whose sequence pairs fit one 64-row table (a candidate with more
+    // rows runs alone, 64 rows at a time).  The LDS path sorts the register
+    // candidates once, with each one's reference_dist (the smallest low bound
+    // of the later ones).  Slab candidates are selected one at a time for
+    // the first kSelectFirst (most pairs with many candidates are decided by
+    // their first few), then the rest of the slab is sorted once.  The
+    // global path selects them one at a time.
     const uint32_t nc = r.ncand;
     double min_dist = kFar;
     Cand c;
     double ref;
-    auto next = [&]() { return spilled ? select_next(cand, nc, lane, c, ref) : select_regs(cr, lane, c, ref); };
-    int slot = next();
+    uint32_t k = 0;      // sorted: position of the next candidate
+    double rref = kFar;  // sorted registers: lane k's candidate's reference_dist
+    uint32_t rids = 0;   // and its pixel ids (id1 | id2 << 16)
+    int slot;
+    uint32_t nsel = 0;          // slab candidates taken by selection (marked apx = +inf)
+    bool slab_sorted = false;
+    if constexpr (SH == 8) {
+        if (!spilled) rref = sort_regs(cr, nc, lane, &rids);
+    }
+    auto next = [&]() -> int {
+        if constexpr (SH == 8) {
+            if (spilled && !slab_sorted) {
+                if (nsel < kSelectFirst) return select_next(cand, nc, lane, c, ref);
+                sort_slab(cand, nc, nc - nsel, lane);
+                slab_sorted = true;
+            }
+            if (k >= nc - nsel) return -1;
+            if (!spilled) {
+                const int kk = (int)k;
+                const uint32_t ids = (uint32_t)__builtin_amdgcn_readlane((int)rids, kk);
+                c.id1 = ids & 0xffffu;
+                c.id2 = ids >> 16;
+                const long long rb = __double_as_longlong(rref);
+                const int lo = __builtin_amdgcn_readlane((int)(rb & 0xffffffffll), kk);
+                const int hi = __builtin_amdgcn_readlane((int)(rb >> 32), kk);
+                ref = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+            } else {
+                const Cand cc = cand[k];  // the same address on every lane
+                c.id1 = uni(cc.id1);
+                c.id2 = uni(cc.id2);
+                ref = uni(cc.lo);  // sort_slab stored reference_dist here
+            }
+            return (int)k;
+        } else {
+            return select_next(cand, nc, lane, c, ref);
+        }
+    };
+    slot = next();
     while (slot >= 0) {
         CandBatch cb;
         bool open = true;
@@ -793,16 +1025,18 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                 } else {
                     cb.s0[q] = s0;
                     cb.t0[q] = t0;
+                    cb.n1[q] = s1 - s0;
                     cb.n2[q] = t1 - t0;
                     cb.ref[q] = ref;
                     cb.row0[q] = cb.nrows;
                     cb.nrows += rows;
                     cb.n = q + 1;
-                    if (spilled) {
+                    if (SH == 8 && (!spilled || slab_sorted)) {
+                        k++;
+                    } else {
                         if (lane == 0) cand[slot].apx = INFINITY;
                         wave_sync();
-                    } else if ((slot & 63) == lane) {
-                        cr.live &= ~(1u << (slot >> 6));
+                        nsel++;
                     }
                     slot = next();
                     if (cb.nrows >= 64) open = false;
@@ -923,6 +1157,9 @@ int within_consts(farm_handle h) {
     static const float mean[20] = {0.0f,  0.7f,  0.62f, 0.55f, 0.52f, 0.45f, 0.42f, 0.4f,  0.38f, 0.34f,
                                    0.31f, 0.28f, 0.25f, 0.22f, 0.18f, 0.15f, 0.1f,  0.05f, 0.0f,  0.0f};
     HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_km_lon), km_lon_table(), sizeof(double) * 900));
+    double inv[900];
+    for (int i = 0; i < 900; i++) inv[i] = 1.0 / km_lon_table()[i];
+    HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_km_lon_inv), inv, sizeof inv));
     HIPCHK(h, hipMemcpyToSymbol(HIP_SYMBOL(c_mean), mean, sizeof mean));
     h->within_consts_ready = true;
     return FARM_OK;
