@@ -647,7 +647,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     // used).  q >= Q, the L1 reach from any vertex of the lane's
                     // edges to the chunk box, is taken from the box of those
                     // vertices.  If every box point p has
-                    // |a.cross(b, p)| > M_lin = 2^-19 q |b - a|_1 in exact
+                    // |a.cross(b, p)| > M_lin = 2^-18 q |b - a|_1 in exact
                     // arithmetic, the reference's f32 a.cross(b, p) has one
                     // nonzero sign on the whole chunk (rounding <= 3 2^-24 Q
                     // |b - a|_1), so its sign test is false for every segment of
@@ -657,36 +657,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     // lane's vertices are apart on an axis; otherwise it needs a
                     // and b within rounding of line cd, which puts c within
                     // 2^-20.7 Q (|b - a|_1 + 2Q) of line ab, and the margin is
-                    // M = 2^-19 q (|b - a|_1 + 2q).  Both margins get 2^-80 added.
-                    // The f32 evaluation below (the centre's cross, its spread
-                    // over the box plus M in two fmas) errs by a few
-                    // 2^-24 q |b - a|_1, covered by M.  (A chunk with a segment of
-                    // length 0 has an infinite box and is never pruned.)
+                    // M = 2^-18 q (|b - a|_1 + 2q).  Both margins get 2^-80 added.
+                    // The cross product at the box centre is evaluated relative
+                    // to the lane's first vertex o: C = s.cross(a - o) per edge
+                    // (s = b - a), then s.cross(c - o) - C in two fmas; with the
+                    // spread over the box and M in two more fmas the f32
+                    // evaluation errs by at most about 13 2^-24 q |b - a|_1
+                    // (every operand is within 2q of o), which M covers twice
+                    // over.  (A chunk with a segment of length 0 has an
+                    // infinite box and is never pruned.)
                     // this lane's slots again, from LDS (not kept in registers
                     // across the evaluation of queued items)
-                    float spx[E], spy[E], sbx[E], sby[E], sab[E];
+                    float sbx[E], sby[E], scc[E];
                     float lxl, lxh, lyl, lyh;  // box of the lane's slot vertices
+                    const float ox = src[lane][0], oy = src[lane][E + 1];
                     {
-                        float nx = src[lane][0], ny = src[lane][E + 1];
+                        float nx = ox, ny = oy;
                         lxl = lxh = nx;
                         lyl = lyh = ny;
 #pragma unroll
                         for (int e = 0; e < E; e++) {
-                            spx[e] = nx;
-                            spy[e] = ny;
+                            const float ax = nx, ay = ny;
                             nx = src[lane][e + 1];
                             ny = src[lane][E + 2 + e];
                             lxl = fminf(lxl, nx);
                             lxh = fmaxf(lxh, nx);
                             lyl = fminf(lyl, ny);
                             lyh = fmaxf(lyh, ny);
-                            sbx[e] = nx - spx[e];
-                            sby[e] = ny - spy[e];
-                            // a slot without an edge gets sab = -inf: its margin and
-                            // threshold are -inf and the box is "pruned" for it (an
-                            // infinite box makes the threshold NaN instead, which
-                            // queues the item; the evaluation skips the slot)
-                            sab[e] = ((evb >> e) & 1u) ? fabsf(sbx[e]) + fabsf(sby[e]) : -INFINITY;
+                            const bool ve = (evb >> e) & 1u;
+                            // a slot without an edge gets s = 0 and C = +inf: its
+                            // centre cross is -inf and the box is "pruned" for it
+                            // (an infinite box makes the threshold NaN instead,
+                            // which queues the item; the evaluation skips the slot)
+                            sbx[e] = ve ? nx - ax : 0.f;
+                            sby[e] = ve ? ny - ay : 0.f;
+                            const float axo = ax - ox, ayo = ay - oy;
+                            scc[e] = ve ? fmaf(sbx[e], ayo, -(sby[e] * axo)) : INFINITY;
                         }
                     }
                     bool need[kRefScreen];
@@ -702,17 +708,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                          (bb.z + bb.w)) * 1.000001f;
                         const float hx1 = bb.z * 1.000001f, hy1 = bb.w * 1.000001f;
                         const bool apart = (ex1 > hx1) | (-ex0 > hx1) | (ey1 > hy1) | (-ey0 > hy1);
-                        const float k1 = q * 0x1p-19f;
                         // (without the prune: the largest finite margin keeps every
                         // chunk for every slot with an edge)
-                        const float k2 = !prune ? 0x1.fffffep127f : (apart ? 0x1p-80f : fmaf(q, q * 0x1p-18f, 0x1p-80f));
+                        const float k1 = !prune ? 0x1.fffffep127f : q * 0x1p-18f;
+                        const float k2 = !prune ? 0x1.fffffep127f : (apart ? 0x1p-80f : fmaf(q, q * 0x1p-17f, 0x1p-80f));
+                        // |s_y| hx + |s_x| hy + (|s_x| + |s_y|) k1 + k2 as
+                        // |s_y| (hx + k1) + |s_x| (hy + k1) + k2
+                        const float hxk = bb.z + k1, hyk = bb.w + k1;
+                        const float cxo = bb.x - ox, cyo = bb.y - oy;
                         need[i] = false;
 #pragma unroll
                         for (int e = 0; e < E; e++) {
-                            const float dxc = bb.x - spx[e], dyc = bb.y - spy[e];
-                            const float oc = fmaf(sbx[e], dyc, -(sby[e] * dxc));  // at the box centre
-                            const float m = fmaf(sab[e], k1, k2);                 // M
-                            const float thr = fmaf(fabsf(sby[e]), bb.z, fmaf(fabsf(sbx[e]), bb.w, m));
+                            const float oc = fmaf(sbx[e], cyo, -fmaf(sby[e], cxo, scc[e]));  // at the box centre
+                            const float thr = fmaf(fabsf(sby[e]), hxk, fmaf(fabsf(sbx[e]), hyk, k2));
                             need[i] = need[i] | !(fabsf(oc) > thr);  // bitwise: no branch per edge
                         }
                     }

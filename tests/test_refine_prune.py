@@ -2,7 +2,7 @@
 
 k_refine (farm_amd/csrc/hip/intersect_kernels.hip) skips a source edge (a, b)
 against a chunk of up to 16 target segments when the chunk's box lies on one
-side of line ab by more than M = 2^-19 q |b - a|_1 (plus 2^-18 q^2 when the
+side of line ab by more than M = 2^-18 q |b - a|_1 (plus 2^-17 q^2 when the
 chunk box overlaps the box of the lane's edge vertices), q >= the L1 reach
 from those vertices to the chunk box.  The skip is only
 allowed if the reference predicate (segment_intersect,
@@ -41,22 +41,27 @@ def fmaf(a, b, c) -> np.float32:
     return f32(np.float64(a) * np.float64(b) + np.float64(c))
 
 
-def pruned_at(a, b, box, k: int = 19, tile=None) -> bool:
+def pruned_at(a, b, box, k: int = 18, tile=None) -> bool:
     """The kernel's per-edge prune decision, in f32, with its margin scaled to
-    2^-k (the kernel uses k = 19).  The reach q is taken from the box of the
+    2^-k (the kernel uses k = 18).  The reach q is taken from the box of the
     vertices of the lane's edges (default: the edge's own two vertices, the
     smallest such box and so the most aggressive prune); the margin is
     2^-k q |b - a|_1 when that box and the chunk box are apart on an axis
     (the reference's collinear branch needs their exact overlap), else
-    2^-k q (|b - a|_1 + 2q)."""
+    2^-k q (|b - a|_1 + 2q).  Cross products are taken relative to the
+    lane's first vertex o (tile[0]): C = s.cross(a - o) per edge, then
+    s.cross(centre - o) - C."""
     cx, cy, hx, hy = box
     ax, ay = f32(a[0]), f32(a[1])
     bx_, by_ = f32(b[0]), f32(b[1])
     pts = np.array([[ax, ay], [bx_, by_]] if tile is None else tile, f32)
+    ox, oy = pts[0]
     lxl, lxh = pts[:, 0].min(), pts[:, 0].max()
     lyl, lyh = pts[:, 1].min(), pts[:, 1].max()
     bax, bay = f32(bx_ - ax), f32(by_ - ay)
     with np.errstate(all="ignore"):
+        axo, ayo = f32(ax - ox), f32(ay - oy)
+        cc = fmaf(bax, ayo, -f32(bay * axo))
         ex0, ex1 = f32(cx - lxl), f32(cx - lxh)
         ey0, ey1 = f32(cy - lyl), f32(cy - lyh)
         q = f32(f32(f32(max(abs(ex0), abs(ex1)) + max(abs(ey0), abs(ey1))) + f32(hx + hy)) * f32(1.000001))
@@ -64,11 +69,10 @@ def pruned_at(a, b, box, k: int = 19, tile=None) -> bool:
         apart = (ex1 > hx1) or (-ex0 > hx1) or (ey1 > hy1) or (-ey0 > hy1)
         k1 = f32(q * f32(2.0 ** -k))
         k2 = f32(2.0 ** -80) if apart else fmaf(q, f32(q * f32(2.0 ** (1 - k))), 2.0 ** -80)
-        dxc, dyc = f32(cx - ax), f32(cy - ay)
-        oc = fmaf(bax, dyc, -f32(bay * dxc))
-        sab = f32(abs(bax) + abs(bay))
-        m = fmaf(sab, k1, k2)
-        thr = fmaf(abs(bay), hx, fmaf(abs(bax), hy, m))
+        hxk, hyk = f32(hx + k1), f32(hy + k1)
+        cxo, cyo = f32(cx - ox), f32(cy - oy)
+        oc = fmaf(bax, cyo, -fmaf(bay, cxo, cc))
+        thr = fmaf(abs(bay), hxk, fmaf(abs(bax), hyk, k2))
         return bool(abs(oc) > thr)
 
 
@@ -211,7 +215,7 @@ def test_constructed_rounding_crossings_need_the_margin():
     # the same side.  Where the reference's binary32 a.cross(b, c) rounds to
     # 0 or to the wrong sign it reports a crossing (a and b are on opposite
     # sides of line cd), so the prune must keep the chunk.  The kernel's
-    # margin (2^-19) keeps every such chunk, even with q taken from a alone;
+    # margin (2^-18) keeps every such chunk, even with q taken from a alone;
     # a margin of 2^-27 already drops some, so this test fails for a margin
     # shrunk that far.
     rng = np.random.default_rng(21)
@@ -239,8 +243,37 @@ def test_constructed_rounding_crossings_need_the_margin():
         # q from a alone: the smallest reach any tile holding this edge can
         # give, so the most aggressive prune the kernel could make
         point = np.array([a, a], f32)
-        assert not pruned_at(a, b, box, 19), (a, b, c, d)
-        assert not pruned_at(a, b, box, 19, tile=point), (a, b, c, d)
+        assert not pruned_at(a, b, box, 18), (a, b, c, d)
+        assert not pruned_at(a, b, box, 18, tile=point), (a, b, c, d)
         dropped_small += pruned_at(a, b, box, 27, tile=point)
     assert ref_true > 200  # the construction reaches the rounding regime
     assert dropped_small > 0  # and a too-small margin would drop reported crossings
+
+
+def test_lane_tiles_with_the_origin_away_from_the_edge():
+    # the kernel takes cross products relative to the lane's first vertex:
+    # every edge of an 8-edge lane tile (not only the first) against chunks
+    # near and on its line, at coordinates up to 1e3 from the origin of the
+    # plane; a prune must never hide a reference crossing
+    rng = np.random.default_rng(31)
+    fired = 0
+    for _ in range(1500):
+        centre = rng.uniform(-1e3, 1e3, 2)
+        lane = polyline(rng, 9, centre, 0.01)
+        e = rng.integers(8)
+        a, b = lane[e], lane[e + 1]
+        if rng.random() < 0.5:  # chunk along the edge's line, a few ulps off
+            d = b.astype(np.float64) - a
+            t = np.sort(rng.uniform(-3, 3, 17))
+            chunk = (a + t[:, None] * d).astype(f32)
+            jig = rng.integers(-1, 2, size=chunk.shape)
+            chunk = np.where(jig != 0, np.nextafter(chunk, np.where(jig > 0, np.inf, -np.inf).astype(f32)),
+                             chunk).astype(f32)
+        else:
+            chunk = polyline(rng, 17, a + rng.normal(0, 0.03, 2), 0.005)
+        box = chunk_box(chunk[:, 0], chunk[:, 1])
+        if pruned_at(a, b, box, tile=lane):
+            fired += 1
+            for k in range(16):
+                assert not oracle.segment_intersect(a, b, chunk[k], chunk[k + 1]), (a, b, chunk[k], chunk[k + 1])
+    assert fired > 200
