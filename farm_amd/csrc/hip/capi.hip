@@ -30,10 +30,11 @@ __global__ void k_bucket_scatter(const uint8_t *, const unsigned long long *, ui
 template <int E>
 __global__ void k_tiles(const PixPair *, const uint32_t *, unsigned long long *, uint64_t, uint32_t *);
 template <int E, int C>
-__global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, const PixPair *,
-                         const uint32_t *, const uint2 *, uint64_t, uint64_t, uint32_t *, uint32_t *,
-                         unsigned long long *, int);
-__global__ void k_tile_map(const uint32_t *, const unsigned long long *, uint64_t, uint64_t, uint2 *);
+__global__ void k_refine(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, const TileDesc *, uint64_t,
+                         uint64_t, uint32_t *, uint32_t *, unsigned long long *, int);
+__global__ void k_tile_map(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *, const PixPair *,
+                           const uint32_t *, const uint32_t *, const unsigned long long *, uint64_t, uint64_t,
+                           TileDesc *);
 __global__ void k_plain(DevSet, const uint32_t *, const uint32_t *, uint32_t, uint32_t *, unsigned long long *);
 __global__ void k_pixedges(DevSet, int64_t, uint32_t *);
 __global__ void k_vdiff(const float *, const float *, uint64_t, float4 *);
@@ -173,7 +174,8 @@ int isx_front(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c)
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = scan_dev(h, W, W.tiles, &W.ctr[kCtrPixPairs], W.cap_pp, W.tile_off))) return rc;
-    hipLaunchKernelGGL(k_tile_map, dim3(g), dim3(256), 0, h->stream, (const uint32_t *)W.tile_off,
+    hipLaunchKernelGGL(k_tile_map, dim3(g), dim3(256), 0, h->stream, S, c.pa, c.pb, (const IsxDesc *)W.desc,
+                       (const PixPair *)W.pps, (const uint32_t *)W.order, (const uint32_t *)W.tile_off,
                        (const unsigned long long *)W.ctr, W.cap_pp, W.cap_tiles, W.tile_map);
     HIPCHK(h, hipGetLastError());
     return fill_u32(h, W.cursor + 63, 1, 0);  // the refinement's tile cursor
@@ -184,8 +186,8 @@ template <int E, int C>
 void launch_refine(farm_handle h, const DevSet &S, const IsxWork &W, const IsxCfg &c) {
     const dim3 grid((unsigned)(h->num_cus * c.bpc)), blk(256);
     hipLaunchKernelGGL((k_refine<E, C>), grid, blk, 0, h->stream, S, c.pa, c.pb, (const IsxDesc *)W.desc,
-                       (const PixPair *)W.pps, (const uint32_t *)W.order, (const uint2 *)W.tile_map, W.cap_pp,
-                       W.cap_tiles, W.cursor + 63, W.result, W.ctr, c.prune);
+                       (const TileDesc *)W.tile_map, W.cap_pp, W.cap_tiles, W.cursor + 63, W.result, W.ctr,
+                       c.prune);
 }
 
 template <int E>
@@ -283,7 +285,7 @@ int isx_reserve(farm_handle h, uint64_t pairs, IsxWork *W) {
 
 // Fixed scratch of farm_intersect_noalloc: at most kNaPairs pairs,
 // kNaPixPairs BORDER x BORDER pixel pairs and kNaTiles refinement tiles per
-// call (about 42 MiB, held by every GPU the library runs on until the
+// call (about 80 MiB, held by every GPU the library runs on until the
 // process ends).  Host code never passes the array's name to a kernel: it
 // uses the address hipGetSymbolAddress returns for the current device.
 constexpr uint64_t kNaPairs = 1ull << 18, kNaPixPairs = 1ull << 20, kNaTiles = 1ull << 20;

@@ -331,16 +331,42 @@ __global__ __launch_bounds__(256) void k_tiles(const PixPair *__restrict__ pps, 
     if ((threadIdx.x & 63) == 0 && t) atomicAdd(&ctr[kCtrTiles], t);
 }
 
-// (ordered pixel pair, tile within it) of every tile, from the scanned
-// counts; nothing is written when the tiles exceed the map's capacity
-__global__ __launch_bounds__(256) void k_tile_map(const uint32_t *__restrict__ tile_off,
+// The descriptor of every tile (its ordered pixel pair, the tile within it,
+// the two pixels' sequence ranges and polygons), from the scanned counts;
+// nothing is written when the tiles exceed the map's capacity
+__global__ __launch_bounds__(256) void k_tile_map(DevSet S, const uint32_t *__restrict__ pa_all,
+                                                  const uint32_t *__restrict__ pb_all,
+                                                  const IsxDesc *__restrict__ desc, const PixPair *__restrict__ pps,
+                                                  const uint32_t *__restrict__ order,
+                                                  const uint32_t *__restrict__ tile_off,
                                                   const unsigned long long *__restrict__ ctr, uint64_t cap_pp,
-                                                  uint64_t cap_tiles, uint2 *__restrict__ map) {
+                                                  uint64_t cap_tiles, TileDesc *__restrict__ map) {
     const uint64_t npp = bounded_count(&ctr[kCtrPixPairs], cap_pp);
     if (ctr[kCtrTiles] > cap_tiles) return;
+    const uint32_t *pa_ids = pa_all + desc->begin;
+    const uint32_t *pb_ids = pb_all + desc->begin;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npp; q += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t t0 = tile_off[q], t1 = tile_off[q + 1];
-        for (uint32_t t = t0; t < t1; t++) map[t] = make_uint2((uint32_t)q, t - t0);
+        if (t0 == t1) continue;
+        const PixPair P = pps[order[q]];
+        const farm_raster_hdr &A = S.hdr[pa_ids[P.pair]];
+        const farm_raster_hdr &B = S.hdr[pb_ids[P.pair]];
+        TileDesc d;
+        d.pair = P.pair;
+        d.a_s0 = A.seq_start + S.pixoff[A.pixoff_start + P.pa];
+        d.a_s1 = A.seq_start + S.pixoff[A.pixoff_start + P.pa + 1];
+        d.b_s0 = B.seq_start + S.pixoff[B.pixoff_start + P.pb];
+        d.b_s1 = B.seq_start + S.pixoff[B.pixoff_start + P.pb + 1];
+        d.a_vert = A.vert_start;
+        d.b_vert = B.vert_start;
+        // the target pixel's sequences once: offsets, sequence entries, vertices
+        const uint32_t nb = d.b_s1 - d.b_s0;
+        d.rbytes = 16 + 8u * nb + 8u * (S.pixedges[B.pix_start + P.pb] + nb);
+        d.pad[0] = d.pad[1] = d.pad[2] = 0;
+        for (uint32_t t = t0; t < t1; t++) {
+            d.tile = t - t0;
+            map[t] = d;
+        }
     }
 }
 
@@ -424,8 +450,7 @@ template <int W, int C>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_refine(DevSet S, const uint32_t *__restrict__ pa_all,
                                                  const uint32_t *__restrict__ pb_all,
                                                  const IsxDesc *__restrict__ desc,
-                                                 const PixPair *__restrict__ pps, const uint32_t *__restrict__ order,
-                                                 const uint2 *__restrict__ tile_map, uint64_t cap_pp,
+                                                 const TileDesc *__restrict__ tile_map, uint64_t cap_pp,
                                                  uint64_t cap_tiles, uint32_t *__restrict__ cursor,
                                                  uint32_t *__restrict__ result, unsigned long long *__restrict__ ctr,
                                                  int prune) {
@@ -433,8 +458,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // host then grows the workspace and repeats the call, or fails it)
     if (ctr[kCtrPixPairs] > cap_pp || ctr[kCtrTiles] > cap_tiles) return;
     const uint32_t total = (uint32_t)ctr[kCtrTiles];
-    const uint32_t *pa_ids = pa_all + desc->begin;
-    const uint32_t *pb_ids = pb_all + desc->begin;
     uint32_t *pair_tests = desc->pair_tests;
     // per wave: the tile's source slots (x then y, W + 1 each) and edge bits
     // of every lane, and the ring of queued items (first vertex of the target
@@ -460,27 +483,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (lane == 0) t = atomicAdd(cursor, 1u);
         t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
         if (t >= total) break;
-        // ordered pixel pair owning tile t, and the tile's place in it
-        const uint2 tm = tile_map[t];
-        const uint32_t lo = __builtin_amdgcn_readfirstlane(tm.x);
-        const uint32_t tile = __builtin_amdgcn_readfirstlane(tm.y);
-        const PixPair P = pps[__builtin_amdgcn_readfirstlane(order[lo])];
-        const uint32_t pair = __builtin_amdgcn_readfirstlane(P.pair);
+        // the tile's descriptor (one uniform load), then its pair's flag
+        const TileDesc td = tile_map[t];
+        const uint32_t pair = __builtin_amdgcn_readfirstlane(td.pair);
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(td.tile);
         if (__hip_atomic_load(&result[pair], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
         items++;
-        const farm_raster_hdr A = S.hdr[pa_ids[pair]];
-        const farm_raster_hdr B = S.hdr[pb_ids[pair]];
-        const uint32_t *pofa = S.pixoff + A.pixoff_start;
-        const uint32_t *pofb = S.pixoff + B.pixoff_start;
-        const uint2 *sqa = S.seqs + A.seq_start;
-        const uint2 *sqb = S.seqs + B.seq_start;
-        const float *ax_ = S.x + A.vert_start;
-        const float *ay_ = S.y + A.vert_start;
-        const float4 *bq_ = S.xyd + B.vert_start;
-        const uint32_t *bchoff = S.choff + B.seq_start;
-        const uint32_t a_s0 = pofa[P.pa], a_s1 = pofa[P.pa + 1];
-        const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(pofb[P.pb]);
-        const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(pofb[P.pb + 1]);
+        const uint2 *sqa = S.seqs;  // absolute sequence indices
+        const uint2 *sqb = S.seqs;
+        const float *ax_ = S.x + __builtin_amdgcn_readfirstlane(td.a_vert);
+        const float *ay_ = S.y + __builtin_amdgcn_readfirstlane(td.a_vert);
+        const float4 *bq_ = S.xyd + __builtin_amdgcn_readfirstlane(td.b_vert);
+        const uint32_t *bchoff = S.choff;
+        const uint32_t a_s0 = __builtin_amdgcn_readfirstlane(td.a_s0), a_s1 = __builtin_amdgcn_readfirstlane(td.a_s1);
+        const uint32_t b_s0 = __builtin_amdgcn_readfirstlane(td.b_s0);
+        const uint32_t b_s1 = __builtin_amdgcn_readfirstlane(td.b_s1);
         // the tile's body, invoked once below (as a lambda it is allocated
         // to fewer registers than as a plain block: 128 against 155 VGPRs at
         // W = 8, four waves per SIMD instead of three)
@@ -490,7 +507,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const uint32_t first = tile * 64u * W + (uint32_t)lane * E;
             float px[E + 1], py[E + 1];
             bool ev[E];
-            {
+            const uint32_t nsq = a_s1 - a_s0;
+            if (nsq <= 64) {
+                // every lane loads one of the pixel's sequences and its first
+                // slot (a wave prefix sum of len + 1) into the wave's LDS slot
+                // array (free until this tile's slots are published); a slot's
+                // sequence is then found by a binary search there and the
+                // lane's later slots by stepping on (LDS reads only, so lanes
+                // may diverge)
+                const uint2 mine = (uint32_t)lane < nsq ? sqa[a_s0 + lane] : make_uint2(0u, 0u);
+                uint32_t nslots;
+                const uint32_t sstart = wave_excl_scan((uint32_t)lane < nsq ? mine.y + 1 : 0u, lane, &nslots);
+                src[lane][0] = __uint_as_float(mine.x);  // (bit copies: the array holds floats)
+                src[lane][1] = __uint_as_float(mine.y);
+                src[lane][2] = __uint_as_float(sstart);
+                wave_sync();
+                auto ent = [&](int j, int f) { return __float_as_uint(src[j][f]); };
+                int k = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int c = k + step;
+                    if (c < (int)nsq && ent(c, 2) <= first) k = c;
+                }
+                uint32_t sbase = ent(k, 2);
+                uint2 sq = make_uint2(ent(k, 0), ent(k, 1));
+                for (int m = 0; m <= E; m++) {
+                    const uint32_t s = first + m;
+                    while (k + 1 < (int)nsq && s >= sbase + sq.y + 1) {
+                        sbase += sq.y + 1;
+                        k++;
+                        sq = make_uint2(ent(k, 0), ent(k, 1));
+                    }
+                    uint32_t o = s - sbase;
+                    const bool inside = s < sbase + sq.y + 1;
+                    if (!inside) o = sq.y;  // past the last slot: clamp (no edge)
+                    const uint32_t v = sq.x + o;
+                    px[m] = ax_[v];
+                    py[m] = ay_[v];
+                    if (m < E) ev[m] = inside && o < sq.y;
+                }
+                wave_sync();  // every lane's reads done before the slots overwrite the array
+            } else {
                 uint32_t k = a_s0, sbase = 0;
                 uint2 sq = sqa[k];
                 for (int m = 0; m <= E; m++) {
@@ -760,12 +817,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 if (lane == 0) __hip_atomic_fetch_or(&result[pair], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
 
-            if (lane == 0) {
-                // algorithmic refine bytes of this tile: its source vertices plus
-                // the target pixel's sequences once
-                const uint32_t nb = b_s1 - b_s0;
-                rbytes += 16 + 8u * nb + 8u * (S.pixedges[B.pix_start + P.pb] + nb);
-            }
+            // algorithmic refine bytes of this tile: its source vertices plus
+            // the target pixel's sequences once
+            if (lane == 0) rbytes += __builtin_amdgcn_readfirstlane(td.rbytes);
             rbytes += (uint32_t)nvalid * 8u;
             // tests of this lane's edges the prune skipped: every screened step
             // that was not queued
@@ -1003,8 +1057,8 @@ __global__ __launch_bounds__(256) void k_finalize(const uint32_t *__restrict__ r
                                          uint32_t *);
 #define FARM_REFINE_INST(EE, CC)                                                                               \
     template __global__ void k_refine<EE, CC>(DevSet, const uint32_t *, const uint32_t *, const IsxDesc *,      \
-                                              const PixPair *, const uint32_t *, const uint2 *, uint64_t,       \
-                                              uint64_t, uint32_t *, uint32_t *, unsigned long long *, int);
+                                              const TileDesc *, uint64_t, uint64_t, uint32_t *, uint32_t *,     \
+                                              unsigned long long *, int);
 FARM_TILES_INST(1)
 FARM_TILES_INST(2)
 FARM_TILES_INST(4)

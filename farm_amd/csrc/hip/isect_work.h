@@ -38,6 +38,21 @@ enum : int {
 // Scan partials: one per block of the three-kernel scan.
 constexpr int kScanParts = 1024;
 
+// What the refinement needs of one tile, gathered by k_tile_map so that the
+// persistent kernel starts a tile with one load instead of a chain of
+// dependent ones (order -> pixel pair -> pair ids -> headers -> pixel
+// offsets).  Sequence indices are absolute (into DevSet::seqs / choff).
+struct TileDesc {
+    uint32_t pair;     // index in the batch
+    uint32_t tile;     // tile number within its pixel pair
+    uint32_t a_s0, a_s1;  // source pixel's edge sequences
+    uint32_t b_s0, b_s1;  // target pixel's edge sequences
+    uint32_t a_vert, b_vert;  // first vertex of the source / target polygon
+    uint32_t rbytes;   // algorithmic refine bytes of the tile's target side
+    uint32_t pad[3];
+};
+static_assert(sizeof(TileDesc) == 48, "tile descriptor layout");
+
 struct IsxWork {
     IsxDesc *desc;
     unsigned long long *ctr;  // kCtrSlots
@@ -53,7 +68,7 @@ struct IsxWork {
     uint32_t *order;          // cap_pp
     uint32_t *tiles;          // cap_pp
     uint32_t *tile_off;       // cap_pp + 1
-    uint2 *tile_map;          // cap_tiles
+    TileDesc *tile_map;       // cap_tiles
     uint64_t cap_pairs = 0, cap_pp = 0, cap_tiles = 0;
 };
 
@@ -65,7 +80,7 @@ constexpr uint64_t isx_bytes(uint64_t cap_pairs, uint64_t cap_pp, uint64_t cap_t
            isx_align(4ull * (kScanParts + 1)) + isx_align(4ull * cap_pairs) + isx_align(4ull * cap_pairs) +
            isx_align(4ull * (cap_pairs + 1)) + isx_align(cap_pairs) + isx_align(sizeof(PixPair) * cap_pp) +
            isx_align(cap_pp) + isx_align(4ull * cap_pp) + isx_align(4ull * cap_pp) + isx_align(4ull * (cap_pp + 1)) +
-           isx_align(8ull * cap_tiles);
+           isx_align(sizeof(TileDesc) * cap_tiles);
 }
 
 inline IsxWork isx_carve(void *base, uint64_t cap_pairs, uint64_t cap_pp, uint64_t cap_tiles) {
@@ -90,7 +105,7 @@ inline IsxWork isx_carve(void *base, uint64_t cap_pairs, uint64_t cap_pp, uint64
     w.order = (uint32_t *)take(4ull * cap_pp);
     w.tiles = (uint32_t *)take(4ull * cap_pp);
     w.tile_off = (uint32_t *)take(4ull * (cap_pp + 1));
-    w.tile_map = (uint2 *)take(8ull * cap_tiles);
+    w.tile_map = (TileDesc *)take(sizeof(TileDesc) * cap_tiles);
     w.cap_pairs = cap_pairs;
     w.cap_pp = cap_pp;
     w.cap_tiles = cap_tiles;

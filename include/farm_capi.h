@@ -275,7 +275,7 @@ int farm_intersect_dev(farm_handle h, uint64_t begin, uint64_t end, const farm_o
 /* farm_intersect_dev without any device allocation, for callers that cannot
  * call hipMalloc (code under HIP graph capture, tight serving loops,
  * harnesses that forbid allocation).  No reference counterpart.  The
- * per-call scratch is a fixed device array of the library (about 42 MiB on
+ * per-call scratch is a fixed device array of the library (about 80 MiB on
  * each GPU the library runs on, held until the process ends): at most
  * 2^18 pairs, 2^20 BORDER x BORDER pixel pairs and 2^20 refinement tiles
  * per call, else FARM_ECAPACITY (out_dev then holds no valid results).
