@@ -25,14 +25,16 @@
 //     divisions by the km table entries, the latitude constant and the
 //     raster steps go through precomputed reciprocals with two exact fma
 //     corrections (div_rcp), which give the same rounded quotient.
-//  3. candidates (apx, low, high, id1, id2) of the last level are handed to
-//     the lanes by lane permutes as they are found, candidate k on lane k, up
-//     to 64 of them, and sorted there once into tuple order (the order of
-//     std::sort over the tuples) with each one's reference_dist by a suffix
-//     minimum; a pair with more keeps them in a per-wave global slab,
-//     selects the first kSelectFirst by argmin (most such pairs are decided
-//     by their first few) and then sorts the rest of the slab once.  The
-//     refinement stops at the reference's first return.
+//  3. candidates (apx, low, high, id1, id2) of the last level are generated
+//     in passes over the last level's pixel pairs: each pass keeps the 64
+//     smallest tuples after a bound (staged in LDS, merged into a sorted set
+//     of 64 by bitonic networks, candidate k on lane k) and only the low
+//     bound of the rest; the refinement takes the 64 in tuple order (the
+//     order of std::sort over the tuples) with each one's reference_dist
+//     from a suffix minimum, and a further pass follows only when all 64 are
+//     refined without a decision.  Nothing is written to global memory.  (A
+//     pair whose layers exceed the LDS lists runs on the global path, with
+//     its candidates in a per-wave slab, taken by argmin selection.)
 //  4. the truncated point-segment distances of one edge-sequence pair are
 //     spread over the lanes; the minimum is taken on the squared distances
 //     and one square root follows (sqrt is monotone and correctly rounded).
@@ -63,8 +65,6 @@ constexpr double kFar = 100000.0;           // the reference's "no distance yet"
 constexpr int kWaves = 4;                   // wavefronts per workgroup
 constexpr int kPixMax = 512;                // pixels of a layer handled in LDS
 constexpr int kBitWords = kPixMax / 32;
-constexpr uint32_t kCandLds = 2048;         // candidate slab of a wave in the LDS path
-constexpr int kCandRegs = 1;                // last-level candidates per lane held in registers
 constexpr uint32_t kChunk = 8;              // pairs per queue grab
 
 struct Cand {
@@ -361,10 +361,10 @@ struct SeqTab {
     uint32_t fa[64], la[64], fb[64], lb[64], w1[64], pre[65];
     unsigned long long mins[64];  // squared distance bits (non-negative doubles order as integers)
     KmC km;                       // the km table entries of the current pair
+    Cand stage[64];               // last-level candidates waiting to be merged into the best 64
 };
 
 constexpr int kBatch = 4;  // candidates refined together (their sequence pairs share one table)
-constexpr uint32_t kSelectFirst = 8;  // slab candidates selected one by one before the slab is sorted
 
 // A run of candidates taken in tuple order: pixel-pair sequence ranges and
 // each one's reference_dist.  All values are uniform across the wave.
@@ -562,36 +562,8 @@ __device__ __forceinline__ int select_next(const Cand *cand, uint32_t n, int lan
     return acc.finish(out, ref);
 }
 
-// Last-level candidates held in registers: slot q of lane l is candidate
-// q * 64 + l, live while bit q of `live` is set.
-struct CandRegs {
-    double apx[kCandRegs], lo[kCandRegs], hi[kCandRegs];
-    uint32_t id1[kCandRegs], id2[kCandRegs];
-    uint32_t live = 0;
-};
 
 
-// Lane index of the r-th (from 0) set bit of m, for r < popcount(m).
-__device__ __forceinline__ int nth_set_bit(uint64_t m, uint32_t r) {
-    uint32_t w = (uint32_t)m;
-    int base = 0;
-    const uint32_t c32 = (uint32_t)__popc(w);
-    if (r >= c32) {
-        r -= c32;
-        w = (uint32_t)(m >> 32);
-        base = 32;
-    }
-#pragma unroll
-    for (int sh = 16; sh > 0; sh >>= 1) {
-        const uint32_t c = (uint32_t)__popc(w & ((1u << sh) - 1u));
-        if (r >= c) {
-            r -= c;
-            w >>= sh;
-            base += sh;
-        }
-    }
-    return base;
-}
 
 __device__ __forceinline__ uint32_t bperm_u(int addr, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
@@ -642,91 +614,8 @@ __device__ __forceinline__ Cand cand_inf() {
     return c;
 }
 
-// Sorts the (at most 64) register candidates into tuple order, candidate k
-// on lane k: *ids gets its pixel ids (id1 | id2 << 16, both below 2^16 on the
-// LDS path), and the return value is its reference_dist, the smallest low
-// bound among the candidates after it, at most kFar (Ideal.cpp:2611-2615).
-__device__ __forceinline__ double sort_regs(const CandRegs &cr, uint32_t nc, int lane, uint32_t *ids) {
-    Cand x = cand_inf();
-    if (cr.live & 1u) {
-        x.apx = cr.apx[0];
-        x.lo = cr.lo[0];
-        x.hi = cr.hi[0];
-        x.id1 = cr.id1[0];
-        x.id2 = cr.id2[0];
-    }
-    bitonic_block(x, 0, 2, lane);
-    *ids = (x.id1 & 0xffffu) | (x.id2 << 16);
-    // suffix minimum of the low bounds, then shifted by one lane
-    double s = (uint32_t)lane < nc ? x.lo : INFINITY;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const double t = __shfl_down(s, o, 64);
-        if (lane + o < 64) s = fmin(s, t);
-    }
-    const double after = __shfl_down(s, 1, 64);
-    return fmin(kFar, lane < 63 ? after : INFINITY);
-}
-
-// Sorts the slab's n candidates (64 < n <= 2048) into tuple order with a
-// bitonic network over the next power of two (+inf padding): strides below
-// 64 in registers, a 64-element block per lane set, wider strides as
-// compare-exchanges in the slab.  Taken candidates (apx = +inf) sort after
-// the live ones.  Then the lo field of each of the first nlive entries is
-// replaced by its reference_dist (smallest low bound of the later live
-// entries, <= kFar).
-__device__ __forceinline__ void sort_slab(Cand *cand, uint32_t n, uint32_t nlive, int lane) {
-    uint32_t N = 128;
-    while (N < n) N <<= 1;
-    for (uint32_t i = n + (uint32_t)lane; i < N; i += 64) cand[i] = cand_inf();
-    wave_sync();
-    for (uint32_t b = 0; b < N; b += 64) {
-        Cand x = cand[b + lane];
-        bitonic_block(x, b, 2, lane);
-        cand[b + lane] = x;
-    }
-    wave_sync();
-    for (uint32_t kk = 128; kk <= N; kk <<= 1) {
-        for (uint32_t j = kk >> 1; j >= 64; j >>= 1) {
-            for (uint32_t p0 = 0; p0 < N / 2; p0 += 64) {
-                const uint32_t p = p0 + (uint32_t)lane;         // compare pair p of N / 2
-                const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));  // its lower element
-                const Cand a = cand[i], b = cand[i + j];
-                const bool up = (i & kk) == 0;
-                if (up ? tuple_less(b, a) : tuple_less(a, b)) {
-                    cand[i] = b;
-                    cand[i + j] = a;
-                }
-            }
-            wave_sync();
-        }
-        for (uint32_t b = 0; b < N; b += 64) {
-            Cand x = cand[b + lane];
-            bitonic_block(x, b, kk, lane);
-            cand[b + lane] = x;
-        }
-        wave_sync();
-    }
-    // reference_dist of every live entry, from the last block to the first
-    double carry = INFINITY;  // smallest low bound past the current block
-    for (int b = (int)((nlive + 63) / 64) - 1; b >= 0; b--) {
-        const uint32_t i = (uint32_t)b * 64 + (uint32_t)lane;
-        double s = i < nlive ? cand[i].lo : INFINITY;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const double t = __shfl_down(s, o, 64);
-            if (lane + o < 64) s = fmin(s, t);
-        }
-        const double after = __shfl_down(s, 1, 64);
-        const double rd = fmin(kFar, fmin(lane < 63 ? after : INFINITY, carry));
-        carry = fmin(carry, uni(__shfl(s, 0, 64)));
-        if (i < nlive) cand[i].lo = rd;
-    }
-    wave_sync();
-}
-
 // Ideal::within for one pair; the wave's buffers: lists / bits (LDS or a
-// global slab) and the candidate slab.
+// global slab) and, on the global path, the candidate slab.
 template <int SH, class T>
 __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uint32_t ib, double d, T *sl,
                                                T *tl, uint32_t *sb, uint32_t *tb, Cand *cand,
@@ -785,10 +674,12 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
     }
     int i = 0, j = 0;
     bool final_level = false;
-    // last-level candidates: in registers while they fit (not on the path
-    // with global buffers), else in the slab (`spilled`)
-    CandRegs cr;
-    bool spilled = SH != 8;
+    // the LDS path keeps the last level's grids, lists and bound for its own
+    // candidate generation (below); the global path writes its candidates to
+    // the slab
+    LGrid fgs = lgrid(S, hs, 0), fgt = lgrid(S, ht, 0);
+    uint32_t fns = 0, fnt = 0;
+    double fkeep2 = -1.0;
     for (;;) {
         const double s_step = lgrid(S, hs, i).step, t_step = lgrid(S, ht, j).step;
         bool s_next = false, t_next = false;
@@ -845,6 +736,18 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         // min distance <= d and < maxd, as bounds on the squared distance
         const double keep2 = fmin(d_le2, sq_lt(maxd));
         r.boxes += total;
+        if constexpr (SH == 8) {
+            if (final_level) {
+                // the last level's candidates are generated below, the 64
+                // smallest at a time (no slab)
+                fgs = gs;
+                fgt = gt;
+                fns = ns;
+                fnt = nt;
+                fkeep2 = keep2;
+                break;
+            }
+        }
         // pass 2: keep pairs with min distance <= d and below maxd; their
         // projections are the next level's sets, or (last level) they are
         // the candidates
@@ -873,7 +776,7 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                     if (mn2 <= keep2) {
                         keep = true;
                         const uint32_t id1 = (uint32_t)(ya * gs.dimx + xa), id2 = (uint32_t)(yb * gt.dimx + xb);
-                        if (!final_level) {
+                        if (SH == 8 || !final_level) {
                             set_bit(sb, id1);
                             set_bit(tb, id2);
                         } else {
@@ -889,54 +792,15 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
                         }
                     }
                 }
-                if (final_level) {
+                if (SH != 8 && final_level) {
+                    // global path: every candidate to the slab, in order
                     const uint64_t m = __ballot(keep);
-                    const uint32_t cnt = (uint32_t)__popcll(m);
-                    if (cnt && !spilled && nc + cnt <= 64u * kCandRegs) {
-                        // candidate nc + r (r-th kept lane of this round) goes
-                        // to lane (nc + r) % 64, slot (nc + r) / 64
-                        const uint32_t rk = (uint32_t)(lane - (int)nc) & 63u;
-                        const bool recv = rk < cnt;
-                        const int addr = (recv ? nth_set_bit(m, rk) : lane) << 2;
-                        const double apx = bperm_d(addr, c.apx), lo = bperm_d(addr, c.lo), hi = bperm_d(addr, c.hi);
-                        const uint32_t i1 = bperm_u(addr, c.id1), i2 = bperm_u(addr, c.id2);
-                        const uint32_t qs = (nc + rk) >> 6;
-#pragma unroll
-                        for (int q = 0; q < kCandRegs; q++) {
-                            if (recv && qs == (uint32_t)q) {
-                                cr.apx[q] = apx;
-                                cr.lo[q] = lo;
-                                cr.hi[q] = hi;
-                                cr.id1[q] = i1;
-                                cr.id2[q] = i2;
-                                cr.live |= 1u << q;
-                            }
-                        }
-                    } else if (cnt) {
-                        if (!spilled) {
-                            // the registers overflow: their candidates go to
-                            // the slab, in order, and the rest follow them
-#pragma unroll
-                            for (int q = 0; q < kCandRegs; q++) {
-                                if ((cr.live >> q) & 1u) {
-                                    Cand t;
-                                    t.apx = cr.apx[q];
-                                    t.lo = cr.lo[q];
-                                    t.hi = cr.hi[q];
-                                    t.id1 = cr.id1[q];
-                                    t.id2 = cr.id2[q];
-                                    cand[q * 64 + lane] = t;  // < nc <= 64 kCandRegs < candcap
-                                }
-                            }
-                            spilled = true;
-                        }
-                        const uint32_t pos = nc + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                        if (keep) {
-                            if (pos < candcap) cand[pos] = c;
-                            else over = true;
-                        }
+                    const uint32_t pos = nc + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    if (keep) {
+                        if (pos < candcap) cand[pos] = c;
+                        else over = true;
                     }
-                    nc += cnt;
+                    nc += (uint32_t)__popcll(m);
                 }
                 a += q;
                 b += rr;
@@ -958,95 +822,222 @@ __device__ __forceinline__ PairOut within_pair(const DevSet &S, uint32_t ia, uin
         ns = compact<SH>(sb, npix_s, gs.dimx, sl, lane);
         nt = compact<SH>(tb, npix_t, gt.dimx, tl, lane);
     }
-    if (!final_level || r.ncand == 0) return r;
-
-    // refinement in tuple order
-    // candidates are taken in tuple order and refined in runs of up to
-    // kBatch whose sequence pairs fit one 64-row table (a candidate with more
-    // rows runs alone, 64 rows at a time).  The LDS path sorts the register
-    // candidates once, with each one's reference_dist (the smallest low bound
-    // of the later ones).  Slab candidates are selected one at a time for
-    // the first kSelectFirst (most pairs with many candidates are decided by
-    // their first few), then the rest of the slab is sorted once.  The
-    // global path selects them one at a time.
-    const uint32_t nc = r.ncand;
-    double min_dist = kFar;
-    Cand c;
-    double ref;
-    uint32_t k = 0;      // sorted: position of the next candidate
-    double rref = kFar;  // sorted registers: lane k's candidate's reference_dist
-    uint32_t rids = 0;   // and its pixel ids (id1 | id2 << 16)
-    int slot;
-    uint32_t nsel = 0;          // slab candidates taken by selection (marked apx = +inf)
-    bool slab_sorted = false;
+    if (!final_level) return r;
     if constexpr (SH == 8) {
-        if (!spilled) rref = sort_regs(cr, nc, lane, &rids);
-    }
-    auto next = [&]() -> int {
-        if constexpr (SH == 8) {
-            if (spilled && !slab_sorted) {
-                if (nsel < kSelectFirst) return select_next(cand, nc, lane, c, ref);
-                sort_slab(cand, nc, nc - nsel, lane);
-                slab_sorted = true;
-            }
-            if (k >= nc - nsel) return -1;
-            if (!spilled) {
-                const int kk = (int)k;
-                const uint32_t ids = (uint32_t)__builtin_amdgcn_readlane((int)rids, kk);
-                c.id1 = ids & 0xffffu;
-                c.id2 = ids >> 16;
-                const long long rb = __double_as_longlong(rref);
-                const int lo = __builtin_amdgcn_readlane((int)(rb & 0xffffffffll), kk);
-                const int hi = __builtin_amdgcn_readlane((int)(rb >> 32), kk);
-                ref = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-            } else {
-                const Cand cc = cand[k];  // the same address on every lane
-                c.id1 = uni(cc.id1);
-                c.id2 = uni(cc.id2);
-                ref = uni(cc.lo);  // sort_slab stored reference_dist here
-            }
-            return (int)k;
-        } else {
-            return select_next(cand, nc, lane, c, ref);
-        }
-    };
-    slot = next();
-    while (slot >= 0) {
-        CandBatch cb;
-        bool open = true;
-#pragma unroll
-        for (int q = 0; q < kBatch; q++) {
-            if (open && slot >= 0) {
-                const uint32_t s0 = S.pixoff[hs.pixoff_start + c.id1], s1 = S.pixoff[hs.pixoff_start + c.id1 + 1];
-                const uint32_t t0 = S.pixoff[ht.pixoff_start + c.id2], t1 = S.pixoff[ht.pixoff_start + c.id2 + 1];
-                const uint32_t rows = (s1 - s0) * (t1 - t0);
-                if (q > 0 && cb.nrows + rows > 64) {
-                    open = false;  // c starts the next run
+        // Last level of the LDS path.  A generation pass runs over the final
+        // lists' pixel pairs, keeps the candidates (min distance <= d and
+        // below the smallest maximum) whose tuple follows `after`, and merges
+        // them 64 at a time through the LDS stage into the 64 smallest
+        // (sorted, candidate k on lane k); the others only leave their low
+        // bound (min_rest).  The refinement takes those 64 in tuple order;
+        // when all are refined without a decision and more candidates remain,
+        // the next pass starts after the last of them.  Every candidate's
+        // reference_dist (the smallest low bound of the candidates after it,
+        // Ideal.cpp:2611-2615) is the suffix minimum over the 64 and min_rest.
+        const double sa = fgs.step * fgs.step, ta = fgt.step * fgt.step;
+        Cand after;
+        after.apx = after.lo = after.hi = -INFINITY;
+        after.id1 = after.id2 = 0u;
+        double min_dist = kFar;
+        bool first_pass = true;
+        for (;;) {
+            Cand best = cand_inf();
+            uint32_t nb = 0, count = 0, nst = 0;
+            bool full = false;
+            Cand th = cand_inf();     // the 64th smallest so far (uniform), once full
+            double mrest = INFINITY;  // this lane's smallest low bound outside the best
+            auto merge = [&](uint32_t n) {
+                wave_sync();
+                Cand x = (uint32_t)lane < n ? tab.stage[lane] : cand_inf();
+                bitonic_block(x, 0, 2, lane);
+                if (nb == 0) {
+                    best = x;
                 } else {
-                    cb.s0[q] = s0;
-                    cb.t0[q] = t0;
-                    cb.n1[q] = s1 - s0;
-                    cb.n2[q] = t1 - t0;
-                    cb.ref[q] = ref;
-                    cb.row0[q] = cb.nrows;
-                    cb.nrows += rows;
-                    cb.n = q + 1;
-                    if (SH == 8 && (!spilled || slab_sorted)) {
-                        k++;
-                    } else {
-                        if (lane == 0) cand[slot].apx = INFINITY;
-                        wave_sync();
-                        nsel++;
+                    // best ascending, x reversed (descending): the elementwise
+                    // smaller is a bitonic sequence of the 64 smallest
+                    Cand y;
+                    const int src = (63 - lane) << 2;
+                    y.apx = bperm_d(src, x.apx);
+                    y.lo = bperm_d(src, x.lo);
+                    y.hi = bperm_d(src, x.hi);
+                    y.id1 = bperm_u(src, x.id1);
+                    y.id2 = bperm_u(src, x.id2);
+                    const bool ys = tuple_less(y, best);
+                    const Cand out = ys ? best : y;  // the larger leaves the 64
+                    if (out.apx != INFINITY) mrest = fmin(mrest, out.lo);
+                    if (ys) best = y;
+#pragma unroll 1
+                    for (int j = 32; j > 0; j >>= 1) bitonic_step(best, (uint32_t)lane, 64u, j);
+                }
+                nb = min(64u, nb + n);
+                if (nb == 64) {
+                    full = true;
+                    th.apx = uni(__shfl(best.apx, 63, 64));
+                    th.lo = uni(__shfl(best.lo, 63, 64));
+                    th.hi = uni(__shfl(best.hi, 63, 64));
+                    th.id1 = uni(__shfl(best.id1, 63, 64));
+                    th.id2 = uni(__shfl(best.id2, 63, 64));
+                }
+                wave_sync();
+            };
+            const uint64_t total = (uint64_t)fns * fnt;
+            if (!first_pass) r.boxes += total;
+            {
+                uint32_t a = lane / fnt, b = lane - (lane / fnt) * fnt;
+                const uint32_t q = 64 / fnt, rr = 64 - q * fnt;
+                const uint64_t rounds = (total + 63) / 64;
+                for (uint64_t it = 0; it < rounds; it++) {
+                    bool keep = false;
+                    Cand c = cand_inf();
+                    if (a < fns) {
+                        const T va = sl[a], vb = tl[b];
+                        const int xa = unpack_x<SH>(va), ya = unpack_y<SH>(va);
+                        const int xb = unpack_x<SH>(vb), yb = unpack_y<SH>(vb);
+                        double b1[4], b2[4];
+                        pbox(fgs, xa, ya, b1);
+                        pbox(fgt, xb, yb, b2);
+                        const double mn2 = box_min_d2(b1, b2, tab.km);
+                        if (mn2 <= fkeep2) {
+                            const uint32_t id1 = (uint32_t)(ya * fgs.dimx + xa), id2 = (uint32_t)(yb * fgt.dimx + xb);
+                            const double mn = sqrt(mn2);
+                            const double hi = sqrt(box_max_d2(b1, b2, tab.km));
+                            const int pf = classify(S.areas[hs.pix_start + id1] + S.areas[ht.pix_start + id2],
+                                                    sa + ta, 20);
+                            c.apx = mn + (double)c_mean[pf] * (hi - mn);
+                            c.lo = mn;
+                            c.hi = hi;
+                            c.id1 = id1;
+                            c.id2 = id2;
+                            keep = tuple_less(after, c);
+                        }
                     }
-                    slot = next();
-                    if (cb.nrows >= 64) open = false;
+                    count += (uint32_t)__popcll(__ballot(keep));
+                    const bool enter = keep && (!full || tuple_less(c, th));
+                    if (keep && !enter) mrest = fmin(mrest, c.lo);
+                    const uint64_t m = __ballot(enter);
+                    const uint32_t cnt = (uint32_t)__popcll(m);
+                    if (nst + cnt > 64) {
+                        merge(nst);
+                        nst = 0;
+                    }
+                    if (enter) tab.stage[nst + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = c;
+                    nst += cnt;
+                    a += q;
+                    b += rr;
+                    if (b >= fnt) {
+                        b -= fnt;
+                        a++;
+                    }
                 }
             }
+            if (nst) merge(nst);
+            if (first_pass) {
+                r.ncand = count;
+                first_pass = false;
+            }
+            if (count == 0) return r;
+            // reference_dist of lane k's candidate, and its pixel ids
+            const double rest = wave_min(mrest);
+            double sfx = (uint32_t)lane < nb ? best.lo : INFINITY;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double t = __shfl_down(sfx, o, 64);
+                if (lane + o < 64) sfx = fmin(sfx, t);
+            }
+            const double later = __shfl_down(sfx, 1, 64);
+            const double rref = fmin(kFar, fmin(lane < 63 ? later : INFINITY, rest));
+            const uint32_t rids = (best.id1 & 0xffffu) | (best.id2 << 16);
+            uint32_t k = 0;
+            while (k < nb) {
+                CandBatch cb;
+                bool open = true;
+#pragma unroll
+                for (int qq = 0; qq < kBatch; qq++) {
+                    if (open && k < nb) {
+                        const int kk = (int)k;
+                        const uint32_t ids = (uint32_t)__builtin_amdgcn_readlane((int)rids, kk);
+                        const uint32_t id1 = ids & 0xffffu, id2 = ids >> 16;
+                        const long long rb = __double_as_longlong(rref);
+                        const int rlo = __builtin_amdgcn_readlane((int)(rb & 0xffffffffll), kk);
+                        const int rhi = __builtin_amdgcn_readlane((int)(rb >> 32), kk);
+                        const double ref = __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
+                        const uint32_t s0 = S.pixoff[hs.pixoff_start + id1], s1 = S.pixoff[hs.pixoff_start + id1 + 1];
+                        const uint32_t t0 = S.pixoff[ht.pixoff_start + id2], t1 = S.pixoff[ht.pixoff_start + id2 + 1];
+                        const uint32_t rows = (s1 - s0) * (t1 - t0);
+                        if (qq > 0 && cb.nrows + rows > 64) {
+                            open = false;  // candidate k starts the next run
+                        } else {
+                            cb.s0[qq] = s0;
+                            cb.t0[qq] = t0;
+                            cb.n1[qq] = s1 - s0;
+                            cb.n2[qq] = t1 - t0;
+                            cb.ref[qq] = ref;
+                            cb.row0[qq] = cb.nrows;
+                            cb.nrows += rows;
+                            cb.n = qq + 1;
+                            k++;
+                            if (cb.nrows >= 64) open = false;
+                        }
+                    }
+                }
+                const int res = refine_batch(S, hs, ht, cb, d, min_dist, r, tab, lane);
+                if (res != 2) {
+                    r.bit = (uint8_t)res;
+                    return r;
+                }
+            }
+            if (count <= nb) return r;  // every candidate refined: not within
+            after.apx = uni(__shfl(best.apx, 63, 64));
+            after.lo = uni(__shfl(best.lo, 63, 64));
+            after.hi = uni(__shfl(best.hi, 63, 64));
+            after.id1 = uni(__shfl(best.id1, 63, 64));
+            after.id2 = uni(__shfl(best.id2, 63, 64));
         }
-        const int res = refine_batch(S, hs, ht, cb, d, min_dist, r, tab, lane);
-        if (res != 2) {
-            r.bit = (uint8_t)res;
-            return r;
+    }
+    if constexpr (SH != 8) {
+        if (r.ncand == 0) return r;
+        // global path: the slab's candidates are taken one at a time by
+        // argmin selection in tuple order (with each one's reference_dist)
+        // and refined in runs of up to kBatch whose sequence pairs fit one
+        // 64-row table (a candidate with more rows runs alone, 64 rows at a
+        // time)
+        const uint32_t nc = r.ncand;
+        double min_dist = kFar;
+        Cand c;
+        double ref;
+        int slot = select_next(cand, nc, lane, c, ref);
+        while (slot >= 0) {
+            CandBatch cb;
+            bool open = true;
+#pragma unroll
+            for (int q = 0; q < kBatch; q++) {
+                if (open && slot >= 0) {
+                    const uint32_t s0 = S.pixoff[hs.pixoff_start + c.id1], s1 = S.pixoff[hs.pixoff_start + c.id1 + 1];
+                    const uint32_t t0 = S.pixoff[ht.pixoff_start + c.id2], t1 = S.pixoff[ht.pixoff_start + c.id2 + 1];
+                    const uint32_t rows = (s1 - s0) * (t1 - t0);
+                    if (q > 0 && cb.nrows + rows > 64) {
+                        open = false;  // c starts the next run
+                    } else {
+                        cb.s0[q] = s0;
+                        cb.t0[q] = t0;
+                        cb.n1[q] = s1 - s0;
+                        cb.n2[q] = t1 - t0;
+                        cb.ref[q] = ref;
+                        cb.row0[q] = cb.nrows;
+                        cb.nrows += rows;
+                        cb.n = q + 1;
+                        if (lane == 0) cand[slot].apx = INFINITY;
+                        wave_sync();
+                        slot = select_next(cand, nc, lane, c, ref);
+                        if (cb.nrows >= 64) open = false;
+                    }
+                }
+            }
+            const int res = refine_batch(S, hs, ht, cb, d, min_dist, r, tab, lane);
+            if (res != 2) {
+                r.bit = (uint8_t)res;
+                return r;
+            }
         }
     }
     return r;
@@ -1080,7 +1071,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         tl = lists[wv][1];
         sb = bits[wv][0];
         tb = bits[wv][1];
-        cand = reinterpret_cast<Cand *>(slab + gw * slab_stride);
+        cand = nullptr;  // the LDS path keeps its candidates in registers and LDS
     } else {
         char *base = slab + gw * slab_stride;
         const uint64_t wbits = ((uint64_t)pixcap + 31) / 32;
@@ -1198,14 +1189,11 @@ int run_within(farm_handle h, uint64_t begin, uint64_t end, double d_km, uint8_t
         per_cu < 1)
         per_cu = 4;
     const unsigned nblocks = (unsigned)h->num_cus * (unsigned)std::min(per_cu, 8);
-    const uint64_t nwaves = (uint64_t)nblocks * kWaves;
-    const uint64_t stride = (uint64_t)kCandLds * sizeof(Cand);
-    if ((rc = ensure(h, h->wslab, nwaves * stride))) return rc;
     DevSet S = devset(h);
     HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
     hipLaunchKernelGGL(k_within<false>, dim3(nblocks), dim3(64 * kWaves), 0, h->stream, S, pa, pb, n,
-                       (const uint32_t *)nullptr, d_km, out_dev, md_dev, P<char>(h->wslab), stride,
-                       (uint32_t)kPixMax, kCandLds, ctr, P<uint32_t>(h->wovf), wst);
+                       (const uint32_t *)nullptr, d_km, out_dev, md_dev, (char *)nullptr, (uint64_t)0,
+                       (uint32_t)kPixMax, (uint32_t)0, ctr, P<uint32_t>(h->wovf), wst);
     HIPCHK(h, hipGetLastError());
     uint32_t novf = 0;
     HIPCHK(h, hipMemcpyAsync(&novf, ctr + 1, 4, hipMemcpyDeviceToHost, h->stream));
