@@ -595,12 +595,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             // antisymmetry of rounded subtraction, o(p) of the next step.  A
             // superset filter (o1*o2 <= 0 or o3 == 0, and o3*o4 <= 0) sends
             // the rare candidates to the exact reference predicate.
+            //
+            // Fewer than 64 items (a tile's last) are split over the lanes: k =
+            // 64 / n rounded down to a power of two (at most C) lanes per item,
+            // part j taking its steps j C / k .. (j + 1) C / k - 1.  A part is
+            // the item's segments from its own first vertex on, so the split
+            // evaluates the same tests in C / k steps instead of C.
             auto drain = [&](uint32_t n) {
-                const bool has = (uint32_t)lane < n;
-                const uint2 it = ring[(qhead + (uint32_t)lane) & (kRefQueue - 1)];
+                uint32_t lk = 0;  // log2 k (uniform)
+                while ((1u << lk) < (uint32_t)C && (n << (lk + 1)) <= 64u) lk++;
+                const uint32_t seg = (uint32_t)C >> lk;
+                const uint32_t item = (uint32_t)lane >> lk, m0 = ((uint32_t)lane & ((1u << lk) - 1u)) * seg;
+                const uint2 it = ring[(qhead + item) & (kRefQueue - 1)];
+                const uint32_t nsf = item < n ? (it.y >> 8) : 0u;
+                const uint32_t ns = nsf > m0 ? min(seg, nsf - m0) : 0u;
+                const bool has = ns > 0u;
                 const uint32_t sl = has ? (it.y & 63u) : (uint32_t)lane;
-                const uint32_t ns = has ? (it.y >> 8) : 0u;
-                const uint32_t vs = has ? it.x : 0u;  // the chunk's first vertex c
+                const uint32_t vs = has ? it.x + m0 : 0u;  // the part's first vertex c
                 float qpx[E + 1], qpy[E + 1];
 #pragma unroll
                 for (int v = 0; v <= E; v++) {
@@ -764,7 +775,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         const float q = ((fmaxf(fabsf(ex0), fabsf(ex1)) + fmaxf(fabsf(ey0), fabsf(ey1))) +
                                          (bb.z + bb.w)) * 1.000001f;
                         const float hx1 = bb.z * 1.000001f, hy1 = bb.w * 1.000001f;
-                        const bool apart = (ex1 > hx1) | (-ex0 > hx1) | (ey1 > hy1) | (-ey0 > hy1);
+                        // (ex1 > hx1 or -ex0 > hx1 or ey1 > hy1 or -ey0 > hy1, as two
+                        // maxima and two compares)
+                        const bool apart = (fmaxf(ex1, -ex0) > hx1) | (fmaxf(ey1, -ey0) > hy1);
                         // (without the prune: the largest finite margin keeps every
                         // chunk for every slot with an edge)
                         const float k1 = !prune ? 0x1.fffffep127f : q * 0x1p-18f;
