@@ -399,6 +399,35 @@ int build_chunk_boxes(farm_handle h, uint32_t C) {
     return FARM_OK;
 }
 
+// The intersection's kIxChunk-segment chunk boxes, when the refinement's are
+// of another size (else those serve).  Synchronises the stream.
+int build_ix_boxes(farm_handle h) {
+    int rc;
+    if (h->chunk == kIxChunk || h->ix_gen == h->set_gen) return FARM_OK;
+    const uint64_t nseqs = h->nseqs;
+    if ((rc = ensure(h, h->ixcnt, 4 * (nseqs + 1)))) return rc;
+    if ((rc = ensure(h, h->ixoff, 4 * (nseqs + 1)))) return rc;
+    if ((rc = fill_u32(h, h->ixcnt.p, nseqs + 1, 0))) return rc;
+    if (h->npolys > 0) {
+        hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+                           kIxChunk, P<uint32_t>(h->ixcnt));
+        HIPCHK(h, hipGetLastError());
+    }
+    if ((rc = exclusive_scan(h, P<uint32_t>(h->ixcnt), P<uint32_t>(h->ixoff), nseqs + 1))) return rc;
+    uint32_t nchunks = 0;
+    HIPCHK(h, hipMemcpyAsync(&nchunks, P<uint32_t>(h->ixoff) + nseqs, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if ((rc = ensure(h, h->ixbox, 16ull * (nchunks + 1)))) return rc;
+    if (h->npolys > 0) {
+        hipLaunchKernelGGL(k_chunk_box, dim3((unsigned)h->npolys), dim3(256), 0, h->stream, devset(h), h->npolys,
+                           kIxChunk, (const uint32_t *)P<uint32_t>(h->ixoff), P<float4>(h->ixbox));
+        HIPCHK(h, hipGetLastError());
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->ix_gen = h->set_gen;
+    return FARM_OK;
+}
+
 }  // namespace farm
 
 extern "C" {
@@ -652,7 +681,7 @@ int farm_close(farm_handle h) {
         if (h->stage[k]) (void)hipHostFree(h->stage[k]);
         if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
     }
-    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->xyd,      &h->fill,     &h->choff,  &h->chcnt,  &h->chbox,  &h->voff,
+    DevBuf *bufs[] = {&h->hdr,    &h->status,  &h->pixoff, &h->seqs,   &h->x,      &h->y,        &h->xyd,      &h->fill,     &h->choff,  &h->chcnt,  &h->chbox,  &h->ixcnt, &h->ixoff, &h->ixbox, &h->voff,
                       &h->vnodes, &h->pixedges, &h->shape_ctr, &h->pa, &h->pb, &h->isx,
                       &h->scan_tmp, &h->pair_tests, &h->mlp_w, &h->mlp_m, &h->mlp_v, &h->mlp_g,
                       &h->mlp_norm, &h->mlp_x, &h->mlp_h1, &h->mlp_h2, &h->mlp_z3, &h->mlp_d3, &h->mlp_dh2,

@@ -25,6 +25,10 @@ struct DevSet {
     // (centre x, y, half extents x, y), refinement only
     const uint32_t *choff;
     const float4 *chbox;
+    // the same for chunks of kIxChunk segments, for the intersection's chunk
+    // skip (the refinement's arrays when C == kIxChunk; null: no skip)
+    const uint32_t *ixoff;
+    const float4 *ixbox;
     const uint32_t *pixedges;  // per pixel: sum of sequence lengths (BORDER), else 0
     const uint32_t *voff;
     const double *vnodes;
@@ -43,6 +47,8 @@ struct DevSet {
 // screens kRefScreen chunk boxes per scalar load (the box array keeps
 // kRefScreen spare entries past its end).
 constexpr uint32_t kRefChunkDefault = 16;
+// target segments per chunk of the intersection's collection screen
+constexpr uint32_t kIxChunk = 16;
 constexpr uint32_t kRefScreen = 4;
 
 // A BORDER x BORDER pixel pair of one candidate pair.

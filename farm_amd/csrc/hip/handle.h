@@ -44,6 +44,10 @@ struct farm_handle_s {
     farm::DevBuf fill;    // -w fill classes (when the set has them)
     int cc = FARM_CATEGORY_COUNT;
     farm::DevBuf hdr, status, pixoff, seqs, x, y, xyd, choff, chcnt, chbox, voff, vnodes, pixedges, shape_ctr;
+    // 16-segment chunk boxes for the intersection's chunk skip when the
+    // refinement's boxes are of another size (built for set generation ix_gen)
+    farm::DevBuf ixcnt, ixoff, ixbox;
+    uint64_t ix_gen = ~0ull;
     uint64_t nstatus = 0;
     // array lengths of the set (farm_polyset's counts)
     uint64_t nverts = 0, nseqs = 0, npixoff = 0, nvoff = 0, nvnodes = 0, nlayers = 0, nlstatus = 0;
@@ -164,6 +168,16 @@ inline DevSet devset(farm_handle h) {
     S.cc = h->cc;
     S.choff = P<uint32_t>(h->choff);
     S.chbox = P<float4>(h->chbox);
+    if (h->chunk == kIxChunk) {
+        S.ixoff = S.choff;
+        S.ixbox = S.chbox;
+    } else if (h->ix_gen == h->set_gen) {
+        S.ixoff = P<uint32_t>(h->ixoff);
+        S.ixbox = P<float4>(h->ixbox);
+    } else {
+        S.ixoff = nullptr;
+        S.ixbox = nullptr;
+    }
     S.pixedges = P<uint32_t>(h->pixedges);
     S.voff = P<uint32_t>(h->voff);
     S.vnodes = P<double>(h->vnodes);

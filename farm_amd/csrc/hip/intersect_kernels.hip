@@ -24,8 +24,9 @@ namespace farm {
 // pass 1: emit the pixel pairs at the offsets of an exclusive scan; it does
 //         nothing when that total exceeds cap_pp (the buffer's capacity: the
 //         host grows it or fails the call, and no offset has wrapped).
-// The window is enumerated in full (no early exit), so the status bytes read
-// equal the algorithmic filter bytes.
+// A lane stops walking its pair's windows at the first IN overlap (or, in
+// approximate mode, the first decisive BORDER pair): the pair is decided and
+// its count is not used.  An undecided pair's windows are walked in full.
 // skip (optional): pairs already decided by the caller (their result is left
 // untouched and they emit no pixel pairs).
 // ------------------------------------------------------------------------
@@ -69,8 +70,8 @@ __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__rest
         uint32_t cnt = 0;
         const uint32_t base = pass == 1 ? pp_off[k] : 0u;
         bytes += 8 + 2 * 56;
-        for (int x = wa.x0; x <= wa.x1; x++) {
-            for (int y = wa.y0; y <= wa.y1; y++) {
+        for (int x = wa.x0; x <= wa.x1 && !(in_hit || heur); x++) {
+            for (int y = wa.y0; y <= wa.y1 && !(in_hit || heur); y++) {
                 const int pa = y * A.dimx + x;
                 const uint8_t fa = sa[pa];
                 bytes += 1;
@@ -79,8 +80,8 @@ __global__ __launch_bounds__(256) void k_filter(DevSet S, const uint32_t *__rest
                 double lx, ly, hx, hy;
                 pixel_box(A, x, y, lx, ly, hx, hy);
                 const Window wb = window_of(B, lx, ly, hx, hy);
-                for (int u = wb.x0; u <= wb.x1; u++) {
-                    for (int v = wb.y0; v <= wb.y1; v++) {
+                for (int u = wb.x0; u <= wb.x1 && !(in_hit || heur); u++) {
+                    for (int v = wb.y0; v <= wb.y1 && !(in_hit || heur); v++) {
                         const int pb = v * B.dimx + u;
                         const uint8_t fb = sb[pb];
                         bytes += 1;

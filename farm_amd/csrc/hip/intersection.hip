@@ -473,6 +473,7 @@ __device__ __forceinline__ bool line_pass(LineProbe p0, LineProbe p1, float k) {
 // loads.
 typedef __attribute__((address_space(4))) const float cfloat;
 constexpr int kScreenChunk = 16;  // target vertices per chunk (<= the vertex padding)
+static_assert(kScreenChunk == (int)kIxChunk, "the chunk skip reads kIxChunk-segment boxes");
 
 // Queues the lanes of mask (vertex indexes ea per lane, eb uniform).
 __device__ __forceinline__ void push_mask(uint64_t mask, uint32_t ea, uint32_t eb, const float *AX, const float *AY,
@@ -545,9 +546,22 @@ __device__ __forceinline__ void screen_chunk(const ScreenLane &L, const cfloat *
 // stream in uniform chunks (scalar loads), else lanes split into phases, each
 // over a contiguous block of the target edges.  Pairs passing the line screen
 // are queued for the exact test.
+//
+// Chunk skip (cbox: the target sequence's kIxChunk-segment chunk boxes,
+// centre and rounded-up half extents; null: none).  Every vertex q of a chunk
+// lies in its box, so the exact o(q) = (q - a) x d1 is within
+// S = |d1y| hx + |d1x| hy of o at the centre c.  The f32 o of the screen errs
+// by at most 2^-22 |d1|_1 R + 2^-24 |o| (R = |q - a|_1 <= Rmax, the box lies
+// in the sequence's box), i.e. by M / 16 and a relative ulp, and so does the
+// f32 o(c).  When the f32 o(c) exceeds T = S (1 + 2^-20) + 2 M + 2^-100 in
+// magnitude, every vertex's f32 o lies beyond M on the side of o(c): the
+// screen would fail every edge of the chunk, and the vertices' side masks are
+// those of the sign of o(c).  A chunk every active lane skips is passed over
+// whole (its vertices are not loaded).  An infinite box (a zero-length
+// segment, a non-finite vertex) makes T infinite or NaN and is never skipped.
 __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *AY, const float *BX, const float *BY,
-                                                 uint2 e1, uint2 e2, EdgeQueue &Q, uint32_t &qn, Rec *R,
-                                                 uint32_t cap, uint32_t *cnt, int lane) {
+                                                 uint2 e1, uint2 e2, const cfloat *cbox, EdgeQueue &Q, uint32_t &qn,
+                                                 Rec *R, uint32_t cap, uint32_t *cnt, int lane) {
     const uint32_t la = e1.x, na = e1.y, lb = e2.x, nb = e2.y;  // first edge, edge count
     if (na == 0 || nb == 0) return;
     if (na >= 48) {
@@ -580,11 +594,27 @@ __device__ __forceinline__ void collect_seq_pair(const float *AX, const float *A
             ScreenLane L{ax, ay, d1x, d1y, M};
             uint64_t gt, lt;
             screen_side(L, bxc[0], byc[0], gt, lt);
+            const float ad1x = fabsf(d1x), ad1y = fabsf(d1y);
+            const float T0 = fmaf(M, 2.f, 0x1p-100f);
+            // true when every active lane skips chunk c (then gt / lt are the
+            // side masks of its last vertex)
+            auto skip_chunk = [&](uint32_t c) -> bool {
+                if (!cbox) return false;
+                const float cx = cbox[4 * c], cy = cbox[4 * c + 1], hx = cbox[4 * c + 2], hy = cbox[4 * c + 3];
+                const float oc = fmaf(cx - ax, d1y, -((cy - ay) * d1x));
+                const float T = fmaf(fmaf(ad1y, hx, ad1x * hy), 1.000001f, T0);
+                if (__ballot(on && !(fabsf(oc) > T))) return false;
+                gt = __ballot(oc > 0.f);
+                lt = __ballot(oc < 0.f);
+                return true;
+            };
             const uint32_t nfull = nb / kScreenChunk * kScreenChunk;
-            for (uint32_t j0 = 0; j0 < nfull; j0 += kScreenChunk)
+            for (uint32_t j0 = 0; j0 < nfull; j0 += kScreenChunk) {
+                if (skip_chunk(j0 / kScreenChunk)) continue;
                 screen_chunk<false>(L, bxc, byc, j0, kScreenChunk, on_mask, gt, lt, ea, lb, AX, AY, BX, BY, Q, qn, R,
                                     cap, cnt, lane);
-            if (nfull < nb)
+            }
+            if (nfull < nb && !skip_chunk(nfull / kScreenChunk))
                 screen_chunk<true>(L, bxc, byc, nfull, nb - nfull, on_mask, gt, lt, ea, lb, AX, AY, BX, BY, Q, qn, R,
                                    cap, cnt, lane);
         }
@@ -672,7 +702,9 @@ __device__ __forceinline__ PairRes intersection_pair(const DevSet &S, const doub
                         for (uint32_t sb = sb0; sb < sb1; sb++) {
                             const uint2 e2 = S.seqs[hB.seq_start + sb];
                             r.tests += (uint64_t)e1.y * e2.y;
-                            collect_seq_pair(AX, AY, BX, BY, e1, e2, Q, qn, R, cap, cnt, lane);
+                            const cfloat *cb =
+                                S.ixbox ? (const cfloat *)(S.ixbox + S.ixoff[hB.seq_start + sb]) : nullptr;
+                            collect_seq_pair(AX, AY, BX, BY, e1, e2, cb, Q, qn, R, cap, cnt, lane);
                         }
                     }
                 }
@@ -842,6 +874,10 @@ template __global__ void k_intersection<true>(DevSet, const double *, const uint
 
 using namespace farm;
 
+namespace farm {
+int build_ix_boxes(farm_handle h);
+}  // namespace farm
+
 namespace {
 
 int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *counts_dev, double *area_dev,
@@ -860,6 +896,7 @@ int run_intersection(farm_handle h, uint64_t begin, uint64_t end, int32_t *count
     if (st) st->pairs = n;
     if (n == 0) return FARM_OK;
     int rc;
+    if ((rc = build_ix_boxes(h))) return rc;
     DevSet S = devset(h);
     if (!h->ring_area_ready) {
         if ((rc = ensure(h, h->ring2a, 8ull * h->npolys))) return rc;

@@ -156,7 +156,7 @@ typedef struct farm_stats {
     uint64_t bb_pixel_pairs;   /* BORDER x BORDER pixel pairs enumerated */
     uint64_t refine_items;     /* refinement work items executed */
     uint64_t segment_tests;    /* segment x segment tests executed */
-    uint64_t filter_bytes;     /* algorithmic filter bytes (full window enumeration) */
+    uint64_t filter_bytes;     /* status and header bytes the filter read (a decided pair stops early) */
     uint64_t refine_bytes;     /* algorithmic refinement bytes */
     uint64_t found;            /* pairs reported true */
     double filter_ms;          /* device time of the filter kernel(s) */

@@ -109,3 +109,115 @@ def test_screen_rejects_most_pairs_of_a_chain():
         scr = screen(*A, *B)
         assert not (acc & ~scr).any()
         assert scr.mean() < 0.05
+
+
+# --- chunk skip ---------------------------------------------------------
+# For long target sequences the kernel first tests each source edge's line
+# against the box of every 16-segment chunk (k_chunk_box: f32 centre, half
+# extents rounded up, infinite for a zero-length segment or a non-finite
+# vertex) and passes over a chunk that every lane skips, taking the side
+# masks of its last vertex from the sign of o at the box centre.  That is
+# exact only if a skipped chunk's vertices all lie beyond M on that side
+# under the per-vertex screen; these checks restate both in f32 numpy.
+
+def _fmaf(a, b, c):
+    return (a.astype(np.float64) * b.astype(np.float64) + np.asarray(c).astype(np.float64)).astype(F)
+
+
+def _chunk_boxes(vx, vy):
+    """k_chunk_box over rows of vertices (n, 17)."""
+    x0, x1 = vx.min(1), vx.max(1)
+    y0, y1 = vy.min(1), vy.max(1)
+    cx = (F(0.5) * x0 + F(0.5) * x1).astype(F)
+    cy = (F(0.5) * y0 + F(0.5) * y1).astype(F)
+    hx = (np.maximum(x1 - cx, cx - x0) * F(1.0000003) + F(1.1754944e-38)).astype(F)
+    hy = (np.maximum(y1 - cy, cy - y0) * F(1.0000003) + F(1.1754944e-38)).astype(F)
+    same = ((vx[:, 1:] == vx[:, :-1]) & (vy[:, 1:] == vy[:, :-1])).any(1)
+    ok = np.isfinite(vx).all(1) & np.isfinite(vy).all(1) & ~same
+    ok &= (np.abs(cx) + hx < F(2.0 ** 40)) & (np.abs(cy) + hy < F(2.0 ** 40))
+    hx = np.where(ok, hx, F(np.inf)).astype(F)
+    hy = np.where(ok, hy, F(np.inf)).astype(F)
+    return cx, cy, hx, hy
+
+
+def _margin(ax, ay, d1x, d1y, vx, vy):
+    # the kernel's M from the box of the target vertices (here the chunk's
+    # own, the smallest box a sequence can have)
+    rmax = (np.maximum(np.abs(vx.min(1) - ax), np.abs(vx.max(1) - ax)) +
+            np.maximum(np.abs(vy.min(1) - ay), np.abs(vy.max(1) - ay))).astype(F)
+    return (F(2.0 ** -18) * (np.abs(d1x) + np.abs(d1y)) * rmax).astype(F)
+
+
+def _skip(ax, ay, d1x, d1y, M, box, k=F(2.0)):
+    cx, cy, hx, hy = box
+    with np.errstate(all="ignore"):
+        oc = _fmaf(cx - ax, d1y, -((cy - ay) * d1x))
+        spread = _fmaf(np.abs(d1y), hx, np.abs(d1x) * hy)
+        T = _fmaf(spread, F(1.000001), _fmaf(M, k, F(2.0 ** -100)))
+    return np.abs(oc) > T, oc
+
+
+def _sides(ax, ay, d1x, d1y, M, vx, vy):
+    # screen_side for every vertex: o > M, o < -M
+    o = _fmaf(vx - ax[:, None], d1y[:, None], -((vy - ay[:, None]) * d1x[:, None]))
+    return o > M[:, None], o < -M[:, None], o
+
+
+def _chunks_near_lines(rng, n, scale, offset):
+    # a source edge a + s d1 and 17 target vertices whose o lies within a few
+    # margins of 0 on either side, spread along the line
+    ax = (rng.random(n) * scale + offset).astype(F)
+    ay = (rng.random(n) * scale - offset).astype(F)
+    ang = rng.random(n) * 2 * np.pi
+    ln = scale * 10.0 ** rng.uniform(-3, 0, n)
+    d1x = (np.cos(ang) * ln).astype(F)
+    d1y = (np.sin(ang) * ln).astype(F)
+    # spread along the line: long chunks, or chunks whose box is within a
+    # few margins across (where the skip decision is close)
+    sp = np.where(rng.random((n, 1)) < 0.5, 0.5, 10.0 ** rng.uniform(-7, -4, (n, 1)))
+    t = rng.uniform(-20, 20, (n, 1)) + np.cumsum(rng.uniform(0, 1, (n, 17)), 1) * sp
+    px = ax[:, None] + t * d1x[:, None]
+    py = ay[:, None] + t * d1y[:, None]
+    m_est = 2.0 ** -18 * (np.abs(d1x) + np.abs(d1y)) * (np.abs(px - ax[:, None]) + np.abs(py - ay[:, None])).max(1)
+    base = rng.uniform(-10, 10, (n, 1))
+    o = m_est[:, None] * (base + rng.uniform(-1.5, 1.5, (n, 17)))
+    nn = (d1x.astype(np.float64) ** 2 + d1y.astype(np.float64) ** 2)[:, None]
+    vx = (px + o * d1y[:, None] / nn).astype(F)
+    vy = (py - o * d1x[:, None] / nn).astype(F)
+    return ax, ay, d1x, d1y, vx, vy
+
+
+def _skip_violations(rng, k):
+    bad = skipped = near = 0
+    for scale, offset in ((1.0, 0.0), (1e-3, 178.0), (0.5, -60.0), (1e-5, 37.5), (20.0, 1.0)):
+        ax, ay, d1x, d1y, vx, vy = _chunks_near_lines(rng, 200_000, scale, offset)
+        M = _margin(ax, ay, d1x, d1y, vx, vy)
+        skip, oc = _skip(ax, ay, d1x, d1y, M, _chunk_boxes(vx, vy), k)
+        gt, lt, o = _sides(ax, ay, d1x, d1y, M, vx, vy)
+        ok = np.where(oc > 0, gt.all(1), lt.all(1))
+        bad += int((skip & ~ok).sum())
+        skipped += int(skip.sum())
+        near += int((skip & (np.abs(o).min(1) < 3 * M)).sum())
+    return bad, skipped, near
+
+
+def test_chunk_skip_keeps_every_vertex_beyond_the_margin():
+    bad, skipped, near = _skip_violations(np.random.default_rng(21), F(2.0))
+    assert bad == 0
+    assert skipped > 50_000 and near > 1_000  # skips happen, also close to the margin
+
+
+def test_chunk_skip_check_fails_with_a_smaller_margin():
+    # the cases reach the margin: half of M in place of 2 M breaks the skip
+    bad, _, _ = _skip_violations(np.random.default_rng(21), F(0.5))
+    assert bad > 0
+
+
+def test_chunk_skip_never_passes_over_a_zero_length_segment():
+    rng = np.random.default_rng(22)
+    ax, ay, d1x, d1y, vx, vy = _chunks_near_lines(rng, 50_000, 1.0, 0.0)
+    vx[:, 5] = vx[:, 4]
+    vy[:, 5] = vy[:, 4]
+    M = _margin(ax, ay, d1x, d1y, vx, vy)
+    skip, _ = _skip(ax, ay, d1x, d1y, M, _chunk_boxes(vx, vy))
+    assert not skip.any()
