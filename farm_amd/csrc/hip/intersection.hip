@@ -258,6 +258,8 @@ __device__ __forceinline__ void path_pt(const float *X, const float *Y, int m, i
     }
 }
 
+constexpr int kLanePath = 16;  // area-piece paths of at most this many segments are searched by one lane
+
 // One ring's pieces: D[0, n) sorted by this ring's (edge, parameter).
 // Counts the reference pieces by status into r and returns this lane's part
 // of the cross-product sum of the area pieces IN the other polygon.
@@ -271,7 +273,7 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
     double acc = 0.0;
     for (uint32_t c0 = 0; c0 < n; c0 += 64) {
         const uint32_t i = c0 + lane;
-        bool piece = false, in_area = false;
+        bool piece = false, in_area = false, long_path = false;
         int st = 0, first = 0, nvert = 0;
         float ax = 0, ay = 0, bx = 0, by = 0;
         float rx = 0, ry = 0, mx = 0, my = 0;  // the two points tested against the other polygon
@@ -309,26 +311,71 @@ __device__ double ring_pieces(const DevSet &S, const farm_raster_hdr &hp, const 
                 if (nvert < 0) nvert = 0;
                 first = ea0 + 1;
                 // the area's test point: the midpoint of the path's longest
-                // sub-segment (the first on ties), well inside the piece
-                float ux = ax, uy = ay, vx = ax, vy = ay, px = ax, py = ay;
-                double best = -1.0;
-                for (int k = 1; k <= nvert + 1; k++) {
-                    float qx, qy;
-                    path_pt(X, Y, m, first, nvert, ax, ay, bx, by, k, qx, qy);
-                    const double dx = (double)qx - px, dy = (double)qy - py;
-                    const double l = dx * dx + dy * dy;
-                    if (l > best) {
-                        best = l;
-                        ux = px;
-                        uy = py;
-                        vx = qx;
-                        vy = qy;
+                // sub-segment (the first on ties), well inside the piece; a
+                // path of more than kLanePath segments is searched by the
+                // whole wave below
+                if (nvert + 1 <= kLanePath) {
+                    float ux = ax, uy = ay, vx = ax, vy = ay, px = ax, py = ay;
+                    double best = -1.0;
+                    for (int k = 1; k <= nvert + 1; k++) {
+                        float qx, qy;
+                        path_pt(X, Y, m, first, nvert, ax, ay, bx, by, k, qx, qy);
+                        const double dx = (double)qx - px, dy = (double)qy - py;
+                        const double l = dx * dx + dy * dy;
+                        if (l > best) {
+                            best = l;
+                            ux = px;
+                            uy = py;
+                            vx = qx;
+                            vy = qy;
+                        }
+                        px = qx;
+                        py = qy;
                     }
-                    px = qx;
-                    py = qy;
+                    mx = (ux + vx) * 0.5f;
+                    my = (uy + vy) * 0.5f;
+                } else {
+                    long_path = true;
                 }
-                mx = (ux + vx) * 0.5f;
-                my = (uy + vy) * 0.5f;
+            }
+        }
+        // long paths, one at a time: segment k (points k - 1 -> k) on lane
+        // (k - 1) % 64; each lane keeps its first longest, then the wave takes
+        // the longest with the smallest k
+        uint64_t lp = __ballot(long_path);
+        while (lp) {
+            const int src = __ffsll((unsigned long long)lp) - 1;
+            lp &= lp - 1;
+            const int f = uni(__shfl(first, src, 64)), nv = uni(__shfl(nvert, src, 64));
+            const float pax = uni(__shfl(ax, src, 64)), pay = uni(__shfl(ay, src, 64));
+            const float pbx = uni(__shfl(bx, src, 64)), pby = uni(__shfl(by, src, 64));
+            double best = -1.0;
+            int bk = INT_MAX;
+            float ux = 0.f, uy = 0.f, vx = 0.f, vy = 0.f;
+            for (int k = 1 + lane; k <= nv + 1; k += 64) {
+                float x0, y0, x1, y1;
+                path_pt(X, Y, m, f, nv, pax, pay, pbx, pby, k - 1, x0, y0);
+                path_pt(X, Y, m, f, nv, pax, pay, pbx, pby, k, x1, y1);
+                const double dx = (double)x1 - x0, dy = (double)y1 - y0;
+                const double l = dx * dx + dy * dy;
+                if (l > best) {
+                    best = l;
+                    bk = k;
+                    ux = x0;
+                    uy = y0;
+                    vx = x1;
+                    vy = y1;
+                }
+            }
+            const double top = -wave_min(-best);
+            int kk = best == top ? bk : INT_MAX;
+            for (int o = 32; o > 0; o >>= 1) kk = min(kk, __shfl_xor(kk, o, 64));
+            const int wl = __ffsll((unsigned long long)__ballot(best == top && bk == kk)) - 1;
+            const float wmx = (__shfl(ux, wl, 64) + __shfl(vx, wl, 64)) * 0.5f;
+            const float wmy = (__shfl(uy, wl, 64) + __shfl(vy, wl, 64)) * 0.5f;
+            if (lane == src) {
+                mx = wmx;
+                my = wmy;
             }
         }
         const uint64_t pm = __ballot(piece);
