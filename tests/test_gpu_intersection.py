@@ -41,6 +41,25 @@ def test_complex_x_lakes_v256(device):
     assert (c[:, 0] > 0).sum() > 20 and (a > 0).sum() > 50
 
 
+def test_chunk_skip_with_refinement_boxes_of_another_size(device):
+    # after an intersect call with 32-segment chunks the refinement's boxes
+    # no longer serve the intersection's 16-segment chunk skip: it builds its
+    # own once per set, and the answers stay the oracle's
+    cro, cxy = datagen.complex_set(7, 12, 0.0, 2.0, nv_cap=20000)
+    lro, lxy = datagen.lake_set(8, 400, 0.0, 2.0)
+    w = Workload((cro, cxy), (lro, lxy), vpr=256)
+    want_c, want_a = oracle.intersection(w.OA, w.OB, w.pa, w.pb)
+    w.upload(device)
+    try:
+        device.intersect(chunk=32)
+        for _ in range(2):  # built on the first call, reused on the second
+            c, a, _ = device.intersection_area()
+            assert np.array_equal(c, want_c)
+            assert (np.abs(a - want_a) <= AREA_RTOL * np.abs(want_a) + AREA_ATOL).all()
+    finally:
+        device.intersect(chunk=16)
+
+
 def test_complex_x_lakes_v10_and_window(device):
     cro, cxy = datagen.complex_set(17, 8, 0.0, 1.5, nv_cap=8000)
     lro, lxy = datagen.lake_set(18, 300, 0.0, 1.5)
